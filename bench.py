@@ -1,0 +1,197 @@
+"""Throughput of the qTIP4P/F Suzuki-Chin PIMD step on one MI355X.
+
+Workload (default): 216 H2O in a cubic 18.64 A box, 32 beads, 298 K, FP64,
+dt = 0.25 fs, PILE-G thermostat (thermostat.cpp:64-133), Ewald parameters
+derived as forces.cpp:574-584 with eps_ewald = 1e-6 (kappa = 0.4103 1/A,
+r_c = 9.059 A, k_max = 9), seeded lattice start with random orientations,
+beads = centroid + free-ring-polymer normal-mode displacements.  One "step"
+is one full update_sc_nospline (update.cpp:271-401): 64 single-bead force
+evaluations plus propagation, thermostat and list checks, state resident in
+HBM, replayed as one HIP graph per step.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+
+Prints one JSON line.  Needs the built library (python -m qtip4pf_water_amd.build).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from qtip4pf_water_amd import config as cfg  # noqa: E402
+from qtip4pf_water_amd import flops  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X FP64 (vector and matrix), AMD published
+KERNELS = ["realspace", "ksum", "sreduce", "kforce", "ljintra", "sites", "finalize", "sc1",
+           "sc2", "prop_a", "prop_b", "check", "nlist"]
+
+
+def workload(name):
+    if name == "c2":
+        return dict(n_side=6, L_angs=18.64, P=32, T_K=298.0, dtfs=0.25, nvt=True,
+                    label="SC-PIMD NVT 216 H2O x 32 beads (PILE-G), 298 K, dt 0.25 fs")
+    if name == "c1":
+        return dict(n_side=6, L_angs=18.64, P=1, T_K=298.0, dtfs=0.25, nvt=False,
+                    label="classical NVE 216 H2O, velocity Verlet, dt 0.25 fs")
+    raise SystemExit(f"unknown workload {name}")
+
+
+def initial_state(wl, seed=2024):
+    q0, box = cfg.lattice_water_box(wl["n_side"], wl["L_angs"], seed=seed)
+    n_mols = wl["n_side"] ** 3
+    beta = 1.0 / (cfg.K2AU * wl["T_K"])
+    beta_n = beta / wl["P"]
+    q = cfg.free_ring_polymer_beads(q0, wl["P"], beta_n, seed=seed + 1)
+    p = cfg.sample_momenta(n_mols, wl["P"], beta_n, seed=seed + 2)
+    return q, p, box, n_mols
+
+
+def cpu_baseline(wl, q, p, box, n_mols, seconds):
+    """The oracle (our C restatement with the reference's OpenMP-over-beads
+    structure, forces.cpp:716) on the host cores, bounded sample."""
+    import oracle
+    oracle.build()
+    threads = max(1, min(16, os.cpu_count() or 1))
+    o = oracle.Oracle(n_mols, wl["P"], box, T_K=wl["T_K"], dtfs=wl["dtfs"])
+    qq, pp = q.copy(), p.copy()
+    point, lst, disp = o.all_lists(qq)
+    P = wl["P"]
+    F, fph = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+    rng = np.random.default_rng(7)
+    if P > 1:
+        o.force_eval_sc(qq, F, fph, point, lst, nthreads=threads)
+    else:
+        o.get_forces(qq, F, point, lst, 0)
+    n, t0 = 0, time.perf_counter()
+    heat = 0.0
+    while True:
+        if P > 1:
+            dev = rng.standard_normal(o.n_deviates()) if wl["nvt"] else None
+            if dev is not None:
+                dev[2] = 0.5 * rng.chisquare(3 * o.n_atoms - 2)
+            _, heat, _, _ = o.update_sc(qq, pp, F, fph, point, lst, disp, wl["nvt"], dev, heat,
+                                        nthreads=threads)
+        else:
+            o.update_class(qq, pp, F, point, lst, disp)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds and n >= 2:
+            break
+    return {"value": n / el, "unit": "steps/s", "cores": threads if P > 1 else 1,
+            "kind": "port",
+            "sample": f"{n} consecutive steps of the same workload on the host "
+                      f"({'OpenMP over beads, ' + str(threads) + ' threads' if P > 1 else '1 thread'}), "
+                      f"{el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--timing-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build shards nothing across GPUs "
+              "(each replica or bead set runs on one device); run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from qtip4pf_water_amd import Water
+
+    wl = workload(a.workload)
+    q, p, box, n_mols = initial_state(wl)
+    w = Water(n_mols, wl["P"], box, T_K=wl["T_K"], dtfs=wl["dtfs"], eps_ewald=1e-6,
+              epsilon_sc=1e-5)
+    d = w.derived
+    w.set_seed(12345)
+    w.set_state(q, p)
+
+    # warm-up (also captures the step graph)
+    w.run(a.warmup, is_nvt=wl["nvt"])
+    w.sync()
+    # timed region: K graph replays, bracketed by device synchronisation
+    t0 = time.perf_counter()
+    w.run(a.steps, is_nvt=wl["nvt"])
+    w.sync()
+    el = time.perf_counter() - t0
+    sps = a.steps / el
+    sc = w.scalars()
+
+    # per-kernel durations: HIP events on the library's stream around every
+    # launch, over a further block of steps launched without the graph
+    w.enable_timing(True)
+    w.run(a.timing_steps, is_nvt=wl["nvt"])
+    w.sync()
+    kt = {}
+    for k in KERNELS:
+        ms, n = w.timing(k)
+        if n:
+            kt[k] = {"ms_total": ms, "launches": n, "us_avg": 1e3 * ms / n}
+    w.enable_timing(False)
+
+    fl = flops.bead_eval_flops(n_mols, box, d["rcut"], d["k_cut"], d["k_max"])
+    evals_per_launch = wl["P"]  # every force-phase kernel launch covers P evaluations
+    dom = max((k for k in ("realspace", "ksum", "kforce", "ljintra") if k in kt),
+              key=lambda k: kt[k]["ms_total"])
+    achieved = fl[dom] * evals_per_launch / (kt[dom]["us_avg"] * 1e-6) / 1e12
+    step_flops = 2 * wl["P"] * fl["total"] if wl["P"] > 1 else fl["total"]
+
+    res = {
+        "metric": "PIMD steps/s 216 H2O x 32 beads" if wl["P"] > 1 else "MD steps/s 216 H2O",
+        "value": sps,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1e3 / sps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded lattice start, free-ring-polymer beads)",
+        "config": {
+            "workload": wl["label"],
+            "n_mols": n_mols, "n_beads": wl["P"], "box_angs": wl["L_angs"],
+            "dt_fs": wl["dtfs"], "T_K": wl["T_K"], "parallelism": "single GPU",
+            "ewald": {"kappa_per_angs": d["kappa"] * cfg.ANGS2BOHR,
+                      "rcut_angs": d["rcut"] / cfg.ANGS2BOHR, "k_max": d["k_max"],
+                      "k_groups": d["n_kgroups"]},
+        },
+        "ns_per_day": sps * wl["dtfs"] * 0.0864,
+        "algorithmic_tflops": step_flops * sps / 1e12,
+        "roofline": {
+            "bound": "mfma",
+            "kernel": dom,
+            "achieved": achieved,
+            "peak": PEAK_FP64_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP64_TFLOPS,
+            "traffic": None,
+            "flops_per_launch": fl[dom] * evals_per_launch,
+            "us_per_launch": kt[dom]["us_avg"],
+        },
+        "kernel_us": {k: round(v["us_avg"], 2) for k, v in kt.items()},
+        "state": {"energy": sc["energy"], "rebuilds": sc["rebuilds"], "steps": sc["steps"],
+                  "status": sc["status"]},
+    }
+    w.close()
+    if not a.no_cpu:
+        res["cpu_baseline"] = cpu_baseline(wl, q, p, box, n_mols, a.cpu_seconds)
+        res["speedup_vs_cpu"] = sps / res["cpu_baseline"]["value"]
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,176 @@
+/*
+ * qtip4pf.h -- C ABI of the MI355X (gfx950) qTIP4P/F path-integral step.
+ *
+ * A drop-in for the hot-path calls of the rayseashyn/qtip4pf_water C++ driver
+ * (simulation.cpp / water.cpp stay the caller).  Every reference routine the
+ * driver calls has an entry point here taking the same arrays in the same
+ * layout (indexing.h:6-24, bead-major, 9 doubles per molecule O,H1,H2):
+ *
+ *   q[b*n_dof + 9*mol + 3*site + d], point[b*n_mols + i],
+ *   list[b*list_size + k], disp[b*3*n_mols + 3*mol + d]
+ *
+ * The reference keeps its parameters in process globals (water.cpp:29-38);
+ * here they live in a context created from qtp_params.  All arithmetic is
+ * FP64.  Functions return QTP_OK (0) or a negative status; the message of the
+ * last failure is available from qtp_last_error().  No exceptions cross the
+ * ABI.  A context serves one host thread at a time.
+ *
+ * Two ways to drive it:
+ *   - reference-shaped calls with host arrays (qtp_get_forces_nospline ...),
+ *     each a full host->device->host round trip, for the reference driver;
+ *   - resident mode (qtp_set_state / qtp_run_sc / qtp_get_state): state stays
+ *     in HBM, the whole step is one replayed HIP graph, no per-step host sync.
+ */
+#ifndef QTIP4PF_H
+#define QTIP4PF_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QTP_OK 0
+#define QTP_ERR_ARG (-1)           /* bad argument or size */
+#define QTP_ERR_HIP (-2)           /* HIP runtime failure */
+#define QTP_ERR_LIST_OVERFLOW (-3) /* Verlet list larger than list_size */
+#define QTP_ERR_NONFINITE (-4)     /* NaN/Inf in positions or forces */
+#define QTP_ERR_STATE (-5)         /* call out of order (e.g. no state set) */
+
+typedef struct qtp_ctx qtp_ctx;
+
+/* Inputs of water.cpp:44-66 that the step depends on, plus the box (the
+ * reference derives it from its lattice, water.cpp:90-120; here it is given).
+ * Zero/negative optional fields take the reference's derivation. */
+typedef struct {
+    int n_mols;
+    int n_beads;          /* 1 = classical, even >= 2 = Suzuki-Chin (water.cpp:69-72) */
+    double boxl[3];       /* bohr */
+    double T_K;           /* config.txt T_K */
+    double dtfs;          /* config.txt dtfs */
+    double eps_ewald;     /* config.txt eps_ewald (water.cpp:148) */
+    double epsilon_sc;    /* config.txt epsilon_sc */
+    int list_size;        /* 0: round(4 r_skin^3 n_mols rho_ox) (water.cpp:135) */
+    /* Ewald override; ewald_kappa <= 0 derives (kappa, r_c, k_cut, k_max)
+     * inside each force call as forces.cpp:574-584 does. */
+    double ewald_kappa, ewald_rcut, ewald_kcut;
+    int ewald_kmax;
+} qtp_params;
+
+/* Derived globals (water.cpp:74-157) and Ewald parameters (forces.cpp:574-584). */
+typedef struct {
+    int n_mols, n_atoms, n_dof, n_beads, list_size;
+    double boxl[3], vol, min_box, max_box;
+    double beta, beta_n, piovern, omega_p, omega_psq, dt, halfdt, epsilonsc;
+    double acc, rho, kwald_prefactor, sd_ox, sd_h;
+    double kappa, rcut, k_cut;
+    int k_max, n_kgroups;
+} qtp_derived;
+
+int qtp_create(const qtp_params *prm, qtp_ctx **out);
+void qtp_destroy(qtp_ctx *ctx);
+const char *qtp_last_error(const qtp_ctx *ctx);
+int qtp_get_derived(const qtp_ctx *ctx, qtp_derived *out);
+/* The HIP stream (hipStream_t) every call of this context runs on. */
+int qtp_get_stream(const qtp_ctx *ctx, void **stream);
+
+/* ---- reference-shaped calls (host arrays, reference layout) ----------- */
+
+/* forces.cpp:468-630 get_forces_nospline(q, F, charge, point, list, bead):
+ * zeroes and fills the bead slice of F, returns the bead energy in *energy. */
+int qtp_get_forces_nospline(qtp_ctx *ctx, const double *q, double *F, const double *charge,
+                            const int *point, const int *list, int bead, double *energy);
+
+/* forces.cpp:698-789 force_eval_sc_nospline: Suzuki-Chin forces F_sc and
+ * physical forces fphys for all beads; *energy = SC conserved potential,
+ * *sc_pot = (2/P) sum_even V.  fphys may be NULL (as the reference default). */
+int qtp_force_eval_sc_nospline(qtp_ctx *ctx, const double *q, double *F_sc,
+                               const double *charge, const double *mass, const int *point,
+                               const int *list, double *sc_pot, double *fphys, double *energy);
+
+/* neighbour.cpp:12-62 neighbour_list(q, point, list, disp, bead): O-O half
+ * list within r_skin, zeroes the bead's disp.  Returns QTP_ERR_LIST_OVERFLOW
+ * (and writes no entry past list_size) where the reference would overflow. */
+int qtp_neighbour_list(qtp_ctx *ctx, const double *q, int *point, int *list, double *disp,
+                       int bead);
+
+/* fourier.cpp:12-69 ftransform(a, length, is_forwards): in-place orthonormal
+ * bead <-> normal-mode transform of a[n_beads*length]. */
+int qtp_ftransform(qtp_ctx *ctx, double *a, int length, int is_forwards);
+
+/* thermostat.cpp:64-133 pile_g(p_nm, mass, rng) with the random numbers
+ * drawn by the caller, in the reference's order:
+ *   dev[0] = R, dev[1] = R_add, dev[2] = gamma((dof-2)/2) draw, dev[3..] = xi
+ *   for bead 1..P-1, dof 0..n_dof-1   (3 + (P-1)*n_dof values).
+ * *heat receives the return value of pile_g (KE before - KE after). */
+int qtp_pile_g(qtp_ctx *ctx, double *p_nm, const double *mass, const double *dev,
+               double *heat);
+
+/* update.cpp:271-401 update_sc_nospline: one B-A-O-A-B Suzuki-Chin step.
+ * dev as qtp_pile_g (ignored unless is_nvt).  *heat -= pile_g's return as in
+ * update.cpp:325; *energy = SC conserved potential. */
+int qtp_update_sc_nospline(qtp_ctx *ctx, double *q, double *p, const double *mass, double *F,
+                           double *fphys, const double *charge, int *point, int *list,
+                           double *disp, const double *dev, int is_nvt, double *heat,
+                           double *sc_pot, double *energy);
+
+/* update.cpp:78-137 update_trajectory_class_nospline: one velocity-Verlet step. */
+int qtp_update_trajectory_class_nospline(qtp_ctx *ctx, double *q, double *p, const double *mass,
+                                         double *F, const double *charge, int *point, int *list,
+                                         double *disp, double *energy);
+
+/* path_integrals.cpp:29-48 get_kcv(q, fphys) */
+int qtp_get_kcv(qtp_ctx *ctx, const double *q, const double *fphys, double *kcv);
+/* path_integrals.cpp:52-81 get_conserved(q, p, mass, energy) (q and p are
+ * not modified here; the reference round-trips them through ftransform). */
+int qtp_get_conserved(qtp_ctx *ctx, const double *q, const double *p, const double *mass,
+                      double energy, double *cons);
+
+/* Host-side draws in the reference's order from a std::mt19937 owned by the
+ * library (for callers without a C++ rng; the C++ adapter uses the caller's). */
+typedef struct qtp_host_rng qtp_host_rng;
+int qtp_host_rng_create(uint32_t seed, qtp_host_rng **out);
+void qtp_host_rng_destroy(qtp_host_rng *r);
+int qtp_host_rng_draw_pile(qtp_host_rng *r, int n_atoms, int n_beads, double *dev);
+
+/* ---- resident mode ------------------------------------------------------ */
+
+/* Upload q and p (reference layout), charges and masses (NULL: qTIP4P/F
+ * values of simulation.cpp:811-822), build every bead's list and the initial
+ * forces (simulation.cpp:839-874). */
+int qtp_set_state(qtp_ctx *ctx, const double *q, const double *p, const double *charge,
+                  const double *mass);
+/* Download state in reference layout; any pointer may be NULL. */
+int qtp_get_state(qtp_ctx *ctx, double *q, double *p, double *F, double *fphys, double *disp);
+/* Download the half lists in reference layout. */
+int qtp_get_lists(qtp_ctx *ctx, int *point, int *list);
+
+/* Thermostat random numbers for resident NVT steps: Philox4x32-10 streams
+ * keyed by seed and the device step counter, Box-Muller normals; the centroid
+ * chi-square is a sum of squared normals.  Default seed 0. */
+int qtp_set_seed(qtp_ctx *ctx, uint64_t seed);
+
+/* Advance n_steps resident steps (SC for n_beads >= 2, velocity Verlet for
+ * n_beads == 1).  is_nvt applies PILE-G (SC only).  Asynchronous: returns once
+ * the work is queued; qtp_sync waits. */
+int qtp_run(qtp_ctx *ctx, int n_steps, int is_nvt);
+int qtp_sync(qtp_ctx *ctx);
+
+/* Scalars of the last resident step: [0] energy (SC conserved potential or
+ * classical V), [1] sc_pot, [2] accumulated heat (update.cpp:325 sign),
+ * [3] list rebuilds so far, [4] steps so far, [5] status flags. */
+int qtp_get_scalars(qtp_ctx *ctx, double out[6]);
+
+/* Per-kernel timing of the last qtp_run (HIP events around each launch of the
+ * named kernel class on the context's stream); for measurement only. */
+int qtp_enable_timing(qtp_ctx *ctx, int on);
+int qtp_get_timing(qtp_ctx *ctx, const char *kernel, double *ms_total, int *launches);
+
+/* The Philox4x32-10 block function the device thermostat uses (host build of
+ * the same code), for known-answer checks.  No GPU needed. */
+int qtp_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

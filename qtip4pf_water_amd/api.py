@@ -1,0 +1,257 @@
+"""Reference-shaped Python interface over the gfx950 library.
+
+`Water` holds what the reference keeps in process globals (water.cpp:29-38)
+and exposes the hot-path routines under the reference's names with the same
+argument meaning and array layout (indexing.h:6-24):
+
+    get_forces_nospline        forces.cpp:468   (forces.h:16)
+    force_eval_sc_nospline     forces.cpp:698   (forces.h:20)
+    neighbour_list             neighbour.cpp:12 (neighbour.h:4)
+    ftransform                 fourier.cpp:12   (fourier.h:4)
+    pile_g                     thermostat.cpp:64 (thermostat.h:6)
+    update_sc_nospline         update.cpp:271   (update.h:7)
+    update_trajectory_class_nospline  update.cpp:78 (update.h:5)
+    get_kcv / get_conserved    path_integrals.cpp:29, :52
+
+Arrays are numpy float64 / int32, modified in place like the reference's.
+Random numbers of pile_g are passed in (`dev`) in the order the reference
+draws them; `HostRng` draws them from std::mt19937 with libstdc++.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import QtpDerived, QtpError, QtpParams, ListOverflow, dp, ip, ptr
+
+
+def _f64(a):
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous float64 numpy array")
+    return a
+
+
+def _i32(a):
+    if not (isinstance(a, np.ndarray) and a.dtype == np.int32 and a.flags.c_contiguous):
+        raise TypeError("expected a C-contiguous int32 numpy array")
+    return a
+
+
+class HostRng:
+    """std::mt19937 owned by the library; draws pile_g's deviates in the
+    reference's order (thermostat.cpp:71, 88-98, 124)."""
+
+    def __init__(self, seed: int):
+        self._L = _lib.load()
+        h = C.c_void_p()
+        rc = self._L.qtp_host_rng_create(C.c_uint32(seed & 0xFFFFFFFF), C.byref(h))
+        if rc != 0:
+            raise QtpError(rc, "host rng")
+        self._h = h
+
+    def draw_pile(self, n_atoms: int, n_beads: int) -> np.ndarray:
+        out = np.zeros(3 + (n_beads - 1) * 3 * n_atoms)
+        rc = self._L.qtp_host_rng_draw_pile(self._h, int(n_atoms), int(n_beads), ptr(out))
+        if rc != 0:
+            raise QtpError(rc, "host rng draw")
+        return out
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._L.qtp_host_rng_destroy(self._h)
+        except Exception:
+            pass
+
+
+def philox4x32_10(ctr, key):
+    L = _lib.load()
+    c = np.asarray(ctr, dtype=np.uint32).copy()
+    k = np.asarray(key, dtype=np.uint32).copy()
+    o = np.zeros(4, dtype=np.uint32)
+    rc = L.qtp_philox4x32_10(ptr(c, _lib.u32p), ptr(k, _lib.u32p), ptr(o, _lib.u32p))
+    if rc != 0:
+        raise QtpError(rc, "philox")
+    return o
+
+
+class Water:
+    """One qTIP4P/F system on the GPU (the reference's globals + device state)."""
+
+    def __init__(self, n_mols, n_beads, boxl, T_K=298.0, dtfs=0.25, eps_ewald=1e-6,
+                 epsilon_sc=1e-5, list_size=0, ewald=None):
+        self._L = _lib.load()
+        p = QtpParams()
+        p.n_mols = int(n_mols)
+        p.n_beads = int(n_beads)
+        for k in range(3):
+            p.boxl[k] = float(boxl[k])
+        p.T_K, p.dtfs, p.eps_ewald, p.epsilon_sc = float(T_K), float(dtfs), float(eps_ewald), float(epsilon_sc)
+        p.list_size = int(list_size)
+        if ewald is not None:
+            p.ewald_kappa, p.ewald_rcut, p.ewald_kcut, p.ewald_kmax = (
+                float(ewald["kappa"]), float(ewald["rcut"]), float(ewald["k_cut"]), int(ewald["k_max"]))
+        h = C.c_void_p()
+        rc = self._L.qtp_create(C.byref(p), C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self._L.qtp_last_error(h).decode() if h else "qtp_create"
+            self.close()
+            raise QtpError(rc, msg)
+        d = QtpDerived()
+        self._chk(self._L.qtp_get_derived(h, C.byref(d)))
+        self.derived = {f: (list(getattr(d, f)) if f == "boxl" else getattr(d, f))
+                        for f, _ in QtpDerived._fields_}
+        self.n_mols, self.n_atoms, self.n_dof = d.n_mols, d.n_atoms, d.n_dof
+        self.n_beads, self.list_size = d.n_beads, d.list_size
+
+    # ---------------------------------------------------------------- misc
+    def _chk(self, rc):
+        if rc != 0:
+            msg = self._L.qtp_last_error(self._h).decode()
+            if rc == -3:
+                raise ListOverflow(rc, msg)
+            raise QtpError(rc, msg)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.qtp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def stream(self) -> int:
+        s = C.c_void_p()
+        self._chk(self._L.qtp_get_stream(self._h, C.byref(s)))
+        return s.value or 0
+
+    def new_lists(self):
+        return (np.zeros(self.n_beads * self.n_mols, np.int32),
+                np.zeros(self.n_beads * self.list_size, np.int32),
+                np.zeros(self.n_beads * 3 * self.n_mols))
+
+    # ------------------------------------------------ reference-shaped calls
+    def get_forces_nospline(self, q, F, charge, point, lst, bead=0) -> float:
+        e = C.c_double()
+        self._chk(self._L.qtp_get_forces_nospline(self._h, ptr(_f64(q)), ptr(_f64(F)),
+                                                  ptr(_f64(charge)) if charge is not None else None,
+                                                  ptr(_i32(point), ip), ptr(_i32(lst), ip),
+                                                  int(bead), C.byref(e)))
+        return e.value
+
+    def force_eval_sc_nospline(self, q, F_sc, charge, mass, point, lst, fphys=None):
+        """Returns (tot_energy, sc_pot) as the reference's return value and
+        sc_pot reference argument."""
+        e, sc = C.c_double(), C.c_double()
+        self._chk(self._L.qtp_force_eval_sc_nospline(
+            self._h, ptr(_f64(q)), ptr(_f64(F_sc)),
+            ptr(_f64(charge)) if charge is not None else None,
+            ptr(_f64(mass)) if mass is not None else None,
+            ptr(_i32(point), ip), ptr(_i32(lst), ip), C.byref(sc),
+            ptr(_f64(fphys)) if fphys is not None else None, C.byref(e)))
+        return e.value, sc.value
+
+    def neighbour_list(self, q, point, lst, disp, bead=0):
+        self._chk(self._L.qtp_neighbour_list(self._h, ptr(_f64(q)), ptr(_i32(point), ip),
+                                             ptr(_i32(lst), ip),
+                                             ptr(_f64(disp)) if disp is not None else None,
+                                             int(bead)))
+
+    def ftransform(self, a, length, is_forwards):
+        self._chk(self._L.qtp_ftransform(self._h, ptr(_f64(a)), int(length), 1 if is_forwards else 0))
+
+    def pile_g(self, p_nm, mass, dev) -> float:
+        h = C.c_double()
+        self._chk(self._L.qtp_pile_g(self._h, ptr(_f64(p_nm)),
+                                     ptr(_f64(mass)) if mass is not None else None,
+                                     ptr(_f64(dev)), C.byref(h)))
+        return h.value
+
+    def update_sc_nospline(self, q, p, mass, F, fphys, charge, point, lst, disp, dev, is_nvt,
+                           heat):
+        """One SC step.  Returns (energy, heat, sc_pot)."""
+        h, sc, e = C.c_double(heat), C.c_double(), C.c_double()
+        self._chk(self._L.qtp_update_sc_nospline(
+            self._h, ptr(_f64(q)), ptr(_f64(p)),
+            ptr(_f64(mass)) if mass is not None else None, ptr(_f64(F)),
+            ptr(_f64(fphys)) if fphys is not None else None,
+            ptr(_f64(charge)) if charge is not None else None,
+            ptr(_i32(point), ip), ptr(_i32(lst), ip), ptr(_f64(disp)),
+            ptr(_f64(dev)) if dev is not None else None, 1 if is_nvt else 0,
+            C.byref(h), C.byref(sc), C.byref(e)))
+        return e.value, h.value, sc.value
+
+    def update_trajectory_class_nospline(self, q, p, mass, F, charge, point, lst, disp) -> float:
+        e = C.c_double()
+        self._chk(self._L.qtp_update_trajectory_class_nospline(
+            self._h, ptr(_f64(q)), ptr(_f64(p)),
+            ptr(_f64(mass)) if mass is not None else None, ptr(_f64(F)),
+            ptr(_f64(charge)) if charge is not None else None,
+            ptr(_i32(point), ip), ptr(_i32(lst), ip), ptr(_f64(disp)), C.byref(e)))
+        return e.value
+
+    def get_kcv(self, q, fphys) -> float:
+        v = C.c_double()
+        self._chk(self._L.qtp_get_kcv(self._h, ptr(_f64(q)), ptr(_f64(fphys)), C.byref(v)))
+        return v.value
+
+    def get_conserved(self, q, p, mass, energy) -> float:
+        v = C.c_double()
+        self._chk(self._L.qtp_get_conserved(self._h, ptr(_f64(q)), ptr(_f64(p)),
+                                            ptr(_f64(mass)) if mass is not None else None,
+                                            float(energy), C.byref(v)))
+        return v.value
+
+    # ------------------------------------------------------- resident mode
+    def set_state(self, q, p, charge=None, mass=None):
+        self._chk(self._L.qtp_set_state(self._h, ptr(_f64(q)), ptr(_f64(p)),
+                                        ptr(_f64(charge)) if charge is not None else None,
+                                        ptr(_f64(mass)) if mass is not None else None))
+
+    def get_state(self):
+        n = self.n_beads * self.n_dof
+        q, p, F, fph = (np.zeros(n) for _ in range(4))
+        disp = np.zeros(self.n_beads * 3 * self.n_mols)
+        self._chk(self._L.qtp_get_state(self._h, ptr(q), ptr(p), ptr(F), ptr(fph), ptr(disp)))
+        return q, p, F, fph, disp
+
+    def get_lists(self):
+        point = np.zeros(self.n_beads * self.n_mols, np.int32)
+        lst = np.zeros(self.n_beads * self.list_size, np.int32)
+        self._chk(self._L.qtp_get_lists(self._h, ptr(point, ip), ptr(lst, ip)))
+        return point, lst
+
+    def set_seed(self, seed: int):
+        self._chk(self._L.qtp_set_seed(self._h, C.c_uint64(seed)))
+
+    def run(self, n_steps: int, is_nvt: bool = False):
+        self._chk(self._L.qtp_run(self._h, int(n_steps), 1 if is_nvt else 0))
+
+    def sync(self):
+        self._chk(self._L.qtp_sync(self._h))
+
+    def scalars(self) -> dict:
+        o = np.zeros(6)
+        self._chk(self._L.qtp_get_scalars(self._h, ptr(o)))
+        return {"energy": o[0], "sc_pot": o[1], "heat": o[2], "rebuilds": int(o[3]),
+                "steps": int(o[4]), "status": int(o[5])}
+
+    def enable_timing(self, on=True):
+        self._chk(self._L.qtp_enable_timing(self._h, 1 if on else 0))
+
+    def timing(self, kernel: str):
+        ms, n = C.c_double(), C.c_int()
+        self._chk(self._L.qtp_get_timing(self._h, kernel.encode(), C.byref(ms), C.byref(n)))
+        return ms.value, n.value

@@ -183,3 +183,32 @@ def read_config(path: str) -> dict:
     if out["from_file"] == "yes":
         out["filename"] = toks[k]
     return out
+
+
+def free_ring_polymer_beads(q0, n_beads, beta_n, seed=0):
+    """Beads = centroid copies plus Gaussian free-ring-polymer normal-mode
+    displacements: for mode k >= 1 (frequency w_k = 2 sin(k pi/P)/beta_n) the
+    normal-mode coordinate is N(0, 1/sqrt(beta_n m w_k^2)); the centroid is
+    unchanged.  Modes use the orthonormal real transform of fourier.cpp.
+    (The reference starts every bead as a copy, initialise.cpp:128-132.)"""
+    P = int(n_beads)
+    q0 = np.asarray(q0, dtype=np.float64)
+    if P == 1:
+        return q0.copy()
+    n_mols = q0.size // 9
+    mass = np.tile(np.repeat([MO, MH, MH], 3), n_mols)
+    rng = np.random.default_rng(seed)
+    j = np.arange(P)
+    C = np.zeros((P, P))
+    C[0] = 1.0 / math.sqrt(P)
+    for k in range(1, (P + 1) // 2):
+        C[k] = math.sqrt(2.0 / P) * np.cos(2 * np.pi * j * k / P)
+        C[P - k] = -math.sqrt(2.0 / P) * np.sin(2 * np.pi * j * k / P)
+    if P % 2 == 0:
+        C[P // 2] = (-1.0) ** j / math.sqrt(P)
+    u = np.zeros((P, q0.size))
+    u[0] = math.sqrt(P) * q0
+    for k in range(1, P):
+        wk = 2.0 * math.sin(k * math.pi / P) / beta_n
+        u[k] = rng.standard_normal(q0.size) / np.sqrt(beta_n * mass * wk * wk)
+    return np.ascontiguousarray((C.T @ u).reshape(-1))

@@ -1,0 +1,1407 @@
+// qtip4pf.hip -- host orchestration and C ABI of the gfx950 qTIP4P/F step.
+//
+// One context = one system (the reference's process globals, water.cpp:29-38)
+// with its device buffers, one HIP stream and (resident mode) one captured
+// HIP graph per step kind.  See include/qtip4pf.h for the entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/qtip4pf.h"
+#include "qtp_kernels.hpp"
+
+using namespace qtp;
+
+namespace {
+
+// qtip4pfparams.h:43-133, evaluated in the reference's order (pi truncated, :45)
+struct RefConsts {
+    double pi, angs2bohr, me, kb, Na, eh, kcal2au, mo, mh, K2au, au2fs, fs2au;
+    double r_cut, r_cut_sq, r_skin, r_skin_sq;
+    double gamma_vs, epslj, sigmalj, sigmalj6, sigmalj12, e_cut;
+    double roheq, ar, ar2, ar3, ar4fac, Dr, theta_eq, ktheta, qo, qh;
+    RefConsts()
+    {
+        pi = 3.14159265359;
+        angs2bohr = 1.0 / 0.529177210903;
+        me = 9.10938356e-31;
+        kb = 1.38064852e-23;
+        Na = 6.0221409e23;
+        eh = 4.3597447222071e-18;
+        kcal2au = 1.5936e-3;
+        const double mhgmol = 1.0080, mogmol = 15.9994;
+        mo = 0.001 * mogmol / (Na * me);
+        mh = 0.001 * mhgmol / (Na * me);
+        K2au = kb / eh;
+        const double hbar_SI = 1.054571817e-34;
+        au2fs = hbar_SI * 1e15 / eh;
+        fs2au = 1.0 / au2fs;
+        r_cut = 9.0 * angs2bohr;
+        r_cut_sq = r_cut * r_cut;
+        r_skin = 1.30 * r_cut;
+        r_skin_sq = r_skin * r_skin;
+        gamma_vs = 0.73612;
+        epslj = 0.1852 * kcal2au;
+        sigmalj = 3.1589 * angs2bohr;
+        sigmalj6 = std::pow(sigmalj, 6.0);
+        sigmalj12 = sigmalj6 * sigmalj6;
+        e_cut = 4.0 * epslj * (std::pow((sigmalj / r_cut), 12.0) - std::pow((sigmalj / r_cut), 6.0));
+        roheq = 0.9419 * angs2bohr;
+        ar = 2.287 / angs2bohr;
+        ar2 = ar * ar;
+        ar3 = ar2 * ar;
+        ar4fac = 7.0 * ar3 * ar / 12.0;
+        Dr = 116.09 * kcal2au;
+        theta_eq = 107.4 * pi / 180.0;
+        ktheta = 87.85 * kcal2au;
+        qo = -1.1128;
+        qh = -0.5 * qo;
+    }
+};
+const RefConsts RC;
+
+struct Timer {
+    double ms = 0.0;
+    int n = 0;
+};
+
+}  // namespace
+
+struct qtp_host_rng {
+    std::mt19937 gen;
+};
+
+struct qtp_ctx {
+    qtp_params prm{};
+    qtp_derived drv{};
+    std::string err;
+    hipStream_t stream = nullptr;
+
+    int P = 0, n_mols = 0, n_atoms = 0, n_dof = 0, NAP = 0, NMP = 0, nblk = 0, list_size = 0;
+    long fcap = 0;
+    Box box{};
+    FF ff{};
+    Ewald ew{};
+    double eself = 0.0;  // sum_i 0.5 * (2 kappa/sqrt(pi)) q_i^2 (forces.cpp:622-624)
+    int G = 0;           // k groups
+    int ntile = 0;       // real-space tiles per evaluation
+    int nmb = 0;         // LJ/intra blocks per evaluation
+    int AC = 64, nac = 0, ngc = 0, ngr = 0;
+    int ep_stride = 0, ep_lj = 0, ep_real = 0, ep_k = 0;
+    int ptile = 0;       // propagation tiles (64 planar dofs each)
+    size_t kforce_lds = 0, ksum_lds = 0, prop_lds = 0;
+
+    // device state (planar SoA)
+    double *q = nullptr, *p = nullptr, *F = nullptr, *fph = nullptr, *disp = nullptr;
+    double *m_atom = nullptr, *sqm_atom = nullptr, *charge = nullptr;
+    int *point = nullptr, *list = nullptr, *fpoint = nullptr, *flist = nullptr, *cnt = nullptr;
+    int *flag = nullptr, *status = nullptr;
+    double *nreb = nullptr;
+    // force-phase scratch (E <= max(P,1))
+    double *qpm = nullptr, *cpos = nullptr, *rpart = nullptr, *spart = nullptr, *sprime = nullptr;
+    double *ef = nullptr, *flj = nullptr, *fpm = nullptr, *Ve = nullptr, *epart = nullptr;
+    double *scal = nullptr, *out = nullptr, *heat = nullptr, *heat_step = nullptr;
+    unsigned long long *step_ctr = nullptr;
+    // constants
+    int4 *groups = nullptr;
+    double *gcf = nullptr;
+    double4 *grk = nullptr;
+    int2 *tiles = nullptr;
+    double *Cm = nullptr;
+    double4 *mode = nullptr;
+    // propagation scratch
+    double *nq = nullptr, *np = nullptr, *ppart = nullptr, *hpart = nullptr, *dev = nullptr;
+    double *obs = nullptr;
+    long dev_cap = 0;
+    uint64_t seed = 0;
+
+    // host copies
+    std::vector<double> h_mass_atom, h_charge;
+    bool state_set = false;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // [is_nvt]
+    bool timing = false;
+    std::map<std::string, Timer> timers;
+    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> evpool;
+    size_t evnext = 0;
+};
+
+// ---------------------------------------------------------------------------
+#define HIPCHK(x)                                                                    \
+    do {                                                                             \
+        hipError_t e_ = (x);                                                         \
+        if (e_ != hipSuccess) {                                                      \
+            ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);               \
+            return QTP_ERR_HIP;                                                      \
+        }                                                                            \
+    } while (0)
+
+#define CHK(x)                                                                       \
+    do {                                                                             \
+        int s_ = (x);                                                                \
+        if (s_ != QTP_OK) return s_;                                                 \
+    } while (0)
+
+template <class T>
+static int dalloc(qtp_ctx *ctx, T **p, size_t n)
+{
+    HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
+    HIPCHK(hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+    return QTP_OK;
+}
+
+// Timed launch bracket (only when timing is enabled; never inside a graph).
+static hipEvent_t ev_get(qtp_ctx *ctx)
+{
+    if (ctx->evnext >= ctx->evpool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->evpool.push_back(e);
+    }
+    return ctx->evpool[ctx->evnext++];
+}
+
+struct TimeScope {
+    qtp_ctx *ctx;
+    const char *name;
+    hipEvent_t a = nullptr, b = nullptr;
+    TimeScope(qtp_ctx *c, const char *n) : ctx(c), name(n)
+    {
+        if (ctx->timing) {
+            a = ev_get(ctx);
+            b = ev_get(ctx);
+            if (a) hipEventRecord(a, ctx->stream);
+        }
+    }
+    ~TimeScope()
+    {
+        if (ctx->timing && a && b) {
+            hipEventRecord(b, ctx->stream);
+            ctx->pending.push_back({name, {a, b}});
+        }
+    }
+};
+
+static int flush_timing(qtp_ctx *ctx)
+{
+    if (ctx->pending.empty()) return QTP_OK;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (auto &pe : ctx->pending) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+        Timer &t = ctx->timers[pe.first];
+        t.ms += ms;
+        t.n += 1;
+    }
+    ctx->pending.clear();
+    ctx->evnext = 0;
+    return QTP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// set-up
+// ---------------------------------------------------------------------------
+static void derive(qtp_ctx *ctx)
+{
+    const qtp_params &P = ctx->prm;
+    qtp_derived &d = ctx->drv;
+    d.n_mols = P.n_mols;
+    d.n_atoms = 3 * P.n_mols;
+    d.n_dof = 3 * d.n_atoms;
+    d.n_beads = P.n_beads;
+    for (int k = 0; k < 3; ++k) d.boxl[k] = P.boxl[k];
+    d.vol = P.boxl[0] * P.boxl[1] * P.boxl[2];
+    const double rho_ox = P.n_mols / d.vol;
+    d.list_size = P.list_size > 0
+                      ? P.list_size
+                      : (int)std::round(4.0 * RC.r_skin * RC.r_skin * RC.r_skin * P.n_mols * rho_ox);
+    // water.cpp:76-83, 138-157
+    const double T_red = RC.K2au * P.T_K;
+    d.beta = 1.0 / T_red;
+    d.piovern = RC.pi / P.n_beads;
+    d.beta_n = d.beta / P.n_beads;
+    d.omega_p = 1.0 / (d.beta_n * 1.0);
+    d.omega_psq = d.omega_p * d.omega_p;
+    d.sd_ox = std::sqrt(RC.mo / d.beta_n);
+    d.sd_h = std::sqrt(RC.mh / d.beta_n);
+    d.dt = P.dtfs * RC.fs2au;
+    d.halfdt = 0.5 * d.dt;
+    d.epsilonsc = P.epsilon_sc;
+    d.acc = std::sqrt(-std::log(P.eps_ewald));
+    d.rho = d.n_atoms / d.vol;
+    const double min_xy = std::min(P.boxl[0], P.boxl[1]);
+    d.min_box = std::min(min_xy, P.boxl[2]);
+    d.kwald_prefactor = 2.0 * RC.pi / d.vol;
+    const double max_xy = std::max(P.boxl[0], P.boxl[1]);
+    d.max_box = std::max(max_xy, P.boxl[2]);
+    // forces.cpp:574-584
+    if (P.ewald_kappa > 0.0) {
+        d.kappa = P.ewald_kappa;
+        d.rcut = P.ewald_rcut;
+        d.k_cut = P.ewald_kcut;
+        d.k_max = P.ewald_kmax;
+    } else {
+        double kappa = 2.6 * std::pow(d.rho / d.vol, 1.0 / 6.0);
+        double rcut = d.acc / kappa;
+        if (rcut > 0.5 * d.min_box) {
+            rcut = 0.5 * d.min_box;
+            kappa = d.acc / rcut;
+        }
+        d.kappa = kappa;
+        d.rcut = rcut;
+        d.k_cut = 2.0 * d.acc * kappa;
+        d.k_max = (int)(d.k_cut * d.max_box / (2.0 * RC.pi));
+    }
+}
+
+// k-vector groups in the reference's inclusion rules (forces.cpp:251-445):
+// kx-only: 0 < k^2 < k_cut^2; (kx,0,kz): kz loop breaks at k^2 > k_cut^2;
+// (kx,ky,0): ky loop breaks likewise; (kx,ky,kz) inside an unbroken ky.
+// Coefficient c = the reference's exp_fac (kx-only: kwald_prefactor * exp_fac).
+static void build_groups(qtp_ctx *ctx, std::vector<int4> &g, std::vector<double> &cf,
+                         std::vector<double4> &rk)
+{
+    const qtp_derived &d = ctx->drv;
+    const int K = d.k_max + 1;
+    const double kc2 = d.k_cut * d.k_cut;
+    const double q4 = 0.25 / (d.kappa * d.kappa);
+    double bk[3];
+    for (int k = 0; k < 3; ++k) bk[k] = 2.0 * RC.pi / d.boxl[k];
+    const double kp = d.kwald_prefactor;
+    for (int kx = 0; kx < K; ++kx) {
+        const double r0 = bk[0] * kx;
+        double ksq = r0 * r0;
+        if (ksq > 0.0 && ksq < kc2) {
+            double ef0 = std::exp(-ksq * q4) / ksq;
+            if (kx != 0) ef0 *= 2.0;
+            g.push_back(make_int4(kx, 0, 0, 1));
+            cf.push_back(kp * ef0);
+            rk.push_back(make_double4(r0, 0.0, 0.0, 0.0));
+        }
+        for (int kz = 1; kz < K; ++kz) {
+            const double r2 = bk[2] * kz;
+            ksq = r0 * r0 + r2 * r2;
+            if (ksq > kc2) break;
+            double ef = kp * std::exp(-ksq * q4) / ksq;
+            if (kx != 0) ef *= 2.0;
+            g.push_back(make_int4(kx, 0, kz, 3));
+            cf.push_back(ef);
+            rk.push_back(make_double4(r0, 0.0, r2, 0.0));
+        }
+        for (int ky = 1; ky < K; ++ky) {
+            const double r1 = bk[1] * ky;
+            ksq = r0 * r0 + r1 * r1;
+            if (ksq > kc2) break;
+            double ef = kp * std::exp(-ksq * q4) / ksq;
+            if (kx != 0) ef *= 2.0;
+            g.push_back(make_int4(kx, ky, 0, 5));
+            cf.push_back(ef);
+            rk.push_back(make_double4(r0, r1, 0.0, 0.0));
+            for (int kz = 1; kz < K; ++kz) {
+                const double r2 = bk[2] * kz;
+                ksq = r0 * r0 + r1 * r1 + r2 * r2;
+                if (ksq > kc2) break;
+                double ef3 = kp * std::exp(-ksq * q4) / ksq;
+                if (kx != 0) ef3 *= 2.0;
+                g.push_back(make_int4(kx, ky, kz, 15));
+                cf.push_back(ef3);
+                rk.push_back(make_double4(r0, r1, r2, 0.0));
+            }
+        }
+    }
+}
+
+// Orthonormal real normal-mode matrix (FFTW R2HC halfcomplex layout with the
+// scalings of fourier.cpp:22-66), entries from long double with exact pi.
+static void build_nm(int P, std::vector<double> &C)
+{
+    C.assign((size_t)P * P, 0.0);
+    const long double twopi = 6.283185307179586476925286766559005768L;
+    const long double s1 = sqrtl(1.0L / P), s2 = sqrtl(2.0L / P);
+    for (int b = 0; b < P; ++b) {
+        C[0 * P + b] = (double)s1;
+        for (int k = 1; k < P - k; ++k) {
+            const long double ang = twopi * (long double)(((long)k * b) % P) / P;
+            C[(size_t)k * P + b] = (double)(s2 * cosl(ang));
+            C[(size_t)(P - k) * P + b] = (double)(-s2 * sinl(ang));
+        }
+        if (P % 2 == 0) C[(size_t)(P / 2) * P + b] = (double)(s1 * ((b % 2) ? -1.0L : 1.0L));
+    }
+}
+
+static int alloc_all(qtp_ctx *ctx)
+{
+    const int P = ctx->P, E = std::max(P, 1);
+    const long N3 = 3L * ctx->NAP;
+    CHK(dalloc(ctx, &ctx->q, (size_t)P * N3));
+    CHK(dalloc(ctx, &ctx->p, (size_t)P * N3));
+    CHK(dalloc(ctx, &ctx->F, (size_t)P * N3));
+    CHK(dalloc(ctx, &ctx->fph, (size_t)P * N3));
+    CHK(dalloc(ctx, &ctx->disp, (size_t)P * 3 * ctx->NMP));
+    CHK(dalloc(ctx, &ctx->m_atom, ctx->NAP));
+    CHK(dalloc(ctx, &ctx->sqm_atom, ctx->NAP));
+    CHK(dalloc(ctx, &ctx->charge, ctx->NAP));
+    CHK(dalloc(ctx, &ctx->point, (size_t)P * ctx->n_mols));
+    CHK(dalloc(ctx, &ctx->list, (size_t)P * ctx->list_size));
+    CHK(dalloc(ctx, &ctx->fpoint, (size_t)P * (ctx->n_mols + 1)));
+    CHK(dalloc(ctx, &ctx->flist, (size_t)P * ctx->fcap));
+    CHK(dalloc(ctx, &ctx->cnt, (size_t)P * 2 * ctx->n_mols));
+    CHK(dalloc(ctx, &ctx->flag, P));
+    CHK(dalloc(ctx, &ctx->status, 1));
+    CHK(dalloc(ctx, &ctx->nreb, P));
+    CHK(dalloc(ctx, &ctx->qpm, (size_t)E * N3));
+    CHK(dalloc(ctx, &ctx->cpos, (size_t)E * N3));
+    CHK(dalloc(ctx, &ctx->rpart, (size_t)E * ctx->nblk * N3));
+    CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));
+    CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * 8));
+    CHK(dalloc(ctx, &ctx->ef, (size_t)E * N3));
+    CHK(dalloc(ctx, &ctx->flj, (size_t)E * N3));
+    CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
+    CHK(dalloc(ctx, &ctx->Ve, E));
+    CHK(dalloc(ctx, &ctx->epart, (size_t)E * ctx->ep_stride));
+    CHK(dalloc(ctx, &ctx->scal, (size_t)E * 4));
+    CHK(dalloc(ctx, &ctx->out, 8));
+    CHK(dalloc(ctx, &ctx->heat, 1));
+    CHK(dalloc(ctx, &ctx->heat_step, 1));
+    CHK(dalloc(ctx, &ctx->step_ctr, 1));
+    CHK(dalloc(ctx, &ctx->nq, (size_t)P * N3));
+    CHK(dalloc(ctx, &ctx->np, (size_t)P * N3));
+    CHK(dalloc(ctx, &ctx->ppart, 2 * (size_t)ctx->ptile + 2));
+    CHK(dalloc(ctx, &ctx->hpart, (size_t)ctx->ptile + 1));
+    CHK(dalloc(ctx, &ctx->obs, (size_t)ctx->ptile * 4 + 64));
+    ctx->dev_cap = 3 + (long)(P > 1 ? P - 1 : 0) * ctx->n_dof;
+    CHK(dalloc(ctx, &ctx->dev, (size_t)ctx->dev_cap));
+    return QTP_OK;
+}
+
+static int upload_constants(qtp_ctx *ctx)
+{
+    // masses and charges (simulation.cpp:811-822)
+    ctx->h_mass_atom.assign(ctx->NAP, 1.0);
+    ctx->h_charge.assign(ctx->NAP, 0.0);
+    std::vector<double> sqm(ctx->NAP, 0.0);
+    for (int i = 0; i < ctx->n_mols; ++i) {
+        ctx->h_mass_atom[3 * i] = RC.mo;
+        ctx->h_mass_atom[3 * i + 1] = RC.mh;
+        ctx->h_mass_atom[3 * i + 2] = RC.mh;
+        ctx->h_charge[3 * i] = RC.qo;
+        ctx->h_charge[3 * i + 1] = RC.qh;
+        ctx->h_charge[3 * i + 2] = RC.qh;
+    }
+    for (int a = 0; a < ctx->n_atoms; ++a) sqm[a] = std::sqrt(ctx->h_mass_atom[a] / ctx->drv.beta_n);
+    HIPCHK(hipMemcpy(ctx->m_atom, ctx->h_mass_atom.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->sqm_atom, sqm.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->charge, ctx->h_charge.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
+    // self energy constant in the reference's per-site order
+    const double pref = 2.0 * ctx->drv.kappa / std::sqrt(RC.pi);
+    double es = 0.0;
+    for (int a = 0; a < ctx->n_atoms; ++a) es += 0.5 * pref * ctx->h_charge[a] * ctx->h_charge[a];
+    ctx->eself = es;
+    // normal modes and per-mode propagator / thermostat constants
+    const int P = ctx->P;
+    std::vector<double> C;
+    build_nm(P, C);
+    HIPCHK(hipMemcpy(ctx->Cm, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<double4> md(P);
+    const double pre = 2.0 / (1.0 * ctx->drv.beta_n), pre_sq = pre * pre;
+    const double tpre = 4.0 / (ctx->drv.beta_n * 1.0);
+    const double dt = ctx->drv.dt;
+    for (int k = 0; k < P; ++k) {
+        const double s = std::sin(k * ctx->drv.piovern);              // update.cpp:302
+        const double wsq = pre_sq * s * s;
+        const double scale = 1.0 / std::sqrt(1.0 + 0.25 * wsq * dt * dt);
+        const double gk = tpre * std::sin(k * RC.pi / P);              // thermostat.cpp:119
+        const double c1 = std::exp(-dt * gk);
+        const double c2 = std::sqrt(1.0 - c1 * c1);
+        md[k] = make_double4(scale, wsq, c1, c2);
+    }
+    HIPCHK(hipMemcpy(ctx->mode, md.data(), P * sizeof(double4), hipMemcpyHostToDevice));
+    return QTP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// force phase: E single-bead evaluations (forces.cpp:468-630)
+// ---------------------------------------------------------------------------
+static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbase, double *fout,
+                     double *eout)
+{
+    hipStream_t s = ctx->stream;
+    const long N3 = 3L * ctx->NAP;
+    {
+        TimeScope ts(ctx, "sites");
+        const int nb = (ctx->NAP + 255) / 256;
+        k_sites<<<E * nb, 256, 0, s>>>(pos, N3, ctx->n_atoms, ctx->NAP, ctx->ff.gamma_vs,
+                                       ctx->ff.half1mg, ctx->cpos);
+    }
+    {
+        TimeScope ts(ctx, "ljintra");
+        k_ljintra<<<E * ctx->nmb, 64, 0, s>>>(pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint,
+                                               ctx->flist, ctx->fcap, lmode, lbase, ctx->box,
+                                               ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride,
+                                               ctx->ep_lj);
+    }
+    {
+        TimeScope ts(ctx, "realspace");
+        k_realspace<<<E * ctx->ntile, 64, 0, s>>>(ctx->cpos, ctx->charge, ctx->n_atoms, ctx->NAP,
+                                                   ctx->nblk, ctx->tiles, ctx->ntile, ctx->box,
+                                                   ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride,
+                                                   ctx->ep_real);
+    }
+    {
+        TimeScope ts(ctx, "ksum");
+        k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, s>>>(
+            ctx->cpos, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->ew, ctx->groups, ctx->G, ctx->AC,
+            ctx->ngc, ctx->nac, ctx->spart);
+    }
+    {
+        TimeScope ts(ctx, "sreduce");
+        k_sreduce<<<E * ctx->ngr, 256, 0, s>>>(ctx->spart, ctx->nac, ctx->G, ctx->groups, ctx->gcf,
+                                                ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k);
+    }
+    {
+        TimeScope ts(ctx, "kforce");
+        k_kforce<<<E * (ctx->NAP / 64), 256, ctx->kforce_lds, s>>>(
+            ctx->cpos, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->ew, ctx->groups, ctx->grk, ctx->G,
+            ctx->sprime, ctx->rpart, ctx->nblk, ctx->ef);
+    }
+    {
+        TimeScope ts(ctx, "finalize");
+        k_finalize<<<E, 256, 0, s>>>(ctx->flj, ctx->ef, ctx->n_atoms, ctx->NAP, ctx->ff.gamma_vs,
+                                     ctx->ff.half1mg, ctx->epart, ctx->ep_stride, ctx->eself, fout,
+                                     eout);
+    }
+    HIPCHK(hipGetLastError());
+    return QTP_OK;
+}
+
+static SCArgs sc_args(qtp_ctx *ctx)
+{
+    SCArgs a;
+    a.P = ctx->P;
+    a.n_atoms = ctx->n_atoms;
+    a.NAP = ctx->NAP;
+    a.omega_psq = ctx->drv.omega_psq;
+    a.epsilonsc = ctx->drv.epsilonsc;
+    a.halfdt = ctx->drv.halfdt;
+    return a;
+}
+
+// SC forces for all beads from ctx->q with the current lists (forces.cpp:698-789)
+static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
+{
+    hipStream_t s = ctx->stream;
+    const int P = ctx->P;
+    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fph, ctx->Ve));
+    SCArgs a = sc_args(ctx);
+    {
+        TimeScope ts(ctx, "sc1");
+        k_sc1<<<P, 256, 0, s>>>(a, ctx->q, ctx->fph, ctx->Ve, ctx->m_atom, ctx->F, ctx->qpm,
+                                ctx->scal);
+    }
+    CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, ctx->Ve + P));
+    {
+        TimeScope ts(ctx, "sc2");
+        k_sc2<<<P, 256, 0, s>>>(a, ctx->fph, ctx->fpm, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
+                                ctr);
+    }
+    HIPCHK(hipGetLastError());
+    return QTP_OK;
+}
+
+static int run_lists(qtp_ctx *ctx, int force_all)
+{
+    hipStream_t s = ctx->stream;
+    const int P = ctx->P;
+    {
+        TimeScope ts(ctx, "check");
+        k_check<<<P, 256, 0, s>>>(ctx->n_mols, ctx->NMP, ctx->disp, RC.r_skin - RC.r_cut, force_all,
+                                  ctx->flag, ctx->nreb);
+    }
+    const int nrb = (ctx->n_mols + 3) / 4;
+    {
+        TimeScope ts(ctx, "nlist");
+        k_nl_count<<<P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box, RC.r_skin_sq,
+                                           ctx->flag, ctx->cnt);
+        k_nl_scan<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->list_size, ctx->fcap, ctx->flag, ctx->cnt,
+                                     ctx->point, ctx->fpoint, ctx->list, ctx->disp, ctx->NMP,
+                                     ctx->status);
+        k_nl_fill<<<P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box, RC.r_skin_sq,
+                                          ctx->list_size, ctx->fcap, ctx->flag, ctx->point,
+                                          ctx->fpoint, ctx->list, ctx->flist);
+    }
+    HIPCHK(hipGetLastError());
+    return QTP_OK;
+}
+
+static PropArgs prop_args(qtp_ctx *ctx, int nvt, int host_dev)
+{
+    PropArgs A;
+    A.P = ctx->P;
+    A.n_atoms = ctx->n_atoms;
+    A.NAP = ctx->NAP;
+    A.ntile = ctx->ptile;
+    A.halfdt = ctx->drv.halfdt;
+    A.dt = ctx->drv.dt;
+    A.beta_n = ctx->drv.beta_n;
+    A.nvt = nvt;
+    A.host_dev = host_dev;
+    A.n_dof = ctx->n_dof;
+    A.seed = ctx->seed;
+    return A;
+}
+
+// One SC step on the device state (update.cpp:271-401).
+static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *ctr)
+{
+    hipStream_t s = ctx->stream;
+    PropArgs A = prop_args(ctx, nvt, host_dev);
+    {
+        TimeScope ts(ctx, "prop_a");
+        k_prop_a<<<ctx->ptile, 256, ctx->prop_lds, s>>>(A, ctx->Cm, ctx->mode, ctx->q, ctx->p,
+                                                        ctx->F, ctx->m_atom, ctx->nq, ctx->np,
+                                                        ctx->ppart, ctr);
+    }
+    {
+        TimeScope ts(ctx, "prop_b");
+        k_prop_b<<<ctx->ptile, 256, ctx->prop_lds, s>>>(
+            A, ctx->Cm, ctx->mode, ctx->nq, ctx->np, ctx->m_atom, ctx->sqm_atom, ctx->ppart,
+            ctx->dev, ctx->q, ctx->p, ctx->disp, ctx->NMP, ctx->hpart, ctr);
+    }
+    if (nvt) k_heat<<<1, 64, 0, s>>>(ctx->ptile, ctx->ppart, ctx->hpart, ctx->heat, ctx->heat_step);
+    HIPCHK(hipGetLastError());
+    CHK(run_lists(ctx, 0));
+    CHK(run_sc_forces(ctx, 1, ctr));
+    return QTP_OK;
+}
+
+// One classical velocity-Verlet step (update.cpp:78-137).
+static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr)
+{
+    hipStream_t s = ctx->stream;
+    const int nb = (3 * ctx->NAP + 255) / 256;
+    {
+        TimeScope ts(ctx, "vv_a");
+        k_vv_a<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->NMP, ctx->drv.halfdt, ctx->drv.dt,
+                                  ctx->m_atom, ctx->F, ctx->q, ctx->p, ctx->disp);
+    }
+    CHK(run_lists(ctx, 0));
+    CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, ctx->Ve));
+    {
+        TimeScope ts(ctx, "vv_b");
+        k_vv_b<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->drv.halfdt, ctx->fph, ctx->Ve, ctx->F,
+                                  ctx->p, ctx->out, ctr);
+    }
+    HIPCHK(hipGetLastError());
+    return QTP_OK;
+}
+
+// ---------------------------------------------------------------------------
+// host <-> device layout conversion
+// ---------------------------------------------------------------------------
+static void to_planar(const qtp_ctx *ctx, const double *ref, int nb, std::vector<double> &out)
+{
+    const long N3 = 3L * ctx->NAP;
+    out.assign((size_t)nb * N3, 0.0);
+    for (int b = 0; b < nb; ++b)
+        for (int a = 0; a < ctx->n_atoms; ++a)
+            for (int d = 0; d < 3; ++d)
+                out[b * N3 + (long)d * ctx->NAP + a] = ref[(long)b * ctx->n_dof + 3 * a + d];
+}
+
+static void from_planar(const qtp_ctx *ctx, const std::vector<double> &pl, int nb, double *ref)
+{
+    const long N3 = 3L * ctx->NAP;
+    for (int b = 0; b < nb; ++b)
+        for (int a = 0; a < ctx->n_atoms; ++a)
+            for (int d = 0; d < 3; ++d)
+                ref[(long)b * ctx->n_dof + 3 * a + d] = pl[b * N3 + (long)d * ctx->NAP + a];
+}
+
+static int upload_planar(qtp_ctx *ctx, double *dst, const double *ref, int nb)
+{
+    std::vector<double> h;
+    to_planar(ctx, ref, nb, h);
+    HIPCHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return QTP_OK;
+}
+
+static int download_planar(qtp_ctx *ctx, const double *src, int nb, double *ref)
+{
+    std::vector<double> h((size_t)nb * 3 * ctx->NAP);
+    HIPCHK(hipMemcpyAsync(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    from_planar(ctx, h, nb, ref);
+    return QTP_OK;
+}
+
+// disp: reference [b][3*mol + d]  <->  device [b][d][NMP]
+static int upload_disp(qtp_ctx *ctx, const double *ref, int b0, int nb)
+{
+    std::vector<double> h((size_t)nb * 3 * ctx->NMP, 0.0);
+    for (int b = 0; b < nb; ++b)
+        for (int i = 0; i < ctx->n_mols; ++i)
+            for (int d = 0; d < 3; ++d)
+                h[((size_t)b * 3 + d) * ctx->NMP + i] = ref[(long)(b0 + b) * 3 * ctx->n_mols + 3 * i + d];
+    HIPCHK(hipMemcpy(ctx->disp + (size_t)b0 * 3 * ctx->NMP, h.data(), h.size() * sizeof(double),
+                     hipMemcpyHostToDevice));
+    return QTP_OK;
+}
+
+static int download_disp(qtp_ctx *ctx, double *ref, int b0, int nb)
+{
+    std::vector<double> h((size_t)nb * 3 * ctx->NMP);
+    HIPCHK(hipMemcpy(h.data(), ctx->disp + (size_t)b0 * 3 * ctx->NMP, h.size() * sizeof(double),
+                     hipMemcpyDeviceToHost));
+    for (int b = 0; b < nb; ++b)
+        for (int i = 0; i < ctx->n_mols; ++i)
+            for (int d = 0; d < 3; ++d)
+                ref[(long)(b0 + b) * 3 * ctx->n_mols + 3 * i + d] = h[((size_t)b * 3 + d) * ctx->NMP + i];
+    return QTP_OK;
+}
+
+// Caller's half lists (reference layout) -> device half + full lists.  Row
+// n_mols-1 is ignored as forces.cpp:489 does.
+static int upload_lists(qtp_ctx *ctx, const int *point, const int *list, int b0, int nb,
+                        int dev_b0)
+{
+    const int n = ctx->n_mols;
+    std::vector<int> fp((size_t)nb * (n + 1)), fl((size_t)nb * ctx->fcap, 0);
+    for (int bb = 0; bb < nb; ++bb) {
+        const int b = b0 + bb;
+        const int *pt = point + (long)b * n;
+        const int *ls = list + (long)b * ctx->list_size;
+        std::vector<std::vector<int>> rows(n);
+        for (int i = 0; i < n - 1; ++i) {
+            const int k0 = pt[i], k1 = pt[i + 1];
+            if (k0 < 0 || k1 > ctx->list_size || k1 < k0) {
+                ctx->err = "neighbour list: point[] out of range";
+                return QTP_ERR_ARG;
+            }
+            for (int k = k0; k < k1; ++k) {
+                const int j = ls[k];
+                if (j < 0 || j >= n) {
+                    ctx->err = "neighbour list: entry out of range";
+                    return QTP_ERR_ARG;
+                }
+                rows[j].push_back(i);
+            }
+        }
+        long pos = 0;
+        for (int i = 0; i < n; ++i) {
+            fp[(size_t)bb * (n + 1) + i] = (int)pos;
+            std::vector<int> r = rows[i];  // partners j < i (i in row j)
+            if (i < n - 1)
+                for (int k = pt[i]; k < pt[i + 1]; ++k) r.push_back(ls[k]);
+            for (int j : r) {
+                if (pos >= ctx->fcap) {
+                    ctx->err = "neighbour list: full list exceeds capacity";
+                    return QTP_ERR_LIST_OVERFLOW;
+                }
+                fl[(size_t)bb * ctx->fcap + pos++] = j;
+            }
+        }
+        fp[(size_t)bb * (n + 1) + n] = (int)pos;
+    }
+    HIPCHK(hipMemcpy(ctx->fpoint + (size_t)dev_b0 * (n + 1), fp.data(), fp.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->flist + (size_t)dev_b0 * ctx->fcap, fl.data(), fl.size() * sizeof(int),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->point + (size_t)dev_b0 * n, point + (long)b0 * n, (size_t)nb * n * sizeof(int),
+                     hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->list + (size_t)dev_b0 * ctx->list_size, list + (long)b0 * ctx->list_size,
+                     (size_t)nb * ctx->list_size * sizeof(int), hipMemcpyHostToDevice));
+    return QTP_OK;
+}
+
+static int check_finite(qtp_ctx *ctx, const double *a, long n, const char *what)
+{
+    for (long i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) {
+            ctx->err = std::string(what) + " contains a non-finite value";
+            return QTP_ERR_NONFINITE;
+        }
+    return QTP_OK;
+}
+
+static int read_status(qtp_ctx *ctx)
+{
+    int st = 0;
+    HIPCHK(hipMemcpy(&st, ctx->status, sizeof(int), hipMemcpyDeviceToHost));
+    if (st & 1) {
+        ctx->err = "list size exceeded! (neighbour.cpp:56-58): increase list_size";
+        return QTP_ERR_LIST_OVERFLOW;
+    }
+    return QTP_OK;
+}
+
+static int upload_charges_masses(qtp_ctx *ctx, const double *charge, const double *mass)
+{
+    if (charge) {
+        std::vector<double> c(ctx->NAP, 0.0);
+        for (int a = 0; a < ctx->n_atoms; ++a) c[a] = charge[a];
+        if (c != ctx->h_charge) {
+            ctx->h_charge = c;
+            HIPCHK(hipMemcpy(ctx->charge, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
+            const double pref = 2.0 * ctx->drv.kappa / std::sqrt(RC.pi);
+            double es = 0.0;
+            for (int a = 0; a < ctx->n_atoms; ++a) es += 0.5 * pref * c[a] * c[a];
+            ctx->eself = es;
+        }
+    }
+    if (mass) {
+        std::vector<double> m(ctx->NAP, 1.0);
+        for (int a = 0; a < ctx->n_atoms; ++a) m[a] = mass[3 * a];
+        if (m != ctx->h_mass_atom) {
+            ctx->h_mass_atom = m;
+            std::vector<double> sq(ctx->NAP, 0.0);
+            for (int a = 0; a < ctx->n_atoms; ++a) sq[a] = std::sqrt(m[a] / ctx->drv.beta_n);
+            HIPCHK(hipMemcpy(ctx->m_atom, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(ctx->sqm_atom, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    return QTP_OK;
+}
+
+static void invalidate_graphs(qtp_ctx *ctx)
+{
+    for (auto &g : ctx->graph)
+        if (g) {
+            hipGraphExecDestroy(g);
+            g = nullptr;
+        }
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+int qtp_create(const qtp_params *prm, qtp_ctx **out)
+{
+    if (!prm || !out) return QTP_ERR_ARG;
+    *out = nullptr;
+    if (prm->n_mols < 2 || prm->n_beads < 1 || (prm->n_beads != 1 && prm->n_beads % 2 == 1) ||
+        prm->boxl[0] <= 0 || prm->boxl[1] <= 0 || prm->boxl[2] <= 0 || prm->eps_ewald <= 0 ||
+        prm->eps_ewald >= 1 || prm->T_K <= 0 || prm->dtfs <= 0 || prm->n_beads > 256)
+        return QTP_ERR_ARG;
+    qtp_ctx *ctx = new qtp_ctx();
+    ctx->prm = *prm;
+    derive(ctx);
+    ctx->P = prm->n_beads;
+    ctx->n_mols = prm->n_mols;
+    ctx->n_atoms = 3 * prm->n_mols;
+    ctx->n_dof = 9 * prm->n_mols;
+    ctx->nblk = (ctx->n_atoms + 63) / 64;
+    ctx->NAP = ctx->nblk * 64;
+    ctx->NMP = ((ctx->n_mols + 63) / 64) * 64;
+    ctx->list_size = ctx->drv.list_size;
+    ctx->fcap = 2L * ctx->list_size;
+    if (2.0 * ctx->drv.rcut > ctx->drv.min_box * (1.0 + 1e-12) || ctx->drv.k_max < 1 ||
+        ctx->drv.k_max > 250) {
+        ctx->err = "Ewald parameters out of range (r_c must be <= L/2, 1 <= k_max <= 250)";
+        *out = ctx;
+        return QTP_ERR_ARG;
+    }
+    for (int k = 0; k < 3; ++k) {
+        ctx->box.L[k] = prm->boxl[k];
+        ctx->box.inv[k] = 1.0 / prm->boxl[k];
+    }
+    FF &f = ctx->ff;
+    f.gamma_vs = RC.gamma_vs;
+    f.epslj4 = 4.0 * RC.epslj;
+    f.epslj24 = 24.0 * RC.epslj;
+    f.sigmalj6 = RC.sigmalj6;
+    f.sigmalj12 = RC.sigmalj12;
+    f.e_cut = RC.e_cut;
+    f.r_cut_sq = RC.r_cut_sq;
+    f.r_skin_sq = RC.r_skin_sq;
+    f.roheq = RC.roheq;
+    f.ar2 = RC.ar2;
+    f.ar3 = RC.ar3;
+    f.ar4fac = RC.ar4fac;
+    f.Dr = RC.Dr;
+    f.theta_eq = RC.theta_eq;
+    f.ktheta = RC.ktheta;
+    f.ar2x2 = 2.0 * RC.ar2;
+    f.ar3x3 = 3.0 * RC.ar3;
+    f.ar4x4 = 4.0 * RC.ar4fac;
+    f.half_ktheta = 0.5 * RC.ktheta;
+    f.half1mg = 0.5 * (1.0 - RC.gamma_vs);
+    ctx->ew.kappa = ctx->drv.kappa;
+    ctx->ew.kappa2 = ctx->drv.kappa * ctx->drv.kappa;
+    ctx->ew.rcut_sq = ctx->drv.rcut * ctx->drv.rcut;
+    ctx->ew.pref2k = 2.0 * ctx->drv.kappa / std::sqrt(RC.pi);
+    for (int k = 0; k < 3; ++k) ctx->ew.bk[k] = 2.0 * RC.pi / prm->boxl[k];
+    ctx->ew.K = ctx->drv.k_max + 1;
+
+    std::vector<int4> g;
+    std::vector<double> cf;
+    std::vector<double4> rk;
+    build_groups(ctx, g, cf, rk);
+    ctx->G = (int)g.size();
+    ctx->drv.n_kgroups = ctx->G;
+    ctx->ngc = (ctx->G + 255) / 256;
+    ctx->ngr = (ctx->G + 255) / 256;
+    const int K = ctx->ew.K;
+    ctx->AC = std::max(1, std::min(64, (int)(48 * 1024 / (3 * K * 16))));
+    ctx->nac = (ctx->n_atoms + ctx->AC - 1) / ctx->AC;
+    ctx->ksum_lds = (size_t)ctx->AC * 3 * K * sizeof(double2);
+    ctx->kforce_lds = (size_t)3 * K * 64 * sizeof(double2) + (size_t)KF_GC * (8 + 3) * sizeof(double) +
+                      (size_t)KF_GC * sizeof(int);
+    std::vector<int2> tl;
+    for (int I = 0; I < ctx->nblk; ++I)
+        for (int J = I; J < ctx->nblk; ++J) tl.push_back(make_int2(I, J));
+    ctx->ntile = (int)tl.size();
+    ctx->nmb = (ctx->n_mols + 63) / 64;
+    ctx->ep_lj = 0;
+    ctx->ep_real = ctx->nmb;
+    ctx->ep_k = ctx->nmb + ctx->ntile;
+    ctx->ep_stride = ctx->nmb + ctx->ntile + ctx->ngr;
+    ctx->ptile = (3 * ctx->NAP + 63) / 64;
+    ctx->prop_lds = (size_t)ctx->P * ctx->P * sizeof(double) + 2 * (size_t)ctx->P * 64 * sizeof(double);
+    *out = ctx;
+
+    // a blocking stream: host-synchronous hipMemcpy/hipMemset on the null
+    // stream are ordered with this stream's kernels
+    if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+        ctx->err = "hipStreamCreate failed";
+        return QTP_ERR_HIP;
+    }
+    if (ctx->kforce_lds > 64 * 1024 || ctx->ksum_lds > 64 * 1024 || ctx->prop_lds > 64 * 1024) {
+        if (hipFuncSetAttribute((const void *)k_kforce, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)std::max<size_t>(ctx->kforce_lds, 64 * 1024)) != hipSuccess ||
+            hipFuncSetAttribute((const void *)k_prop_a, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)std::max<size_t>(ctx->prop_lds, 64 * 1024)) != hipSuccess ||
+            hipFuncSetAttribute((const void *)k_prop_b, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)std::max<size_t>(ctx->prop_lds, 64 * 1024)) != hipSuccess) {
+            ctx->err = "LDS request above the device limit";
+            return QTP_ERR_ARG;
+        }
+    }
+    int st = QTP_OK;
+    if ((st = dalloc(ctx, &ctx->groups, ctx->G)) || (st = dalloc(ctx, &ctx->gcf, ctx->G)) ||
+        (st = dalloc(ctx, &ctx->grk, ctx->G)) || (st = dalloc(ctx, &ctx->tiles, ctx->ntile)) ||
+        (st = dalloc(ctx, &ctx->Cm, (size_t)ctx->P * ctx->P)) ||
+        (st = dalloc(ctx, &ctx->mode, ctx->P)) || (st = alloc_all(ctx)))
+        return st;
+    if (hipMemcpy(ctx->groups, g.data(), g.size() * sizeof(int4), hipMemcpyHostToDevice) ||
+        hipMemcpy(ctx->gcf, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice) ||
+        hipMemcpy(ctx->grk, rk.data(), rk.size() * sizeof(double4), hipMemcpyHostToDevice) ||
+        hipMemcpy(ctx->tiles, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice)) {
+        ctx->err = "constant upload failed";
+        return QTP_ERR_HIP;
+    }
+    CHK(upload_constants(ctx));
+    HIPCHK(hipDeviceSynchronize());
+    return QTP_OK;
+}
+
+void qtp_destroy(qtp_ctx *ctx)
+{
+    if (!ctx) return;
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    invalidate_graphs(ctx);
+    for (auto e : ctx->evpool) hipEventDestroy(e);
+    void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
+                    ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
+                    ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->cpos, ctx->rpart,
+                    ctx->spart, ctx->sprime, ctx->ef, ctx->flj, ctx->fpm, ctx->Ve, ctx->epart,
+                    ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
+                    ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
+                    ctx->ppart, ctx->hpart, ctx->dev, ctx->obs};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *qtp_last_error(const qtp_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int qtp_get_derived(const qtp_ctx *ctx, qtp_derived *out)
+{
+    if (!ctx || !out) return QTP_ERR_ARG;
+    *out = ctx->drv;
+    return QTP_OK;
+}
+
+int qtp_get_stream(const qtp_ctx *ctx, void **stream)
+{
+    if (!ctx || !stream) return QTP_ERR_ARG;
+    *stream = (void *)ctx->stream;
+    return QTP_OK;
+}
+
+// ---- reference-shaped calls --------------------------------------------------
+
+int qtp_get_forces_nospline(qtp_ctx *ctx, const double *q, double *F, const double *charge,
+                            const int *point, const int *list, int bead, double *energy)
+{
+    if (!ctx || !q || !F || !point || !list || bead < 0 || bead >= ctx->P) return QTP_ERR_ARG;
+    CHK(check_finite(ctx, q + (long)bead * ctx->n_dof, ctx->n_dof, "positions"));
+    CHK(upload_charges_masses(ctx, charge, nullptr));
+    // stage the bead's positions in evaluation slot 0 of qpm, its lists in slot `bead`
+    CHK(upload_planar(ctx, ctx->qpm, q + (long)bead * ctx->n_dof, 1));
+    CHK(upload_lists(ctx, point, list, bead, 1, bead));
+    CHK(run_phase(ctx, ctx->qpm, 1, 0, bead, ctx->fpm, ctx->Ve));
+    double e = 0.0;
+    HIPCHK(hipMemcpyAsync(&e, ctx->Ve, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CHK(download_planar(ctx, ctx->fpm, 1, F + (long)bead * ctx->n_dof));
+    CHK(flush_timing(ctx));
+    if (energy) *energy = e;
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+int qtp_force_eval_sc_nospline(qtp_ctx *ctx, const double *q, double *F_sc, const double *charge,
+                               const double *mass, const int *point, const int *list,
+                               double *sc_pot, double *fphys, double *energy)
+{
+    if (!ctx || !q || !F_sc || !point || !list || ctx->P < 2) return QTP_ERR_ARG;
+    const int P = ctx->P;
+    CHK(check_finite(ctx, q, (long)P * ctx->n_dof, "positions"));
+    CHK(upload_charges_masses(ctx, charge, mass));
+    CHK(upload_planar(ctx, ctx->q, q, P));
+    CHK(upload_lists(ctx, point, list, 0, P, 0));
+    CHK(run_sc_forces(ctx, 0, nullptr));
+    double o[2];
+    HIPCHK(hipMemcpyAsync(o, ctx->out, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CHK(download_planar(ctx, ctx->F, P, F_sc));
+    if (fphys) CHK(download_planar(ctx, ctx->fph, P, fphys));
+    CHK(flush_timing(ctx));
+    if (energy) *energy = o[0];
+    if (sc_pot) *sc_pot = o[1];
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+int qtp_neighbour_list(qtp_ctx *ctx, const double *q, int *point, int *list, double *disp,
+                       int bead)
+{
+    if (!ctx || !q || !point || !list || bead < 0 || bead >= ctx->P) return QTP_ERR_ARG;
+    CHK(check_finite(ctx, q + (long)bead * ctx->n_dof, ctx->n_dof, "positions"));
+    const long N3 = 3L * ctx->NAP;
+    CHK(upload_planar(ctx, ctx->q + bead * N3, q + (long)bead * ctx->n_dof, 1));
+    std::vector<int> fl(ctx->P, 0);
+    fl[bead] = 1;
+    HIPCHK(hipMemcpy(ctx->flag, fl.data(), ctx->P * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ctx->status, 0, sizeof(int)));
+    const int nrb = (ctx->n_mols + 3) / 4;
+    hipStream_t s = ctx->stream;
+    {
+        TimeScope ts(ctx, "nlist");
+        k_nl_count<<<ctx->P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box,
+                                                RC.r_skin_sq, ctx->flag, ctx->cnt);
+        k_nl_scan<<<ctx->P, 1024, 0, s>>>(ctx->n_mols, ctx->list_size, ctx->fcap, ctx->flag,
+                                          ctx->cnt, ctx->point, ctx->fpoint, ctx->list, ctx->disp,
+                                          ctx->NMP, ctx->status);
+        k_nl_fill<<<ctx->P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box,
+                                               RC.r_skin_sq, ctx->list_size, ctx->fcap, ctx->flag,
+                                               ctx->point, ctx->fpoint, ctx->list, ctx->flist);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(s));
+    CHK(flush_timing(ctx));
+    int st = read_status(ctx);
+    HIPCHK(hipMemcpy(point + (long)bead * ctx->n_mols, ctx->point + (size_t)bead * ctx->n_mols,
+                     ctx->n_mols * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(list + (long)bead * ctx->list_size, ctx->list + (size_t)bead * ctx->list_size,
+                     ctx->list_size * sizeof(int), hipMemcpyDeviceToHost));
+    if (disp)
+        for (int i = 0; i < 3 * ctx->n_mols; ++i) disp[(long)bead * 3 * ctx->n_mols + i] = 0.0;
+    ctx->state_set = false;
+    return st;
+}
+
+int qtp_ftransform(qtp_ctx *ctx, double *a, int length, int is_forwards)
+{
+    if (!ctx || !a || length < 1) return QTP_ERR_ARG;
+    const int P = ctx->P;
+    const size_t n = (size_t)P * length;
+    double *d = ctx->nq;  // scratch of P * 3 * NAP doubles
+    bool own = false;
+    if (n > (size_t)P * 3 * ctx->NAP) {
+        HIPCHK(hipMalloc((void **)&d, n * sizeof(double)));
+        own = true;
+    }
+    HIPCHK(hipMemcpyAsync(d, a, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    const size_t lds = (size_t)P * P * sizeof(double) + (size_t)P * 64 * sizeof(double);
+    {
+        TimeScope ts(ctx, "nm");
+        k_nm_generic<<<(length + 63) / 64, 256, lds, ctx->stream>>>(P, length, ctx->Cm, d,
+                                                                   is_forwards ? 1 : 0);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(a, d, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (own) HIPCHK(hipFree(d));
+    ctx->state_set = false;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    CHK(flush_timing(ctx));
+    return QTP_OK;
+}
+
+int qtp_pile_g(qtp_ctx *ctx, double *p_nm, const double *mass, const double *dev, double *heat)
+{
+    if (!ctx || !p_nm || !dev || ctx->P < 2) return QTP_ERR_ARG;
+    const int P = ctx->P;
+    CHK(upload_charges_masses(ctx, nullptr, mass));
+    CHK(upload_planar(ctx, ctx->np, p_nm, P));
+    HIPCHK(hipMemcpy(ctx->dev, dev, ctx->dev_cap * sizeof(double), hipMemcpyHostToDevice));
+    const int nt = (3 * ctx->NAP + 255) / 256;
+    {
+        TimeScope ts(ctx, "pile");
+        k_pile_a<<<nt, 256, 0, ctx->stream>>>(ctx->n_atoms, ctx->NAP, nt, ctx->np, ctx->m_atom,
+                                              ctx->ppart);
+        k_pile_b<<<nt, 256, 0, ctx->stream>>>(P, ctx->n_atoms, ctx->NAP, ctx->n_dof, nt,
+                                              ctx->drv.beta_n, ctx->mode, ctx->m_atom,
+                                              ctx->sqm_atom, ctx->ppart, ctx->dev, ctx->np,
+                                              ctx->hpart);
+        k_heat<<<1, 64, 0, ctx->stream>>>(nt, ctx->ppart, ctx->hpart, nullptr, ctx->heat_step);
+    }
+    HIPCHK(hipGetLastError());
+    double h = 0.0;
+    HIPCHK(hipMemcpyAsync(&h, ctx->heat_step, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CHK(download_planar(ctx, ctx->np, P, p_nm));
+    CHK(flush_timing(ctx));
+    if (heat) *heat = h;
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+int qtp_update_sc_nospline(qtp_ctx *ctx, double *q, double *p, const double *mass, double *F,
+                           double *fphys, const double *charge, int *point, int *list,
+                           double *disp, const double *dev, int is_nvt, double *heat,
+                           double *sc_pot, double *energy)
+{
+    if (!ctx || !q || !p || !F || !point || !list || !disp || ctx->P < 2 || (is_nvt && !dev))
+        return QTP_ERR_ARG;
+    const int P = ctx->P;
+    CHK(check_finite(ctx, q, (long)P * ctx->n_dof, "positions"));
+    CHK(upload_charges_masses(ctx, charge, mass));
+    CHK(upload_planar(ctx, ctx->q, q, P));
+    CHK(upload_planar(ctx, ctx->p, p, P));
+    CHK(upload_planar(ctx, ctx->F, F, P));
+    CHK(upload_disp(ctx, disp, 0, P));
+    CHK(upload_lists(ctx, point, list, 0, P, 0));
+    if (is_nvt)
+        HIPCHK(hipMemcpy(ctx->dev, dev, ctx->dev_cap * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ctx->status, 0, sizeof(int)));
+    double h0 = heat ? *heat : 0.0;
+    HIPCHK(hipMemcpy(ctx->heat, &h0, sizeof(double), hipMemcpyHostToDevice));
+    CHK(run_sc_step(ctx, is_nvt, 1, nullptr));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    CHK(flush_timing(ctx));
+    CHK(read_status(ctx));
+    double o[2];
+    HIPCHK(hipMemcpy(o, ctx->out, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&h0, ctx->heat, sizeof(double), hipMemcpyDeviceToHost));
+    CHK(download_planar(ctx, ctx->q, P, q));
+    CHK(download_planar(ctx, ctx->p, P, p));
+    CHK(download_planar(ctx, ctx->F, P, F));
+    if (fphys) CHK(download_planar(ctx, ctx->fph, P, fphys));
+    CHK(download_disp(ctx, disp, 0, P));
+    std::vector<int> fl(P);
+    HIPCHK(hipMemcpy(fl.data(), ctx->flag, P * sizeof(int), hipMemcpyDeviceToHost));
+    for (int b = 0; b < P; ++b)
+        if (fl[b]) {
+            HIPCHK(hipMemcpy(point + (long)b * ctx->n_mols, ctx->point + (size_t)b * ctx->n_mols,
+                             ctx->n_mols * sizeof(int), hipMemcpyDeviceToHost));
+            HIPCHK(hipMemcpy(list + (long)b * ctx->list_size, ctx->list + (size_t)b * ctx->list_size,
+                             ctx->list_size * sizeof(int), hipMemcpyDeviceToHost));
+        }
+    if (heat) *heat = h0;
+    if (energy) *energy = o[0];
+    if (sc_pot) *sc_pot = o[1];
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+int qtp_update_trajectory_class_nospline(qtp_ctx *ctx, double *q, double *p, const double *mass,
+                                         double *F, const double *charge, int *point, int *list,
+                                         double *disp, double *energy)
+{
+    if (!ctx || !q || !p || !F || !point || !list || !disp || ctx->P != 1) return QTP_ERR_ARG;
+    CHK(check_finite(ctx, q, ctx->n_dof, "positions"));
+    CHK(upload_charges_masses(ctx, charge, mass));
+    CHK(upload_planar(ctx, ctx->q, q, 1));
+    CHK(upload_planar(ctx, ctx->p, p, 1));
+    CHK(upload_planar(ctx, ctx->F, F, 1));
+    CHK(upload_disp(ctx, disp, 0, 1));
+    CHK(upload_lists(ctx, point, list, 0, 1, 0));
+    HIPCHK(hipMemset(ctx->status, 0, sizeof(int)));
+    CHK(run_class_step(ctx, nullptr));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    CHK(flush_timing(ctx));
+    CHK(read_status(ctx));
+    double o[2];
+    HIPCHK(hipMemcpy(o, ctx->out, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    CHK(download_planar(ctx, ctx->q, 1, q));
+    CHK(download_planar(ctx, ctx->p, 1, p));
+    CHK(download_planar(ctx, ctx->F, 1, F));
+    CHK(download_disp(ctx, disp, 0, 1));
+    int fl = 0;
+    HIPCHK(hipMemcpy(&fl, ctx->flag, sizeof(int), hipMemcpyDeviceToHost));
+    if (fl) {
+        HIPCHK(hipMemcpy(point, ctx->point, ctx->n_mols * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(list, ctx->list, ctx->list_size * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    if (energy) *energy = o[0];
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+int qtp_get_kcv(qtp_ctx *ctx, const double *q, const double *fphys, double *kcv)
+{
+    if (!ctx || !q || !fphys || !kcv) return QTP_ERR_ARG;
+    const int P = ctx->P;
+    CHK(upload_planar(ctx, ctx->nq, q, P));
+    CHK(upload_planar(ctx, ctx->np, fphys, P));
+    const int nb = (3 * ctx->NAP + 255) / 256;
+    k_kcv<<<nb, 256, 0, ctx->stream>>>(P, ctx->n_atoms, ctx->NAP, ctx->nq, ctx->np, ctx->obs);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(nb);
+    HIPCHK(hipMemcpyAsync(part.data(), ctx->obs, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double tq = 0.0;
+    for (double v : part) tq += v;
+    tq /= P;
+    *kcv = 0.5 * ctx->n_dof / ctx->drv.beta + tq;
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+int qtp_get_conserved(qtp_ctx *ctx, const double *q, const double *p, const double *mass,
+                      double energy, double *cons)
+{
+    if (!ctx || !q || !p || !cons) return QTP_ERR_ARG;
+    const int P = ctx->P;
+    CHK(upload_charges_masses(ctx, nullptr, mass));
+    CHK(upload_planar(ctx, ctx->nq, q, P));
+    CHK(upload_planar(ctx, ctx->np, p, P));
+    const int len = 3 * ctx->NAP;
+    const size_t lds = (size_t)P * P * sizeof(double) + (size_t)P * 64 * sizeof(double);
+    k_nm_generic<<<(len + 63) / 64, 256, lds, ctx->stream>>>(P, len, ctx->Cm, ctx->nq, 1);
+    k_nm_generic<<<(len + 63) / 64, 256, lds, ctx->stream>>>(P, len, ctx->Cm, ctx->np, 1);
+    const int nb = (3 * ctx->NAP + 255) / 256;
+    k_cons<<<nb, 256, 0, ctx->stream>>>(P, ctx->n_atoms, ctx->NAP, ctx->nq, ctx->np, ctx->mode,
+                                        ctx->m_atom, ctx->obs);
+    HIPCHK(hipGetLastError());
+    std::vector<double> part(nb);
+    HIPCHK(hipMemcpyAsync(part.data(), ctx->obs, nb * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *cons = energy + s;
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+// ---- host rng --------------------------------------------------------------
+
+int qtp_host_rng_create(uint32_t seed, qtp_host_rng **out)
+{
+    if (!out) return QTP_ERR_ARG;
+    *out = new qtp_host_rng();
+    (*out)->gen.seed(seed);
+    return QTP_OK;
+}
+
+void qtp_host_rng_destroy(qtp_host_rng *r) { delete r; }
+
+// thermostat.cpp:71, 88-98, 124: one normal_distribution object for R, R_add
+// and every xi; a separate gamma_distribution for the centroid.
+int qtp_host_rng_draw_pile(qtp_host_rng *r, int n_atoms, int n_beads, double *dev)
+{
+    if (!r || !dev || n_atoms < 1 || n_beads < 1) return QTP_ERR_ARG;
+    std::normal_distribution<double> gauss_dev(0.0, 1.0);
+    const int dof = 3 * n_atoms;
+    dev[0] = gauss_dev(r->gen);
+    if (dof % 2 == 0) {
+        std::gamma_distribution<double> gamma_dev((dof - 2) / 2, 1.0);
+        dev[1] = gauss_dev(r->gen);
+        dev[2] = gamma_dev(r->gen);
+    } else {
+        std::gamma_distribution<double> gamma_dev((dof - 1) / 2, 1.0);
+        dev[1] = 0.0;
+        dev[2] = gamma_dev(r->gen);
+    }
+    const long n = (long)(n_beads - 1) * dof;
+    for (long i = 0; i < n; ++i) dev[3 + i] = gauss_dev(r->gen);
+    return QTP_OK;
+}
+
+// ---- resident mode ------------------------------------------------------------
+
+int qtp_set_state(qtp_ctx *ctx, const double *q, const double *p, const double *charge,
+                  const double *mass)
+{
+    if (!ctx || !q || !p) return QTP_ERR_ARG;
+    const int P = ctx->P;
+    CHK(check_finite(ctx, q, (long)P * ctx->n_dof, "positions"));
+    CHK(check_finite(ctx, p, (long)P * ctx->n_dof, "momenta"));
+    invalidate_graphs(ctx);
+    CHK(upload_charges_masses(ctx, charge, mass));
+    CHK(upload_planar(ctx, ctx->q, q, P));
+    CHK(upload_planar(ctx, ctx->p, p, P));
+    HIPCHK(hipMemset(ctx->disp, 0, (size_t)P * 3 * ctx->NMP * sizeof(double)));
+    HIPCHK(hipMemset(ctx->status, 0, sizeof(int)));
+    HIPCHK(hipMemset(ctx->heat, 0, sizeof(double)));
+    HIPCHK(hipMemset(ctx->nreb, 0, P * sizeof(double)));
+    HIPCHK(hipMemset(ctx->step_ctr, 0, sizeof(unsigned long long)));
+    CHK(run_lists(ctx, 1));
+    if (P >= 2) {
+        CHK(run_sc_forces(ctx, 0, nullptr));
+    } else {
+        CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, ctx->Ve));
+        HIPCHK(hipMemcpyAsync(ctx->F, ctx->fph, 3 * ctx->NAP * sizeof(double),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->out, ctx->Ve, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIPCHK(hipMemsetAsync(ctx->nreb, 0, P * sizeof(double), ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    CHK(flush_timing(ctx));
+    CHK(read_status(ctx));
+    ctx->state_set = true;
+    return QTP_OK;
+}
+
+int qtp_get_state(qtp_ctx *ctx, double *q, double *p, double *F, double *fphys, double *disp)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int P = ctx->P;
+    if (q) CHK(download_planar(ctx, ctx->q, P, q));
+    if (p) CHK(download_planar(ctx, ctx->p, P, p));
+    if (F) CHK(download_planar(ctx, ctx->F, P, F));
+    if (fphys) CHK(download_planar(ctx, ctx->fph, P, fphys));
+    if (disp) CHK(download_disp(ctx, disp, 0, P));
+    return QTP_OK;
+}
+
+int qtp_get_lists(qtp_ctx *ctx, int *point, int *list)
+{
+    if (!ctx || !point || !list) return QTP_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    HIPCHK(hipMemcpy(point, ctx->point, (size_t)ctx->P * ctx->n_mols * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(list, ctx->list, (size_t)ctx->P * ctx->list_size * sizeof(int), hipMemcpyDeviceToHost));
+    return QTP_OK;
+}
+
+int qtp_set_seed(qtp_ctx *ctx, uint64_t seed)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    ctx->seed = seed;
+    invalidate_graphs(ctx);
+    return QTP_OK;
+}
+
+static int step_once(qtp_ctx *ctx, int nvt)
+{
+    if (ctx->P >= 2) return run_sc_step(ctx, nvt, 0, ctx->step_ctr);
+    return run_class_step(ctx, ctx->step_ctr);
+}
+
+int qtp_run(qtp_ctx *ctx, int n_steps, int is_nvt)
+{
+    if (!ctx || n_steps < 0) return QTP_ERR_ARG;
+    if (!ctx->state_set) {
+        ctx->err = "qtp_run: no resident state (call qtp_set_state)";
+        return QTP_ERR_STATE;
+    }
+    const int nvt = (is_nvt && ctx->P >= 2) ? 1 : 0;
+    if (ctx->timing) {
+        for (int i = 0; i < n_steps; ++i) CHK(step_once(ctx, nvt));
+        CHK(flush_timing(ctx));
+        return QTP_OK;
+    }
+    if (!ctx->graph[nvt]) {
+        hipGraph_t g;
+        HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int st = step_once(ctx, nvt);
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (st != QTP_OK) return st;
+        if (e != hipSuccess) {
+            ctx->err = std::string("graph capture: ") + hipGetErrorString(e);
+            return QTP_ERR_HIP;
+        }
+        hipError_t ei = hipGraphInstantiate(&ctx->graph[nvt], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ei);
+            ctx->graph[nvt] = nullptr;
+            return QTP_ERR_HIP;
+        }
+    }
+    for (int i = 0; i < n_steps; ++i) HIPCHK(hipGraphLaunch(ctx->graph[nvt], ctx->stream));
+    return QTP_OK;
+}
+
+int qtp_sync(qtp_ctx *ctx)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return read_status(ctx);
+}
+
+int qtp_get_scalars(qtp_ctx *ctx, double out[6])
+{
+    if (!ctx || !out) return QTP_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    double o[2], h;
+    unsigned long long n;
+    int st;
+    std::vector<double> nr(ctx->P);
+    HIPCHK(hipMemcpy(o, ctx->out, 2 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&h, ctx->heat, sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&n, ctx->step_ctr, sizeof(n), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&st, ctx->status, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nr.data(), ctx->nreb, ctx->P * sizeof(double), hipMemcpyDeviceToHost));
+    double tot = 0.0;
+    for (double v : nr) tot += v;
+    out[0] = o[0];
+    out[1] = o[1];
+    out[2] = h;
+    out[3] = tot;
+    out[4] = (double)n;
+    out[5] = (double)st;
+    return QTP_OK;
+}
+
+int qtp_enable_timing(qtp_ctx *ctx, int on)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    ctx->timing = on != 0;
+    ctx->timers.clear();
+    return QTP_OK;
+}
+
+int qtp_get_timing(qtp_ctx *ctx, const char *kernel, double *ms_total, int *launches)
+{
+    if (!ctx || !kernel) return QTP_ERR_ARG;
+    CHK(flush_timing(ctx));
+    auto it = ctx->timers.find(kernel);
+    if (ms_total) *ms_total = it == ctx->timers.end() ? 0.0 : it->second.ms;
+    if (launches) *launches = it == ctx->timers.end() ? 0 : it->second.n;
+    return QTP_OK;
+}
+
+int qtp_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    if (!ctr || !key || !out) return QTP_ERR_ARG;
+    uint32_t c[4] = {ctr[0], ctr[1], ctr[2], ctr[3]};
+    philox4x32_10(c, key[0], key[1]);
+    for (int i = 0; i < 4; ++i) out[i] = c[i];
+    return QTP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1284 @@
+// qtp_kernels.hpp -- gfx950 device code of the qTIP4P/F path-integral step.
+//
+// Layout in HBM (per bead / per force evaluation, "planar SoA"):
+//   x[(b*3 + d)*NAP + a]   a = 3*mol + site (O,H1,H2), NAP = n_atoms padded
+//                          to a multiple of 64 (one wavefront of sites)
+// Every kernel is FP64.  The translation unit is compiled with
+// -ffp-contract=off: products and sums that the reference evaluates
+// separately stay separate (bit-exact neighbour lists depend on it), and the
+// hot loops use explicit fma() where rounding-level differences are allowed.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace qtp {
+
+constexpr int WAVE = 64;
+
+struct Box {
+    double L[3];
+    double inv[3];
+};
+
+// qtip4pfparams.h:106-133 values needed on the device
+struct FF {
+    double gamma_vs, epslj4, epslj24, sigmalj6, sigmalj12, e_cut, r_cut_sq, r_skin_sq;
+    double roheq, ar2, ar3, ar4fac, Dr, theta_eq, ktheta;
+    double ar2x2, ar3x3, ar4x4;  // 2*ar2, 3*ar3, 4*ar4fac (forces.cpp:83 products)
+    double half_ktheta;          // 0.5*ktheta
+    double half1mg;              // 0.5*(1-gamma_vs)
+};
+
+struct Ewald {
+    double kappa, kappa2, rcut_sq, pref2k;  // pref2k = 2 kappa / sqrt(pi)
+    double bk[3];                           // 2 pi / L_d (reference pi)
+    int K;                                  // k_max + 1
+};
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide sum (blockDim multiple of 64, <= 1024); result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0) {
+        const int nw = (blockDim.x + 63) >> 6;
+        for (int i = 0; i < nw; ++i) r += sh[i];
+    }
+    return r;
+}
+
+// Minimum image as neighbour.cpp:46 / forces.cpp:503: IEEE divide, round half
+// away from zero, separate multiply and subtract.
+__device__ __forceinline__ double mimg_exact(double r, double L)
+{
+    return r - L * round(r / L);
+}
+
+// Minimum image for force kernels: r*(1/L) and round-to-even differ from the
+// exact form only when r/L lies within an ulp of a half-integer, i.e. for
+// separations of half a box, which are beyond every force cutoff (<= L/2).
+__device__ __forceinline__ double mimg_fast(double r, double L, double invL)
+{
+    return fma(-L, rint(r * invL), r);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) and Box-Muller normals
+// ---------------------------------------------------------------------------
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1)
+{
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0];
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+// One standard normal for (seed, step, stream, index).
+__device__ inline double philox_normal(uint64_t seed, uint64_t step, uint32_t stream, uint32_t idx)
+{
+    uint32_t c[4] = {idx, (uint32_t)step, stream, (uint32_t)(step >> 32)};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double inv53 = 1.0 / 9007199254740992.0;  // 2^-53
+    const double u1 = ((double)(((uint64_t)(c[0] >> 5) << 26) | (c[1] >> 6)) + 0.5) * inv53;
+    const double u2 = ((double)(((uint64_t)(c[2] >> 5) << 26) | (c[3] >> 6)) + 0.5) * inv53;
+    return sqrt(-2.0 * log(u1)) * cospi(2.0 * u2);
+}
+
+// ---------------------------------------------------------------------------
+// Force evaluation.  A "phase" evaluates E independent single-bead forces
+// (forces.cpp:468-630); eval e reads positions pos + e*pstride and bead
+// lst_bead(e)'s Verlet list.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lst_bead(int mode, int base, int e)
+{
+    return mode == 0 ? base + e : 2 * (e >> 1) + 1;
+}
+
+// M-site (charge) positions, forces.cpp:558-564.  grid: E * ceil(NAP/256).
+__global__ __launch_bounds__(256) void k_sites(const double *__restrict__ pos, long pstride,
+                                               int n_atoms, int NAP, double gamma, double half1mg,
+                                               double *__restrict__ cpos)
+{
+    const int nb = (NAP + 255) / 256;
+    const int e = blockIdx.x / nb;
+    const int a = (blockIdx.x - e * nb) * 256 + threadIdx.x;
+    if (a >= NAP) return;
+    const double *P = pos + (long)e * pstride;
+    double *Cp = cpos + (long)e * 3 * NAP;
+    for (int d = 0; d < 3; ++d) {
+        double v = 0.0;
+        if (a < n_atoms) {
+            v = P[d * NAP + a];
+            if (a % 3 == 0)
+                v = gamma * v + half1mg * (P[d * NAP + a + 1] + P[d * NAP + a + 2]);
+        }
+        Cp[d * NAP + a] = v;
+    }
+}
+
+// Real-space Ewald over all site pairs i<j within r_c (forces.cpp:586-608,
+// rwald :141-177).  One wavefront per 64x64 tile (I<=J) of the pair matrix.
+// Lane l owns site I*64+l; at step s it meets site J*64+((l+s)&63), so the 64
+// lanes touch 64 distinct j and the reaction-force accumulator of each j is
+// handed lane to lane with the j it belongs to (deterministic, no atomics).
+// Partial site forces go to slot J (for block I's sites) and slot I (for
+// block J's sites): every (site, slot) pair has exactly one writer.
+__global__ __launch_bounds__(64) void k_realspace(const double *__restrict__ cpos,
+                                                  const double *__restrict__ charge, int n_atoms,
+                                                  int NAP, int nblk, const int2 *__restrict__ tiles,
+                                                  int ntile, Box box, Ewald ew,
+                                                  double *__restrict__ rpart,
+                                                  double *__restrict__ epart, int ep_stride,
+                                                  int ep_off)
+{
+    const int e = blockIdx.x / ntile, t = blockIdx.x - e * ntile;
+    const int I = tiles[t].x, J = tiles[t].y;
+    const int l = threadIdx.x;
+    const double *X = cpos + (long)e * 3 * NAP;
+    __shared__ double sx[64], sy[64], sz[64], sq[64], sacc[3][64];
+    const int i = I * 64 + l, jj = J * 64 + l;
+    const bool vi = i < n_atoms;
+    const double xi = X[i], yi = X[NAP + i], zi = X[2 * NAP + i];
+    const double qi = vi ? charge[i] : 0.0;
+    sx[l] = X[jj];
+    sy[l] = X[NAP + jj];
+    sz[l] = X[2 * NAP + jj];
+    sq[l] = jj < n_atoms ? charge[jj] : 0.0;
+    __syncthreads();
+    const bool diag = (I == J);
+    const int s0 = diag ? 1 : 0, s1 = diag ? 33 : 64;
+    const int mi = i / 3;
+    double fx = 0.0, fy = 0.0, fz = 0.0, ax = 0.0, ay = 0.0, az = 0.0, en = 0.0;
+    for (int s = s0; s < s1; ++s) {
+        const int jl = (l + s) & 63;
+        const int j = J * 64 + jl;
+        const bool act = vi && (j < n_atoms) && !(diag && s == 32 && l >= 32);
+        double dx = mimg_fast(xi - sx[jl], box.L[0], box.inv[0]);
+        double dy = mimg_fast(yi - sy[jl], box.L[1], box.inv[1]);
+        double dz = mimg_fast(zi - sz[jl], box.L[2], box.inv[2]);
+        const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+        if (act && rsq < ew.rcut_sq) {
+            const double r = sqrt(rsq);
+            const double rinv = 1.0 / r;
+            const double qq = qi * sq[jl];
+            double ec = erfc(ew.kappa * r);
+            if (mi == j / 3) ec -= 1.0;  // same molecule: -erf (forces.cpp:158-166)
+            const double ep = qq * rinv * ec;
+            const double pf = (ep + qq * ew.pref2k * exp(-ew.kappa2 * rsq)) * (rinv * rinv);
+            en += ep;
+            fx = fma(pf, dx, fx);
+            fy = fma(pf, dy, fy);
+            fz = fma(pf, dz, fz);
+            ax = fma(-pf, dx, ax);
+            ay = fma(-pf, dy, ay);
+            az = fma(-pf, dz, az);
+        }
+        const int src = (l + 1) & 63;
+        ax = __shfl(ax, src, 64);
+        ay = __shfl(ay, src, 64);
+        az = __shfl(az, src, 64);
+    }
+    // lane l now holds the accumulator of j-local (l + s1) & 63
+    const int jl_end = (l + s1) & 63;
+    const long base_e = (long)e * nblk * 3 * NAP;
+    if (diag) {
+        sacc[0][jl_end] = ax;
+        sacc[1][jl_end] = ay;
+        sacc[2][jl_end] = az;
+        __syncthreads();
+        double *o = rpart + base_e + (long)I * 3 * NAP;
+        o[i] = fx + sacc[0][l];
+        o[NAP + i] = fy + sacc[1][l];
+        o[2 * NAP + i] = fz + sacc[2][l];
+    } else {
+        double *oI = rpart + base_e + (long)J * 3 * NAP;  // slot J, sites of block I
+        oI[i] = fx;
+        oI[NAP + i] = fy;
+        oI[2 * NAP + i] = fz;
+        double *oJ = rpart + base_e + (long)I * 3 * NAP;  // slot I, sites of block J
+        const int j = J * 64 + jl_end;
+        oJ[j] = ax;
+        oJ[NAP + j] = ay;
+        oJ[2 * NAP + j] = az;
+    }
+    en = wave_sum(en);
+    if (l == 0) epart[(long)e * ep_stride + ep_off + t] = en;
+}
+
+// Reciprocal-space structure factors (kwald, forces.cpp:180-448), partial
+// over a chunk of AC sites.  A k-"group" (kx>=0, ky>=0, kz>=0) carries the
+// four sign variants (kx,+-ky,+-kz) of the reference's branches; lane <-> group.
+// Per site the factors e^{i m b_d r_d} (m = 0..k_max, charge folded into x)
+// come from the same cos/sin recurrence as forces.cpp:212-245 and sit in LDS.
+// Output: 8 sums per group: Re/Im of A Z, A Z*, B Z, B Z* with A = X Y,
+// B = X Y* (X,Y,Z the x,y,z factors).   grid: E * ngc * nac.
+__global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ cpos,
+                                              const double *__restrict__ charge, int n_atoms,
+                                              int NAP, Ewald ew, const int4 *__restrict__ groups,
+                                              int G, int AC, int ngc, int nac,
+                                              double *__restrict__ spart)
+{
+    extern __shared__ double2 T[];  // [AC][3][K]
+    const int per_e = ngc * nac;
+    const int e = blockIdx.x / per_e;
+    const int r = blockIdx.x - e * per_e;
+    const int gc = r / nac, ac = r - gc * nac;
+    const int K = ew.K;
+    const double *X = cpos + (long)e * 3 * NAP;
+    for (int t = threadIdx.x; t < AC * 3; t += blockDim.x) {
+        const int a = t / 3, d = t - 3 * a;
+        const int atom = ac * AC + a;
+        double2 *row = T + (a * 3 + d) * K;
+        if (atom < n_atoms) {
+            const double kr = ew.bk[d] * X[d * NAP + atom];
+            double sn, cs;
+            sincos(kr, &sn, &cs);
+            const double qf = (d == 0) ? charge[atom] : 1.0;
+            double cm = 1.0, sm = 0.0;
+            row[0] = make_double2(cm * qf, sm * qf);
+            for (int m = 1; m < K; ++m) {
+                const double cn = cs * cm - sn * sm;
+                const double snn = sn * cm + cs * sm;
+                cm = cn;
+                sm = snn;
+                row[m] = make_double2(cm * qf, sm * qf);
+            }
+        } else {
+            for (int m = 0; m < K; ++m) row[m] = make_double2(0.0, 0.0);
+        }
+    }
+    __syncthreads();
+    const int g = gc * blockDim.x + threadIdx.x;
+    if (g >= G) return;
+    const int4 gr = groups[g];
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0, s6 = 0.0, s7 = 0.0;
+    for (int a = 0; a < AC; ++a) {
+        const double2 Xv = T[(a * 3 + 0) * K + gr.x];
+        const double2 Yv = T[(a * 3 + 1) * K + gr.y];
+        const double2 Zv = T[(a * 3 + 2) * K + gr.z];
+        const double cc = Xv.x * Yv.x, csn = Xv.x * Yv.y, sc = Xv.y * Yv.x, ss = Xv.y * Yv.y;
+        const double Ar = cc - ss, Ai = csn + sc, Br = cc + ss, Bi = sc - csn;
+        s0 = fma(Ar, Zv.x, fma(-Ai, Zv.y, s0));
+        s1 = fma(Ai, Zv.x, fma(Ar, Zv.y, s1));
+        s2 = fma(Ar, Zv.x, fma(Ai, Zv.y, s2));
+        s3 = fma(Ai, Zv.x, fma(-Ar, Zv.y, s3));
+        s4 = fma(Br, Zv.x, fma(-Bi, Zv.y, s4));
+        s5 = fma(Bi, Zv.x, fma(Br, Zv.y, s5));
+        s6 = fma(Br, Zv.x, fma(Bi, Zv.y, s6));
+        s7 = fma(Bi, Zv.x, fma(-Br, Zv.y, s7));
+    }
+    double *o = spart + (((long)e * nac + ac) * G + g) * 8;
+    o[0] = s0; o[1] = s1; o[2] = s2; o[3] = s3;
+    o[4] = s4; o[5] = s5; o[6] = s6; o[7] = s7;
+}
+
+// Sum the partial structure factors, energy per group (forces.cpp:274, 315,
+// 364, 428) and the force-ready S' = 2 c S of the variants the reference
+// evaluates (gr.w bit v set); others set to zero.   grid: E * ceil(G/256).
+__global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spart, int nac, int G,
+                                                 const int4 *__restrict__ groups,
+                                                 const double *__restrict__ gcf,
+                                                 double *__restrict__ sprime,
+                                                 double *__restrict__ epart, int ep_stride,
+                                                 int ep_off)
+{
+    __shared__ double sh[16];
+    const int nb = (G + 255) / 256;
+    const int e = blockIdx.x / nb, gb = blockIdx.x - e * nb;
+    const int g = gb * 256 + threadIdx.x;
+    double en = 0.0;
+    if (g < G) {
+        double s[8];
+        for (int v = 0; v < 8; ++v) s[v] = 0.0;
+        for (int ac = 0; ac < nac; ++ac) {
+            const double *p = spart + (((long)e * nac + ac) * G + g) * 8;
+            for (int v = 0; v < 8; ++v) s[v] += p[v];
+        }
+        const int mask = groups[g].w;
+        const double cf = gcf[g];
+        double sq = 0.0;
+        for (int v = 0; v < 4; ++v) {
+            if ((mask >> v) & 1) {
+                sq += s[2 * v] * s[2 * v];
+                sq += s[2 * v + 1] * s[2 * v + 1];
+            } else {
+                s[2 * v] = 0.0;
+                s[2 * v + 1] = 0.0;
+            }
+        }
+        en = cf * sq;
+        const double c2 = 2.0 * cf;
+        double *o = sprime + ((long)e * G + g) * 8;
+        for (int v = 0; v < 8; ++v) o[v] = c2 * s[v];
+    }
+    en = block_sum(en, sh);
+    if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + gb] = en;
+}
+
+// Reciprocal-space forces (forces.cpp:276-279, 319-324, 368-373, 434-442) for
+// a block of 64 sites (lane <-> site), the four waves splitting the groups;
+// then the real-space partial slots are added: ef = total Ewald site force.
+// grid: E * (NAP/64).   LDS: tables [3][K][64] + a chunk of GC groups.
+constexpr int KF_GC = 256;
+__global__ __launch_bounds__(256) void k_kforce(const double *__restrict__ cpos,
+                                                const double *__restrict__ charge, int n_atoms,
+                                                int NAP, Ewald ew, const int4 *__restrict__ groups,
+                                                const double4 *__restrict__ grk, int G,
+                                                const double *__restrict__ sprime,
+                                                const double *__restrict__ rpart, int nblk,
+                                                double *__restrict__ ef)
+{
+    extern __shared__ double smem[];
+    const int K = ew.K;
+    double2 *T = (double2 *)smem;            // [3][K][64]
+    double *SP = smem + 2 * 3 * K * 64;      // [GC][8]
+    double *RK = SP + KF_GC * 8;             // [GC][3]
+    int *GR = (int *)(RK + KF_GC * 3);       // [GC]
+    const int nab = NAP / 64;
+    const int e = blockIdx.x / nab, ab = blockIdx.x - e * nab;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int atom = ab * 64 + lane;
+    const double *X = cpos + (long)e * 3 * NAP;
+    if (threadIdx.x < 192) {
+        const int d = threadIdx.x >> 6;
+        if (atom < n_atoms) {
+            const double kr = ew.bk[d] * X[d * NAP + atom];
+            double sn, cs;
+            sincos(kr, &sn, &cs);
+            const double qf = (d == 0) ? charge[atom] : 1.0;
+            double cm = 1.0, sm = 0.0;
+            T[(d * K + 0) * 64 + lane] = make_double2(cm * qf, sm * qf);
+            for (int m = 1; m < K; ++m) {
+                const double cn = cs * cm - sn * sm;
+                const double snn = sn * cm + cs * sm;
+                cm = cn;
+                sm = snn;
+                T[(d * K + m) * 64 + lane] = make_double2(cm * qf, sm * qf);
+            }
+        } else {
+            for (int m = 0; m < K; ++m) T[(d * K + m) * 64 + lane] = make_double2(0.0, 0.0);
+        }
+    }
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int g0 = 0; g0 < G; g0 += KF_GC) {
+        const int gn = min(KF_GC, G - g0);
+        __syncthreads();
+        for (int t = threadIdx.x; t < gn * 8; t += 256) SP[t] = sprime[((long)e * G + g0) * 8 + t];
+        for (int t = threadIdx.x; t < gn; t += 256) {
+            const double4 rk = grk[g0 + t];
+            RK[t * 3 + 0] = rk.x;
+            RK[t * 3 + 1] = rk.y;
+            RK[t * 3 + 2] = rk.z;
+            const int4 gr = groups[g0 + t];
+            GR[t] = gr.x | (gr.y << 8) | (gr.z << 16);
+        }
+        __syncthreads();
+        const int qn = (gn + 3) >> 2;
+        const int gb = w * qn, ge = min(gn, gb + qn);
+        int pkxy = -1;
+        double Ar = 0.0, Ai = 0.0, Br = 0.0, Bi = 0.0;
+        for (int g = gb; g < ge; ++g) {
+            const int pk = GR[g];
+            const int kxy = pk & 0xFFFF, kz = pk >> 16;
+            if (kxy != pkxy) {
+                const double2 Xv = T[(0 * K + (pk & 255)) * 64 + lane];
+                const double2 Yv = T[(1 * K + ((pk >> 8) & 255)) * 64 + lane];
+                const double cc = Xv.x * Yv.x, csn = Xv.x * Yv.y, sc = Xv.y * Yv.x,
+                             ss = Xv.y * Yv.y;
+                Ar = cc - ss;
+                Ai = csn + sc;
+                Br = cc + ss;
+                Bi = sc - csn;
+                pkxy = kxy;
+            }
+            const double2 Zv = T[(2 * K + kz) * 64 + lane];
+            const double *S = SP + g * 8;
+            const double e0r = fma(Ar, Zv.x, -Ai * Zv.y), e0i = fma(Ai, Zv.x, Ar * Zv.y);
+            const double e1r = fma(Ar, Zv.x, Ai * Zv.y), e1i = fma(Ai, Zv.x, -Ar * Zv.y);
+            const double e2r = fma(Br, Zv.x, -Bi * Zv.y), e2i = fma(Bi, Zv.x, Br * Zv.y);
+            const double e3r = fma(Br, Zv.x, Bi * Zv.y), e3i = fma(Bi, Zv.x, -Br * Zv.y);
+            const double t0 = fma(S[1], e0r, -S[0] * e0i);
+            const double t1 = fma(S[3], e1r, -S[2] * e1i);
+            const double t2 = fma(S[5], e2r, -S[4] * e2i);
+            const double t3 = fma(S[7], e3r, -S[6] * e3i);
+            const double p01 = t0 + t1, p23 = t2 + t3, m01 = t0 - t1, m23 = t2 - t3;
+            fx = fma(-RK[g * 3 + 0], p01 + p23, fx);
+            fy = fma(-RK[g * 3 + 1], p01 - p23, fy);
+            fz = fma(-RK[g * 3 + 2], m01 + m23, fz);
+        }
+    }
+    __syncthreads();
+    double *red = SP;  // [4][3][64]
+    red[(w * 3 + 0) * 64 + lane] = fx;
+    red[(w * 3 + 1) * 64 + lane] = fy;
+    red[(w * 3 + 2) * 64 + lane] = fz;
+    __syncthreads();
+    if (w == 0) {
+        for (int d = 0; d < 3; ++d) {
+            double v = red[(0 * 3 + d) * 64 + lane] + red[(1 * 3 + d) * 64 + lane] +
+                       red[(2 * 3 + d) * 64 + lane] + red[(3 * 3 + d) * 64 + lane];
+            const double *rp = rpart + (long)e * nblk * 3 * NAP + d * NAP + atom;
+            for (int s = 0; s < nblk; ++s) v += rp[(long)s * 3 * NAP];
+            ef[((long)e * 3 + d) * NAP + atom] = v;
+        }
+    }
+}
+
+// LJ over the full O-O list (forces.cpp:489-515, lj_oxygen :25-59) and the
+// intramolecular terms (forces.cpp:519-554, :65-138); lane <-> molecule.
+// Each pair is met from both sides; its energy is halved.
+// grid: E * ceil(n_mols/64).
+__global__ __launch_bounds__(64) void k_ljintra(const double *__restrict__ pos, long pstride,
+                                                int n_mols, int NAP,
+                                                const int *__restrict__ fpoint,
+                                                const int *__restrict__ flist, long fcap,
+                                                int lmode, int lbase, Box box, FF ff,
+                                                double *__restrict__ flj,
+                                                double *__restrict__ epart, int ep_stride,
+                                                int ep_off)
+{
+    const int nmb = (n_mols + 63) / 64;
+    const int e = blockIdx.x / nmb, mb = blockIdx.x - e * nmb;
+    const int i = mb * 64 + threadIdx.x;
+    const double *P = pos + (long)e * pstride;
+    double en = 0.0;
+    if (i < n_mols) {
+        const int b = lst_bead(lmode, lbase, e);
+        const int a = 3 * i;
+        double o[3], h1[3], h2[3];
+        for (int d = 0; d < 3; ++d) {
+            o[d] = P[d * NAP + a];
+            h1[d] = P[d * NAP + a + 1];
+            h2[d] = P[d * NAP + a + 2];
+        }
+        double fo[3] = {0.0, 0.0, 0.0}, fh1[3] = {0.0, 0.0, 0.0}, fh2[3] = {0.0, 0.0, 0.0};
+        const int *fp = fpoint + (long)b * (n_mols + 1);
+        const int *fl = flist + (long)b * fcap;
+        // an overflowed list (status flag set by k_nl_scan) is never read past its capacity
+        const int k0 = (int)min((long)fp[i], fcap), k1 = (int)min((long)fp[i + 1], fcap);
+        double elj = 0.0;
+        for (int k = k0; k < k1; ++k) {
+            const int j = fl[k];
+            const int aj = 3 * j;
+            double r[3];
+            for (int d = 0; d < 3; ++d)
+                r[d] = mimg_fast(o[d] - P[d * NAP + aj], box.L[d], box.inv[d]);
+            const double rsq = fma(r[0], r[0], fma(r[1], r[1], r[2] * r[2]));
+            if (rsq < ff.r_cut_sq) {
+                const double rm2 = 1.0 / rsq;
+                const double rm6 = rm2 * rm2 * rm2;
+                const double a12 = rm6 * rm6 * ff.sigmalj12;
+                const double a6 = rm6 * ff.sigmalj6;
+                const double u = a12 - a6;
+                elj += u * ff.epslj4 - ff.e_cut;
+                const double pf = (u + a12) * ff.epslj24 * rm2;
+                for (int d = 0; d < 3; ++d) fo[d] = fma(pf, r[d], fo[d]);
+            }
+        }
+        en = 0.5 * elj;
+        // intramolecular (forces.cpp:519-554)
+        double r1[3], r2[3];
+        for (int d = 0; d < 3; ++d) {
+            r1[d] = mimg_fast(o[d] - h1[d], box.L[d], box.inv[d]);
+            r2[d] = mimg_fast(o[d] - h2[d], box.L[d], box.inv[d]);
+        }
+        const double roh1 = sqrt(r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+        const double roh2 = sqrt(r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+        // bonds (intra_morse_bond)
+        for (int bnd = 0; bnd < 2; ++bnd) {
+            const double roh = bnd ? roh2 : roh1;
+            const double *rr = bnd ? r2 : r1;
+            double *fh = bnd ? fh2 : fh1;
+            const double rm1 = 1.0 / roh;
+            const double x = roh - ff.roheq;
+            const double x2 = x * x, x3 = x2 * x, x4 = x2 * x2;
+            en += (ff.ar2 * x2 - ff.ar3 * x3 + ff.ar4fac * x4) * ff.Dr;
+            const double pf = (ff.ar2x2 * x - ff.ar3x3 * x2 + ff.ar4x4 * x3) * (ff.Dr * rm1);
+            for (int d = 0; d < 3; ++d) {
+                fo[d] -= pf * rr[d];
+                fh[d] += pf * rr[d];
+            }
+        }
+        // angle (intra_harm_angle)
+        {
+            const double dot = r1[0] * r2[0] + r1[1] * r2[1] + r1[2] * r2[2];
+            const double r1m1 = 1.0 / roh1, r2m1 = 1.0 / roh2;
+            const double r1m2 = r1m1 * r1m1, r2m2 = r2m1 * r2m1, r12 = r1m1 * r2m1;
+            const double c = dot * r12;
+            const double dxdc = 1.0 / sqrt(1.0 - c * c);
+            const double c2r1 = dot * r1m2 * r12, c2r2 = dot * r2m2 * r12;
+            const double td = acos(c) - ff.theta_eq;
+            en += ff.half_ktheta * td * td;
+            const double pf = ff.ktheta * td * dxdc;
+            for (int d = 0; d < 3; ++d) {
+                fo[d] += pf * (r12 * (r1[d] + r2[d]) - c2r1 * r1[d] - c2r2 * r2[d]);
+                fh1[d] -= pf * (r12 * r2[d] - c2r1 * r1[d]);
+                fh2[d] -= pf * (r12 * r1[d] - c2r2 * r2[d]);
+            }
+        }
+        double *O = flj + (long)e * 3 * NAP;
+        for (int d = 0; d < 3; ++d) {
+            O[d * NAP + a] = fo[d];
+            O[d * NAP + a + 1] = fh1[d];
+            O[d * NAP + a + 2] = fh2[d];
+        }
+    }
+    en = wave_sum(en);
+    if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + mb] = en;
+}
+
+// Physical forces: LJ + intra + chain rule of the M-site Ewald force
+// (forces.cpp:613-619); energy = ordered sum of the partials - self term
+// (forces.cpp:622-624).   grid: E, 256 threads.
+__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ flj,
+                                                  const double *__restrict__ ef, int n_atoms,
+                                                  int NAP, double gamma, double half1mg,
+                                                  const double *__restrict__ epart, int ep_stride,
+                                                  double eself, double *__restrict__ fout,
+                                                  double *__restrict__ eout)
+{
+    const int e = blockIdx.x;
+    const double *L = flj + (long)e * 3 * NAP;
+    const double *W = ef + (long)e * 3 * NAP;
+    double *O = fout + (long)e * 3 * NAP;
+    for (int t = threadIdx.x; t < 3 * NAP; t += 256) {
+        const int d = t / NAP, a = t - d * NAP;
+        double v = 0.0;
+        if (a < n_atoms) {
+            const int ao = a - a % 3;
+            if (a == ao)
+                v = L[t] + gamma * W[t];
+            else
+                v = L[t] + (W[t] + half1mg * W[d * NAP + ao]);
+        }
+        O[t] = v;
+    }
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+        const double *p = epart + (long)e * ep_stride;
+        for (int k = 0; k < ep_stride; ++k) s += p[k];
+        eout[e] = s - eself;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Suzuki-Chin combination (forces.cpp:698-789)
+// ---------------------------------------------------------------------------
+struct SCArgs {
+    int P, n_atoms, NAP;
+    double omega_psq, epsilonsc, halfdt;
+};
+
+// After the physical-force phase.  grid: P, 256 threads.
+//  even b: F_sc = 2 f / 3; tot_b = 2V; vop_b = V
+//  odd  b: tot_b = 4V, fsq_b = sum f^2/m / (3 w^2); delta and the displaced
+//          positions q +- eps f/m for the finite-difference phase.
+__global__ __launch_bounds__(256) void k_sc1(SCArgs a, const double *__restrict__ q,
+                                             const double *__restrict__ fph,
+                                             const double *__restrict__ Vb,
+                                             const double *__restrict__ m_atom,
+                                             double *__restrict__ Fsc, double *__restrict__ qpm,
+                                             double *__restrict__ scal /*[P][4]*/)
+{
+    __shared__ double sh[16];
+    __shared__ double s_scale;
+    const int b = blockIdx.x;
+    const long o = (long)b * 3 * a.NAP;
+    const double V = Vb[b];
+    if ((b & 1) == 0) {
+        for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) Fsc[o + t] = 2.0 * fph[o + t] / 3.0;
+        if (threadIdx.x == 0) {
+            scal[b * 4 + 0] = 2.0 * V;
+            scal[b * 4 + 1] = V;
+            scal[b * 4 + 2] = 0.0;
+            scal[b * 4 + 3] = 0.0;
+        }
+        return;
+    }
+    double fsq = 0.0, del = 0.0;
+    for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) {
+        const int at = t % a.NAP;
+        if (at < a.n_atoms) {
+            const double f = fph[o + t];
+            const double m = m_atom[at];
+            fsq += f * f / m;
+            const double fm = f / m;
+            del += fm * fm;
+        }
+    }
+    fsq = block_sum(fsq, sh);
+    del = block_sum(del, sh);
+    if (threadIdx.x == 0) {
+        fsq /= 3.0 * a.omega_psq;
+        del /= (double)(a.n_atoms * a.P);
+        del = 1.0 / sqrt(del);
+        const double sc_scale = a.epsilonsc * del;
+        scal[b * 4 + 0] = 4.0 * V;
+        scal[b * 4 + 1] = 0.0;
+        scal[b * 4 + 2] = fsq;
+        scal[b * 4 + 3] = 1.0 / (9.0 * a.omega_psq * sc_scale);
+        s_scale = sc_scale;
+    }
+    __syncthreads();
+    const double sc_scale = s_scale;
+    const int k = b >> 1;
+    double *Pp = qpm + (long)(2 * k) * 3 * a.NAP;
+    double *Pm = qpm + (long)(2 * k + 1) * 3 * a.NAP;
+    for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) {
+        const int at = t % a.NAP;
+        const double dq = sc_scale * fph[o + t] / m_atom[at];
+        Pp[t] = q[o + t] + dq;
+        Pm[t] = q[o + t] - dq;
+    }
+}
+
+// After the displaced-force phase: odd-bead F_sc = 4 f/3 - fscale (f+ - f-),
+// optional final half kick p += dt/2 F_sc (update.cpp:391-397), and (block 0)
+// the bead-ordered reduction of the SC potential and sc_pot (forces.cpp:780-781).
+// grid: P, 256 threads.   out: [0] tot_energy, [1] sc_pot
+__global__ __launch_bounds__(256) void k_sc2(SCArgs a, const double *__restrict__ fph,
+                                             const double *__restrict__ fpm,
+                                             const double *__restrict__ scal,
+                                             double *__restrict__ Fsc, double *__restrict__ p,
+                                             int kick, double *__restrict__ out,
+                                             unsigned long long *__restrict__ step_ctr)
+{
+    const int b = blockIdx.x;
+    const long o = (long)b * 3 * a.NAP;
+    if (b & 1) {
+        const int k = b >> 1;
+        const double fscale = scal[b * 4 + 3];
+        const double *fp = fpm + (long)(2 * k) * 3 * a.NAP;
+        const double *fm = fpm + (long)(2 * k + 1) * 3 * a.NAP;
+        for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) {
+            double F = 4.0 * fph[o + t] / 3.0;
+            F -= fscale * (fp[t] - fm[t]);
+            Fsc[o + t] = F;
+            if (kick) p[o + t] += a.halfdt * F;
+        }
+    } else if (kick) {
+        for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) p[o + t] += a.halfdt * Fsc[o + t];
+    }
+    if (b == 0 && threadIdx.x == 0) {
+        double tot = 0.0, vop = 0.0;
+        for (int bb = 0; bb < a.P; ++bb) {
+            if ((bb & 1) == 0) {
+                tot += scal[bb * 4 + 0];
+                vop += scal[bb * 4 + 1];
+            } else {
+                tot += scal[bb * 4 + 0];
+                tot += scal[bb * 4 + 2];
+            }
+        }
+        tot /= 3.0;
+        out[0] = tot;
+        out[1] = 2.0 * vop / a.P;
+        if (step_ctr) step_ctr[0] += 1ull;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Ring-polymer propagation (update.cpp:281-352), PILE-G (thermostat.cpp:64-133)
+// ---------------------------------------------------------------------------
+struct PropArgs {
+    int P, n_atoms, NAP, ntile;
+    double halfdt, dt, beta_n;
+    int nvt, host_dev, n_dof;  // host_dev: deviates from the host buffer
+    uint64_t seed;
+};
+
+// Per-mode constants: [k] = {scale_k, wsq_k, c1_k, c2_k}
+// (update.cpp:302-304, thermostat.cpp:119-121)
+
+// Kick + forward normal-mode transform + first sqrt-Cayley half step.
+// A tile = 64 consecutive planar dofs for all P beads, staged in LDS.
+// grid: ntile, 256 threads.  LDS: C[P][P], Xq[P][64], Xp[P][64].
+__global__ __launch_bounds__(256) void k_prop_a(PropArgs A, const double *__restrict__ Cm,
+                                                const double4 *__restrict__ mode,
+                                                const double *__restrict__ q,
+                                                const double *__restrict__ p,
+                                                const double *__restrict__ F,
+                                                const double *__restrict__ m_atom,
+                                                double *__restrict__ nq, double *__restrict__ np,
+                                                double *__restrict__ part /*[2][ntile]*/,
+                                                const unsigned long long *__restrict__ step_ctr)
+{
+    extern __shared__ double sm[];
+    const int P = A.P;
+    double *Cs = sm;
+    double *Xq = Cs + P * P;
+    double *Xp = Xq + P * 64;
+    __shared__ double sh[16];
+    const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
+    const int N3 = 3 * A.NAP;
+    const int f = blockIdx.x * 64 + lane;
+    const bool vf = f < N3;
+    for (int t = threadIdx.x; t < P * P; t += 256) Cs[t] = Cm[t];
+    for (int b = kg; b < P; b += 4) {
+        double vq = 0.0, vp = 0.0;
+        if (vf) {
+            vq = q[(long)b * N3 + f];
+            vp = p[(long)b * N3 + f] + A.halfdt * F[(long)b * N3 + f];
+        }
+        Xq[b * 64 + lane] = vq;
+        Xp[b * 64 + lane] = vp;
+    }
+    __syncthreads();
+    const int at = vf ? f % A.NAP : 0;
+    const bool va = vf && at < A.n_atoms;
+    const double m = va ? m_atom[at] : 1.0;
+    double ke0 = 0.0, chi = 0.0;
+    for (int k = kg; k < P; k += 4) {
+        double uq = 0.0, up = 0.0;
+        const double *cr = Cs + k * P;
+        for (int b = 0; b < P; ++b) {
+            uq = fma(cr[b], Xq[b * 64 + lane], uq);
+            up = fma(cr[b], Xp[b * 64 + lane], up);
+        }
+        const double4 md = mode[k];
+        double temp = A.halfdt * up / m + uq;
+        temp *= md.x;
+        up = up - A.halfdt * md.y * uq * m;
+        up *= md.x;
+        uq = temp;
+        if (vf) {
+            nq[(long)k * N3 + f] = uq;
+            np[(long)k * N3 + f] = up;
+        }
+        if (k == 0 && va) ke0 += 0.5 * up * up / m;
+    }
+    if (A.nvt && !A.host_dev && kg == 0 && va) {
+        // centroid chi-square: normals for reference dof index 1..n_dof-1
+        const int d = f / A.NAP;
+        const int ridx = 3 * at + d;
+        if (ridx >= 1) {
+            const double z = philox_normal(A.seed, *step_ctr, (uint32_t)(P + 1), (uint32_t)ridx);
+            chi += z * z;
+        }
+    }
+    ke0 = block_sum(ke0, sh);
+    chi = block_sum(chi, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = ke0;
+        part[A.ntile + blockIdx.x] = chi;
+    }
+}
+
+// PILE-G in normal-mode space, second sqrt-Cayley half step, inverse
+// transform, displacement tracking.   grid: ntile, 256 threads.
+// heat partials: hpart[tile] = sum over modes of KE_before - KE_after,
+// excluding the centroid's KE_before (added once from part[]).
+__global__ __launch_bounds__(256) void k_prop_b(PropArgs A, const double *__restrict__ Cm,
+                                                const double4 *__restrict__ mode,
+                                                const double *__restrict__ nq,
+                                                const double *__restrict__ np,
+                                                const double *__restrict__ m_atom,
+                                                const double *__restrict__ sqm_atom,
+                                                const double *__restrict__ part,
+                                                const double *__restrict__ dev,
+                                                double *__restrict__ q, double *__restrict__ p,
+                                                double *__restrict__ disp, int NMP,
+                                                double *__restrict__ hpart,
+                                                const unsigned long long *__restrict__ step_ctr)
+{
+    extern __shared__ double sm[];
+    const int P = A.P;
+    double *Cs = sm;
+    double *Uq = Cs + P * P;
+    double *Up = Uq + P * 64;
+    __shared__ double sh[16];
+    __shared__ double s_alpha;
+    const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
+    const int N3 = 3 * A.NAP;
+    const int f = blockIdx.x * 64 + lane;
+    const bool vf = f < N3;
+    const int at = vf ? f % A.NAP : 0;
+    const int d = vf ? f / A.NAP : 0;
+    const bool va = vf && at < A.n_atoms;
+    const double m = va ? m_atom[at] : 1.0;
+    const double sqm = va ? sqm_atom[at] : 0.0;
+    const uint64_t step = step_ctr ? *step_ctr : 0ull;
+    for (int t = threadIdx.x; t < P * P; t += 256) Cs[t] = Cm[t];
+    if (A.nvt && threadIdx.x == 0) {
+        double ke = 0.0, chi = 0.0;
+        for (int t = 0; t < A.ntile; ++t) ke += part[t];
+        double R, g2;
+        if (A.host_dev) {
+            R = dev[0];
+            if (A.n_dof % 2 == 0)
+                g2 = 2.0 * dev[2] + dev[1] * dev[1];
+            else
+                g2 = 2.0 * dev[2];
+        } else {
+            for (int t = 0; t < A.ntile; ++t) chi += part[A.ntile + t];
+            R = philox_normal(A.seed, step, (uint32_t)P, 0u);
+            g2 = chi;
+        }
+        const double alpha_sq = (R * R + g2) / (2.0 * A.beta_n * ke);
+        s_alpha = sqrt(alpha_sq);
+    }
+    for (int k = kg; k < P; k += 4) {
+        Uq[k * 64 + lane] = vf ? nq[(long)k * N3 + f] : 0.0;
+        Up[k * 64 + lane] = vf ? np[(long)k * N3 + f] : 0.0;
+    }
+    __syncthreads();
+    double heat = 0.0;
+    if (A.nvt) {
+        const double alpha = s_alpha;
+        const int ridx = 3 * at + d;
+        for (int k = kg; k < P; k += 4) {
+            double up = Up[k * 64 + lane];
+            if (k == 0) {
+                up *= alpha;
+                if (va) heat -= 0.5 * up * up / m;
+            } else {
+                const double4 md = mode[k];
+                if (va) heat += 0.5 * up * up / m;
+                double xi = 0.0;
+                if (va) {
+                    if (A.host_dev)
+                        xi = dev[3 + (long)(k - 1) * A.n_dof + ridx];
+                    else
+                        xi = philox_normal(A.seed, step, (uint32_t)k, (uint32_t)ridx);
+                }
+                up = md.z * up + sqm * md.w * xi;
+                if (va) heat -= 0.5 * up * up / m;
+            }
+            Up[k * 64 + lane] = up;
+        }
+    }
+    // second half step
+    for (int k = kg; k < P; k += 4) {
+        const double4 md = mode[k];
+        double uq = Uq[k * 64 + lane], up = Up[k * 64 + lane];
+        double temp = A.halfdt * up / m + uq;
+        temp *= md.x;
+        up = up - A.halfdt * md.y * uq * m;
+        up *= md.x;
+        Uq[k * 64 + lane] = temp;
+        Up[k * 64 + lane] = up;
+    }
+    __syncthreads();
+    // inverse transform: x_b = sum_k C[k][b] u_k
+    for (int b = kg; b < P; b += 4) {
+        double xq = 0.0, xp = 0.0;
+        for (int k = 0; k < P; ++k) {
+            const double c = Cs[k * P + b];
+            xq = fma(c, Uq[k * 64 + lane], xq);
+            xp = fma(c, Up[k * 64 + lane], xp);
+        }
+        if (vf) {
+            const long idx = (long)b * N3 + f;
+            const double qold = q[idx];
+            q[idx] = xq;
+            p[idx] = xp;
+            if (va && (at % 3) == 0) disp[((long)b * 3 + d) * NMP + at / 3] += xq - qold;
+        }
+    }
+    if (A.nvt) {
+        heat = block_sum(heat, sh);
+        if (threadIdx.x == 0) hpart[blockIdx.x] = heat;
+    }
+}
+
+// Accumulate the thermostat heat of one step: heat -= pile_g's return
+// (update.cpp:325).  Single thread.
+__global__ void k_heat(int ntile, const double *__restrict__ part,
+                       const double *__restrict__ hpart, double *__restrict__ heat_acc,
+                       double *__restrict__ heat_step)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double ke = 0.0, h = 0.0;
+    for (int t = 0; t < ntile; ++t) ke += part[t];
+    for (int t = 0; t < ntile; ++t) h += hpart[t];
+    const double ret = ke + h;  // pile_g's return value
+    if (heat_step) heat_step[0] = ret;
+    if (heat_acc) heat_acc[0] -= ret;
+}
+
+// Generic orthonormal transform along the bead axis of a[P][len] (fourier.cpp:12-69).
+// grid: ceil(len/64), 256 threads.  fwd: u_k = sum_b C[k][b] x_b, else the transpose.
+__global__ __launch_bounds__(256) void k_nm_generic(int P, int len, const double *__restrict__ Cm,
+                                                    double *__restrict__ a, int fwd)
+{
+    extern __shared__ double sm[];
+    double *Cs = sm;
+    double *X = Cs + P * P;
+    const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + lane;
+    for (int t = threadIdx.x; t < P * P; t += 256) Cs[t] = Cm[t];
+    for (int b = kg; b < P; b += 4) X[b * 64 + lane] = c < len ? a[(long)b * len + c] : 0.0;
+    __syncthreads();
+    for (int k = kg; k < P; k += 4) {
+        double u = 0.0;
+        for (int b = 0; b < P; ++b) {
+            const double cf = fwd ? Cs[k * P + b] : Cs[b * P + k];
+            u = fma(cf, X[b * 64 + lane], u);
+        }
+        if (c < len) a[(long)k * len + c] = u;
+    }
+}
+
+// PILE-G on normal-mode momenta in planar layout (drop-in pile_g).
+// Pass 1: centroid KE partials.  grid: ntile, 256 threads.
+__global__ __launch_bounds__(256) void k_pile_a(int n_atoms, int NAP, int ntile,
+                                                const double *__restrict__ pnm,
+                                                const double *__restrict__ m_atom,
+                                                double *__restrict__ part)
+{
+    __shared__ double sh[16];
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    double ke = 0.0;
+    if (f < 3 * NAP) {
+        const int at = f % NAP;
+        if (at < n_atoms) ke = 0.5 * pnm[f] * pnm[f] / m_atom[at];
+    }
+    ke = block_sum(ke, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = ke;
+}
+
+// Pass 2: rescale the centroid, Langevin on modes 1..P-1 with host deviates.
+// grid: ntile, 256 threads (same tiling as pass 1).
+__global__ __launch_bounds__(256) void k_pile_b(int P, int n_atoms, int NAP, int n_dof, int ntile,
+                                                double beta_n, const double4 *__restrict__ mode,
+                                                const double *__restrict__ m_atom,
+                                                const double *__restrict__ sqm_atom,
+                                                const double *__restrict__ part,
+                                                const double *__restrict__ dev,
+                                                double *__restrict__ pnm,
+                                                double *__restrict__ hpart)
+{
+    __shared__ double sh[16];
+    __shared__ double s_alpha;
+    if (threadIdx.x == 0) {
+        double ke = 0.0;
+        for (int t = 0; t < ntile; ++t) ke += part[t];
+        const double R = dev[0];
+        const double g2 = (n_dof % 2 == 0) ? 2.0 * dev[2] + dev[1] * dev[1] : 2.0 * dev[2];
+        s_alpha = sqrt((R * R + g2) / (2.0 * beta_n * ke));
+    }
+    __syncthreads();
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    double heat = 0.0;
+    if (f < 3 * NAP) {
+        const int at = f % NAP, d = f / NAP;
+        if (at < n_atoms) {
+            const double m = m_atom[at], sq = sqm_atom[at];
+            const int ridx = 3 * at + d;
+            double up = pnm[f] * s_alpha;
+            pnm[f] = up;
+            heat -= 0.5 * up * up / m;
+            for (int k = 1; k < P; ++k) {
+                const double4 md = mode[k];
+                const long idx = (long)k * 3 * NAP + f;
+                double v = pnm[idx];
+                heat += 0.5 * v * v / m;
+                v = md.z * v + sq * md.w * dev[3 + (long)(k - 1) * n_dof + ridx];
+                heat -= 0.5 * v * v / m;
+                pnm[idx] = v;
+            }
+        }
+    }
+    heat = block_sum(heat, sh);
+    if (threadIdx.x == 0) hpart[blockIdx.x] = heat;
+}
+
+// ---------------------------------------------------------------------------
+// Verlet lists (neighbour.cpp:12-62) and the rebuild test (update.cpp:356-385)
+// ---------------------------------------------------------------------------
+
+// Per bead: two largest O displacement norms; flag = d1 + d2 > r_skin - r_cut.
+// grid: P, 256 threads.
+__global__ __launch_bounds__(256) void k_check(int n_mols, int NMP, const double *__restrict__ disp,
+                                               double skin, int force_all, int *__restrict__ flag,
+                                               double *__restrict__ nreb)
+{
+    __shared__ double s1[256], s2[256];
+    const int b = blockIdx.x;
+    double m1 = 0.0, m2 = 0.0;
+    for (int i = threadIdx.x; i < n_mols; i += 256) {
+        const double *D = disp + (long)b * 3 * NMP;
+        const double dx = D[i], dy = D[NMP + i], dz = D[2 * NMP + i];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > m1) {
+            m2 = m1;
+            m1 = d2;
+        } else if (d2 > m2) {
+            m2 = d2;
+        }
+    }
+    s1[threadIdx.x] = m1;
+    s2[threadIdx.x] = m2;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if (threadIdx.x < o) {
+            const double a1 = s1[threadIdx.x], a2 = s2[threadIdx.x];
+            const double b1 = s1[threadIdx.x + o], b2 = s2[threadIdx.x + o];
+            // top two of {a1 >= a2, b1 >= b2}
+            double t1, t2;
+            if (a1 >= b1) {
+                t1 = a1;
+                t2 = a2 >= b1 ? a2 : b1;
+            } else {
+                t1 = b1;
+                t2 = b2 >= a1 ? b2 : a1;
+            }
+            s1[threadIdx.x] = t1;
+            s2[threadIdx.x] = t2;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const int f = force_all || (sqrt(s1[0]) + sqrt(s2[0]) > skin);
+        flag[b] = f;
+        if (f && nreb) nreb[b] += 1.0;
+    }
+}
+
+// Count O-O pairs within r_skin per row (half: j > i; full: j != i).
+// The distance test is bit-for-bit neighbour.cpp:42-53.
+// grid: P * ceil(n_mols/4), 256 threads (one row per wavefront).
+__global__ __launch_bounds__(256) void k_nl_count(int n_mols, int NAP, const double *__restrict__ q,
+                                                  Box box, double r_skin_sq,
+                                                  const int *__restrict__ flag,
+                                                  int *__restrict__ cnt /*[P][2][n_mols]*/)
+{
+    const int nrb = (n_mols + 3) / 4;
+    const int b = blockIdx.x / nrb;
+    if (!flag[b]) return;
+    const int i = (blockIdx.x - b * nrb) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n_mols) return;
+    const double *Q = q + (long)b * 3 * NAP;
+    const double xi = Q[3 * i], yi = Q[NAP + 3 * i], zi = Q[2 * NAP + 3 * i];
+    int nh = 0, nf = 0;
+    for (int j0 = 0; j0 < n_mols; j0 += 64) {
+        const int j = j0 + lane;
+        bool in = false;
+        if (j < n_mols && j != i) {
+            const double rx = mimg_exact(xi - Q[3 * j], box.L[0]);
+            const double ry = mimg_exact(yi - Q[NAP + 3 * j], box.L[1]);
+            const double rz = mimg_exact(zi - Q[2 * NAP + 3 * j], box.L[2]);
+            const double rsq = rx * rx + ry * ry + rz * rz;
+            in = rsq < r_skin_sq;
+        }
+        const unsigned long long bf = __ballot(in);
+        const unsigned long long bh = __ballot(in && j > i);
+        nf += __popcll(bf);
+        nh += __popcll(bh);
+    }
+    if (lane == 0) {
+        cnt[((long)b * 2 + 0) * n_mols + i] = nh;
+        cnt[((long)b * 2 + 1) * n_mols + i] = nf;
+    }
+}
+
+// Exclusive scans -> point (reference layout) and fpoint; zero the bead's
+// displacements and the unused tail of its half list (neighbour.cpp:23-32).
+// grid: P, 1024 threads.
+__global__ __launch_bounds__(1024) void k_nl_scan(int n_mols, int list_size, long fcap,
+                                                  const int *__restrict__ flag,
+                                                  const int *__restrict__ cnt,
+                                                  int *__restrict__ point, int *__restrict__ fpoint,
+                                                  int *__restrict__ list, double *__restrict__ disp,
+                                                  int NMP, int *__restrict__ status)
+{
+    const int b = blockIdx.x;
+    if (!flag[b]) return;
+    __shared__ int sh[1024];
+    __shared__ int carry[2];
+    if (threadIdx.x < 2) carry[threadIdx.x] = 0;
+    __syncthreads();
+    for (int which = 0; which < 2; ++which) {
+        const int *c = cnt + ((long)b * 2 + which) * n_mols;
+        int *outp = which == 0 ? point + (long)b * n_mols : fpoint + (long)b * (n_mols + 1);
+        for (int i0 = 0; i0 < n_mols; i0 += 1024) {
+            const int i = i0 + threadIdx.x;
+            const int v = i < n_mols ? c[i] : 0;
+            sh[threadIdx.x] = v;
+            __syncthreads();
+            for (int o = 1; o < 1024; o <<= 1) {
+                const int add = threadIdx.x >= o ? sh[threadIdx.x - o] : 0;
+                __syncthreads();
+                sh[threadIdx.x] += add;
+                __syncthreads();
+            }
+            const int incl = sh[threadIdx.x];
+            const int base = carry[which];
+            if (i < n_mols) outp[i] = base + incl - v;
+            __syncthreads();
+            if (threadIdx.x == 1023) carry[which] = base + incl;
+            __syncthreads();
+        }
+        if (which == 1 && threadIdx.x == 0) outp[n_mols] = carry[1];
+    }
+    __syncthreads();
+    const int nh = carry[0], nf = carry[1];
+    if (threadIdx.x == 0) {
+        if (nh > list_size || (long)nf > fcap) atomicOr(status, 1);
+    }
+    int *L = list + (long)b * list_size;
+    for (int k = nh + threadIdx.x; k < list_size; k += 1024) L[k] = 0;
+    for (int t = threadIdx.x; t < 3 * NMP; t += 1024) disp[(long)b * 3 * NMP + t] = 0.0;
+}
+
+// Write the list entries in ascending j with ballot + prefix popcount.
+// grid: as k_nl_count.
+__global__ __launch_bounds__(256) void k_nl_fill(int n_mols, int NAP, const double *__restrict__ q,
+                                                 Box box, double r_skin_sq, int list_size, long fcap,
+                                                 const int *__restrict__ flag,
+                                                 const int *__restrict__ point,
+                                                 const int *__restrict__ fpoint,
+                                                 int *__restrict__ list, int *__restrict__ flist)
+{
+    const int nrb = (n_mols + 3) / 4;
+    const int b = blockIdx.x / nrb;
+    if (!flag[b]) return;
+    const int i = (blockIdx.x - b * nrb) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n_mols) return;
+    const double *Q = q + (long)b * 3 * NAP;
+    const double xi = Q[3 * i], yi = Q[NAP + 3 * i], zi = Q[2 * NAP + 3 * i];
+    int kh = point[(long)b * n_mols + i];
+    long kf = fpoint[(long)b * (n_mols + 1) + i];
+    int *L = list + (long)b * list_size;
+    int *FL = flist + (long)b * fcap;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int j0 = 0; j0 < n_mols; j0 += 64) {
+        const int j = j0 + lane;
+        bool in = false;
+        if (j < n_mols && j != i) {
+            const double rx = mimg_exact(xi - Q[3 * j], box.L[0]);
+            const double ry = mimg_exact(yi - Q[NAP + 3 * j], box.L[1]);
+            const double rz = mimg_exact(zi - Q[2 * NAP + 3 * j], box.L[2]);
+            const double rsq = rx * rx + ry * ry + rz * rz;
+            in = rsq < r_skin_sq;
+        }
+        const bool inh = in && j > i;
+        const unsigned long long bf = __ballot(in);
+        const unsigned long long bh = __ballot(inh);
+        if (in) {
+            const long pos = kf + __popcll(bf & below);
+            if (pos < fcap) FL[pos] = j;
+        }
+        if (inh) {
+            const int pos = kh + __popcll(bh & below);
+            if (pos < list_size) L[pos] = j;
+        }
+        kf += __popcll(bf);
+        kh += __popcll(bh);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Classical velocity Verlet (update.cpp:78-137)
+// ---------------------------------------------------------------------------
+// p += dt/2 F; q += p dt / m; O displacements.  grid: ceil(3 NAP / 256).
+__global__ __launch_bounds__(256) void k_vv_a(int n_atoms, int NAP, int NMP, double halfdt,
+                                              double dt, const double *__restrict__ m_atom,
+                                              const double *__restrict__ F, double *__restrict__ q,
+                                              double *__restrict__ p, double *__restrict__ disp)
+{
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    if (f >= 3 * NAP) return;
+    const int at = f % NAP, d = f / NAP;
+    if (at >= n_atoms) return;
+    double pp = p[f] + halfdt * F[f];
+    p[f] = pp;
+    const double m = m_atom[at];
+    q[f] += pp * dt / m;
+    if (at % 3 == 0) disp[d * NMP + at / 3] += pp * dt / m;
+}
+
+// F = f_phys of the single evaluation, p += dt/2 F, energy out.
+__global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfdt,
+                                              const double *__restrict__ fe,
+                                              const double *__restrict__ Ve,
+                                              double *__restrict__ F, double *__restrict__ p,
+                                              double *__restrict__ out,
+                                              unsigned long long *__restrict__ step_ctr)
+{
+    const int f = blockIdx.x * 256 + threadIdx.x;
+    if (f < 3 * NAP) {
+        const double v = fe[f];
+        F[f] = v;
+        const int at = f % NAP;
+        if (at < n_atoms) p[f] += halfdt * v;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        out[0] = Ve[0];
+        out[1] = 0.0;
+        if (step_ctr) step_ctr[0] += 1ull;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Observables (path_integrals.cpp:10-81)
+// ---------------------------------------------------------------------------
+// get_kcv: 0.5 n_dof / beta + (1/P) sum_{even b} sum_j -(q_bj - qc_j) f_bj.
+// grid: ceil(3 NAP / 256); out: partials per block.
+__global__ __launch_bounds__(256) void k_kcv(int P, int n_atoms, int NAP,
+                                             const double *__restrict__ q,
+                                             const double *__restrict__ f,
+                                             double *__restrict__ part)
+{
+    __shared__ double sh[16];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (t < 3 * NAP && (t % NAP) < n_atoms) {
+        const long N3 = 3L * NAP;
+        double qc = 0.0;
+        for (int b = 0; b < P; ++b) qc += q[b * N3 + t];
+        qc /= (double)(float)P;
+        for (int b = 0; b < P; b += 2) acc -= (q[b * N3 + t] - qc) * f[b * N3 + t];
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// get_conserved on normal-mode q, p: sum_k sum_j p^2/2m + m w_k^2 q^2 / 2.
+__global__ __launch_bounds__(256) void k_cons(int P, int n_atoms, int NAP,
+                                              const double *__restrict__ nq,
+                                              const double *__restrict__ np,
+                                              const double4 *__restrict__ mode,
+                                              const double *__restrict__ m_atom,
+                                              double *__restrict__ part)
+{
+    __shared__ double sh[16];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double acc = 0.0;
+    if (t < 3 * NAP && (t % NAP) < n_atoms) {
+        const long N3 = 3L * NAP;
+        const double m = m_atom[t % NAP];
+        for (int k = 0; k < P; ++k) {
+            const double pv = np[k * N3 + t], qv = nq[k * N3 + t];
+            acc += 0.5 * pv * pv / m + 0.5 * mode[k].y * (m * qv * qv);
+        }
+    }
+    acc = block_sum(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+}  // namespace qtp

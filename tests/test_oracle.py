@@ -405,3 +405,37 @@ def test_free_ring_polymer_step_is_exact_cayley(oracle_mod):
         cay = np.linalg.solve(np.eye(2) - 0.5 * s.dt * np.array([[0, 1 / m], [-w2 * m, 0]]),
                               np.eye(2) + 0.5 * s.dt * np.array([[0, 1 / m], [-w2 * m, 0]]))
         assert np.allclose(full, cay, atol=1e-12)
+
+
+# ------------------------------------------------------- golden vectors
+def _golden(name):
+    import os
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", name))
+
+
+def test_oracle_neighbour_list_matches_golden(oracle_mod):
+    g = _golden("nlist_216.npz")
+    P = int(g["n_beads"])
+    o = oracle_mod.Oracle(216, P, tuple(g["boxl"]))
+    assert o.list_size == int(g["list_size"])
+    point, lst, disp = o.new_lists()
+    for b in range(P):
+        o.neighbour_list(g["q"], point, lst, disp, b)
+    assert np.array_equal(point, g["point"])
+    assert np.array_equal(lst, g["list"])
+
+
+def test_oracle_pile_g_matches_golden(oracle_mod):
+    """Reference pile_g with std::mt19937(seed), two consecutive calls; the
+    oracle is fed the library's host draws of the same stream."""
+    from qtip4pf_water_amd import HostRng
+    g = _golden("pile_g_27x8.npz")
+    P = int(g["n_beads"])
+    o = oracle_mod.Oracle(27, P, tuple(g["boxl"]), T_K=float(g["T_K"]), dtfs=float(g["dtfs"]))
+    assert np.array_equal(o.mass, g["mass"])
+    rng = HostRng(int(g["seed"]))
+    p = g["p_in"].copy()
+    h1 = o.pile_g(p, rng.draw_pile(o.n_atoms, P))
+    assert np.array_equal(p, g["p_out1"]) and h1 == float(g["heat1"])
+    h2 = o.pile_g(p, rng.draw_pile(o.n_atoms, P))
+    assert np.array_equal(p, g["p_out2"]) and h2 == float(g["heat2"])
