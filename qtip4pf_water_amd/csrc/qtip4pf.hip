@@ -116,6 +116,7 @@ struct qtp_ctx {
     int2 *tiles = nullptr;
     double *Cm = nullptr;
     double4 *mode = nullptr;
+    double *etab = nullptr;  // erfcx polynomial table
     // propagation scratch
     double *nq = nullptr, *np = nullptr, *ppart = nullptr, *hpart = nullptr, *dev = nullptr;
     double *obs = nullptr;
@@ -374,7 +375,7 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->nq, (size_t)P * N3));
     CHK(dalloc(ctx, &ctx->np, (size_t)P * N3));
     CHK(dalloc(ctx, &ctx->ppart, 2 * (size_t)ctx->ptile + 2));
-    CHK(dalloc(ctx, &ctx->hpart, (size_t)ctx->ptile + 1));
+    CHK(dalloc(ctx, &ctx->hpart, (size_t)ctx->P * ctx->ptile + 1));
     CHK(dalloc(ctx, &ctx->obs, (size_t)ctx->ptile * 4 + 64));
     ctx->dev_cap = 3 + (long)(P > 1 ? P - 1 : 0) * ctx->n_dof;
     CHK(dalloc(ctx, &ctx->dev, (size_t)ctx->dev_cap));
@@ -434,31 +435,26 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
 {
     hipStream_t s = ctx->stream;
     const long N3 = 3L * ctx->NAP;
-    {
-        TimeScope ts(ctx, "sites");
-        const int nb = (ctx->NAP + 255) / 256;
-        k_sites<<<E * nb, 256, 0, s>>>(pos, N3, ctx->n_atoms, ctx->NAP, ctx->ff.gamma_vs,
-                                       ctx->ff.half1mg, ctx->cpos);
-    }
+    const double g = ctx->ff.gamma_vs, h = ctx->ff.half1mg;
     {
         TimeScope ts(ctx, "ljintra");
-        k_ljintra<<<E * ctx->nmb, 64, 0, s>>>(pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint,
-                                               ctx->flist, ctx->fcap, lmode, lbase, ctx->box,
-                                               ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride,
-                                               ctx->ep_lj);
+        k_ljintra<<<E * ctx->nmb, 256, 0, s>>>(pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint,
+                                                ctx->flist, ctx->fcap, lmode, lbase, ctx->box,
+                                                ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride,
+                                                ctx->ep_lj);
     }
     {
         TimeScope ts(ctx, "realspace");
-        k_realspace<<<E * ctx->ntile, 64, 0, s>>>(ctx->cpos, ctx->charge, ctx->n_atoms, ctx->NAP,
-                                                   ctx->nblk, ctx->tiles, ctx->ntile, ctx->box,
-                                                   ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride,
-                                                   ctx->ep_real);
+        k_realspace<<<E * ctx->ntile, 64, 0, s>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
+                                                   ctx->nblk, g, h, ctx->tiles, ctx->ntile,
+                                                   ctx->box, ctx->ew, ctx->etab, ctx->rpart,
+                                                   ctx->epart, ctx->ep_stride, ctx->ep_real);
     }
     {
         TimeScope ts(ctx, "ksum");
         k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, s>>>(
-            ctx->cpos, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->ew, ctx->groups, ctx->G, ctx->AC,
-            ctx->ngc, ctx->nac, ctx->spart);
+            pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->G,
+            ctx->AC, ctx->ngc, ctx->nac, ctx->spart);
     }
     {
         TimeScope ts(ctx, "sreduce");
@@ -467,15 +463,14 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     }
     {
         TimeScope ts(ctx, "kforce");
-        k_kforce<<<E * (ctx->NAP / 64), 256, ctx->kforce_lds, s>>>(
-            ctx->cpos, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->ew, ctx->groups, ctx->grk, ctx->G,
-            ctx->sprime, ctx->rpart, ctx->nblk, ctx->ef);
+        const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
+        k_kforce<<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
+            pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->grk,
+            ctx->G, ctx->sprime, ctx->rpart, ctx->nblk, ctx->flj, fout);
     }
-    {
-        TimeScope ts(ctx, "finalize");
-        k_finalize<<<E, 256, 0, s>>>(ctx->flj, ctx->ef, ctx->n_atoms, ctx->NAP, ctx->ff.gamma_vs,
-                                     ctx->ff.half1mg, ctx->epart, ctx->ep_stride, ctx->eself, fout,
-                                     eout);
+    if (eout) {
+        TimeScope ts(ctx, "esum");
+        k_esum<<<E, 64, 0, s>>>(E, ctx->epart, ctx->ep_stride, ctx->eself, eout);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;
@@ -498,14 +493,14 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
 {
     hipStream_t s = ctx->stream;
     const int P = ctx->P;
-    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fph, ctx->Ve));
+    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fph, nullptr));
     SCArgs a = sc_args(ctx);
     {
         TimeScope ts(ctx, "sc1");
-        k_sc1<<<P, 256, 0, s>>>(a, ctx->q, ctx->fph, ctx->Ve, ctx->m_atom, ctx->F, ctx->qpm,
-                                ctx->scal);
+        k_sc1<<<P, 256, 0, s>>>(a, ctx->q, ctx->fph, ctx->epart, ctx->ep_stride, ctx->eself,
+                                ctx->Ve, ctx->m_atom, ctx->F, ctx->qpm, ctx->scal);
     }
-    CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, ctx->Ve + P));
+    CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr));
     {
         TimeScope ts(ctx, "sc2");
         k_sc2<<<P, 256, 0, s>>>(a, ctx->fph, ctx->fpm, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
@@ -519,6 +514,16 @@ static int run_lists(qtp_ctx *ctx, int force_all)
 {
     hipStream_t s = ctx->stream;
     const int P = ctx->P;
+    if (ctx->n_mols <= NL_MAXMOL) {
+        TimeScope ts(ctx, "nlist");
+        k_nlist_bead<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box,
+                                        RC.r_skin_sq, RC.r_skin - RC.r_cut, force_all,
+                                        ctx->list_size, ctx->fcap, ctx->disp, ctx->flag, ctx->nreb,
+                                        ctx->point, ctx->fpoint, ctx->list, ctx->flist,
+                                        ctx->status);
+        HIPCHK(hipGetLastError());
+        return QTP_OK;
+    }
     {
         TimeScope ts(ctx, "check");
         k_check<<<P, 256, 0, s>>>(ctx->n_mols, ctx->NMP, ctx->disp, RC.r_skin - RC.r_cut, force_all,
@@ -562,19 +567,22 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
 {
     hipStream_t s = ctx->stream;
     PropArgs A = prop_args(ctx, nvt, host_dev);
+    const dim3 grid(ctx->ptile, ctx->P);
     {
         TimeScope ts(ctx, "prop_a");
-        k_prop_a<<<ctx->ptile, 256, ctx->prop_lds, s>>>(A, ctx->Cm, ctx->mode, ctx->q, ctx->p,
-                                                        ctx->F, ctx->m_atom, ctx->nq, ctx->np,
-                                                        ctx->ppart, ctr);
+        k_prop_fwd<<<grid, 64, 0, s>>>(A, ctx->Cm, ctx->mode, ctx->q, ctx->p, ctx->F, ctx->m_atom,
+                                       ctx->sqm_atom, ctx->dev, ctx->nq, ctx->np, ctx->ppart,
+                                       ctx->hpart, ctr);
     }
     {
         TimeScope ts(ctx, "prop_b");
-        k_prop_b<<<ctx->ptile, 256, ctx->prop_lds, s>>>(
-            A, ctx->Cm, ctx->mode, ctx->nq, ctx->np, ctx->m_atom, ctx->sqm_atom, ctx->ppart,
-            ctx->dev, ctx->q, ctx->p, ctx->disp, ctx->NMP, ctx->hpart, ctr);
+        k_prop_inv<<<grid, 64, 0, s>>>(A, ctx->Cm, ctx->mode, ctx->nq, ctx->np, ctx->m_atom,
+                                       ctx->ppart, ctx->dev, ctx->q, ctx->p, ctx->disp, ctx->NMP,
+                                       ctx->hpart, ctr);
     }
-    if (nvt) k_heat<<<1, 64, 0, s>>>(ctx->ptile, ctx->ppart, ctx->hpart, ctx->heat, ctx->heat_step);
+    if (nvt)
+        k_heat<<<1, 64, 0, s>>>(ctx->ptile, ctx->P * ctx->ptile, ctx->ppart, ctx->hpart, ctx->heat,
+                                ctx->heat_step);
     HIPCHK(hipGetLastError());
     CHK(run_lists(ctx, 0));
     CHK(run_sc_forces(ctx, 1, ctr));
@@ -592,11 +600,11 @@ static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr)
                                   ctx->m_atom, ctx->F, ctx->q, ctx->p, ctx->disp);
     }
     CHK(run_lists(ctx, 0));
-    CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, ctx->Ve));
+    CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, nullptr));
     {
         TimeScope ts(ctx, "vv_b");
-        k_vv_b<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->drv.halfdt, ctx->fph, ctx->Ve, ctx->F,
-                                  ctx->p, ctx->out, ctr);
+        k_vv_b<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->drv.halfdt, ctx->fph, ctx->epart,
+                                  ctx->ep_stride, ctx->eself, ctx->F, ctx->p, ctx->out, ctr);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;
@@ -856,19 +864,18 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->AC = std::max(1, std::min(64, (int)(48 * 1024 / (3 * K * 16))));
     ctx->nac = (ctx->n_atoms + ctx->AC - 1) / ctx->AC;
     ctx->ksum_lds = (size_t)ctx->AC * 3 * K * sizeof(double2);
-    ctx->kforce_lds = (size_t)3 * K * 64 * sizeof(double2) + (size_t)KF_GC * (8 + 3) * sizeof(double) +
-                      (size_t)KF_GC * sizeof(int);
+    ctx->kforce_lds = (size_t)3 * K * 64 * sizeof(double2) + (size_t)KF_WAVES * 3 * 64 * sizeof(double);
     std::vector<int2> tl;
     for (int I = 0; I < ctx->nblk; ++I)
         for (int J = I; J < ctx->nblk; ++J) tl.push_back(make_int2(I, J));
     ctx->ntile = (int)tl.size();
-    ctx->nmb = (ctx->n_mols + 63) / 64;
+    ctx->nmb = (ctx->n_mols + (256 / LJ_LPM) - 1) / (256 / LJ_LPM);
     ctx->ep_lj = 0;
     ctx->ep_real = ctx->nmb;
     ctx->ep_k = ctx->nmb + ctx->ntile;
     ctx->ep_stride = ctx->nmb + ctx->ntile + ctx->ngr;
     ctx->ptile = (3 * ctx->NAP + 63) / 64;
-    ctx->prop_lds = (size_t)ctx->P * ctx->P * sizeof(double) + 2 * (size_t)ctx->P * 64 * sizeof(double);
+    ctx->prop_lds = 0;
     *out = ctx;
 
     // a blocking stream: host-synchronous hipMemcpy/hipMemset on the null
@@ -877,14 +884,11 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
         ctx->err = "hipStreamCreate failed";
         return QTP_ERR_HIP;
     }
-    if (ctx->kforce_lds > 64 * 1024 || ctx->ksum_lds > 64 * 1024 || ctx->prop_lds > 64 * 1024) {
-        if (hipFuncSetAttribute((const void *)k_kforce, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)std::max<size_t>(ctx->kforce_lds, 64 * 1024)) != hipSuccess ||
-            hipFuncSetAttribute((const void *)k_prop_a, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)std::max<size_t>(ctx->prop_lds, 64 * 1024)) != hipSuccess ||
-            hipFuncSetAttribute((const void *)k_prop_b, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)std::max<size_t>(ctx->prop_lds, 64 * 1024)) != hipSuccess) {
-            ctx->err = "LDS request above the device limit";
+    if (ctx->kforce_lds > 64 * 1024 || ctx->ksum_lds > 64 * 1024) {
+        if (ctx->kforce_lds > 160 * 1024 ||
+            hipFuncSetAttribute((const void *)k_kforce, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ctx->kforce_lds) != hipSuccess) {
+            ctx->err = "LDS request above the device limit (k_max too large)";
             return QTP_ERR_ARG;
         }
     }
@@ -892,12 +896,14 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     if ((st = dalloc(ctx, &ctx->groups, ctx->G)) || (st = dalloc(ctx, &ctx->gcf, ctx->G)) ||
         (st = dalloc(ctx, &ctx->grk, ctx->G)) || (st = dalloc(ctx, &ctx->tiles, ctx->ntile)) ||
         (st = dalloc(ctx, &ctx->Cm, (size_t)ctx->P * ctx->P)) ||
-        (st = dalloc(ctx, &ctx->mode, ctx->P)) || (st = alloc_all(ctx)))
+        (st = dalloc(ctx, &ctx->mode, ctx->P)) ||
+        (st = dalloc(ctx, &ctx->etab, QTP_ERFCX_NINT * QTP_ERFCX_NCOEF)) || (st = alloc_all(ctx)))
         return st;
     if (hipMemcpy(ctx->groups, g.data(), g.size() * sizeof(int4), hipMemcpyHostToDevice) ||
         hipMemcpy(ctx->gcf, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice) ||
         hipMemcpy(ctx->grk, rk.data(), rk.size() * sizeof(double4), hipMemcpyHostToDevice) ||
-        hipMemcpy(ctx->tiles, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice)) {
+        hipMemcpy(ctx->tiles, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice) ||
+        hipMemcpy(ctx->etab, qtp_erfcx_coef, sizeof(qtp_erfcx_coef), hipMemcpyHostToDevice)) {
         ctx->err = "constant upload failed";
         return QTP_ERR_HIP;
     }
@@ -917,7 +923,7 @@ void qtp_destroy(qtp_ctx *ctx)
                     ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->cpos, ctx->rpart,
                     ctx->spart, ctx->sprime, ctx->ef, ctx->flj, ctx->fpm, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
-                    ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
+                    ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->etab, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -1066,7 +1072,7 @@ int qtp_pile_g(qtp_ctx *ctx, double *p_nm, const double *mass, const double *dev
                                               ctx->drv.beta_n, ctx->mode, ctx->m_atom,
                                               ctx->sqm_atom, ctx->ppart, ctx->dev, ctx->np,
                                               ctx->hpart);
-        k_heat<<<1, 64, 0, ctx->stream>>>(nt, ctx->ppart, ctx->hpart, nullptr, ctx->heat_step);
+        k_heat<<<1, 64, 0, ctx->stream>>>(nt, nt, ctx->ppart, ctx->hpart, nullptr, ctx->heat_step);
     }
     HIPCHK(hipGetLastError());
     double h = 0.0;

@@ -11,6 +11,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "qtp_erfcx_table.h"
+
 namespace qtp {
 
 constexpr int WAVE = 64;
@@ -109,46 +111,71 @@ __device__ inline double philox_normal(uint64_t seed, uint64_t step, uint32_t st
 // ---------------------------------------------------------------------------
 // Force evaluation.  A "phase" evaluates E independent single-bead forces
 // (forces.cpp:468-630); eval e reads positions pos + e*pstride and bead
-// lst_bead(e)'s Verlet list.
+// lst_bead(e)'s Verlet list.  Energies are left as per-block partials
+// epart[e][0 .. ep_stride) (LJ+intra blocks, real-space tiles, k-space
+// blocks); eval_energy() sums them in that fixed order.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lst_bead(int mode, int base, int e)
 {
     return mode == 0 ? base + e : 2 * (e >> 1) + 1;
 }
 
-// M-site (charge) positions, forces.cpp:558-564.  grid: E * ceil(NAP/256).
-__global__ __launch_bounds__(256) void k_sites(const double *__restrict__ pos, long pstride,
-                                               int n_atoms, int NAP, double gamma, double half1mg,
-                                               double *__restrict__ cpos)
+// Charge-site coordinate d of site a (a < n_atoms): the M-site for O slots,
+// r_M = gamma r_O + (1-gamma)/2 (r_H1 + r_H2) (forces.cpp:558-564).
+__device__ __forceinline__ double site_coord(const double *__restrict__ P, int NAP, int a, int d,
+                                             double gamma, double half1mg)
 {
-    const int nb = (NAP + 255) / 256;
-    const int e = blockIdx.x / nb;
-    const int a = (blockIdx.x - e * nb) * 256 + threadIdx.x;
-    if (a >= NAP) return;
-    const double *P = pos + (long)e * pstride;
-    double *Cp = cpos + (long)e * 3 * NAP;
-    for (int d = 0; d < 3; ++d) {
-        double v = 0.0;
-        if (a < n_atoms) {
-            v = P[d * NAP + a];
-            if (a % 3 == 0)
-                v = gamma * v + half1mg * (P[d * NAP + a + 1] + P[d * NAP + a + 2]);
-        }
-        Cp[d * NAP + a] = v;
-    }
+    const double v = P[d * NAP + a];
+    if (a % 3 != 0) return v;
+    return gamma * v + half1mg * (P[d * NAP + a + 1] + P[d * NAP + a + 2]);
+}
+
+// Sum of n partials by one wavefront (lane-strided, then a fixed xor tree):
+// deterministic, the same value in every lane.
+__device__ __forceinline__ double wave_sum_array(const double *__restrict__ p, int n)
+{
+    const int lane = threadIdx.x & 63;
+    double s = 0.0;
+    for (int k = lane; k < n; k += 64) s += p[k];
+    return wave_sum(s);
+}
+
+// Energy of evaluation e: its partials (LJ+intra blocks, real-space tiles,
+// k-space blocks) minus the self term.  Called by a whole wavefront.
+__device__ __forceinline__ double eval_energy(const double *__restrict__ epart, int e, int ep_stride,
+                                              double eself)
+{
+    return wave_sum_array(epart + (long)e * ep_stride, ep_stride) - eself;
+}
+
+// erfc(x) = exp(-x^2) erfcx(x); erfcx from the piecewise table (qtp_erfcx_table.h)
+// held in LDS.  ex = exp(-x^2) is passed in (the caller needs it anyway).
+__device__ __forceinline__ double erfc_from_table(double x, double ex, const double *tab)
+{
+    if (x >= QTP_ERFCX_NINT * QTP_ERFCX_H) return erfc(x);
+    const int i = (int)(x * QTP_ERFCX_INVH);
+    const double t = x - (i + 0.5) * QTP_ERFCX_H;
+    const double *c = tab + i * QTP_ERFCX_NCOEF;
+    double g = c[QTP_ERFCX_NCOEF - 1];
+    for (int k = QTP_ERFCX_NCOEF - 2; k >= 0; --k) g = fma(g, t, c[k]);
+    return ex * g;
 }
 
 // Real-space Ewald over all site pairs i<j within r_c (forces.cpp:586-608,
 // rwald :141-177).  One wavefront per 64x64 tile (I<=J) of the pair matrix.
-// Lane l owns site I*64+l; at step s it meets site J*64+((l+s)&63), so the 64
-// lanes touch 64 distinct j and the reaction-force accumulator of each j is
-// handed lane to lane with the j it belongs to (deterministic, no atomics).
-// Partial site forces go to slot J (for block I's sites) and slot I (for
-// block J's sites): every (site, slot) pair has exactly one writer.
-__global__ __launch_bounds__(64) void k_realspace(const double *__restrict__ cpos,
+// Lane l owns site I*64+l and meets site J*64+((l+s)&63) at step s.  Pass 1
+// records which steps fall inside r_c (a 64-bit mask per lane); pass 2 pops
+// the set bits, so lanes evaluate only in-cutoff pairs.  Reaction forces on j
+// accumulate in LDS with ds_add_f64 from this single wavefront (a fixed order
+// for given inputs: bit-reproducible).  Partial site forces go to slot J (for
+// block I's sites) and slot I (for block J's sites): every (site, slot) has
+// exactly one writer.   grid: E * ntile, 64 threads.
+__global__ __launch_bounds__(64) void k_realspace(const double *__restrict__ pos, long pstride,
                                                   const double *__restrict__ charge, int n_atoms,
-                                                  int NAP, int nblk, const int2 *__restrict__ tiles,
-                                                  int ntile, Box box, Ewald ew,
+                                                  int NAP, int nblk, double gamma, double half1mg,
+                                                  const int2 *__restrict__ tiles, int ntile,
+                                                  Box box, Ewald ew,
+                                                  const double *__restrict__ etab,
                                                   double *__restrict__ rpart,
                                                   double *__restrict__ epart, int ep_stride,
                                                   int ep_off)
@@ -156,89 +183,119 @@ __global__ __launch_bounds__(64) void k_realspace(const double *__restrict__ cpo
     const int e = blockIdx.x / ntile, t = blockIdx.x - e * ntile;
     const int I = tiles[t].x, J = tiles[t].y;
     const int l = threadIdx.x;
-    const double *X = cpos + (long)e * 3 * NAP;
-    __shared__ double sx[64], sy[64], sz[64], sq[64], sacc[3][64];
+    const double *X = pos + (long)e * pstride;
+    __shared__ double sx[64], sy[64], sz[64], sq[64], ax[64], ay[64], az[64];
+    __shared__ double tab[QTP_ERFCX_NINT * QTP_ERFCX_NCOEF];
+    for (int k = l; k < QTP_ERFCX_NINT * QTP_ERFCX_NCOEF; k += 64) tab[k] = etab[k];
     const int i = I * 64 + l, jj = J * 64 + l;
     const bool vi = i < n_atoms;
-    const double xi = X[i], yi = X[NAP + i], zi = X[2 * NAP + i];
-    const double qi = vi ? charge[i] : 0.0;
-    sx[l] = X[jj];
-    sy[l] = X[NAP + jj];
-    sz[l] = X[2 * NAP + jj];
-    sq[l] = jj < n_atoms ? charge[jj] : 0.0;
+    double xi = 0.0, yi = 0.0, zi = 0.0, qi = 0.0;
+    if (vi) {
+        xi = site_coord(X, NAP, i, 0, gamma, half1mg);
+        yi = site_coord(X, NAP, i, 1, gamma, half1mg);
+        zi = site_coord(X, NAP, i, 2, gamma, half1mg);
+        qi = charge[i];
+    }
+    const bool vj = jj < n_atoms;
+    sx[l] = vj ? site_coord(X, NAP, jj, 0, gamma, half1mg) : 0.0;
+    sy[l] = vj ? site_coord(X, NAP, jj, 1, gamma, half1mg) : 0.0;
+    sz[l] = vj ? site_coord(X, NAP, jj, 2, gamma, half1mg) : 0.0;
+    sq[l] = vj ? charge[jj] : 0.0;
+    ax[l] = 0.0;
+    ay[l] = 0.0;
+    az[l] = 0.0;
     __syncthreads();
     const bool diag = (I == J);
     const int s0 = diag ? 1 : 0, s1 = diag ? 33 : 64;
-    const int mi = i / 3;
-    double fx = 0.0, fy = 0.0, fz = 0.0, ax = 0.0, ay = 0.0, az = 0.0, en = 0.0;
-    for (int s = s0; s < s1; ++s) {
-        const int jl = (l + s) & 63;
-        const int j = J * 64 + jl;
-        const bool act = vi && (j < n_atoms) && !(diag && s == 32 && l >= 32);
-        double dx = mimg_fast(xi - sx[jl], box.L[0], box.inv[0]);
-        double dy = mimg_fast(yi - sy[jl], box.L[1], box.inv[1]);
-        double dz = mimg_fast(zi - sz[jl], box.L[2], box.inv[2]);
-        const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-        if (act && rsq < ew.rcut_sq) {
-            const double r = sqrt(rsq);
-            const double rinv = 1.0 / r;
-            const double qq = qi * sq[jl];
-            double ec = erfc(ew.kappa * r);
-            if (mi == j / 3) ec -= 1.0;  // same molecule: -erf (forces.cpp:158-166)
-            const double ep = qq * rinv * ec;
-            const double pf = (ep + qq * ew.pref2k * exp(-ew.kappa2 * rsq)) * (rinv * rinv);
-            en += ep;
-            fx = fma(pf, dx, fx);
-            fy = fma(pf, dy, fy);
-            fz = fma(pf, dz, fz);
-            ax = fma(-pf, dx, ax);
-            ay = fma(-pf, dy, ay);
-            az = fma(-pf, dz, az);
+    // pass 1: in-cutoff mask
+    unsigned long long mask = 0ull;
+    if (vi) {
+        for (int s = s0; s < s1; ++s) {
+            const int jl = (l + s) & 63;
+            if (J * 64 + jl >= n_atoms || (diag && s == 32 && l >= 32)) continue;
+            const double dx = mimg_fast(xi - sx[jl], box.L[0], box.inv[0]);
+            const double dy = mimg_fast(yi - sy[jl], box.L[1], box.inv[1]);
+            const double dz = mimg_fast(zi - sz[jl], box.L[2], box.inv[2]);
+            const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (rsq < ew.rcut_sq) mask |= 1ull << s;
         }
-        const int src = (l + 1) & 63;
-        ax = __shfl(ax, src, 64);
-        ay = __shfl(ay, src, 64);
-        az = __shfl(az, src, 64);
     }
-    // lane l now holds the accumulator of j-local (l + s1) & 63
-    const int jl_end = (l + s1) & 63;
+    // pass 2: in-cutoff pairs only
+    const int mi = i / 3;
+    double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
+    while (mask) {
+        const int s = __ffsll((unsigned long long)mask) - 1;
+        mask &= mask - 1ull;
+        const int jl = (l + s) & 63;
+        const double dx = mimg_fast(xi - sx[jl], box.L[0], box.inv[0]);
+        const double dy = mimg_fast(yi - sy[jl], box.L[1], box.inv[1]);
+        const double dz = mimg_fast(zi - sz[jl], box.L[2], box.inv[2]);
+        const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double rinv = rsqrt(rsq);
+        const double ex = exp(-ew.kappa2 * rsq);
+        double ec = erfc_from_table(ew.kappa * (rsq * rinv), ex, tab);
+        if (mi == (J * 64 + jl) / 3) ec -= 1.0;  // same molecule: -erf (forces.cpp:158-166)
+        const double qq = qi * sq[jl];
+        const double ep = qq * rinv * ec;
+        const double pf = (ep + qq * ew.pref2k * ex) * (rinv * rinv);
+        en += ep;
+        fx = fma(pf, dx, fx);
+        fy = fma(pf, dy, fy);
+        fz = fma(pf, dz, fz);
+        atomicAdd(&ax[jl], -pf * dx);
+        atomicAdd(&ay[jl], -pf * dy);
+        atomicAdd(&az[jl], -pf * dz);
+    }
+    __syncthreads();
     const long base_e = (long)e * nblk * 3 * NAP;
     if (diag) {
-        sacc[0][jl_end] = ax;
-        sacc[1][jl_end] = ay;
-        sacc[2][jl_end] = az;
-        __syncthreads();
         double *o = rpart + base_e + (long)I * 3 * NAP;
-        o[i] = fx + sacc[0][l];
-        o[NAP + i] = fy + sacc[1][l];
-        o[2 * NAP + i] = fz + sacc[2][l];
+        o[i] = fx + ax[l];
+        o[NAP + i] = fy + ay[l];
+        o[2 * NAP + i] = fz + az[l];
     } else {
         double *oI = rpart + base_e + (long)J * 3 * NAP;  // slot J, sites of block I
         oI[i] = fx;
         oI[NAP + i] = fy;
         oI[2 * NAP + i] = fz;
         double *oJ = rpart + base_e + (long)I * 3 * NAP;  // slot I, sites of block J
-        const int j = J * 64 + jl_end;
-        oJ[j] = ax;
-        oJ[NAP + j] = ay;
-        oJ[2 * NAP + j] = az;
+        oJ[jj] = ax[l];
+        oJ[NAP + jj] = ay[l];
+        oJ[2 * NAP + jj] = az[l];
     }
     en = wave_sum(en);
     if (l == 0) epart[(long)e * ep_stride + ep_off + t] = en;
 }
 
+// Per-site factor table e^{i m b_d r_d}, m = 0..K-1, by the cos/sin recurrence
+// of forces.cpp:212-245 (charge folded into the x factor, :241-244).
+template <int STRIDE>
+__device__ __forceinline__ void kw_table(double2 *row, double r, double bk, double qf, int K)
+{
+    const double kr = bk * r;
+    double sn, cs;
+    sincos(kr, &sn, &cs);
+    double cm = 1.0, sm = 0.0;
+    row[0] = make_double2(cm * qf, sm * qf);
+    for (int m = 1; m < K; ++m) {
+        const double cn = cs * cm - sn * sm;
+        const double snn = sn * cm + cs * sm;
+        cm = cn;
+        sm = snn;
+        row[m * STRIDE] = make_double2(cm * qf, sm * qf);
+    }
+}
+
 // Reciprocal-space structure factors (kwald, forces.cpp:180-448), partial
 // over a chunk of AC sites.  A k-"group" (kx>=0, ky>=0, kz>=0) carries the
 // four sign variants (kx,+-ky,+-kz) of the reference's branches; lane <-> group.
-// Per site the factors e^{i m b_d r_d} (m = 0..k_max, charge folded into x)
-// come from the same cos/sin recurrence as forces.cpp:212-245 and sit in LDS.
 // Output: 8 sums per group: Re/Im of A Z, A Z*, B Z, B Z* with A = X Y,
 // B = X Y* (X,Y,Z the x,y,z factors).   grid: E * ngc * nac.
-__global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ cpos,
+__global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, long pstride,
                                               const double *__restrict__ charge, int n_atoms,
-                                              int NAP, Ewald ew, const int4 *__restrict__ groups,
-                                              int G, int AC, int ngc, int nac,
-                                              double *__restrict__ spart)
+                                              int NAP, double gamma, double half1mg, Ewald ew,
+                                              const int4 *__restrict__ groups, int G, int AC,
+                                              int ngc, int nac, double *__restrict__ spart)
 {
     extern __shared__ double2 T[];  // [AC][3][K]
     const int per_e = ngc * nac;
@@ -246,25 +303,14 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ cpos,
     const int r = blockIdx.x - e * per_e;
     const int gc = r / nac, ac = r - gc * nac;
     const int K = ew.K;
-    const double *X = cpos + (long)e * 3 * NAP;
+    const double *X = pos + (long)e * pstride;
     for (int t = threadIdx.x; t < AC * 3; t += blockDim.x) {
         const int a = t / 3, d = t - 3 * a;
         const int atom = ac * AC + a;
         double2 *row = T + (a * 3 + d) * K;
         if (atom < n_atoms) {
-            const double kr = ew.bk[d] * X[d * NAP + atom];
-            double sn, cs;
-            sincos(kr, &sn, &cs);
-            const double qf = (d == 0) ? charge[atom] : 1.0;
-            double cm = 1.0, sm = 0.0;
-            row[0] = make_double2(cm * qf, sm * qf);
-            for (int m = 1; m < K; ++m) {
-                const double cn = cs * cm - sn * sm;
-                const double snn = sn * cm + cs * sm;
-                cm = cn;
-                sm = snn;
-                row[m] = make_double2(cm * qf, sm * qf);
-            }
+            kw_table<1>(row, site_coord(X, NAP, atom, d, gamma, half1mg), ew.bk[d],
+                        d == 0 ? charge[atom] : 1.0, K);
         } else {
             for (int m = 0; m < K; ++m) row[m] = make_double2(0.0, 0.0);
         }
@@ -337,151 +383,162 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
     if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + gb] = en;
 }
 
-// Reciprocal-space forces (forces.cpp:276-279, 319-324, 368-373, 434-442) for
-// a block of 64 sites (lane <-> site), the four waves splitting the groups;
-// then the real-space partial slots are added: ef = total Ewald site force.
-// grid: E * (NAP/64).   LDS: tables [3][K][64] + a chunk of GC groups.
-constexpr int KF_GC = 256;
-__global__ __launch_bounds__(256) void k_kforce(const double *__restrict__ cpos,
+// Reciprocal-space forces (forces.cpp:276-279, 319-324, 368-373, 434-442),
+// plus the real-space partial slots, the M-site chain rule (forces.cpp:613-619)
+// and the LJ+intra forces: the physical force of every site of a block of
+// 21 molecules (63 sites, lane <-> site; lane 63 idle).  The 8 waves split the
+// k-groups; S', k and the group table are read with wave-uniform addresses
+// (scalar loads).   grid: E * ceil(n_atoms/63), 512 threads.
+constexpr int KF_WAVES = 16;
+constexpr int KF_SITES = 63;
+__global__ __launch_bounds__(1024) void k_kforce(const double *__restrict__ pos, long pstride,
                                                 const double *__restrict__ charge, int n_atoms,
-                                                int NAP, Ewald ew, const int4 *__restrict__ groups,
+                                                int NAP, double gamma, double half1mg, Ewald ew,
+                                                const int4 *__restrict__ groups,
                                                 const double4 *__restrict__ grk, int G,
                                                 const double *__restrict__ sprime,
                                                 const double *__restrict__ rpart, int nblk,
-                                                double *__restrict__ ef)
+                                                const double *__restrict__ flj,
+                                                double *__restrict__ fout)
 {
     extern __shared__ double smem[];
     const int K = ew.K;
-    double2 *T = (double2 *)smem;            // [3][K][64]
-    double *SP = smem + 2 * 3 * K * 64;      // [GC][8]
-    double *RK = SP + KF_GC * 8;             // [GC][3]
-    int *GR = (int *)(RK + KF_GC * 3);       // [GC]
-    const int nab = NAP / 64;
+    double2 *T = (double2 *)smem;        // [3][K][64]
+    double *red = smem + 2 * 3 * K * 64;  // [KF_WAVES][3][64]
+    const int nab = (n_atoms + KF_SITES - 1) / KF_SITES;
     const int e = blockIdx.x / nab, ab = blockIdx.x - e * nab;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int atom = ab * 64 + lane;
-    const double *X = cpos + (long)e * 3 * NAP;
+    const int lane = threadIdx.x & 63;
+    // wave index made explicitly uniform: the group loop below then runs on
+    // scalar control with scalar (SMEM) loads of the group data
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int atom = ab * KF_SITES + lane;
+    const bool va = lane < KF_SITES && atom < n_atoms;
+    const double *X = pos + (long)e * pstride;
     if (threadIdx.x < 192) {
         const int d = threadIdx.x >> 6;
-        if (atom < n_atoms) {
-            const double kr = ew.bk[d] * X[d * NAP + atom];
-            double sn, cs;
-            sincos(kr, &sn, &cs);
-            const double qf = (d == 0) ? charge[atom] : 1.0;
-            double cm = 1.0, sm = 0.0;
-            T[(d * K + 0) * 64 + lane] = make_double2(cm * qf, sm * qf);
-            for (int m = 1; m < K; ++m) {
-                const double cn = cs * cm - sn * sm;
-                const double snn = sn * cm + cs * sm;
-                cm = cn;
-                sm = snn;
-                T[(d * K + m) * 64 + lane] = make_double2(cm * qf, sm * qf);
-            }
-        } else {
+        if (va)
+            kw_table<64>(T + d * K * 64 + lane, site_coord(X, NAP, atom, d, gamma, half1mg),
+                         ew.bk[d], d == 0 ? charge[atom] : 1.0, K);
+        else
             for (int m = 0; m < K; ++m) T[(d * K + m) * 64 + lane] = make_double2(0.0, 0.0);
-        }
-    }
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int g0 = 0; g0 < G; g0 += KF_GC) {
-        const int gn = min(KF_GC, G - g0);
-        __syncthreads();
-        for (int t = threadIdx.x; t < gn * 8; t += 256) SP[t] = sprime[((long)e * G + g0) * 8 + t];
-        for (int t = threadIdx.x; t < gn; t += 256) {
-            const double4 rk = grk[g0 + t];
-            RK[t * 3 + 0] = rk.x;
-            RK[t * 3 + 1] = rk.y;
-            RK[t * 3 + 2] = rk.z;
-            const int4 gr = groups[g0 + t];
-            GR[t] = gr.x | (gr.y << 8) | (gr.z << 16);
-        }
-        __syncthreads();
-        const int qn = (gn + 3) >> 2;
-        const int gb = w * qn, ge = min(gn, gb + qn);
-        int pkxy = -1;
-        double Ar = 0.0, Ai = 0.0, Br = 0.0, Bi = 0.0;
-        for (int g = gb; g < ge; ++g) {
-            const int pk = GR[g];
-            const int kxy = pk & 0xFFFF, kz = pk >> 16;
-            if (kxy != pkxy) {
-                const double2 Xv = T[(0 * K + (pk & 255)) * 64 + lane];
-                const double2 Yv = T[(1 * K + ((pk >> 8) & 255)) * 64 + lane];
-                const double cc = Xv.x * Yv.x, csn = Xv.x * Yv.y, sc = Xv.y * Yv.x,
-                             ss = Xv.y * Yv.y;
-                Ar = cc - ss;
-                Ai = csn + sc;
-                Br = cc + ss;
-                Bi = sc - csn;
-                pkxy = kxy;
-            }
-            const double2 Zv = T[(2 * K + kz) * 64 + lane];
-            const double *S = SP + g * 8;
-            const double e0r = fma(Ar, Zv.x, -Ai * Zv.y), e0i = fma(Ai, Zv.x, Ar * Zv.y);
-            const double e1r = fma(Ar, Zv.x, Ai * Zv.y), e1i = fma(Ai, Zv.x, -Ar * Zv.y);
-            const double e2r = fma(Br, Zv.x, -Bi * Zv.y), e2i = fma(Bi, Zv.x, Br * Zv.y);
-            const double e3r = fma(Br, Zv.x, Bi * Zv.y), e3i = fma(Bi, Zv.x, -Br * Zv.y);
-            const double t0 = fma(S[1], e0r, -S[0] * e0i);
-            const double t1 = fma(S[3], e1r, -S[2] * e1i);
-            const double t2 = fma(S[5], e2r, -S[4] * e2i);
-            const double t3 = fma(S[7], e3r, -S[6] * e3i);
-            const double p01 = t0 + t1, p23 = t2 + t3, m01 = t0 - t1, m23 = t2 - t3;
-            fx = fma(-RK[g * 3 + 0], p01 + p23, fx);
-            fy = fma(-RK[g * 3 + 1], p01 - p23, fy);
-            fz = fma(-RK[g * 3 + 2], m01 + m23, fz);
-        }
     }
     __syncthreads();
-    double *red = SP;  // [4][3][64]
+    const int per = (G + KF_WAVES - 1) / KF_WAVES;
+    const int g0 = w * per, g1 = min(G, g0 + per);
+    const double4 *S4 = (const double4 *)(sprime + (long)e * G * 8);
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    int pkxy = -1;
+    double Ar = 0.0, Ai = 0.0, Br = 0.0, Bi = 0.0;
+    int4 gr = groups[min(g0, G - 1)];
+    for (int g = g0; g < g1; ++g) {
+        // one batch of scalar loads per group; the next group's indices prefetched
+        const double4 sa = S4[2 * g], sb = S4[2 * g + 1];
+        const double4 rk = grk[g];
+        const int4 grn = groups[min(g + 1, G - 1)];
+        const int kxy = gr.x | (gr.y << 8);
+        if (kxy != pkxy) {
+            const double2 Xv = T[(0 * K + gr.x) * 64 + lane];
+            const double2 Yv = T[(1 * K + gr.y) * 64 + lane];
+            const double cc = Xv.x * Yv.x, csn = Xv.x * Yv.y, sc = Xv.y * Yv.x, ss = Xv.y * Yv.y;
+            Ar = cc - ss;
+            Ai = csn + sc;
+            Br = cc + ss;
+            Bi = sc - csn;
+            pkxy = kxy;
+        }
+        const double2 Zv = T[(2 * K + gr.z) * 64 + lane];
+        // variant 0: A Z ; 1: A Z* ; 2: B Z ; 3: B Z*  (gr.w: variants the reference sums)
+        const double e0r = fma(Ar, Zv.x, -Ai * Zv.y), e0i = fma(Ai, Zv.x, Ar * Zv.y);
+        const double t0 = fma(sa.y, e0r, -sa.x * e0i);
+        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
+        if (gr.w & 2) {
+            const double e1r = fma(Ar, Zv.x, Ai * Zv.y), e1i = fma(Ai, Zv.x, -Ar * Zv.y);
+            t1 = fma(sa.w, e1r, -sa.z * e1i);
+        }
+        if (gr.w & 4) {
+            const double e2r = fma(Br, Zv.x, -Bi * Zv.y), e2i = fma(Bi, Zv.x, Br * Zv.y);
+            t2 = fma(sb.y, e2r, -sb.x * e2i);
+        }
+        if (gr.w & 8) {
+            const double e3r = fma(Br, Zv.x, Bi * Zv.y), e3i = fma(Bi, Zv.x, -Br * Zv.y);
+            t3 = fma(sb.w, e3r, -sb.z * e3i);
+        }
+        const double p01 = t0 + t1, p23 = t2 + t3, m01 = t0 - t1, m23 = t2 - t3;
+        fx = fma(-rk.x, p01 + p23, fx);
+        fy = fma(-rk.y, p01 - p23, fy);
+        fz = fma(-rk.z, m01 + m23, fz);
+        gr = grn;
+    }
     red[(w * 3 + 0) * 64 + lane] = fx;
     red[(w * 3 + 1) * 64 + lane] = fy;
     red[(w * 3 + 2) * 64 + lane] = fz;
     __syncthreads();
-    if (w == 0) {
+    // wave 0: total Ewald site force, then the chain rule within the block
+    double efd[3] = {0.0, 0.0, 0.0};
+    if (w == 0 && va) {
         for (int d = 0; d < 3; ++d) {
-            double v = red[(0 * 3 + d) * 64 + lane] + red[(1 * 3 + d) * 64 + lane] +
-                       red[(2 * 3 + d) * 64 + lane] + red[(3 * 3 + d) * 64 + lane];
+            double v = 0.0;
+            for (int ww = 0; ww < KF_WAVES; ++ww) v += red[(ww * 3 + d) * 64 + lane];
             const double *rp = rpart + (long)e * nblk * 3 * NAP + d * NAP + atom;
-            for (int s = 0; s < nblk; ++s) v += rp[(long)s * 3 * NAP];
-            ef[((long)e * 3 + d) * NAP + atom] = v;
+            for (int sl = 0; sl < nblk; ++sl) v += rp[(long)sl * 3 * NAP];
+            efd[d] = v;
+        }
+    }
+    __syncthreads();
+    double *ef = red;  // [3][64]
+    if (w == 0) {
+        for (int d = 0; d < 3; ++d) ef[d * 64 + lane] = efd[d];
+    }
+    __syncthreads();
+    if (w == 0 && va) {
+        const int site = lane % 3;  // 63 = 21 molecules: lane%3 == atom%3
+        const int lo = lane - site;
+        for (int d = 0; d < 3; ++d) {
+            const long idx = ((long)e * 3 + d) * NAP + atom;
+            double v;
+            if (site == 0)
+                v = flj[idx] + gamma * ef[d * 64 + lane];
+            else
+                v = flj[idx] + (ef[d * 64 + lane] + half1mg * ef[d * 64 + lo]);
+            fout[idx] = v;
         }
     }
 }
 
 // LJ over the full O-O list (forces.cpp:489-515, lj_oxygen :25-59) and the
-// intramolecular terms (forces.cpp:519-554, :65-138); lane <-> molecule.
-// Each pair is met from both sides; its energy is halved.
-// grid: E * ceil(n_mols/64).
-__global__ __launch_bounds__(64) void k_ljintra(const double *__restrict__ pos, long pstride,
-                                                int n_mols, int NAP,
-                                                const int *__restrict__ fpoint,
-                                                const int *__restrict__ flist, long fcap,
-                                                int lmode, int lbase, Box box, FF ff,
-                                                double *__restrict__ flj,
-                                                double *__restrict__ epart, int ep_stride,
-                                                int ep_off)
+// intramolecular terms (forces.cpp:519-554, :65-138).  16 lanes per molecule
+// share its list row; lane 0 of the group adds the intramolecular terms.
+// Each LJ pair is met from both sides; its energy is halved.
+// grid: E * ceil(n_mols/16), 256 threads.
+constexpr int LJ_LPM = 16;
+__global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos, long pstride,
+                                                 int n_mols, int NAP,
+                                                 const int *__restrict__ fpoint,
+                                                 const int *__restrict__ flist, long fcap,
+                                                 int lmode, int lbase, Box box, FF ff,
+                                                 double *__restrict__ flj,
+                                                 double *__restrict__ epart, int ep_stride,
+                                                 int ep_off)
 {
-    const int nmb = (n_mols + 63) / 64;
+    __shared__ double sh[16];
+    constexpr int MPB = 256 / LJ_LPM;
+    const int nmb = (n_mols + MPB - 1) / MPB;
     const int e = blockIdx.x / nmb, mb = blockIdx.x - e * nmb;
-    const int i = mb * 64 + threadIdx.x;
+    const int gi = threadIdx.x / LJ_LPM, t = threadIdx.x % LJ_LPM;
+    const int i = mb * MPB + gi;
     const double *P = pos + (long)e * pstride;
-    double en = 0.0;
+    double fo[3] = {0.0, 0.0, 0.0}, elj = 0.0, en = 0.0;
+    double o[3] = {0.0, 0.0, 0.0};
+    const int a = 3 * i;
     if (i < n_mols) {
         const int b = lst_bead(lmode, lbase, e);
-        const int a = 3 * i;
-        double o[3], h1[3], h2[3];
-        for (int d = 0; d < 3; ++d) {
-            o[d] = P[d * NAP + a];
-            h1[d] = P[d * NAP + a + 1];
-            h2[d] = P[d * NAP + a + 2];
-        }
-        double fo[3] = {0.0, 0.0, 0.0}, fh1[3] = {0.0, 0.0, 0.0}, fh2[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < 3; ++d) o[d] = P[d * NAP + a];
         const int *fp = fpoint + (long)b * (n_mols + 1);
         const int *fl = flist + (long)b * fcap;
-        // an overflowed list (status flag set by k_nl_scan) is never read past its capacity
+        // an overflowed list (status flag set by the list kernels) is never read past capacity
         const int k0 = (int)min((long)fp[i], fcap), k1 = (int)min((long)fp[i + 1], fcap);
-        double elj = 0.0;
-        for (int k = k0; k < k1; ++k) {
-            const int j = fl[k];
-            const int aj = 3 * j;
+        for (int k = k0 + t; k < k1; k += LJ_LPM) {
+            const int aj = 3 * fl[k];
             double r[3];
             for (int d = 0; d < 3; ++d)
                 r[d] = mimg_fast(o[d] - P[d * NAP + aj], box.L[d], box.inv[d]);
@@ -497,6 +554,18 @@ __global__ __launch_bounds__(64) void k_ljintra(const double *__restrict__ pos, 
                 for (int d = 0; d < 3; ++d) fo[d] = fma(pf, r[d], fo[d]);
             }
         }
+    }
+    for (int off = LJ_LPM / 2; off > 0; off >>= 1) {
+        for (int d = 0; d < 3; ++d) fo[d] += __shfl_xor(fo[d], off, 64);
+        elj += __shfl_xor(elj, off, 64);
+    }
+    if (i < n_mols && t == 0) {
+        double h1[3], h2[3];
+        for (int d = 0; d < 3; ++d) {
+            h1[d] = P[d * NAP + a + 1];
+            h2[d] = P[d * NAP + a + 2];
+        }
+        double fh1[3] = {0.0, 0.0, 0.0}, fh2[3] = {0.0, 0.0, 0.0};
         en = 0.5 * elj;
         // intramolecular (forces.cpp:519-554)
         double r1[3], r2[3];
@@ -545,42 +614,18 @@ __global__ __launch_bounds__(64) void k_ljintra(const double *__restrict__ pos, 
             O[d * NAP + a + 2] = fh2[d];
         }
     }
-    en = wave_sum(en);
+    en = block_sum(en, sh);
     if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + mb] = en;
 }
 
-// Physical forces: LJ + intra + chain rule of the M-site Ewald force
-// (forces.cpp:613-619); energy = ordered sum of the partials - self term
-// (forces.cpp:622-624).   grid: E, 256 threads.
-__global__ __launch_bounds__(256) void k_finalize(const double *__restrict__ flj,
-                                                  const double *__restrict__ ef, int n_atoms,
-                                                  int NAP, double gamma, double half1mg,
-                                                  const double *__restrict__ epart, int ep_stride,
-                                                  double eself, double *__restrict__ fout,
-                                                  double *__restrict__ eout)
+// Energies of E evaluations (ordered sum of the partials minus the self term).
+// grid: 1 block of 64 threads per 64 evaluations.
+__global__ __launch_bounds__(64) void k_esum(int E, const double *__restrict__ epart, int ep_stride,
+                                             double eself, double *__restrict__ eout)
 {
     const int e = blockIdx.x;
-    const double *L = flj + (long)e * 3 * NAP;
-    const double *W = ef + (long)e * 3 * NAP;
-    double *O = fout + (long)e * 3 * NAP;
-    for (int t = threadIdx.x; t < 3 * NAP; t += 256) {
-        const int d = t / NAP, a = t - d * NAP;
-        double v = 0.0;
-        if (a < n_atoms) {
-            const int ao = a - a % 3;
-            if (a == ao)
-                v = L[t] + gamma * W[t];
-            else
-                v = L[t] + (W[t] + half1mg * W[d * NAP + ao]);
-        }
-        O[t] = v;
-    }
-    if (threadIdx.x == 0) {
-        double s = 0.0;
-        const double *p = epart + (long)e * ep_stride;
-        for (int k = 0; k < ep_stride; ++k) s += p[k];
-        eout[e] = s - eself;
-    }
+    const double v = eval_energy(epart, e, ep_stride, eself);
+    if (threadIdx.x == 0) eout[e] = v;
 }
 
 // ---------------------------------------------------------------------------
@@ -597,16 +642,26 @@ struct SCArgs {
 //          positions q +- eps f/m for the finite-difference phase.
 __global__ __launch_bounds__(256) void k_sc1(SCArgs a, const double *__restrict__ q,
                                              const double *__restrict__ fph,
-                                             const double *__restrict__ Vb,
+                                             const double *__restrict__ epart, int ep_stride,
+                                             double eself, double *__restrict__ Vout,
                                              const double *__restrict__ m_atom,
                                              double *__restrict__ Fsc, double *__restrict__ qpm,
                                              double *__restrict__ scal /*[P][4]*/)
 {
     __shared__ double sh[16];
     __shared__ double s_scale;
+    __shared__ double s_V;
     const int b = blockIdx.x;
     const long o = (long)b * 3 * a.NAP;
-    const double V = Vb[b];
+    if (threadIdx.x < 64) {
+        const double v = eval_energy(epart, b, ep_stride, eself);
+        if (threadIdx.x == 0) {
+            s_V = v;
+            Vout[b] = v;
+        }
+    }
+    __syncthreads();
+    const double V = s_V;
     if ((b & 1) == 0) {
         for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) Fsc[o + t] = 2.0 * fph[o + t] / 3.0;
         if (threadIdx.x == 0) {
@@ -709,213 +764,167 @@ struct PropArgs {
     uint64_t seed;
 };
 
-// Per-mode constants: [k] = {scale_k, wsq_k, c1_k, c2_k}
+// Per-mode constants: mode[k] = {scale_k, wsq_k, c1_k, c2_k}
 // (update.cpp:302-304, thermostat.cpp:119-121)
 
-// Kick + forward normal-mode transform + first sqrt-Cayley half step.
-// A tile = 64 consecutive planar dofs for all P beads, staged in LDS.
-// grid: ntile, 256 threads.  LDS: C[P][P], Xq[P][64], Xp[P][64].
-__global__ __launch_bounds__(256) void k_prop_a(PropArgs A, const double *__restrict__ Cm,
-                                                const double4 *__restrict__ mode,
-                                                const double *__restrict__ q,
-                                                const double *__restrict__ p,
-                                                const double *__restrict__ F,
-                                                const double *__restrict__ m_atom,
-                                                double *__restrict__ nq, double *__restrict__ np,
-                                                double *__restrict__ part /*[2][ntile]*/,
-                                                const unsigned long long *__restrict__ step_ctr)
+// sqrt-Cayley half step of one normal mode (update.cpp:307-317)
+__device__ __forceinline__ void cayley_half(double &uq, double &up, double m, double halfdt,
+                                            double4 md)
 {
-    extern __shared__ double sm[];
-    const int P = A.P;
-    double *Cs = sm;
-    double *Xq = Cs + P * P;
-    double *Xp = Xq + P * 64;
-    __shared__ double sh[16];
-    const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
-    const int N3 = 3 * A.NAP;
-    const int f = blockIdx.x * 64 + lane;
-    const bool vf = f < N3;
-    for (int t = threadIdx.x; t < P * P; t += 256) Cs[t] = Cm[t];
-    for (int b = kg; b < P; b += 4) {
-        double vq = 0.0, vp = 0.0;
-        if (vf) {
-            vq = q[(long)b * N3 + f];
-            vp = p[(long)b * N3 + f] + A.halfdt * F[(long)b * N3 + f];
-        }
-        Xq[b * 64 + lane] = vq;
-        Xp[b * 64 + lane] = vp;
-    }
-    __syncthreads();
-    const int at = vf ? f % A.NAP : 0;
-    const bool va = vf && at < A.n_atoms;
-    const double m = va ? m_atom[at] : 1.0;
-    double ke0 = 0.0, chi = 0.0;
-    for (int k = kg; k < P; k += 4) {
-        double uq = 0.0, up = 0.0;
-        const double *cr = Cs + k * P;
-        for (int b = 0; b < P; ++b) {
-            uq = fma(cr[b], Xq[b * 64 + lane], uq);
-            up = fma(cr[b], Xp[b * 64 + lane], up);
-        }
-        const double4 md = mode[k];
-        double temp = A.halfdt * up / m + uq;
-        temp *= md.x;
-        up = up - A.halfdt * md.y * uq * m;
-        up *= md.x;
-        uq = temp;
-        if (vf) {
-            nq[(long)k * N3 + f] = uq;
-            np[(long)k * N3 + f] = up;
-        }
-        if (k == 0 && va) ke0 += 0.5 * up * up / m;
-    }
-    if (A.nvt && !A.host_dev && kg == 0 && va) {
-        // centroid chi-square: normals for reference dof index 1..n_dof-1
-        const int d = f / A.NAP;
-        const int ridx = 3 * at + d;
-        if (ridx >= 1) {
-            const double z = philox_normal(A.seed, *step_ctr, (uint32_t)(P + 1), (uint32_t)ridx);
-            chi += z * z;
-        }
-    }
-    ke0 = block_sum(ke0, sh);
-    chi = block_sum(chi, sh);
-    if (threadIdx.x == 0) {
-        part[blockIdx.x] = ke0;
-        part[A.ntile + blockIdx.x] = chi;
-    }
+    double temp = halfdt * up / m + uq;
+    temp *= md.x;
+    up = up - halfdt * md.y * uq * m;
+    up *= md.x;
+    uq = temp;
 }
 
-// PILE-G in normal-mode space, second sqrt-Cayley half step, inverse
-// transform, displacement tracking.   grid: ntile, 256 threads.
-// heat partials: hpart[tile] = sum over modes of KE_before - KE_after,
-// excluding the centroid's KE_before (added once from part[]).
-__global__ __launch_bounds__(256) void k_prop_b(PropArgs A, const double *__restrict__ Cm,
-                                                const double4 *__restrict__ mode,
-                                                const double *__restrict__ nq,
-                                                const double *__restrict__ np,
-                                                const double *__restrict__ m_atom,
-                                                const double *__restrict__ sqm_atom,
-                                                const double *__restrict__ part,
-                                                const double *__restrict__ dev,
-                                                double *__restrict__ q, double *__restrict__ p,
-                                                double *__restrict__ disp, int NMP,
-                                                double *__restrict__ hpart,
-                                                const unsigned long long *__restrict__ step_ctr)
+// Kick, forward normal-mode transform and first half step for mode k of 64
+// planar dofs; modes k >= 1 also get their Langevin step and second half
+// step here (they do not wait for the centroid).  One wavefront per
+// (dof tile, mode); the transform rows are wave-uniform (scalar loads).
+// grid: (ntile, P), 64 threads.
+// part[tile] = centroid KE after the first half step (tile partial);
+// part[ntile + tile] = chi-square partial (device random numbers);
+// hpart[k * ntile + tile] = KE before - KE after of mode k's Langevin step.
+__global__ __launch_bounds__(64) void k_prop_fwd(PropArgs A, const double *__restrict__ Cm,
+                                                 const double4 *__restrict__ mode,
+                                                 const double *__restrict__ q,
+                                                 const double *__restrict__ p,
+                                                 const double *__restrict__ F,
+                                                 const double *__restrict__ m_atom,
+                                                 const double *__restrict__ sqm_atom,
+                                                 const double *__restrict__ dev,
+                                                 double *__restrict__ nq, double *__restrict__ np,
+                                                 double *__restrict__ part,
+                                                 double *__restrict__ hpart,
+                                                 const unsigned long long *__restrict__ step_ctr)
 {
-    extern __shared__ double sm[];
-    const int P = A.P;
-    double *Cs = sm;
-    double *Uq = Cs + P * P;
-    double *Up = Uq + P * 64;
-    __shared__ double sh[16];
-    __shared__ double s_alpha;
-    const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
-    const int N3 = 3 * A.NAP;
-    const int f = blockIdx.x * 64 + lane;
-    const bool vf = f < N3;
-    const int at = vf ? f % A.NAP : 0;
-    const int d = vf ? f / A.NAP : 0;
-    const bool va = vf && at < A.n_atoms;
+    const int P = A.P, k = blockIdx.y, tile = blockIdx.x, lane = threadIdx.x;
+    const long N3 = 3L * A.NAP;
+    const int f = tile * 64 + lane;
+    const int at = f % A.NAP, d = f / A.NAP;
+    const bool va = at < A.n_atoms;
     const double m = va ? m_atom[at] : 1.0;
-    const double sqm = va ? sqm_atom[at] : 0.0;
-    const uint64_t step = step_ctr ? *step_ctr : 0ull;
-    for (int t = threadIdx.x; t < P * P; t += 256) Cs[t] = Cm[t];
-    if (A.nvt && threadIdx.x == 0) {
-        double ke = 0.0, chi = 0.0;
-        for (int t = 0; t < A.ntile; ++t) ke += part[t];
+    const double *cr = Cm + (long)k * P;
+    double uq = 0.0, up = 0.0;
+#pragma unroll 8
+    for (int b = 0; b < P; ++b) {
+        const double c = cr[b];
+        const long idx = b * N3 + f;
+        const double pk = p[idx] + A.halfdt * F[idx];  // update.cpp:281-287
+        uq = fma(c, q[idx], uq);
+        up = fma(c, pk, up);
+    }
+    const double4 md = mode[k];
+    cayley_half(uq, up, m, A.halfdt, md);
+    if (k == 0) {
+        nq[f] = uq;
+        np[f] = up;
+        double ke0 = va ? 0.5 * up * up / m : 0.0;
+        double chi = 0.0;
+        if (A.nvt && !A.host_dev && va) {
+            const int ridx = 3 * at + d;
+            if (ridx >= 1) {
+                const double z = philox_normal(A.seed, *step_ctr, (uint32_t)(P + 1), (uint32_t)ridx);
+                chi = z * z;
+            }
+        }
+        ke0 = wave_sum(ke0);
+        chi = wave_sum(chi);
+        if (lane == 0) {
+            part[tile] = ke0;
+            part[A.ntile + tile] = chi;
+        }
+        return;
+    }
+    if (A.nvt) {
+        double h = 0.0;
+        if (va) {
+            h += 0.5 * up * up / m;
+            const int ridx = 3 * at + d;
+            const double xi = A.host_dev ? dev[3 + (long)(k - 1) * A.n_dof + ridx]
+                                         : philox_normal(A.seed, *step_ctr, (uint32_t)k, (uint32_t)ridx);
+            up = md.z * up + sqm_atom[at] * md.w * xi;
+            h -= 0.5 * up * up / m;
+        }
+        h = wave_sum(h);
+        if (lane == 0) hpart[(long)k * A.ntile + tile] = h;
+    }
+    cayley_half(uq, up, m, A.halfdt, md);
+    nq[k * N3 + f] = uq;
+    np[k * N3 + f] = up;
+}
+
+// Centroid rescale (Bussi, tau -> 0), its second half step, inverse
+// transform to bead b and displacement tracking.  One wavefront per
+// (dof tile, bead).   grid: (ntile, P), 64 threads.
+// hpart[tile] (mode-0 slot) = - centroid KE after the rescale (bead-0 waves).
+__global__ __launch_bounds__(64) void k_prop_inv(PropArgs A, const double *__restrict__ Cm,
+                                                 const double4 *__restrict__ mode,
+                                                 const double *__restrict__ nq,
+                                                 const double *__restrict__ np,
+                                                 const double *__restrict__ m_atom,
+                                                 const double *__restrict__ part,
+                                                 const double *__restrict__ dev,
+                                                 double *__restrict__ q, double *__restrict__ p,
+                                                 double *__restrict__ disp, int NMP,
+                                                 double *__restrict__ hpart,
+                                                 const unsigned long long *__restrict__ step_ctr)
+{
+    const int P = A.P, b = blockIdx.y, tile = blockIdx.x, lane = threadIdx.x;
+    const long N3 = 3L * A.NAP;
+    const int f = tile * 64 + lane;
+    const int at = f % A.NAP, d = f / A.NAP;
+    const bool va = at < A.n_atoms;
+    const double m = va ? m_atom[at] : 1.0;
+    double u0q = nq[f], u0p = np[f];
+    if (A.nvt) {
+        const double ke = wave_sum_array(part, A.ntile);
         double R, g2;
         if (A.host_dev) {
             R = dev[0];
-            if (A.n_dof % 2 == 0)
-                g2 = 2.0 * dev[2] + dev[1] * dev[1];
-            else
-                g2 = 2.0 * dev[2];
+            g2 = (A.n_dof % 2 == 0) ? 2.0 * dev[2] + dev[1] * dev[1] : 2.0 * dev[2];
         } else {
-            for (int t = 0; t < A.ntile; ++t) chi += part[A.ntile + t];
-            R = philox_normal(A.seed, step, (uint32_t)P, 0u);
-            g2 = chi;
+            g2 = wave_sum_array(part + A.ntile, A.ntile);
+            R = philox_normal(A.seed, *step_ctr, (uint32_t)P, 0u);
         }
-        const double alpha_sq = (R * R + g2) / (2.0 * A.beta_n * ke);
-        s_alpha = sqrt(alpha_sq);
-    }
-    for (int k = kg; k < P; k += 4) {
-        Uq[k * 64 + lane] = vf ? nq[(long)k * N3 + f] : 0.0;
-        Up[k * 64 + lane] = vf ? np[(long)k * N3 + f] : 0.0;
-    }
-    __syncthreads();
-    double heat = 0.0;
-    if (A.nvt) {
-        const double alpha = s_alpha;
-        const int ridx = 3 * at + d;
-        for (int k = kg; k < P; k += 4) {
-            double up = Up[k * 64 + lane];
-            if (k == 0) {
-                up *= alpha;
-                if (va) heat -= 0.5 * up * up / m;
-            } else {
-                const double4 md = mode[k];
-                if (va) heat += 0.5 * up * up / m;
-                double xi = 0.0;
-                if (va) {
-                    if (A.host_dev)
-                        xi = dev[3 + (long)(k - 1) * A.n_dof + ridx];
-                    else
-                        xi = philox_normal(A.seed, step, (uint32_t)k, (uint32_t)ridx);
-                }
-                up = md.z * up + sqm * md.w * xi;
-                if (va) heat -= 0.5 * up * up / m;
-            }
-            Up[k * 64 + lane] = up;
+        const double alpha = sqrt((R * R + g2) / (2.0 * A.beta_n * ke));
+        u0p *= alpha;
+        if (b == 0) {
+            double h = va ? -0.5 * u0p * u0p / m : 0.0;
+            h = wave_sum(h);
+            if (lane == 0) hpart[tile] = h;
         }
     }
-    // second half step
-    for (int k = kg; k < P; k += 4) {
-        const double4 md = mode[k];
-        double uq = Uq[k * 64 + lane], up = Up[k * 64 + lane];
-        double temp = A.halfdt * up / m + uq;
-        temp *= md.x;
-        up = up - A.halfdt * md.y * uq * m;
-        up *= md.x;
-        Uq[k * 64 + lane] = temp;
-        Up[k * 64 + lane] = up;
+    cayley_half(u0q, u0p, m, A.halfdt, mode[0]);
+    double xq = Cm[b] * u0q, xp = Cm[b] * u0p;
+#pragma unroll 8
+    for (int k = 1; k < P; ++k) {
+        const double c = Cm[(long)k * P + b];
+        xq = fma(c, nq[k * N3 + f], xq);
+        xp = fma(c, np[k * N3 + f], xp);
     }
-    __syncthreads();
-    // inverse transform: x_b = sum_k C[k][b] u_k
-    for (int b = kg; b < P; b += 4) {
-        double xq = 0.0, xp = 0.0;
-        for (int k = 0; k < P; ++k) {
-            const double c = Cs[k * P + b];
-            xq = fma(c, Uq[k * 64 + lane], xq);
-            xp = fma(c, Up[k * 64 + lane], xp);
-        }
-        if (vf) {
-            const long idx = (long)b * N3 + f;
-            const double qold = q[idx];
-            q[idx] = xq;
-            p[idx] = xp;
-            if (va && (at % 3) == 0) disp[((long)b * 3 + d) * NMP + at / 3] += xq - qold;
-        }
-    }
-    if (A.nvt) {
-        heat = block_sum(heat, sh);
-        if (threadIdx.x == 0) hpart[blockIdx.x] = heat;
-    }
+    const long idx = b * N3 + f;
+    const double qold = q[idx];
+    q[idx] = xq;
+    p[idx] = xp;
+    if (va && (at % 3) == 0) disp[((long)b * 3 + d) * NMP + at / 3] += xq - qold;
 }
 
-// Accumulate the thermostat heat of one step: heat -= pile_g's return
-// (update.cpp:325).  Single thread.
-__global__ void k_heat(int ntile, const double *__restrict__ part,
-                       const double *__restrict__ hpart, double *__restrict__ heat_acc,
-                       double *__restrict__ heat_step)
+// The thermostat's heat for one step: pile_g's return value (KE before -
+// KE after, thermostat.cpp:86-129); heat_acc -= it (update.cpp:325).
+__global__ __launch_bounds__(64) void k_heat(int ntile, int nh, const double *__restrict__ part,
+                                             const double *__restrict__ hpart,
+                                             double *__restrict__ heat_acc,
+                                             double *__restrict__ heat_step)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    double ke = 0.0, h = 0.0;
-    for (int t = 0; t < ntile; ++t) ke += part[t];
-    for (int t = 0; t < ntile; ++t) h += hpart[t];
-    const double ret = ke + h;  // pile_g's return value
-    if (heat_step) heat_step[0] = ret;
-    if (heat_acc) heat_acc[0] -= ret;
+    const double ke = wave_sum_array(part, ntile);
+    const double h = wave_sum_array(hpart, nh);
+    if (threadIdx.x == 0) {
+        const double ret = ke + h;
+        if (heat_step) heat_step[0] = ret;
+        if (heat_acc) heat_acc[0] -= ret;
+    }
 }
 
 // Generic orthonormal transform along the bead axis of a[P][len] (fourier.cpp:12-69).
@@ -1192,6 +1201,162 @@ __global__ __launch_bounds__(256) void k_nl_fill(int n_mols, int NAP, const doub
     }
 }
 
+// Rebuild test (update.cpp:356-385) and, when it fires, the bead's Verlet
+// lists (neighbour.cpp:12-62) in one launch: one block of 1024 threads per
+// bead.  Row counts live in LDS (n_mols <= NL_MAXMOL), rows are handled one
+// per wavefront with ballot + prefix-popcount compaction; the distance test
+// is bit-for-bit neighbour.cpp:42-53.   grid: P.
+constexpr int NL_MAXMOL = 4096;
+__global__ __launch_bounds__(1024) void k_nlist_bead(int n_mols, int NAP, int NMP,
+                                                     const double *__restrict__ q, Box box,
+                                                     double r_skin_sq, double skin, int force_all,
+                                                     int list_size, long fcap,
+                                                     double *__restrict__ disp,
+                                                     int *__restrict__ flag,
+                                                     double *__restrict__ nreb,
+                                                     int *__restrict__ point,
+                                                     int *__restrict__ fpoint,
+                                                     int *__restrict__ list,
+                                                     int *__restrict__ flist,
+                                                     int *__restrict__ status)
+{
+    __shared__ double s1[1024], s2[1024];
+    __shared__ int ch[NL_MAXMOL], cf[NL_MAXMOL];
+    __shared__ int s_flag, s_tot[2];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    double *D = disp + (long)b * 3 * NMP;
+    // --- two largest O displacements
+    double m1 = 0.0, m2 = 0.0;
+    for (int i = tid; i < n_mols; i += 1024) {
+        const double dx = D[i], dy = D[NMP + i], dz = D[2 * NMP + i];
+        const double d2 = dx * dx + dy * dy + dz * dz;
+        if (d2 > m1) {
+            m2 = m1;
+            m1 = d2;
+        } else if (d2 > m2) {
+            m2 = d2;
+        }
+    }
+    s1[tid] = m1;
+    s2[tid] = m2;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if (tid < o) {
+            const double a1 = s1[tid], a2 = s2[tid], b1 = s1[tid + o], b2 = s2[tid + o];
+            double t1, t2;
+            if (a1 >= b1) {
+                t1 = a1;
+                t2 = a2 >= b1 ? a2 : b1;
+            } else {
+                t1 = b1;
+                t2 = b2 >= a1 ? b2 : a1;
+            }
+            s1[tid] = t1;
+            s2[tid] = t2;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int f = force_all || (sqrt(s1[0]) + sqrt(s2[0]) > skin);
+        s_flag = f;
+        flag[b] = f;
+        if (f && nreb) nreb[b] += 1.0;
+    }
+    __syncthreads();
+    if (!s_flag) return;
+    // --- counts
+    const double *Q = q + (long)b * 3 * NAP;
+    const int lane = tid & 63, w = tid >> 6;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int i = w; i < n_mols; i += 16) {
+        const double xi = Q[3 * i], yi = Q[NAP + 3 * i], zi = Q[2 * NAP + 3 * i];
+        int nh = 0, nf = 0;
+        for (int j0 = 0; j0 < n_mols; j0 += 64) {
+            const int j = j0 + lane;
+            bool in = false;
+            if (j < n_mols && j != i) {
+                const double rx = mimg_exact(xi - Q[3 * j], box.L[0]);
+                const double ry = mimg_exact(yi - Q[NAP + 3 * j], box.L[1]);
+                const double rz = mimg_exact(zi - Q[2 * NAP + 3 * j], box.L[2]);
+                in = rx * rx + ry * ry + rz * rz < r_skin_sq;
+            }
+            nf += __popcll(__ballot(in));
+            nh += __popcll(__ballot(in && j > i));
+        }
+        if (lane == 0) {
+            ch[i] = nh;
+            cf[i] = nf;
+        }
+    }
+    __syncthreads();
+    // --- exclusive scans (wave 0: chunks of 64 with a running carry)
+    if (w == 0) {
+        int carry_h = 0, carry_f = 0;
+        for (int i0 = 0; i0 < n_mols; i0 += 64) {
+            const int i = i0 + lane;
+            const int vh = i < n_mols ? ch[i] : 0, vf = i < n_mols ? cf[i] : 0;
+            int ih = vh, iff = vf;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int th = __shfl_up(ih, o, 64), tf = __shfl_up(iff, o, 64);
+                if (lane >= o) {
+                    ih += th;
+                    iff += tf;
+                }
+            }
+            if (i < n_mols) {
+                ch[i] = carry_h + ih - vh;
+                cf[i] = carry_f + iff - vf;
+            }
+            carry_h += __shfl(ih, 63, 64);
+            carry_f += __shfl(iff, 63, 64);
+        }
+        if (lane == 0) {
+            s_tot[0] = carry_h;
+            s_tot[1] = carry_f;
+        }
+    }
+    __syncthreads();
+    const int nh_tot = s_tot[0], nf_tot = s_tot[1];
+    if (tid == 0 && (nh_tot > list_size || (long)nf_tot > fcap)) atomicOr(status, 1);
+    for (int i = tid; i < n_mols; i += 1024) {
+        point[(long)b * n_mols + i] = ch[i];
+        fpoint[(long)b * (n_mols + 1) + i] = cf[i];
+    }
+    if (tid == 0) fpoint[(long)b * (n_mols + 1) + n_mols] = nf_tot;
+    int *L = list + (long)b * list_size;
+    int *FL = flist + (long)b * fcap;
+    for (int k = nh_tot + tid; k < list_size; k += 1024) L[k] = 0;
+    for (int t = tid; t < 3 * NMP; t += 1024) D[t] = 0.0;
+    // --- fill
+    for (int i = w; i < n_mols; i += 16) {
+        const double xi = Q[3 * i], yi = Q[NAP + 3 * i], zi = Q[2 * NAP + 3 * i];
+        int kh = ch[i];
+        long kf = cf[i];
+        for (int j0 = 0; j0 < n_mols; j0 += 64) {
+            const int j = j0 + lane;
+            bool in = false;
+            if (j < n_mols && j != i) {
+                const double rx = mimg_exact(xi - Q[3 * j], box.L[0]);
+                const double ry = mimg_exact(yi - Q[NAP + 3 * j], box.L[1]);
+                const double rz = mimg_exact(zi - Q[2 * NAP + 3 * j], box.L[2]);
+                in = rx * rx + ry * ry + rz * rz < r_skin_sq;
+            }
+            const bool inh = in && j > i;
+            const unsigned long long bf = __ballot(in), bh = __ballot(inh);
+            if (in) {
+                const long pos = kf + __popcll(bf & below);
+                if (pos < fcap) FL[pos] = j;
+            }
+            if (inh) {
+                const int pos = kh + __popcll(bh & below);
+                if (pos < list_size) L[pos] = j;
+            }
+            kf += __popcll(bf);
+            kh += __popcll(bh);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Classical velocity Verlet (update.cpp:78-137)
 // ---------------------------------------------------------------------------
@@ -1215,7 +1380,8 @@ __global__ __launch_bounds__(256) void k_vv_a(int n_atoms, int NAP, int NMP, dou
 // F = f_phys of the single evaluation, p += dt/2 F, energy out.
 __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfdt,
                                               const double *__restrict__ fe,
-                                              const double *__restrict__ Ve,
+                                              const double *__restrict__ epart, int ep_stride,
+                                              double eself,
                                               double *__restrict__ F, double *__restrict__ p,
                                               double *__restrict__ out,
                                               unsigned long long *__restrict__ step_ctr)
@@ -1227,10 +1393,13 @@ __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfd
         const int at = f % NAP;
         if (at < n_atoms) p[f] += halfdt * v;
     }
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        out[0] = Ve[0];
-        out[1] = 0.0;
-        if (step_ctr) step_ctr[0] += 1ull;
+    if (blockIdx.x == 0 && threadIdx.x < 64) {
+        const double v = eval_energy(epart, 0, ep_stride, eself);
+        if (threadIdx.x == 0) {
+            out[0] = v;
+            out[1] = 0.0;
+            if (step_ctr) step_ctr[0] += 1ull;
+        }
     }
 }
 
