@@ -445,7 +445,7 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     }
     {
         TimeScope ts(ctx, "realspace");
-        k_realspace<<<E * ctx->ntile, 64, 0, s>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
+        k_realspace<<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
                                                    ctx->nblk, g, h, ctx->tiles, ctx->ntile,
                                                    ctx->box, ctx->ew, ctx->etab, ctx->rpart,
                                                    ctx->epart, ctx->ep_stride, ctx->ep_real);

@@ -148,46 +148,67 @@ __device__ __forceinline__ double eval_energy(const double *__restrict__ epart, 
     return wave_sum_array(epart + (long)e * ep_stride, ep_stride) - eself;
 }
 
-// erfc(x) = exp(-x^2) erfcx(x); erfcx from the piecewise table (qtp_erfcx_table.h)
-// held in LDS.  ex = exp(-x^2) is passed in (the caller needs it anyway).
-__device__ __forceinline__ double erfc_from_table(double x, double ex, const double *tab)
+// erfc(x) = exp(-x^2) erfcx(x), with erfcx(x) = exp(x^2) erfc(x) from the
+// rational fit P8/Q8 of qtp_erfcx_table.h on [0.2, 4.6) (relative error
+// < 9e-16); ex = exp(-x^2) is passed in (the force needs it too).  Outside
+// that range (separations no water configuration reaches at the reference's
+// eps_ewald) the library erfc is used.
+__device__ __forceinline__ double erfc_via_erfcx(double x, double ex)
 {
-    if (x >= QTP_ERFCX_NINT * QTP_ERFCX_H) return erfc(x);
-    const int i = (int)(x * QTP_ERFCX_INVH);
-    const double t = x - (i + 0.5) * QTP_ERFCX_H;
-    const double *c = tab + i * QTP_ERFCX_NCOEF;
-    double g = c[QTP_ERFCX_NCOEF - 1];
-    for (int k = QTP_ERFCX_NCOEF - 2; k >= 0; --k) g = fma(g, t, c[k]);
-    return ex * g;
+    if (x < QTP_ERFCX_RLO || x >= QTP_ERFCX_RHI) return erfc(x);
+    constexpr double P[QTP_ERFCX_RDEG + 1] = {QTP_ERFCX_P};
+    constexpr double Q[QTP_ERFCX_RDEG + 1] = {QTP_ERFCX_Q};
+    double p = P[QTP_ERFCX_RDEG], q = Q[QTP_ERFCX_RDEG];
+#pragma unroll
+    for (int k = QTP_ERFCX_RDEG - 1; k >= 0; --k) {
+        p = fma(p, x, P[k]);
+        q = fma(q, x, Q[k]);
+    }
+    return ex * (p / q);
 }
 
 // Real-space Ewald over all site pairs i<j within r_c (forces.cpp:586-608,
-// rwald :141-177).  One wavefront per 64x64 tile (I<=J) of the pair matrix.
-// Lane l owns site I*64+l and meets site J*64+((l+s)&63) at step s.  Pass 1
-// records which steps fall inside r_c (a 64-bit mask per lane); pass 2 pops
-// the set bits, so lanes evaluate only in-cutoff pairs.  Reaction forces on j
-// accumulate in LDS with ds_add_f64 from this single wavefront (a fixed order
-// for given inputs: bit-reproducible).  Partial site forces go to slot J (for
-// block I's sites) and slot I (for block J's sites): every (site, slot) has
-// exactly one writer.   grid: E * ntile, 64 threads.
-__global__ __launch_bounds__(64) void k_realspace(const double *__restrict__ pos, long pstride,
-                                                  const double *__restrict__ charge, int n_atoms,
-                                                  int NAP, int nblk, double gamma, double half1mg,
-                                                  const int2 *__restrict__ tiles, int ntile,
-                                                  Box box, Ewald ew,
-                                                  const double *__restrict__ etab,
-                                                  double *__restrict__ rpart,
-                                                  double *__restrict__ epart, int ep_stride,
-                                                  int ep_off)
+// rwald :141-177).  One 256-thread block per 64x64 tile (I<=J) of the site
+// pair matrix; wave w takes the 16 j-columns 16w..16w+15 against all 64 i
+// (lane <-> i).  Pass 1 marks the in-cutoff columns of each lane (16-bit
+// mask); pass 2 pops the set bits, so lanes evaluate only in-cutoff pairs.
+// Reaction forces accumulate in LDS with ds_add_f64; each column belongs to
+// one wavefront, so the accumulation order is fixed for given inputs.
+// Partial site forces go to slot J (block I's sites) and slot I (block J's
+// sites): every (site, slot) has exactly one writer.
+// grid: E * ntile, 256 threads.
+constexpr int RS_WAVES = 4;
+__global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ pos, long pstride,
+                                                   const double *__restrict__ charge, int n_atoms,
+                                                   int NAP, int nblk, double gamma, double half1mg,
+                                                   const int2 *__restrict__ tiles, int ntile,
+                                                   Box box, Ewald ew,
+                                                   const double *__restrict__ etab,
+                                                   double *__restrict__ rpart,
+                                                   double *__restrict__ epart, int ep_stride,
+                                                   int ep_off)
 {
     const int e = blockIdx.x / ntile, t = blockIdx.x - e * ntile;
     const int I = tiles[t].x, J = tiles[t].y;
-    const int l = threadIdx.x;
+    const int l = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double *X = pos + (long)e * pstride;
-    __shared__ double sx[64], sy[64], sz[64], sq[64], ax[64], ay[64], az[64];
-    __shared__ double tab[QTP_ERFCX_NINT * QTP_ERFCX_NCOEF];
-    for (int k = l; k < QTP_ERFCX_NINT * QTP_ERFCX_NCOEF; k += 64) tab[k] = etab[k];
-    const int i = I * 64 + l, jj = J * 64 + l;
+    __shared__ double sx[64], sy[64], sz[64], sq[64];
+    __shared__ double fj[3][64];
+    __shared__ double fi[RS_WAVES][3][64];
+    __shared__ double ewv[RS_WAVES];
+    if (w == 0) {
+        const int jj = J * 64 + l;
+        const bool vj = jj < n_atoms;
+        sx[l] = vj ? site_coord(X, NAP, jj, 0, gamma, half1mg) : 0.0;
+        sy[l] = vj ? site_coord(X, NAP, jj, 1, gamma, half1mg) : 0.0;
+        sz[l] = vj ? site_coord(X, NAP, jj, 2, gamma, half1mg) : 0.0;
+        sq[l] = vj ? charge[jj] : 0.0;
+        fj[0][l] = 0.0;
+        fj[1][l] = 0.0;
+        fj[2][l] = 0.0;
+    }
+    const int i = I * 64 + l;
     const bool vi = i < n_atoms;
     double xi = 0.0, yi = 0.0, zi = 0.0, qi = 0.0;
     if (vi) {
@@ -196,44 +217,35 @@ __global__ __launch_bounds__(64) void k_realspace(const double *__restrict__ pos
         zi = site_coord(X, NAP, i, 2, gamma, half1mg);
         qi = charge[i];
     }
-    const bool vj = jj < n_atoms;
-    sx[l] = vj ? site_coord(X, NAP, jj, 0, gamma, half1mg) : 0.0;
-    sy[l] = vj ? site_coord(X, NAP, jj, 1, gamma, half1mg) : 0.0;
-    sz[l] = vj ? site_coord(X, NAP, jj, 2, gamma, half1mg) : 0.0;
-    sq[l] = vj ? charge[jj] : 0.0;
-    ax[l] = 0.0;
-    ay[l] = 0.0;
-    az[l] = 0.0;
     __syncthreads();
-    const bool diag = (I == J);
-    const int s0 = diag ? 1 : 0, s1 = diag ? 33 : 64;
-    // pass 1: in-cutoff mask
-    unsigned long long mask = 0ull;
+    const int c0 = 16 * w;
+    // pass 1: in-cutoff columns (j > i; j < n_atoms)
+    unsigned int mask = 0u;
     if (vi) {
-        for (int s = s0; s < s1; ++s) {
-            const int jl = (l + s) & 63;
-            if (J * 64 + jl >= n_atoms || (diag && s == 32 && l >= 32)) continue;
+        for (int c = 0; c < 16; ++c) {
+            const int jl = c0 + c;
+            const int j = J * 64 + jl;
             const double dx = mimg_fast(xi - sx[jl], box.L[0], box.inv[0]);
             const double dy = mimg_fast(yi - sy[jl], box.L[1], box.inv[1]);
             const double dz = mimg_fast(zi - sz[jl], box.L[2], box.inv[2]);
             const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-            if (rsq < ew.rcut_sq) mask |= 1ull << s;
+            if (j > i && j < n_atoms && rsq < ew.rcut_sq) mask |= 1u << c;
         }
     }
     // pass 2: in-cutoff pairs only
     const int mi = i / 3;
     double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
     while (mask) {
-        const int s = __ffsll((unsigned long long)mask) - 1;
-        mask &= mask - 1ull;
-        const int jl = (l + s) & 63;
+        const int c = __ffs(mask) - 1;
+        mask &= mask - 1u;
+        const int jl = c0 + c;
         const double dx = mimg_fast(xi - sx[jl], box.L[0], box.inv[0]);
         const double dy = mimg_fast(yi - sy[jl], box.L[1], box.inv[1]);
         const double dz = mimg_fast(zi - sz[jl], box.L[2], box.inv[2]);
         const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
         const double rinv = rsqrt(rsq);
         const double ex = exp(-ew.kappa2 * rsq);
-        double ec = erfc_from_table(ew.kappa * (rsq * rinv), ex, tab);
+        double ec = erfc_via_erfcx(ew.kappa * (rsq * rinv), ex);
         if (mi == (J * 64 + jl) / 3) ec -= 1.0;  // same molecule: -erf (forces.cpp:158-166)
         const double qq = qi * sq[jl];
         const double ep = qq * rinv * ec;
@@ -242,29 +254,30 @@ __global__ __launch_bounds__(64) void k_realspace(const double *__restrict__ pos
         fx = fma(pf, dx, fx);
         fy = fma(pf, dy, fy);
         fz = fma(pf, dz, fz);
-        atomicAdd(&ax[jl], -pf * dx);
-        atomicAdd(&ay[jl], -pf * dy);
-        atomicAdd(&az[jl], -pf * dz);
+        atomicAdd(&fj[0][jl], -pf * dx);
+        atomicAdd(&fj[1][jl], -pf * dy);
+        atomicAdd(&fj[2][jl], -pf * dz);
     }
+    fi[w][0][l] = fx;
+    fi[w][1][l] = fy;
+    fi[w][2][l] = fz;
+    en = wave_sum(en);
+    if (l == 0) ewv[w] = en;
     __syncthreads();
     const long base_e = (long)e * nblk * 3 * NAP;
-    if (diag) {
-        double *o = rpart + base_e + (long)I * 3 * NAP;
-        o[i] = fx + ax[l];
-        o[NAP + i] = fy + ay[l];
-        o[2 * NAP + i] = fz + az[l];
-    } else {
-        double *oI = rpart + base_e + (long)J * 3 * NAP;  // slot J, sites of block I
-        oI[i] = fx;
-        oI[NAP + i] = fy;
-        oI[2 * NAP + i] = fz;
-        double *oJ = rpart + base_e + (long)I * 3 * NAP;  // slot I, sites of block J
-        oJ[jj] = ax[l];
-        oJ[NAP + jj] = ay[l];
-        oJ[2 * NAP + jj] = az[l];
+    if (w < 3) {
+        // wave d writes component d of both slot entries of this tile
+        const int d = w;
+        const double si = fi[0][d][l] + fi[1][d][l] + fi[2][d][l] + fi[3][d][l];
+        if (I == J) {
+            rpart[base_e + (long)I * 3 * NAP + d * NAP + i] = si + fj[d][l];
+        } else {
+            rpart[base_e + (long)J * 3 * NAP + d * NAP + i] = si;                 // slot J
+            rpart[base_e + (long)I * 3 * NAP + d * NAP + J * 64 + l] = fj[d][l];  // slot I
+        }
+    } else if (l == 0) {
+        epart[(long)e * ep_stride + ep_off + t] = ewv[0] + ewv[1] + ewv[2] + ewv[3];
     }
-    en = wave_sum(en);
-    if (l == 0) epart[(long)e * ep_stride + ep_off + t] = en;
 }
 
 // Per-site factor table e^{i m b_d r_d}, m = 0..K-1, by the cos/sin recurrence
