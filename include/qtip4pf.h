@@ -18,7 +18,7 @@
  * Two ways to drive it:
  *   - reference-shaped calls with host arrays (qtp_get_forces_nospline ...),
  *     each a full host->device->host round trip, for the reference driver;
- *   - resident mode (qtp_set_state / qtp_run_sc / qtp_get_state): state stays
+ *   - resident mode (qtp_set_state / qtp_run / qtp_get_state): state stays
  *     in HBM, the whole step is one replayed HIP graph, no per-step host sync.
  */
 #ifndef QTIP4PF_H
@@ -80,6 +80,16 @@ int qtp_get_stream(const qtp_ctx *ctx, void **stream);
  * zeroes and fills the bead slice of F, returns the bead energy in *energy. */
 int qtp_get_forces_nospline(qtp_ctx *ctx, const double *q, double *F, const double *charge,
                             const int *point, const int *list, int bead, double *energy);
+
+/* Pressure tensor support (no counterpart in the reference, which computes
+ * no virial): get_forces_nospline plus the virial of the bead,
+ *   virial[6] = W_xx, W_yy, W_zz, W_xy, W_xz, W_yz,   W = -dV/d(strain)
+ * at fixed Ewald parameters (kappa, r_c, k-set), i.e. the minimum-image pair
+ * sum over sites of r_ij (x) f_ij plus the intramolecular and reciprocal
+ * terms.  Pressure: (N_mol k T + tr W / 3) / V for a classical system. */
+int qtp_get_forces_virial(qtp_ctx *ctx, const double *q, double *F, const double *charge,
+                          const int *point, const int *list, int bead, double *energy,
+                          double virial[6]);
 
 /* forces.cpp:698-789 force_eval_sc_nospline: Suzuki-Chin forces F_sc and
  * physical forces fphys for all beads; *energy = SC conserved potential,
@@ -144,6 +154,11 @@ int qtp_set_state(qtp_ctx *ctx, const double *q, const double *p, const double *
 int qtp_get_state(qtp_ctx *ctx, double *q, double *p, double *F, double *fphys, double *disp);
 /* Download the half lists in reference layout. */
 int qtp_get_lists(qtp_ctx *ctx, int *point, int *list);
+
+/* Virial of every bead at the resident positions (as qtp_get_forces_virial):
+ * virial[b*6 + c]; energy[b] = V of bead b if energy is not NULL.  No
+ * counterpart in the reference.  Leaves the resident state untouched. */
+int qtp_get_virial(qtp_ctx *ctx, double *virial, double *energy);
 
 /* Thermostat random numbers for resident NVT steps: Philox4x32-10 streams
  * keyed by seed and the device step counter, Box-Muller normals; the centroid

@@ -60,6 +60,7 @@ SIGNATURES = [
     ("qtp_get_derived", C.c_int, [_vp, C.POINTER(QtpDerived)]),
     ("qtp_get_stream", C.c_int, [_vp, C.POINTER(_vp)]),
     ("qtp_get_forces_nospline", C.c_int, [_vp, dp, dp, dp, ip, ip, C.c_int, dp]),
+    ("qtp_get_forces_virial", C.c_int, [_vp, dp, dp, dp, ip, ip, C.c_int, dp, dp]),
     ("qtp_force_eval_sc_nospline", C.c_int, [_vp, dp, dp, dp, dp, ip, ip, dp, dp, dp]),
     ("qtp_neighbour_list", C.c_int, [_vp, dp, ip, ip, dp, C.c_int]),
     ("qtp_ftransform", C.c_int, [_vp, dp, C.c_int, C.c_int]),
@@ -75,6 +76,7 @@ SIGNATURES = [
     ("qtp_set_state", C.c_int, [_vp, dp, dp, dp, dp]),
     ("qtp_get_state", C.c_int, [_vp, dp, dp, dp, dp, dp]),
     ("qtp_get_lists", C.c_int, [_vp, ip, ip]),
+    ("qtp_get_virial", C.c_int, [_vp, dp, dp]),
     ("qtp_set_seed", C.c_int, [_vp, C.c_uint64]),
     ("qtp_run", C.c_int, [_vp, C.c_int, C.c_int]),
     ("qtp_sync", C.c_int, [_vp]),

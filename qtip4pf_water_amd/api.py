@@ -39,6 +39,12 @@ def _i32(a):
     return a
 
 
+def voigt_to_matrix(v):
+    """(xx, yy, zz, xy, xz, yz) -> symmetric 3x3."""
+    xx, yy, zz, xy, xz, yz = (float(x) for x in v)
+    return np.array([[xx, xy, xz], [xy, yy, yz], [xz, yz, zz]])
+
+
 class HostRng:
     """std::mt19937 owned by the library; draws pile_g's deviates in the
     reference's order (thermostat.cpp:71, 88-98, 124)."""
@@ -151,6 +157,16 @@ class Water:
                                                   int(bead), C.byref(e)))
         return e.value
 
+    def get_forces_virial(self, q, F, charge, point, lst, bead=0):
+        """get_forces_nospline plus the bead's virial tensor W = -dV/d(strain)
+        at fixed Ewald parameters.  Returns (energy, W as a 3x3 array)."""
+        e, v = C.c_double(), np.zeros(6)
+        self._chk(self._L.qtp_get_forces_virial(self._h, ptr(_f64(q)), ptr(_f64(F)),
+                                                ptr(_f64(charge)) if charge is not None else None,
+                                                ptr(_i32(point), ip), ptr(_i32(lst), ip),
+                                                int(bead), C.byref(e), ptr(v)))
+        return e.value, voigt_to_matrix(v)
+
     def force_eval_sc_nospline(self, q, F_sc, charge, mass, point, lst, fphys=None):
         """Returns (tot_energy, sc_pot) as the reference's return value and
         sc_pot reference argument."""
@@ -232,6 +248,12 @@ class Water:
         lst = np.zeros(self.n_beads * self.list_size, np.int32)
         self._chk(self._L.qtp_get_lists(self._h, ptr(point, ip), ptr(lst, ip)))
         return point, lst
+
+    def virial(self):
+        """Virial tensor (P x 3 x 3) and energy (P) of every resident bead."""
+        v, e = np.zeros(6 * self.n_beads), np.zeros(self.n_beads)
+        self._chk(self._L.qtp_get_virial(self._h, ptr(v), ptr(e)))
+        return np.stack([voigt_to_matrix(v[6 * b:6 * b + 6]) for b in range(self.n_beads)]), e
 
     def set_seed(self, seed: int):
         self._chk(self._L.qtp_set_seed(self._h, C.c_uint64(seed)))

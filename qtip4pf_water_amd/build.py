@@ -53,5 +53,21 @@ def build(verbose: bool = False) -> str:
     return OUT
 
 
+DRIVER_SRC = os.path.join(ROOT, "tests", "adapter", "adapter_driver.cpp")
+DRIVER_BIN = os.path.join(ROOT, "tests", "_bin", "qtp_adapter_driver")
+
+
+def build_adapter_driver(verbose: bool = False) -> str:
+    """Test driver for include/qtip4pf_ref_adapter.hpp (links libqtip4pf.so)."""
+    os.makedirs(os.path.dirname(DRIVER_BIN), exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), DRIVER_SRC,
+           "-L" + PKG, "-lqtip4pf", "-Wl,-rpath,$ORIGIN/../../qtip4pf_water_amd", "-o", DRIVER_BIN]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+    return DRIVER_BIN
+
+
 if __name__ == "__main__":
     print(build(verbose=True))
+    print(build_adapter_driver(verbose=True))

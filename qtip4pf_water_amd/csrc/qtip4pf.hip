@@ -105,8 +105,8 @@ struct qtp_ctx {
     int *flag = nullptr, *status = nullptr;
     double *nreb = nullptr;
     // force-phase scratch (E <= max(P,1))
-    double *qpm = nullptr, *cpos = nullptr, *rpart = nullptr, *spart = nullptr, *sprime = nullptr;
-    double *ef = nullptr, *flj = nullptr, *fpm = nullptr, *Ve = nullptr, *epart = nullptr;
+    double *qpm = nullptr, *vpart = nullptr, *vout = nullptr, *rpart = nullptr, *spart = nullptr, *sprime = nullptr;
+    double *flj = nullptr, *fpm = nullptr, *Ve = nullptr, *epart = nullptr;
     double *scal = nullptr, *out = nullptr, *heat = nullptr, *heat_step = nullptr;
     unsigned long long *step_ctr = nullptr;
     // constants
@@ -116,7 +116,6 @@ struct qtp_ctx {
     int2 *tiles = nullptr;
     double *Cm = nullptr;
     double4 *mode = nullptr;
-    double *etab = nullptr;  // erfcx polynomial table
     // propagation scratch
     double *nq = nullptr, *np = nullptr, *ppart = nullptr, *hpart = nullptr, *dev = nullptr;
     double *obs = nullptr;
@@ -358,11 +357,11 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->status, 1));
     CHK(dalloc(ctx, &ctx->nreb, P));
     CHK(dalloc(ctx, &ctx->qpm, (size_t)E * N3));
-    CHK(dalloc(ctx, &ctx->cpos, (size_t)E * N3));
+    CHK(dalloc(ctx, &ctx->vpart, (size_t)E * 6 * ctx->ep_stride));
+    CHK(dalloc(ctx, &ctx->vout, (size_t)E * 6));
     CHK(dalloc(ctx, &ctx->rpart, (size_t)E * ctx->nblk * N3));
     CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));
     CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * 8));
-    CHK(dalloc(ctx, &ctx->ef, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->flj, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->Ve, E));
@@ -430,25 +429,39 @@ static int upload_constants(qtp_ctx *ctx)
 // ---------------------------------------------------------------------------
 // force phase: E single-bead evaluations (forces.cpp:468-630)
 // ---------------------------------------------------------------------------
+// vout (optional): per-evaluation virial W_ab = sum_i r_i (x) F_i in the
+// minimum-image pair form (6 components xx yy zz xy xz yz), at fixed Ewald
+// parameters.
 static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbase, double *fout,
-                     double *eout)
+                     double *eout, double *vout = nullptr)
 {
     hipStream_t s = ctx->stream;
     const long N3 = 3L * ctx->NAP;
     const double g = ctx->ff.gamma_vs, h = ctx->ff.half1mg;
+    double *vp = vout ? ctx->vpart : nullptr;
     {
         TimeScope ts(ctx, "ljintra");
-        k_ljintra<<<E * ctx->nmb, 256, 0, s>>>(pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint,
-                                                ctx->flist, ctx->fcap, lmode, lbase, ctx->box,
-                                                ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride,
-                                                ctx->ep_lj);
+        if (vout)
+            k_ljintra<true><<<E * ctx->nmb, 256, 0, s>>>(
+                pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, ctx->flist, ctx->fcap, lmode, lbase,
+                ctx->box, ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride, ctx->ep_lj, vp);
+        else
+            k_ljintra<false><<<E * ctx->nmb, 256, 0, s>>>(
+                pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, ctx->flist, ctx->fcap, lmode, lbase,
+                ctx->box, ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride, ctx->ep_lj, vp);
     }
     {
         TimeScope ts(ctx, "realspace");
-        k_realspace<<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
-                                                   ctx->nblk, g, h, ctx->tiles, ctx->ntile,
-                                                   ctx->box, ctx->ew, ctx->etab, ctx->rpart,
-                                                   ctx->epart, ctx->ep_stride, ctx->ep_real);
+        if (vout)
+            k_realspace<true><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(
+                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles,
+                ctx->ntile, ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real,
+                vp);
+        else
+            k_realspace<false><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(
+                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles,
+                ctx->ntile, ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real,
+                vp);
     }
     {
         TimeScope ts(ctx, "ksum");
@@ -459,7 +472,8 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     {
         TimeScope ts(ctx, "sreduce");
         k_sreduce<<<E * ctx->ngr, 256, 0, s>>>(ctx->spart, ctx->nac, ctx->G, ctx->groups, ctx->gcf,
-                                                ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k);
+                                                ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
+                                                ctx->grk, 0.25 / ctx->ew.kappa2, vp);
     }
     {
         TimeScope ts(ctx, "kforce");
@@ -472,6 +486,7 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         TimeScope ts(ctx, "esum");
         k_esum<<<E, 64, 0, s>>>(E, ctx->epart, ctx->ep_stride, ctx->eself, eout);
     }
+    if (vout) k_vsum<<<E, 64, 0, s>>>(ctx->vpart, ctx->ep_stride, vout);
     HIPCHK(hipGetLastError());
     return QTP_OK;
 }
@@ -897,13 +912,12 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
         (st = dalloc(ctx, &ctx->grk, ctx->G)) || (st = dalloc(ctx, &ctx->tiles, ctx->ntile)) ||
         (st = dalloc(ctx, &ctx->Cm, (size_t)ctx->P * ctx->P)) ||
         (st = dalloc(ctx, &ctx->mode, ctx->P)) ||
-        (st = dalloc(ctx, &ctx->etab, QTP_ERFCX_NINT * QTP_ERFCX_NCOEF)) || (st = alloc_all(ctx)))
+        (st = alloc_all(ctx)))
         return st;
     if (hipMemcpy(ctx->groups, g.data(), g.size() * sizeof(int4), hipMemcpyHostToDevice) ||
         hipMemcpy(ctx->gcf, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice) ||
         hipMemcpy(ctx->grk, rk.data(), rk.size() * sizeof(double4), hipMemcpyHostToDevice) ||
-        hipMemcpy(ctx->tiles, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice) ||
-        hipMemcpy(ctx->etab, qtp_erfcx_coef, sizeof(qtp_erfcx_coef), hipMemcpyHostToDevice)) {
+        hipMemcpy(ctx->tiles, tl.data(), tl.size() * sizeof(int2), hipMemcpyHostToDevice)) {
         ctx->err = "constant upload failed";
         return QTP_ERR_HIP;
     }
@@ -920,10 +934,10 @@ void qtp_destroy(qtp_ctx *ctx)
     for (auto e : ctx->evpool) hipEventDestroy(e);
     void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
                     ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
-                    ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->cpos, ctx->rpart,
-                    ctx->spart, ctx->sprime, ctx->ef, ctx->flj, ctx->fpm, ctx->Ve, ctx->epart,
+                    ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart,
+                    ctx->spart, ctx->sprime, ctx->flj, ctx->fpm, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
-                    ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->etab, ctx->nq, ctx->np,
+                    ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -961,6 +975,27 @@ int qtp_get_forces_nospline(qtp_ctx *ctx, const double *q, double *F, const doub
     CHK(run_phase(ctx, ctx->qpm, 1, 0, bead, ctx->fpm, ctx->Ve));
     double e = 0.0;
     HIPCHK(hipMemcpyAsync(&e, ctx->Ve, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    CHK(download_planar(ctx, ctx->fpm, 1, F + (long)bead * ctx->n_dof));
+    CHK(flush_timing(ctx));
+    if (energy) *energy = e;
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+int qtp_get_forces_virial(qtp_ctx *ctx, const double *q, double *F, const double *charge,
+                          const int *point, const int *list, int bead, double *energy,
+                          double virial[6])
+{
+    if (!ctx || !q || !F || !point || !list || !virial || bead < 0 || bead >= ctx->P)
+        return QTP_ERR_ARG;
+    CHK(check_finite(ctx, q + (long)bead * ctx->n_dof, ctx->n_dof, "positions"));
+    CHK(upload_charges_masses(ctx, charge, nullptr));
+    CHK(upload_planar(ctx, ctx->qpm, q + (long)bead * ctx->n_dof, 1));
+    CHK(upload_lists(ctx, point, list, bead, 1, bead));
+    CHK(run_phase(ctx, ctx->qpm, 1, 0, bead, ctx->fpm, ctx->Ve, ctx->vout));
+    double e = 0.0;
+    HIPCHK(hipMemcpyAsync(&e, ctx->Ve, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(virial, ctx->vout, 6 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     CHK(download_planar(ctx, ctx->fpm, 1, F + (long)bead * ctx->n_dof));
     CHK(flush_timing(ctx));
     if (energy) *energy = e;
@@ -1309,6 +1344,26 @@ int qtp_set_seed(qtp_ctx *ctx, uint64_t seed)
     ctx->seed = seed;
     invalidate_graphs(ctx);
     return QTP_OK;
+}
+
+int qtp_get_virial(qtp_ctx *ctx, double *virial, double *energy)
+{
+    if (!ctx || !virial) return QTP_ERR_ARG;
+    if (!ctx->state_set) {
+        ctx->err = "qtp_get_virial: no resident state (call qtp_set_state)";
+        return QTP_ERR_STATE;
+    }
+    const int P = ctx->P;
+    // physical forces go to scratch: the resident force arrays are left as they are
+    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fpm, ctx->Ve, ctx->vout));
+    HIPCHK(hipMemcpyAsync(virial, ctx->vout, (size_t)P * 6 * sizeof(double),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    if (energy)
+        HIPCHK(hipMemcpyAsync(energy, ctx->Ve, (size_t)P * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    CHK(flush_timing(ctx));
+    return read_status(ctx);
 }
 
 static int step_once(qtp_ctx *ctx, int nvt)

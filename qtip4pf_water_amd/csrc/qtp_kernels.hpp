@@ -177,21 +177,24 @@ __device__ __forceinline__ double erfc_via_erfcx(double x, double ex)
 // Partial site forces go to slot J (block I's sites) and slot I (block J's
 // sites): every (site, slot) has exactly one writer.
 // grid: E * ntile, 256 threads.
+// VIR: also accumulate the pair virial r_ij (x) f_ij (6 components) into
+// vpart[(e*6 + c)*ep_stride + ep_off + tile].
 constexpr int RS_WAVES = 4;
+template <bool VIR>
 __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ pos, long pstride,
                                                    const double *__restrict__ charge, int n_atoms,
                                                    int NAP, int nblk, double gamma, double half1mg,
                                                    const int2 *__restrict__ tiles, int ntile,
                                                    Box box, Ewald ew,
-                                                   const double *__restrict__ etab,
                                                    double *__restrict__ rpart,
                                                    double *__restrict__ epart, int ep_stride,
-                                                   int ep_off)
+                                                   int ep_off, double *__restrict__ vpart)
 {
     const int e = blockIdx.x / ntile, t = blockIdx.x - e * ntile;
     const int I = tiles[t].x, J = tiles[t].y;
     const int l = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ double wv[RS_WAVES][6];
     const double *X = pos + (long)e * pstride;
     __shared__ double sx[64], sy[64], sz[64], sq[64];
     __shared__ double fj[3][64];
@@ -235,6 +238,7 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
     // pass 2: in-cutoff pairs only
     const int mi = i / 3;
     double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
+    double vxx = 0.0, vyy = 0.0, vzz = 0.0, vxy = 0.0, vxz = 0.0, vyz = 0.0;
     while (mask) {
         const int c = __ffs(mask) - 1;
         mask &= mask - 1u;
@@ -254,6 +258,14 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
         fx = fma(pf, dx, fx);
         fy = fma(pf, dy, fy);
         fz = fma(pf, dz, fz);
+        if (VIR) {
+            vxx = fma(pf * dx, dx, vxx);
+            vyy = fma(pf * dy, dy, vyy);
+            vzz = fma(pf * dz, dz, vzz);
+            vxy = fma(pf * dx, dy, vxy);
+            vxz = fma(pf * dx, dz, vxz);
+            vyz = fma(pf * dy, dz, vyz);
+        }
         atomicAdd(&fj[0][jl], -pf * dx);
         atomicAdd(&fj[1][jl], -pf * dy);
         atomicAdd(&fj[2][jl], -pf * dz);
@@ -263,7 +275,16 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
     fi[w][2][l] = fz;
     en = wave_sum(en);
     if (l == 0) ewv[w] = en;
+    if (VIR) {
+        const double v6[6] = {vxx, vyy, vzz, vxy, vxz, vyz};
+        for (int cc = 0; cc < 6; ++cc) {
+            const double v = wave_sum(v6[cc]);
+            if (l == 0) wv[w][cc] = v;
+        }
+    }
     __syncthreads();
+    if (VIR && w == 0 && l < 6)
+        vpart[((long)e * 6 + l) * ep_stride + ep_off + t] = wv[0][l] + wv[1][l] + wv[2][l] + wv[3][l];
     const long base_e = (long)e * nblk * 3 * NAP;
     if (w < 3) {
         // wave d writes component d of both slot entries of this tile
@@ -356,18 +377,24 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 // Sum the partial structure factors, energy per group (forces.cpp:274, 315,
 // 364, 428) and the force-ready S' = 2 c S of the variants the reference
 // evaluates (gr.w bit v set); others set to zero.   grid: E * ceil(G/256).
+// With vpart set, also the reciprocal-space virial tensor
+//   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 kappa^2)))
+// (E_k = c |S(k)|^2 per sign variant; the strain derivative of kwald's sum at
+// fixed Ewald parameters).
 __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spart, int nac, int G,
                                                  const int4 *__restrict__ groups,
                                                  const double *__restrict__ gcf,
                                                  double *__restrict__ sprime,
                                                  double *__restrict__ epart, int ep_stride,
-                                                 int ep_off)
+                                                 int ep_off, const double4 *__restrict__ grk,
+                                                 double q4, double *__restrict__ vpart)
 {
     __shared__ double sh[16];
     const int nb = (G + 255) / 256;
     const int e = blockIdx.x / nb, gb = blockIdx.x - e * nb;
     const int g = gb * 256 + threadIdx.x;
     double en = 0.0;
+    double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     if (g < G) {
         double s[8];
         for (int v = 0; v < 8; ++v) s[v] = 0.0;
@@ -388,12 +415,33 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
             }
         }
         en = cf * sq;
+        if (vpart) {
+            const double4 rk = grk[g];
+            const double k2 = rk.x * rk.x + rk.y * rk.y + rk.z * rk.z;
+            const double bb = 2.0 * (1.0 / k2 + q4);
+            for (int v = 0; v < 4; ++v) {
+                const double Ev = cf * (s[2 * v] * s[2 * v] + s[2 * v + 1] * s[2 * v + 1]);
+                const double ky = (v < 2) ? rk.y : -rk.y, kz = (v & 1) ? -rk.z : rk.z;
+                v6[0] += Ev * (1.0 - bb * rk.x * rk.x);
+                v6[1] += Ev * (1.0 - bb * ky * ky);
+                v6[2] += Ev * (1.0 - bb * kz * kz);
+                v6[3] -= Ev * bb * rk.x * ky;
+                v6[4] -= Ev * bb * rk.x * kz;
+                v6[5] -= Ev * bb * ky * kz;
+            }
+        }
         const double c2 = 2.0 * cf;
         double *o = sprime + ((long)e * G + g) * 8;
         for (int v = 0; v < 8; ++v) o[v] = c2 * s[v];
     }
     en = block_sum(en, sh);
     if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + gb] = en;
+    if (vpart) {
+        for (int c = 0; c < 6; ++c) {
+            const double v = block_sum(v6[c], sh);
+            if (threadIdx.x == 0) vpart[((long)e * 6 + c) * ep_stride + ep_off + gb] = v;
+        }
+    }
 }
 
 // Reciprocal-space forces (forces.cpp:276-279, 319-324, 368-373, 434-442),
@@ -524,6 +572,7 @@ __global__ __launch_bounds__(1024) void k_kforce(const double *__restrict__ pos,
 // Each LJ pair is met from both sides; its energy is halved.
 // grid: E * ceil(n_mols/16), 256 threads.
 constexpr int LJ_LPM = 16;
+template <bool VIR>
 __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos, long pstride,
                                                  int n_mols, int NAP,
                                                  const int *__restrict__ fpoint,
@@ -531,10 +580,11 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
                                                  int lmode, int lbase, Box box, FF ff,
                                                  double *__restrict__ flj,
                                                  double *__restrict__ epart, int ep_stride,
-                                                 int ep_off)
+                                                 int ep_off, double *__restrict__ vpart)
 {
     __shared__ double sh[16];
     constexpr int MPB = 256 / LJ_LPM;
+    double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int nmb = (n_mols + MPB - 1) / MPB;
     const int e = blockIdx.x / nmb, mb = blockIdx.x - e * nmb;
     const int gi = threadIdx.x / LJ_LPM, t = threadIdx.x % LJ_LPM;
@@ -565,6 +615,15 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
                 elj += u * ff.epslj4 - ff.e_cut;
                 const double pf = (u + a12) * ff.epslj24 * rm2;
                 for (int d = 0; d < 3; ++d) fo[d] = fma(pf, r[d], fo[d]);
+                if (VIR) {  // each pair is met twice: half of r (x) f per side
+                    const double hp = 0.5 * pf;
+                    v6[0] += hp * r[0] * r[0];
+                    v6[1] += hp * r[1] * r[1];
+                    v6[2] += hp * r[2] * r[2];
+                    v6[3] += hp * r[0] * r[1];
+                    v6[4] += hp * r[0] * r[2];
+                    v6[5] += hp * r[1] * r[2];
+                }
             }
         }
     }
@@ -602,6 +661,14 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
                 fo[d] -= pf * rr[d];
                 fh[d] += pf * rr[d];
             }
+            if (VIR) {  // r_O (x) F_O + r_H (x) F_H = -pf r (x) r, r = r_O - r_H
+                v6[0] -= pf * rr[0] * rr[0];
+                v6[1] -= pf * rr[1] * rr[1];
+                v6[2] -= pf * rr[2] * rr[2];
+                v6[3] -= pf * rr[0] * rr[1];
+                v6[4] -= pf * rr[0] * rr[2];
+                v6[5] -= pf * rr[1] * rr[2];
+            }
         }
         // angle (intra_harm_angle)
         {
@@ -614,10 +681,21 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
             const double td = acos(c) - ff.theta_eq;
             en += ff.half_ktheta * td * td;
             const double pf = ff.ktheta * td * dxdc;
+            double ga[3], gb[3];  // angle forces on H1, H2
             for (int d = 0; d < 3; ++d) {
                 fo[d] += pf * (r12 * (r1[d] + r2[d]) - c2r1 * r1[d] - c2r2 * r2[d]);
-                fh1[d] -= pf * (r12 * r2[d] - c2r1 * r1[d]);
-                fh2[d] -= pf * (r12 * r1[d] - c2r2 * r2[d]);
+                ga[d] = -pf * (r12 * r2[d] - c2r1 * r1[d]);
+                gb[d] = -pf * (r12 * r1[d] - c2r2 * r2[d]);
+                fh1[d] += ga[d];
+                fh2[d] += gb[d];
+            }
+            if (VIR) {  // sum_i r_i (x) F_i = -(r1 (x) F_H1 + r2 (x) F_H2), r_k = r_O - r_Hk
+                v6[0] -= r1[0] * ga[0] + r2[0] * gb[0];
+                v6[1] -= r1[1] * ga[1] + r2[1] * gb[1];
+                v6[2] -= r1[2] * ga[2] + r2[2] * gb[2];
+                v6[3] -= 0.5 * (r1[0] * ga[1] + r2[0] * gb[1] + r1[1] * ga[0] + r2[1] * gb[0]);
+                v6[4] -= 0.5 * (r1[0] * ga[2] + r2[0] * gb[2] + r1[2] * ga[0] + r2[2] * gb[0]);
+                v6[5] -= 0.5 * (r1[1] * ga[2] + r2[1] * gb[2] + r1[2] * ga[1] + r2[2] * gb[1]);
             }
         }
         double *O = flj + (long)e * 3 * NAP;
@@ -629,6 +707,24 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
     }
     en = block_sum(en, sh);
     if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + mb] = en;
+    if (VIR) {
+        for (int c = 0; c < 6; ++c) {
+            const double v = block_sum(v6[c], sh);
+            if (threadIdx.x == 0) vpart[((long)e * 6 + c) * ep_stride + ep_off + mb] = v;
+        }
+    }
+}
+
+// Virial of E evaluations: the 6 components' partials summed per evaluation.
+// grid: E, 64 threads.   out[e*6 + c], c = xx, yy, zz, xy, xz, yz
+__global__ __launch_bounds__(64) void k_vsum(const double *__restrict__ vpart, int ep_stride,
+                                             double *__restrict__ out)
+{
+    const int e = blockIdx.x;
+    for (int c = 0; c < 6; ++c) {
+        const double v = wave_sum_array(vpart + ((long)e * 6 + c) * ep_stride, ep_stride);
+        if (threadIdx.x == 0) out[(long)e * 6 + c] = v;
+    }
 }
 
 // Energies of E evaluations (ordered sum of the partials minus the self term).

@@ -295,3 +295,52 @@ def test_pile_g_matches_reference_golden(W):
     ke = 0.5 * np.sum(g["p_in"].reshape(P, -1) ** 2 / g["mass"])
     assert abs(h1 - float(g["heat1"])) <= 1e-12 * ke
     assert abs(h2 - float(g["heat2"])) <= 1e-12 * ke
+
+
+@pytest.mark.parametrize("P,nvt", [(4, True), (1, False)])
+def test_reference_adapter_driver(oracle_mod, tmp_path, P, nvt):
+    """The reference-named entry points of include/qtip4pf_ref_adapter.hpp,
+    driven by a C++ loop in the role of simulation.cpp, against the oracle
+    fed the same std::mt19937 stream."""
+    import os
+    import struct
+    import subprocess
+    from qtip4pf_water_amd import HostRng
+    drv = os.path.join(os.path.dirname(__file__), "_bin", "qtp_adapter_driver")
+    assert os.path.exists(drv), "build the driver: python -m qtip4pf_water_amd.build"
+    n_side, L, steps, seed = 3, 9.4, 4, 77
+    say("adapter driver 27 mols P", P, "nvt", nvt)
+    q, box = system(n_side, L, P, 121)
+    o = oracle_mod.Oracle(27, P, box)
+    p = cfg.sample_momenta(27, P, o.sys.beta_n, seed=5)
+    fin = tmp_path / "in.bin"
+    with open(fin, "wb") as f:
+        f.write(struct.pack("5i", 27, P, steps, 1 if nvt else 0, seed))
+        f.write(struct.pack("7d", *box, 298.0, 0.25, 1e-6, 1e-5))
+        f.write(q.tobytes())
+        f.write(p.tobytes())
+    fout = tmp_path / "out.bin"
+    r = subprocess.run([drv, str(fin), str(fout)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    out = np.fromfile(fout, dtype=np.float64)
+    N = P * o.n_dof
+    qg, pg, Fg, eg = out[:N], out[N:2 * N], out[2 * N:3 * N], out[3 * N:3 * N + steps]
+    # oracle run with the same stream
+    point, lst, disp = o.all_lists(q)
+    qr, pr = q.copy(), p.copy()
+    F, fph = np.zeros(N), np.zeros(N)
+    rng = HostRng(seed)
+    heat = 0.0
+    if P > 1:
+        o.force_eval_sc(qr, F, fph, point, lst)
+    else:
+        o.get_forces(qr, F, point, lst, 0)
+    for s in range(steps):
+        if P > 1:
+            dev = rng.draw_pile(o.n_atoms, P) if nvt else None
+            er, heat, _, _ = o.update_sc(qr, pr, F, fph, point, lst, disp, nvt, dev, heat)
+        else:
+            er, _ = o.update_class(qr, pr, F, point, lst, disp)
+        assert abs(eg[s] - er) <= 1e-12 * e_self_scale(o) * P, s
+    assert np.max(np.abs(qg - qr)) <= 1e-12 * np.max(np.abs(qr))
+    assert np.max(np.abs(pg - pr)) <= 1e-10 * np.max(np.abs(pr))

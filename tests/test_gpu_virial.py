@@ -1,0 +1,117 @@
+"""GPU virial tensor (qtp_get_forces_virial, qtp_get_virial).
+
+The reference computes no virial; the tensor W = -dV/d(strain) at fixed
+Ewald parameters is pinned by
+  * central differences of the GPU's own energy in stretched orthorhombic
+    boxes (diagonal components), at the production Ewald parameters
+    (eps_ewald = 1e-6) and at converged ones;
+  * all six components against central differences of an independent
+    periodic-cell energy (tests/ewald_bruteforce.py, itself pinned to the
+    oracle in test_virial_pin.py) at converged Ewald parameters, where the
+    virial no longer depends on how the Coulomb sum is split.
+Tolerance: 1e-6 of the largest component (the differences carry ~1e-9
+rounding and truncation error).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import ewald_bruteforce as bf
+from qtip4pf_water_amd import config as cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _system(P):
+    q0, box = cfg.lattice_water_box(4, 18.64, seed=21, jitter_bohr=0.1)
+    q = cfg.replicate_beads(q0, P)
+    if P > 1:
+        q = q + np.random.default_rng(121).normal(0, 0.03, q.shape)
+    return np.ascontiguousarray(q), box
+
+
+def _override(w):
+    d = w.derived
+    return {"kappa": d["kappa"], "rcut": d["rcut"], "k_cut": d["k_cut"], "k_max": d["k_max"]}
+
+
+def _gpu_energy(q0, box, eps, ew, oracle_mod):
+    from qtip4pf_water_amd import Water
+    o = oracle_mod.Oracle(64, 1, box)
+    point, lst, _ = o.all_lists(q0)
+    with Water(64, 1, box, eps_ewald=eps, ewald=ew) as w:
+        e, W = w.get_forces_virial(q0, np.zeros(o.n_dof), o.charge, point, lst, 0)
+    return e, W
+
+
+def _diag_fd(q0, box, eps, ew, h, oracle_mod):
+    out = np.zeros(3)
+    for a in range(3):
+        es = []
+        for sgn in (1.0, -1.0):
+            b2 = list(box)
+            b2[a] *= 1.0 + sgn * h
+            q2 = q0.reshape(-1, 3).copy()
+            q2[:, a] *= 1.0 + sgn * h
+            es.append(_gpu_energy(q2.ravel(), tuple(b2), eps, ew, oracle_mod)[0])
+        out[a] = -(es[0] - es[1]) / (2.0 * h)
+    return out
+
+
+@pytest.mark.parametrize("eps,h", [(1e-6, 1e-6), (1e-16, 1e-5)])
+def test_virial_diagonal_matches_energy_differences(oracle_mod, eps, h):
+    """No site pair of this configuration lies within 3.8e-5 (relative) of
+    r_c at eps 1e-6, so a strain of 1e-6 crosses no cutoff."""
+    from qtip4pf_water_amd import Water
+    say("virial diagonal FD", eps)
+    q0, box = _system(1)
+    with Water(64, 1, box, eps_ewald=eps) as w0:
+        ew = _override(w0)
+    e, W = _gpu_energy(q0, box, eps, ew, oracle_mod)
+    fd = _diag_fd(q0, box, eps, ew, h, oracle_mod)
+    scale = np.max(np.abs(W))
+    assert np.max(np.abs(np.diag(W) - fd)) <= 1e-6 * scale, (np.diag(W), fd)
+
+
+def test_virial_all_components_match_bruteforce(oracle_mod):
+    from qtip4pf_water_amd import Water
+    say("virial vs brute force")
+    q0, box = _system(1)
+    with Water(64, 1, box, eps_ewald=1e-16) as w0:
+        ew = _override(w0)
+    e, W = _gpu_energy(q0, box, 1e-16, ew, oracle_mod)
+    c = oracle_mod.consts()
+    Wb = bf.virial_fd(q0, box, c, ew["kappa"], ew["rcut"], ew["k_cut"], h=1e-5)
+    Wg = np.array([W[a, b] for a, b in bf.VOIGT])
+    say("W_gpu", Wg, "W_bf", Wb)
+    assert np.max(np.abs(Wg - Wb)) <= 1e-6 * np.max(np.abs(Wb))
+    eb = bf.energy(q0, box, c, ew["kappa"], ew["rcut"], ew["k_cut"])
+    assert abs(e - eb) <= 1e-10 * 23.0
+
+
+def test_resident_virial_matches_per_bead_calls(oracle_mod):
+    from qtip4pf_water_amd import Water
+    say("resident virial P=2")
+    P = 2
+    q, box = _system(P)
+    o = oracle_mod.Oracle(64, P, box)
+    point, lst, _ = o.all_lists(q)
+    p = cfg.sample_momenta(64, P, o.sys.beta_n, seed=5)
+    with Water(64, P, box) as w:
+        ref = [w.get_forces_virial(q, np.zeros(P * o.n_dof), o.charge, point, lst, b)
+               for b in range(P)]
+        w.set_state(q, p)
+        Wr, er = w.virial()
+    for b in range(P):
+        assert abs(er[b] - ref[b][0]) <= 1e-13 * 23.0
+        assert np.max(np.abs(Wr[b] - ref[b][1])) <= 1e-13 * np.max(np.abs(ref[b][1]))
+    # and the bead energies are the oracle's
+    for b in range(P):
+        e_or = o.get_forces(q, np.zeros(P * o.n_dof), point, lst, b)
+        assert abs(er[b] - e_or) <= 1e-12 * 23.0
