@@ -218,7 +218,11 @@ def ref_available() -> bool:
 def ref():
     global _ref
     if _ref is None:
-        L = C.CDLL(REF_SO)
+        # lazy binding: get_conserved's ftransform (fourier.cpp, not built) is
+        # never called and stays unresolved
+        L = C.CDLL(REF_SO, mode=os.RTLD_LAZY)
+        L.ref_get_kcv.restype = C.c_double
+        L.ref_get_kcv.argtypes = [_dp, _dp]
         L.ref_set_globals.argtypes = [C.c_int, C.c_int, _dp, C.c_int, C.c_double, C.c_double,
                                       C.c_double]
         L.ref_seed.argtypes = [C.c_uint]

@@ -1,5 +1,6 @@
 // ref_harness.cpp -- harness around the reference's own neighbour.cpp and
-// thermostat.cpp, compiled unmodified from the reference tree (see Makefile).
+// thermostat.cpp (and path_integrals.cpp's get_kcv), compiled unmodified
+// from the reference tree (see Makefile).
 //
 // TEST INFRASTRUCTURE ONLY.  The reference keeps its state in process globals
 // defined by water.cpp:29-38; water.cpp itself needs FFTW and Boost headers
@@ -15,6 +16,7 @@
 
 #include "qtip4pfparams.h"
 #include "neighbour.h"
+#include "path_integrals.h"
 #include "thermostat.h"
 #include "indexing.h"
 
@@ -50,6 +52,9 @@ void ref_set_globals(int n_mols_, int n_beads_, const double *boxl_, int list_si
 }
 
 void ref_seed(unsigned int seed) { g_rng.seed(seed); }
+
+// path_integrals.cpp:29-48 (reads n_beads, n_dof, beta)
+double ref_get_kcv(const double *q, const double *f) { return get_kcv(q, f); }
 
 // neighbour.cpp:12-62
 void ref_neighbour_list(const double *q, int *point, int *list, double *disp, int bead)

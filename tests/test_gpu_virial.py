@@ -66,13 +66,15 @@ def _diag_fd(q0, box, eps, ew, h, oracle_mod):
 
 @pytest.mark.parametrize("eps,h", [(1e-6, 1e-6), (1e-16, 1e-5)])
 def test_virial_diagonal_matches_energy_differences(oracle_mod, eps, h):
-    """No site pair of this configuration lies within 3.8e-5 (relative) of
-    r_c at eps 1e-6, so a strain of 1e-6 crosses no cutoff."""
+    """r_c is the derived L/2 pulled in by 2h so that it stays <= L/2 in the
+    compressed boxes.  No site pair of this configuration lies within 3.6e-5
+    (relative) of that r_c at eps 1e-6, so a strain of 1e-6 crosses no cutoff."""
     from qtip4pf_water_amd import Water
     say("virial diagonal FD", eps)
     q0, box = _system(1)
     with Water(64, 1, box, eps_ewald=eps) as w0:
         ew = _override(w0)
+    ew["rcut"] *= 1.0 - 2.0 * h
     e, W = _gpu_energy(q0, box, eps, ew, oracle_mod)
     fd = _diag_fd(q0, box, eps, ew, h, oracle_mod)
     scale = np.max(np.abs(W))

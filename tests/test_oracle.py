@@ -407,6 +407,77 @@ def test_free_ring_polymer_step_is_exact_cayley(oracle_mod):
         assert np.allclose(full, cay, atol=1e-12)
 
 
+@needs_ref
+def test_get_kcv_bit_exact_vs_reference(oracle_mod):
+    """path_integrals.cpp:29-48 compiled from the reference (even beads only,
+    centroid divided by float(n_beads), sum divided by n_beads)."""
+    q0, box = small_box(3, 9.4, 5)
+    for P in (2, 4, 8):
+        o = oracle_mod.Oracle(27, P, box)
+        oracle_mod.ref_set_from(o)
+        rng = np.random.default_rng(40 + P)
+        q = cfg.replicate_beads(q0, P) + rng.normal(0, 0.1, P * o.n_dof)
+        f = rng.normal(0, 0.05, P * o.n_dof)
+        kr = oracle_mod.ref().ref_get_kcv(oracle_mod._ptr(q), oracle_mod._ptr(f))
+        assert o.get_kcv(q, f) == kr
+
+
+def _nve_conserved_spreads(oracle_mod, dtfs, n_steps):
+    P = 8
+    q0, box = cfg.lattice_water_box(4, 18.64, seed=5)   # 64 molecules, L/2 > 9 A
+    o = oracle_mod.Oracle(64, P, box, dtfs=dtfs)
+    q = cfg.replicate_beads(q0, P)
+    p = 0.3 * cfg.sample_momenta(64, P, o.sys.beta_n, seed=2)
+    point, lst, disp = o.all_lists(q)
+    F, fph = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+    o.force_eval_sc(q, F, fph, point, lst, nthreads=8)
+    H, E, S, K = [], [], [], []
+    for _ in range(n_steps):
+        e, _, _, _ = o.update_sc(q, p, F, fph, point, lst, disp, False, None, 0.0, nthreads=8)
+        H.append(o.get_conserved(q, p, e))
+        E.append(e)
+        S.append(o.get_conserved(q, np.zeros_like(p), 0.0))   # spring energy alone
+        K.append(o.get_conserved(np.zeros_like(q), p, 0.0))   # normal-mode KE alone
+    return [float(np.ptp(np.array(x))) for x in (H, E, S, K)]
+
+
+def test_conserved_quantity_is_conserved_in_nve(oracle_mod):
+    """get_conserved (path_integrals.cpp:52-81) along NVE Suzuki-Chin steps:
+    H_RP = E_SC + KE_nm + V_spring is the integrator's conserved quantity.
+    Its spread stays below 1 % of the spread of each of its three terms (a
+    dropped or mis-weighted term shows at that term's scale) and falls as
+    dt^2 (ratio > 3 when dt halves), as a second-order splitting's must."""
+    h1, e1, s1, k1 = _nve_conserved_spreads(oracle_mod, 0.25, 80)
+    h2, _, _, _ = _nve_conserved_spreads(oracle_mod, 0.125, 160)
+    assert h1 < 1e-2 * min(e1, s1, k1), (h1, e1, s1, k1)
+    assert h1 / h2 > 3.0, (h1, h2)
+
+
+def _classical_spreads(oracle_mod, dtfs, n_steps):
+    q0, box = cfg.lattice_water_box(4, 18.64, seed=5)
+    o = oracle_mod.Oracle(64, 1, box, dtfs=dtfs)
+    q = q0.copy()
+    p = 0.3 * cfg.sample_momenta(64, 1, o.sys.beta, seed=2)
+    point, lst, disp = o.all_lists(q)
+    F = np.zeros(o.n_dof)
+    o.get_forces(q, F, point, lst, 0)
+    H, V = [], []
+    for _ in range(n_steps):
+        e, _ = o.update_class(q, p, F, point, lst, disp)
+        H.append(e + 0.5 * np.sum(p * p / o.mass))
+        V.append(e)
+    return float(np.ptp(H)), float(np.ptp(V))
+
+
+def test_classical_step_conserves_energy_to_second_order(oracle_mod):
+    """update_trajectory_class_nospline (update.cpp:78-137) is velocity
+    Verlet: V + KE is conserved to O(dt^2)."""
+    h1, v1 = _classical_spreads(oracle_mod, 0.25, 80)
+    h2, _ = _classical_spreads(oracle_mod, 0.125, 160)
+    assert h1 < 2e-3 * v1, (h1, v1)
+    assert 3.0 < h1 / h2 < 5.0, (h1, h2)
+
+
 # ------------------------------------------------------- golden vectors
 def _golden(name):
     import os
