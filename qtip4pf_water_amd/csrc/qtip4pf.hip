@@ -83,6 +83,9 @@ struct qtp_ctx {
     qtp_derived drv{};
     std::string err;
     hipStream_t stream = nullptr;
+    // second stream for the k-space half of a force phase (fork/join by events)
+    hipStream_t s2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
 
     int P = 0, n_mols = 0, n_atoms = 0, n_dof = 0, NAP = 0, NMP = 0, nblk = 0, list_size = 0;
     long fcap = 0;
@@ -439,6 +442,28 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     const long N3 = 3L * ctx->NAP;
     const double g = ctx->ff.gamma_vs, h = ctx->ff.half1mg;
     double *vp = vout ? ctx->vpart : nullptr;
+    // The k-space sums (ksum, sreduce) do not depend on the pair kernels: they
+    // run on the second stream, joined before kforce.  With per-kernel timing
+    // on, everything stays on one stream so each launch is timed alone.
+    const bool fork = !ctx->timing;
+    hipStream_t sk = fork ? ctx->s2 : s;
+    if (fork) {
+        HIPCHK(hipEventRecord(ctx->ev_fork, s));
+        HIPCHK(hipStreamWaitEvent(sk, ctx->ev_fork, 0));
+    }
+    {
+        TimeScope ts(ctx, "ksum");
+        k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, sk>>>(
+            pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->G,
+            ctx->AC, ctx->ngc, ctx->nac, ctx->spart);
+    }
+    {
+        TimeScope ts(ctx, "sreduce");
+        k_sreduce<<<E * ctx->ngr, 256, 0, sk>>>(ctx->spart, ctx->nac, ctx->G, ctx->groups, ctx->gcf,
+                                                 ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
+                                                 ctx->grk, 0.25 / ctx->ew.kappa2, vp);
+    }
+    if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
     {
         TimeScope ts(ctx, "ljintra");
         if (vout)
@@ -463,18 +488,7 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
                 ctx->ntile, ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real,
                 vp);
     }
-    {
-        TimeScope ts(ctx, "ksum");
-        k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, s>>>(
-            pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->G,
-            ctx->AC, ctx->ngc, ctx->nac, ctx->spart);
-    }
-    {
-        TimeScope ts(ctx, "sreduce");
-        k_sreduce<<<E * ctx->ngr, 256, 0, s>>>(ctx->spart, ctx->nac, ctx->G, ctx->groups, ctx->gcf,
-                                                ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
-                                                ctx->grk, 0.25 / ctx->ew.kappa2, vp);
-    }
+    if (fork) HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
     {
         TimeScope ts(ctx, "kforce");
         const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
@@ -829,9 +843,12 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->NMP = ((ctx->n_mols + 63) / 64) * 64;
     ctx->list_size = ctx->drv.list_size;
     ctx->fcap = 2L * ctx->list_size;
+    // the real-space erfc approximation covers kappa * r <= QTP_ERFCX_XMAX
+    // (derived parameters: kappa * r_c = sqrt(-ln eps_ewald) <= 6.2 for eps >= 2e-17)
     if (2.0 * ctx->drv.rcut > ctx->drv.min_box * (1.0 + 1e-12) || ctx->drv.k_max < 1 ||
-        ctx->drv.k_max > 250) {
-        ctx->err = "Ewald parameters out of range (r_c must be <= L/2, 1 <= k_max <= 250)";
+        ctx->drv.k_max > 250 || !(ctx->drv.kappa * ctx->drv.rcut <= QTP_ERFCX_XMAX)) {
+        ctx->err = "Ewald parameters out of range (r_c must be <= L/2, 1 <= k_max <= 250, "
+                   "kappa * r_c <= 6.2)";
         *out = ctx;
         return QTP_ERR_ARG;
     }
@@ -895,7 +912,9 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
 
     // a blocking stream: host-synchronous hipMemcpy/hipMemset on the null
     // stream are ordered with this stream's kernels
-    if (hipStreamCreate(&ctx->stream) != hipSuccess) {
+    if (hipStreamCreate(&ctx->stream) != hipSuccess || hipStreamCreate(&ctx->s2) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "hipStreamCreate failed";
         return QTP_ERR_HIP;
     }
@@ -941,6 +960,10 @@ void qtp_destroy(qtp_ctx *ctx)
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs};
     for (void *p : ptrs)
         if (p) hipFree(p);
+    if (ctx->s2) hipStreamSynchronize(ctx->s2);
+    if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
+    if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+    if (ctx->s2) hipStreamDestroy(ctx->s2);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
