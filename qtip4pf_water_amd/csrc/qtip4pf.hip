@@ -364,7 +364,8 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->vout, (size_t)E * 6));
     CHK(dalloc(ctx, &ctx->rpart, (size_t)E * ctx->nblk * N3));
     CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));
-    CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * 8));
+    CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));
+
     CHK(dalloc(ctx, &ctx->flj, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->Ve, E));
@@ -493,8 +494,8 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         TimeScope ts(ctx, "kforce");
         const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
         k_kforce<<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
-            pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->grk,
-            ctx->G, ctx->sprime, ctx->rpart, ctx->nblk, ctx->flj, fout);
+            pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
+            ctx->rpart, ctx->nblk, ctx->flj, fout);
     }
     if (eout) {
         TimeScope ts(ctx, "esum");
@@ -896,7 +897,7 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->AC = std::max(1, std::min(64, (int)(48 * 1024 / (3 * K * 16))));
     ctx->nac = (ctx->n_atoms + ctx->AC - 1) / ctx->AC;
     ctx->ksum_lds = (size_t)ctx->AC * 3 * K * sizeof(double2);
-    ctx->kforce_lds = (size_t)3 * K * 64 * sizeof(double2) + (size_t)KF_WAVES * 3 * 64 * sizeof(double);
+    ctx->kforce_lds = kforce_lds_bytes(K);
     std::vector<int2> tl;
     for (int I = 0; I < ctx->nblk; ++I)
         for (int J = I; J < ctx->nblk; ++J) tl.push_back(make_int2(I, J));

@@ -328,6 +328,8 @@ __device__ __forceinline__ void kw_table(double2 *row, double r, double bk, doub
     }
 }
 
+constexpr int KF_REC = 9;   // doubles per k-group record read by k_kforce
+
 // Reciprocal-space structure factors (kwald, forces.cpp:180-448), partial
 // over a chunk of AC sites.  A k-"group" (kx>=0, ky>=0, kz>=0) carries the
 // four sign variants (kx,+-ky,+-kz) of the reference's branches; lane <-> group.
@@ -383,8 +385,9 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 }
 
 // Sum the partial structure factors, energy per group (forces.cpp:274, 315,
-// 364, 428) and the force-ready S' = 2 c S of the variants the reference
-// evaluates (gr.w bit v set); others set to zero.   grid: E * ceil(G/256).
+// 364, 428) and the force record of k_kforce from S' = 2 c S of the variants
+// the reference evaluates (gr.w bit v set; others zero).
+// grid: E * ceil(G/256).
 // With vpart set, also the reciprocal-space virial tensor
 //   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 kappa^2)))
 // (E_k = c |S(k)|^2 per sign variant; the strain derivative of kwald's sum at
@@ -438,9 +441,22 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
                 v6[5] -= Ev * bb * ky * kz;
             }
         }
+        // force record of k_kforce (9 doubles): with S_v = 2c S(k_v) per variant,
+        // A side (variants 0, 1): c1 = S0i+S1i, c2 = S1r-S0r, c3 = -(S0r+S1r),
+        // c4 = S1i-S0i; B side (variants 2, 3) likewise from S2, S3; then the
+        // group's (kx, ky, kz) packed into the ninth slot
         const double c2 = 2.0 * cf;
-        double *o = sprime + ((long)e * G + g) * 8;
-        for (int v = 0; v < 8; ++v) o[v] = c2 * s[v];
+        double *o = sprime + ((long)e * G + g) * KF_REC;
+        for (int side = 0; side < 2; ++side) {
+            const double S0r = c2 * s[4 * side], S0i = c2 * s[4 * side + 1];
+            const double S1r = c2 * s[4 * side + 2], S1i = c2 * s[4 * side + 3];
+            o[4 * side + 0] = S0i + S1i;
+            o[4 * side + 1] = S1r - S0r;
+            o[4 * side + 2] = -(S0r + S1r);
+            o[4 * side + 3] = S1i - S0i;
+        }
+        const int4 gg = groups[g];
+        o[8] = __longlong_as_double((long long)(gg.x | (gg.y << 10) | (gg.z << 20)));
     }
     en = block_sum(en, sh);
     if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + gb] = en;
@@ -458,31 +474,63 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
 // 21 molecules (63 sites, lane <-> site; lane 63 idle).  The 8 waves split the
 // k-groups; S', k and the group table are read with wave-uniform addresses
 // (scalar loads).   grid: E * ceil(n_atoms/63), 512 threads.
-constexpr int KF_WAVES = 16;
+constexpr int KF_WAVES = 12;   // waves per block: two blocks fit one CU
 constexpr int KF_SITES = 63;
-__global__ __launch_bounds__(1024) void k_kforce(const double *__restrict__ pos, long pstride,
-                                                const double *__restrict__ charge, int n_atoms,
-                                                int NAP, double gamma, double half1mg, Ewald ew,
-                                                const int4 *__restrict__ groups,
-                                                const double4 *__restrict__ grk, int G,
-                                                const double *__restrict__ sprime,
-                                                const double *__restrict__ rpart, int nblk,
-                                                const double *__restrict__ flj,
-                                                double *__restrict__ fout)
+constexpr int KF_CHUNK = 48;   // k-group records staged per wave and round
+// LDS bytes of k_kforce for a K-entry factor table
+inline size_t kforce_lds_bytes(int K)
+{
+    const size_t cf = (size_t)KF_WAVES * KF_CHUNK * KF_REC;
+    const size_t red = (size_t)KF_WAVES * 3 * 64;
+    return (size_t)3 * K * 64 * 16 + (cf > red ? cf : red) * 8 + (size_t)3 * 64 * 8;
+}
+// Reciprocal-space forces and the final per-atom forces of E evaluations.
+// Block = 63 sites (21 whole molecules) x KF_WAVES waves; lanes <-> sites,
+// wave w takes a contiguous part of the k-groups.  Groups in build order
+// form columns of fixed (kx, ky) and rising kz.  For site j the group's
+// force is minus the k-weighted imaginary parts of S'_v* e^{i k_v r_j} over
+// its sign variants; with A = X Y, B = X Y* (per column) and Z the z factor
+// it is linear in Z:
+//   p01 = Ar (c1 Zr + c2 Zi) + Ai (c3 Zr + c4 Zi)          (fx, fy; variants 0, 1)
+//   m01 = Ar kz (-c4 Zr + c3 Zi) + Ai kz (c2 Zr - c1 Zi)   (fz)
+// and p23, m23 likewise with B and c5..c8.  A column accumulates the four
+// Z-linear sums per side over its groups and applies A, B, kx, ky once.
+// Group records (k_sreduce) are staged per wave in LDS with vector loads,
+// KF_CHUNK at a time; the real-space slot partials are loaded at kernel
+// start (wave w takes slots w, w + KF_WAVES, ...).  Epilogue: waves 0-2 (one
+// per dimension) sum the wave partials, apply the M-site chain rule
+// (forces.cpp:613-619) and add the LJ/intramolecular forces.
+// grid: E * ceil(n_atoms / 63), 64 * KF_WAVES threads, kforce_lds_bytes(K).
+__global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
+    const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
+    int NAP, double gamma, double half1mg, Ewald ew, int G, const double *__restrict__ krec,
+    const double *__restrict__ rpart, int nblk, const double *__restrict__ flj,
+    double *__restrict__ fout)
 {
     extern __shared__ double smem[];
     const int K = ew.K;
-    double2 *T = (double2 *)smem;        // [3][K][64]
-    double *red = smem + 2 * 3 * K * 64;  // [KF_WAVES][3][64]
+    double2 *T = (double2 *)smem;           // [3][K][64]
+    double *CF = smem + 2 * 3 * K * 64;     // [KF_WAVES][KF_CHUNK][KF_REC]; later red [W][3][64]
+    const size_t cfn = (size_t)KF_WAVES * KF_CHUNK * KF_REC;
+    const size_t redn = (size_t)KF_WAVES * 3 * 64;
+    double *ef = CF + (cfn > redn ? cfn : redn);  // [3][64]
     const int nab = (n_atoms + KF_SITES - 1) / KF_SITES;
     const int e = blockIdx.x / nab, ab = blockIdx.x - e * nab;
     const int lane = threadIdx.x & 63;
-    // wave index made explicitly uniform: the group loop below then runs on
-    // scalar control with scalar (SMEM) loads of the group data
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int atom = ab * KF_SITES + lane;
     const bool va = lane < KF_SITES && atom < n_atoms;
     const double *X = pos + (long)e * pstride;
+    double fx = 0.0, fy = 0.0, fz = 0.0, fl = 0.0;
+    if (va) {
+        for (int sl = w; sl < nblk; sl += KF_WAVES) {
+            const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + atom;
+            fx += rp[0];
+            fy += rp[NAP];
+            fz += rp[2 * NAP];
+        }
+        if (w < 3) fl = flj[((long)e * 3 + w) * NAP + atom];
+    }
     if (threadIdx.x < 192) {
         const int d = threadIdx.x >> 6;
         if (va)
@@ -491,86 +539,86 @@ __global__ __launch_bounds__(1024) void k_kforce(const double *__restrict__ pos,
         else
             for (int m = 0; m < K; ++m) T[(d * K + m) * 64 + lane] = make_double2(0.0, 0.0);
     }
-    __syncthreads();
     const int per = (G + KF_WAVES - 1) / KF_WAVES;
     const int g0 = w * per, g1 = min(G, g0 + per);
-    const double4 *S4 = (const double4 *)(sprime + (long)e * G * 8);
-    double fx = 0.0, fy = 0.0, fz = 0.0;
+    const int rounds = (per + KF_CHUNK - 1) / KF_CHUNK;
+    double *cw = CF + (size_t)w * KF_CHUNK * KF_REC;
+    const double *R = krec + (long)e * G * KF_REC;
+    double Ar = 0.0, Ai = 0.0, Br = 0.0, Bi = 0.0, kxr = 0.0, kyr = 0.0;
+    double u1 = 0.0, u2 = 0.0, v1 = 0.0, v2 = 0.0, u3 = 0.0, u4 = 0.0, v3 = 0.0, v4 = 0.0;
     int pkxy = -1;
-    double Ar = 0.0, Ai = 0.0, Br = 0.0, Bi = 0.0;
-    int4 gr = groups[min(g0, G - 1)];
-    for (int g = g0; g < g1; ++g) {
-        // one batch of scalar loads per group; the next group's indices prefetched
-        const double4 sa = S4[2 * g], sb = S4[2 * g + 1];
-        const double4 rk = grk[g];
-        const int4 grn = groups[min(g + 1, G - 1)];
-        const int kxy = gr.x | (gr.y << 8);
-        if (kxy != pkxy) {
-            const double2 Xv = T[(0 * K + gr.x) * 64 + lane];
-            const double2 Yv = T[(1 * K + gr.y) * 64 + lane];
-            const double cc = Xv.x * Yv.x, csn = Xv.x * Yv.y, sc = Xv.y * Yv.x, ss = Xv.y * Yv.y;
-            Ar = cc - ss;
-            Ai = csn + sc;
-            Br = cc + ss;
-            Bi = sc - csn;
-            pkxy = kxy;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int c0 = g0 + rd * KF_CHUNK;
+        const int nc = max(0, min(KF_CHUNK, g1 - c0));
+        for (int t = lane; t < nc * KF_REC; t += 64) cw[t] = R[(long)c0 * KF_REC + t];
+        __syncthreads();  // tables (first round) and this round's records
+        for (int i = 0; i < nc; ++i) {
+            const double *c = cw + i * KF_REC;
+            // the same for every lane: made scalar so the column logic is uniform
+            const int tag = __builtin_amdgcn_readfirstlane((int)__double_as_longlong(c[8]));
+            const int kx = tag & 1023, ky = (tag >> 10) & 1023, kz = tag >> 20;
+            const int kxy = kx | (ky << 10);
+            if (kxy != pkxy) {
+                // close the previous column
+                const double p01 = fma(Ar, u1, Ai * u2), m01 = fma(Ar, v1, Ai * v2);
+                const double p23 = fma(Br, u3, Bi * u4), m23 = fma(Br, v3, Bi * v4);
+                fx = fma(-kxr, p01 + p23, fx);
+                fy = fma(-kyr, p01 - p23, fy);
+                fz -= m01 + m23;
+                u1 = u2 = v1 = v2 = u3 = u4 = v3 = v4 = 0.0;
+                const double2 Xv = T[(0 * K + kx) * 64 + lane];
+                const double2 Yv = T[(1 * K + ky) * 64 + lane];
+                const double cc = Xv.x * Yv.x, csn = Xv.x * Yv.y, sc = Xv.y * Yv.x,
+                             ss = Xv.y * Yv.y;
+                Ar = cc - ss;
+                Ai = csn + sc;
+                Br = cc + ss;
+                Bi = sc - csn;
+                kxr = ew.bk[0] * kx;
+                kyr = ew.bk[1] * ky;
+                pkxy = kxy;
+            }
+            const double2 Zv = T[(2 * K + kz) * 64 + lane];
+            const double kzr = ew.bk[2] * kz;
+            const double zkr = kzr * Zv.x, zki = kzr * Zv.y;
+            u1 = fma(c[0], Zv.x, fma(c[1], Zv.y, u1));
+            u2 = fma(c[2], Zv.x, fma(c[3], Zv.y, u2));
+            v1 = fma(-c[3], zkr, fma(c[2], zki, v1));
+            v2 = fma(c[1], zkr, fma(-c[0], zki, v2));
+            if (ky > 0) {  // B-side variants exist only off the ky = 0 plane
+                u3 = fma(c[4], Zv.x, fma(c[5], Zv.y, u3));
+                u4 = fma(c[6], Zv.x, fma(c[7], Zv.y, u4));
+                v3 = fma(-c[7], zkr, fma(c[6], zki, v3));
+                v4 = fma(c[5], zkr, fma(-c[4], zki, v4));
+            }
         }
-        const double2 Zv = T[(2 * K + gr.z) * 64 + lane];
-        // variant 0: A Z ; 1: A Z* ; 2: B Z ; 3: B Z*  (gr.w: variants the reference sums)
-        const double e0r = fma(Ar, Zv.x, -Ai * Zv.y), e0i = fma(Ai, Zv.x, Ar * Zv.y);
-        const double t0 = fma(sa.y, e0r, -sa.x * e0i);
-        double t1 = 0.0, t2 = 0.0, t3 = 0.0;
-        if (gr.w & 2) {
-            const double e1r = fma(Ar, Zv.x, Ai * Zv.y), e1i = fma(Ai, Zv.x, -Ar * Zv.y);
-            t1 = fma(sa.w, e1r, -sa.z * e1i);
-        }
-        if (gr.w & 4) {
-            const double e2r = fma(Br, Zv.x, -Bi * Zv.y), e2i = fma(Bi, Zv.x, Br * Zv.y);
-            t2 = fma(sb.y, e2r, -sb.x * e2i);
-        }
-        if (gr.w & 8) {
-            const double e3r = fma(Br, Zv.x, Bi * Zv.y), e3i = fma(Bi, Zv.x, -Br * Zv.y);
-            t3 = fma(sb.w, e3r, -sb.z * e3i);
-        }
-        const double p01 = t0 + t1, p23 = t2 + t3, m01 = t0 - t1, m23 = t2 - t3;
-        fx = fma(-rk.x, p01 + p23, fx);
-        fy = fma(-rk.y, p01 - p23, fy);
-        fz = fma(-rk.z, m01 + m23, fz);
-        gr = grn;
+        __syncthreads();  // records consumed before the next round (or red) overwrites them
     }
+    if (rounds == 0) __syncthreads();
+    {
+        const double p01 = fma(Ar, u1, Ai * u2), m01 = fma(Ar, v1, Ai * v2);
+        const double p23 = fma(Br, u3, Bi * u4), m23 = fma(Br, v3, Bi * v4);
+        fx = fma(-kxr, p01 + p23, fx);
+        fy = fma(-kyr, p01 - p23, fy);
+        fz -= m01 + m23;
+    }
+    double *red = CF;  // [KF_WAVES][3][64]
     red[(w * 3 + 0) * 64 + lane] = fx;
     red[(w * 3 + 1) * 64 + lane] = fy;
     red[(w * 3 + 2) * 64 + lane] = fz;
     __syncthreads();
-    // wave 0: total Ewald site force, then the chain rule within the block
-    double efd[3] = {0.0, 0.0, 0.0};
-    if (w == 0 && va) {
-        for (int d = 0; d < 3; ++d) {
-            double v = 0.0;
-            for (int ww = 0; ww < KF_WAVES; ++ww) v += red[(ww * 3 + d) * 64 + lane];
-            const double *rp = rpart + (long)e * nblk * 3 * NAP + d * NAP + atom;
-            for (int sl = 0; sl < nblk; ++sl) v += rp[(long)sl * 3 * NAP];
-            efd[d] = v;
-        }
+    if (w < 3) {  // wave d: total Ewald site force along d
+        double v = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < KF_WAVES; ++ww) v += red[(ww * 3 + w) * 64 + lane];
+        ef[w * 64 + lane] = v;
     }
     __syncthreads();
-    double *ef = red;  // [3][64]
-    if (w == 0) {
-        for (int d = 0; d < 3; ++d) ef[d * 64 + lane] = efd[d];
-    }
-    __syncthreads();
-    if (w == 0 && va) {
+    if (w < 3 && va) {
         const int site = lane % 3;  // 63 = 21 molecules: lane%3 == atom%3
-        const int lo = lane - site;
-        for (int d = 0; d < 3; ++d) {
-            const long idx = ((long)e * 3 + d) * NAP + atom;
-            double v;
-            if (site == 0)
-                v = flj[idx] + gamma * ef[d * 64 + lane];
-            else
-                v = flj[idx] + (ef[d * 64 + lane] + half1mg * ef[d * 64 + lo]);
-            fout[idx] = v;
-        }
+        const double v = site == 0 ? fl + gamma * ef[w * 64 + lane]
+                                   : fl + (ef[w * 64 + lane] + half1mg * ef[w * 64 + lane - site]);
+        fout[((long)e * 3 + w) * NAP + atom] = v;
     }
 }
 
