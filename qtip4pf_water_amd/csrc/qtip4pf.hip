@@ -443,9 +443,10 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     const long N3 = 3L * ctx->NAP;
     const double g = ctx->ff.gamma_vs, h = ctx->ff.half1mg;
     double *vp = vout ? ctx->vpart : nullptr;
-    // The k-space sums (ksum, sreduce) do not depend on the pair kernels: they
-    // run on the second stream, joined before kforce.  With per-kernel timing
-    // on, everything stays on one stream so each launch is timed alone.
+    // The k-space sums (ksum, sreduce) and the LJ/intramolecular kernel do not
+    // depend on the real-space kernel: they run on the second stream, joined
+    // before kforce.  With per-kernel timing on, everything stays on one
+    // stream so each launch is timed alone.
     const bool fork = !ctx->timing;
     hipStream_t sk = fork ? ctx->s2 : s;
     if (fork) {
@@ -464,18 +465,18 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
                                                  ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
                                                  ctx->grk, 0.25 / ctx->ew.kappa2, vp);
     }
-    if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
     {
         TimeScope ts(ctx, "ljintra");
         if (vout)
-            k_ljintra<true><<<E * ctx->nmb, 256, 0, s>>>(
+            k_ljintra<true><<<E * ctx->nmb, 256, 0, sk>>>(
                 pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, ctx->flist, ctx->fcap, lmode, lbase,
                 ctx->box, ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride, ctx->ep_lj, vp);
         else
-            k_ljintra<false><<<E * ctx->nmb, 256, 0, s>>>(
+            k_ljintra<false><<<E * ctx->nmb, 256, 0, sk>>>(
                 pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, ctx->flist, ctx->fcap, lmode, lbase,
                 ctx->box, ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride, ctx->ep_lj, vp);
     }
+    if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
     {
         TimeScope ts(ctx, "realspace");
         if (vout)

@@ -174,11 +174,16 @@ __device__ __forceinline__ double erfc_via_erfcx(double x, double ex)
 
 // Real-space Ewald over all site pairs i<j within r_c (forces.cpp:586-608,
 // rwald :141-177).  One 256-thread block per 64x64 tile (I<=J) of the site
-// pair matrix; wave w takes the 16 j-columns 16w..16w+15 against all 64 i
-// (lane <-> i).  Pass 1 marks the in-cutoff columns of each lane (16-bit
-// mask); pass 2 pops the set bits, so lanes evaluate only in-cutoff pairs.
-// Reaction forces accumulate in LDS with ds_add_f64; each column belongs to
-// one wavefront, so the accumulation order is fixed for given inputs.
+// pair matrix; wave w takes the 16 j-columns 16w..16w+15 against all 64 i.
+// Pass 1 (lane <-> i) marks each lane's in-cutoff columns (16-bit mask).
+// The wave then lays its in-cutoff pairs out in an LDS queue, lane-major
+// (offsets by a wave prefix sum of the mask popcounts), and pass 2 gives
+// every lane the same number of consecutive queue entries: lanes stay busy
+// however unevenly the pairs fall on the rows.  Consecutive entries mostly
+// share i, so a lane keeps its i-force in registers and adds it to the
+// wave's LDS row when i changes; j reactions go to the LDS column array with
+// ds_add_f64.  All LDS adds of a row or column come from one wavefront, in
+// program order and fixed lane order, so the result is reproducible.
 // Partial site forces go to slot J (block I's sites) and slot I (block J's
 // sites): every (site, slot) has exactly one writer.
 // grid: E * ntile, 256 threads.
@@ -199,67 +204,95 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
     const int I = tiles[t].x, J = tiles[t].y;
     const int l = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    __shared__ double wv[RS_WAVES][6];
     const double *X = pos + (long)e * pstride;
-    __shared__ double sx[64], sy[64], sz[64], sq[64];
+    __shared__ double4 sI[64], sJ[64];  // x, y, z, charge
     __shared__ double fj[3][64];
     __shared__ double fi[RS_WAVES][3][64];
+    __shared__ unsigned short qe[RS_WAVES][64 * 16];
     __shared__ double ewv[RS_WAVES];
-    if (w == 0) {
-        const int jj = J * 64 + l;
-        const bool vj = jj < n_atoms;
-        sx[l] = vj ? site_coord(X, NAP, jj, 0, gamma, half1mg) : 0.0;
-        sy[l] = vj ? site_coord(X, NAP, jj, 1, gamma, half1mg) : 0.0;
-        sz[l] = vj ? site_coord(X, NAP, jj, 2, gamma, half1mg) : 0.0;
-        sq[l] = vj ? charge[jj] : 0.0;
-        fj[0][l] = 0.0;
-        fj[1][l] = 0.0;
-        fj[2][l] = 0.0;
+    __shared__ double wv[RS_WAVES][6];
+    if (w < 2) {
+        const int s = (w == 0 ? J : I) * 64 + l;
+        double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (s < n_atoms)
+            v = make_double4(site_coord(X, NAP, s, 0, gamma, half1mg),
+                             site_coord(X, NAP, s, 1, gamma, half1mg),
+                             site_coord(X, NAP, s, 2, gamma, half1mg), charge[s]);
+        (w == 0 ? sJ : sI)[l] = v;
+        if (w == 0) {
+            fj[0][l] = 0.0;
+            fj[1][l] = 0.0;
+            fj[2][l] = 0.0;
+        }
     }
-    const int i = I * 64 + l;
-    const bool vi = i < n_atoms;
-    double xi = 0.0, yi = 0.0, zi = 0.0, qi = 0.0;
-    if (vi) {
-        xi = site_coord(X, NAP, i, 0, gamma, half1mg);
-        yi = site_coord(X, NAP, i, 1, gamma, half1mg);
-        zi = site_coord(X, NAP, i, 2, gamma, half1mg);
-        qi = charge[i];
-    }
+    fi[w][0][l] = 0.0;
+    fi[w][1][l] = 0.0;
+    fi[w][2][l] = 0.0;
     __syncthreads();
     const int c0 = 16 * w;
-    // pass 1: in-cutoff columns (j > i; j < n_atoms)
+    const int i = I * 64 + l;
+    // pass 1: in-cutoff columns of row i (j > i; j < n_atoms)
     unsigned int mask = 0u;
-    if (vi) {
+    if (i < n_atoms) {
+        const double4 a = sI[l];
         for (int c = 0; c < 16; ++c) {
             const int jl = c0 + c;
             const int j = J * 64 + jl;
-            const double dx = mimg_fast(xi - sx[jl], box.L[0], box.inv[0]);
-            const double dy = mimg_fast(yi - sy[jl], box.L[1], box.inv[1]);
-            const double dz = mimg_fast(zi - sz[jl], box.L[2], box.inv[2]);
+            const double4 b = sJ[jl];
+            const double dx = mimg_fast(a.x - b.x, box.L[0], box.inv[0]);
+            const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
+            const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
             const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
             if (j > i && j < n_atoms && rsq < ew.rcut_sq) mask |= 1u << c;
         }
     }
-    // pass 2: in-cutoff pairs only.  Lane l visits its columns starting at
-    // l mod 16, so at most 4 lanes of a wave add into one LDS column at a time.
-    const int mi = i / 3;
+    // the wave's pair queue, lane-major
+    const int pc = __popc(mask);
+    int incl = pc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (l >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    {
+        int pos_q = incl - pc;
+        unsigned int m = mask;
+        while (m) {
+            const int c = __ffs(m) - 1;
+            m &= m - 1u;
+            qe[w][pos_q++] = (unsigned short)(l | (c << 6));
+        }
+    }
+    __syncthreads();
+    // pass 2: lane l takes queue entries [l*S, l*S + S)
+    const int S = (total + 63) >> 6;
+    const int q0 = min(l * S, total), q1 = min(q0 + S, total);
     double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
     double vxx = 0.0, vyy = 0.0, vzz = 0.0, vxy = 0.0, vxz = 0.0, vyz = 0.0;
-    const int rot = l & 15;
-    mask = ((mask >> rot) | (mask << (16 - rot))) & 0xffffu;
-    while (mask) {
-        const int c = (__ffs(mask) - 1 + rot) & 15;
-        mask &= mask - 1u;
-        const int jl = c0 + c;
-        const double dx = mimg_fast(xi - sx[jl], box.L[0], box.inv[0]);
-        const double dy = mimg_fast(yi - sy[jl], box.L[1], box.inv[1]);
-        const double dz = mimg_fast(zi - sz[jl], box.L[2], box.inv[2]);
+    int icur = -1;
+    for (int q = q0; q < q1; ++q) {
+        const int ent = qe[w][q];
+        const int il = ent & 63, jl = c0 + (ent >> 6);
+        if (il != icur) {
+            if (icur >= 0) {
+                atomicAdd(&fi[w][0][icur], fx);
+                atomicAdd(&fi[w][1][icur], fy);
+                atomicAdd(&fi[w][2][icur], fz);
+            }
+            fx = fy = fz = 0.0;
+            icur = il;
+        }
+        const double4 a = sI[il];
+        const double4 b = sJ[jl];
+        const double dx = mimg_fast(a.x - b.x, box.L[0], box.inv[0]);
+        const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
+        const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
         const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
         const double rinv = rsqrt(rsq);
         const double ex = exp(-ew.kappa2 * rsq);
         double ec = erfc_via_erfcx(ew.kappa * (rsq * rinv), ex);
-        if (mi == (J * 64 + jl) / 3) ec -= 1.0;  // same molecule: -erf (forces.cpp:158-166)
-        const double qq = qi * sq[jl];
+        if ((I * 64 + il) / 3 == (J * 64 + jl) / 3) ec -= 1.0;  // same molecule: -erf (forces.cpp:158-166)
+        const double qq = a.w * b.w;
         const double ep = qq * rinv * ec;
         const double pf = (ep + qq * ew.pref2k * ex) * (rinv * rinv);
         en += ep;
@@ -278,9 +311,11 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
         atomicAdd(&fj[1][jl], -pf * dy);
         atomicAdd(&fj[2][jl], -pf * dz);
     }
-    fi[w][0][l] = fx;
-    fi[w][1][l] = fy;
-    fi[w][2][l] = fz;
+    if (icur >= 0) {
+        atomicAdd(&fi[w][0][icur], fx);
+        atomicAdd(&fi[w][1][icur], fy);
+        atomicAdd(&fi[w][2][icur], fz);
+    }
     en = wave_sum(en);
     if (l == 0) ewv[w] = en;
     if (VIR) {
