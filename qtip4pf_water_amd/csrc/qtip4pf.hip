@@ -528,13 +528,13 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
     SCArgs a = sc_args(ctx);
     {
         TimeScope ts(ctx, "sc1");
-        k_sc1<<<P, 256, 0, s>>>(a, ctx->q, ctx->fph, ctx->epart, ctx->ep_stride, ctx->eself,
+        k_sc1<<<P, SC_THREADS, 0, s>>>(a, ctx->q, ctx->fph, ctx->epart, ctx->ep_stride, ctx->eself,
                                 ctx->Ve, ctx->m_atom, ctx->F, ctx->qpm, ctx->scal);
     }
     CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr));
     {
         TimeScope ts(ctx, "sc2");
-        k_sc2<<<P, 256, 0, s>>>(a, ctx->fph, ctx->fpm, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
+        k_sc2<<<P, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
                                 ctr);
     }
     HIPCHK(hipGetLastError());

@@ -840,13 +840,19 @@ struct SCArgs {
 //  even b: F_sc = 2 f / 3; tot_b = 2V; vop_b = V
 //  odd  b: tot_b = 4V, fsq_b = sum f^2/m / (3 w^2); delta and the displaced
 //          positions q +- eps f/m for the finite-difference phase.
-__global__ __launch_bounds__(256) void k_sc1(SCArgs a, const double *__restrict__ q,
-                                             const double *__restrict__ fph,
-                                             const double *__restrict__ epart, int ep_stride,
-                                             double eself, double *__restrict__ Vout,
-                                             const double *__restrict__ m_atom,
-                                             double *__restrict__ Fsc, double *__restrict__ qpm,
-                                             double *__restrict__ scal /*[P][4]*/)
+// Suzuki-Chin combination, first half (forces.cpp:738-775), one 1024-thread
+// block per bead: bead energy; even beads F_sc = 2f/3; odd beads |f|^2_M,
+// the finite-difference step eps / sqrt(sum (f/m)^2 / (n_atoms P)) and the
+// displaced positions q +- step f/m of the second force phase.
+constexpr int SC_THREADS = 1024;
+__global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__restrict__ q,
+                                                    const double *__restrict__ fph,
+                                                    const double *__restrict__ epart, int ep_stride,
+                                                    double eself, double *__restrict__ Vout,
+                                                    const double *__restrict__ m_atom,
+                                                    double *__restrict__ Fsc,
+                                                    double *__restrict__ qpm,
+                                                    double *__restrict__ scal /*[P][4]*/)
 {
     __shared__ double sh[16];
     __shared__ double s_scale;
@@ -863,7 +869,7 @@ __global__ __launch_bounds__(256) void k_sc1(SCArgs a, const double *__restrict_
     __syncthreads();
     const double V = s_V;
     if ((b & 1) == 0) {
-        for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) Fsc[o + t] = 2.0 * fph[o + t] / 3.0;
+        for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) Fsc[o + t] = 2.0 * fph[o + t] / 3.0;
         if (threadIdx.x == 0) {
             scal[b * 4 + 0] = 2.0 * V;
             scal[b * 4 + 1] = V;
@@ -873,10 +879,9 @@ __global__ __launch_bounds__(256) void k_sc1(SCArgs a, const double *__restrict_
         return;
     }
     double fsq = 0.0, del = 0.0;
-    for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) {
-        const int at = t % a.NAP;
-        if (at < a.n_atoms) {
-            const double f = fph[o + t];
+    for (int d = 0; d < 3; ++d) {
+        for (int at = threadIdx.x; at < a.n_atoms; at += SC_THREADS) {
+            const double f = fph[o + d * a.NAP + at];
             const double m = m_atom[at];
             fsq += f * f / m;
             const double fm = f / m;
@@ -901,24 +906,24 @@ __global__ __launch_bounds__(256) void k_sc1(SCArgs a, const double *__restrict_
     const int k = b >> 1;
     double *Pp = qpm + (long)(2 * k) * 3 * a.NAP;
     double *Pm = qpm + (long)(2 * k + 1) * 3 * a.NAP;
-    for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) {
-        const int at = t % a.NAP;
-        const double dq = sc_scale * fph[o + t] / m_atom[at];
-        Pp[t] = q[o + t] + dq;
-        Pm[t] = q[o + t] - dq;
+    for (int d = 0; d < 3; ++d) {
+        for (int at = threadIdx.x; at < a.NAP; at += SC_THREADS) {
+            const long t = (long)d * a.NAP + at;
+            const double dq = sc_scale * fph[o + t] / m_atom[at];
+            Pp[t] = q[o + t] + dq;
+            Pm[t] = q[o + t] - dq;
+        }
     }
 }
-
-// After the displaced-force phase: odd-bead F_sc = 4 f/3 - fscale (f+ - f-),
-// optional final half kick p += dt/2 F_sc (update.cpp:391-397), and (block 0)
-// the bead-ordered reduction of the SC potential and sc_pot (forces.cpp:780-781).
-// grid: P, 256 threads.   out: [0] tot_energy, [1] sc_pot
-__global__ __launch_bounds__(256) void k_sc2(SCArgs a, const double *__restrict__ fph,
-                                             const double *__restrict__ fpm,
-                                             const double *__restrict__ scal,
-                                             double *__restrict__ Fsc, double *__restrict__ p,
-                                             int kick, double *__restrict__ out,
-                                             unsigned long long *__restrict__ step_ctr)
+// Second half (forces.cpp:776-789): odd beads F_sc = 4f/3 - (f+ - f-) / (9 w^2
+// step); the half-step kick p += dt/2 F_sc when requested; block 0 sums the
+// SC potential and sc_pot in bead order.   grid: P, SC_THREADS threads.
+__global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__restrict__ fph,
+                                                    const double *__restrict__ fpm,
+                                                    const double *__restrict__ scal,
+                                                    double *__restrict__ Fsc, double *__restrict__ p,
+                                                    int kick, double *__restrict__ out,
+                                                    unsigned long long *__restrict__ step_ctr)
 {
     const int b = blockIdx.x;
     const long o = (long)b * 3 * a.NAP;
@@ -927,14 +932,14 @@ __global__ __launch_bounds__(256) void k_sc2(SCArgs a, const double *__restrict_
         const double fscale = scal[b * 4 + 3];
         const double *fp = fpm + (long)(2 * k) * 3 * a.NAP;
         const double *fm = fpm + (long)(2 * k + 1) * 3 * a.NAP;
-        for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) {
+        for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) {
             double F = 4.0 * fph[o + t] / 3.0;
             F -= fscale * (fp[t] - fm[t]);
             Fsc[o + t] = F;
             if (kick) p[o + t] += a.halfdt * F;
         }
     } else if (kick) {
-        for (int t = threadIdx.x; t < 3 * a.NAP; t += 256) p[o + t] += a.halfdt * Fsc[o + t];
+        for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) p[o + t] += a.halfdt * Fsc[o + t];
     }
     if (b == 0 && threadIdx.x == 0) {
         double tot = 0.0, vop = 0.0;
