@@ -171,6 +171,21 @@ int qtp_set_seed(qtp_ctx *ctx, uint64_t seed);
 int qtp_run(qtp_ctx *ctx, int n_steps, int is_nvt);
 int qtp_sync(qtp_ctx *ctx);
 
+/* Batched replicas (no counterpart in the reference, which runs one system
+ * per process): n contexts, each with its own size, bead count, temperature
+ * and resident state (qtp_set_state done), advanced together.  Each replica
+ * keeps its own captured step graph and streams; qtp_batch_run queues every
+ * replica's step, replica after replica, step after step, so the GPU
+ * overlaps replicas.  is_nvt[i] (NULL: all NVE) selects PILE-G per replica.
+ * While a batch runs, call no other function on its contexts before
+ * qtp_batch_sync; the batch does not own the contexts (destroy it first). */
+typedef struct qtp_batch qtp_batch;
+int qtp_batch_create(qtp_ctx *const *ctxs, const int *is_nvt, int n, qtp_batch **out);
+int qtp_batch_run(qtp_batch *b, int n_steps);
+int qtp_batch_sync(qtp_batch *b);
+const char *qtp_batch_last_error(const qtp_batch *b);
+void qtp_batch_destroy(qtp_batch *b);
+
 /* Scalars of the last resident step: [0] energy (SC conserved potential or
  * classical V), [1] sc_pot, [2] accumulated heat (update.cpp:325 sign),
  * [3] list rebuilds so far, [4] steps so far, [5] status flags. */

@@ -80,6 +80,11 @@ SIGNATURES = [
     ("qtp_set_seed", C.c_int, [_vp, C.c_uint64]),
     ("qtp_run", C.c_int, [_vp, C.c_int, C.c_int]),
     ("qtp_sync", C.c_int, [_vp]),
+    ("qtp_batch_create", C.c_int, [C.POINTER(_vp), ip, C.c_int, C.POINTER(_vp)]),
+    ("qtp_batch_run", C.c_int, [_vp, C.c_int]),
+    ("qtp_batch_sync", C.c_int, [_vp]),
+    ("qtp_batch_last_error", C.c_char_p, [_vp]),
+    ("qtp_batch_destroy", None, [_vp]),
     ("qtp_get_scalars", C.c_int, [_vp, dp]),
     ("qtp_enable_timing", C.c_int, [_vp, C.c_int]),
     ("qtp_get_timing", C.c_int, [_vp, C.c_char_p, dp, ip]),

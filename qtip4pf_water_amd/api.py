@@ -277,3 +277,54 @@ class Water:
         ms, n = C.c_double(), C.c_int()
         self._chk(self._L.qtp_get_timing(self._h, kernel.encode(), C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Batch:
+    """Independent replicas (Water objects with resident state) advanced
+    together (qtp_batch_*): every replica's step graph is queued on its own
+    streams so the GPU overlaps them.  The Water objects must outlive the
+    batch; call sync() before touching them again."""
+
+    def __init__(self, waters, is_nvt=None):
+        self._L = _lib.load()
+        self.waters = list(waters)
+        n = len(self.waters)
+        arr = (C.c_void_p * n)(*[w._h for w in self.waters])
+        nvt = None
+        if is_nvt is not None:
+            flags = [int(bool(is_nvt))] * n if np.isscalar(is_nvt) else [int(bool(x)) for x in is_nvt]
+            nvt = np.ascontiguousarray(flags, dtype=np.int32)
+        h = C.c_void_p()
+        rc = self._L.qtp_batch_create(arr, ptr(nvt, ip) if nvt is not None else None, n, C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self._L.qtp_batch_last_error(h).decode() if h else "qtp_batch_create"
+            self.close()
+            raise QtpError(rc, msg)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise QtpError(rc, self._L.qtp_batch_last_error(self._h).decode())
+
+    def run(self, n_steps: int):
+        self._chk(self._L.qtp_batch_run(self._h, int(n_steps)))
+
+    def sync(self):
+        self._chk(self._L.qtp_batch_sync(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.qtp_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
