@@ -10,6 +10,8 @@ evaluations plus propagation, thermostat and list checks, state resident in
 HBM, replayed as one HIP graph per step.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
+    python bench.py --workload batch [--replicas 64] [--steps K]   # batched replicas
+    python bench.py --workload c1                                  # classical 216 NVE
 
 Prints one JSON line.  Needs the built library (python -m qtip4pf_water_amd.build).
 """
@@ -92,12 +94,78 @@ def cpu_baseline(wl, q, p, box, n_mols, seconds):
                       f"{el:.1f} s"}
 
 
+# 64 independent replicas (qtp_batch_*): molecule counts from {64, ..., 512}
+# at 0.997 g/cm^3 (cubic boxes; lattice sites, 128 of a 6^3 lattice for 128
+# molecules), bead counts from {8, 16, 32, 64},
+# temperatures uniform in 250-350 K, NVT, one master seed.
+BATCH_SIZES = {64: 12.43, 128: 15.66, 216: 18.64, 343: 21.75, 512: 24.86}
+
+
+def batch_replicas(n_rep, seed):
+    rng = np.random.default_rng(seed)
+    reps = []
+    for r in range(n_rep):
+        n = int(rng.choice(sorted(BATCH_SIZES)))
+        reps.append(dict(n_mols=n, L_angs=BATCH_SIZES[n], P=int(rng.choice([8, 16, 32, 64])),
+                         T_K=float(rng.uniform(250.0, 350.0)), seed=int(seed * 1000 + r)))
+    return reps
+
+
+def run_batch(a):
+    """Replica-steps/s of a batch; value = total replica steps per second."""
+    from qtip4pf_water_amd import Batch, Water
+    reps = batch_replicas(a.replicas, 2024)
+    ws = []
+    for r in reps:
+        q0, box = cfg.water_box(r["n_mols"], r["L_angs"], seed=r["seed"])
+        beta_n = 1.0 / (cfg.K2AU * r["T_K"]) / r["P"]
+        q = cfg.free_ring_polymer_beads(q0, r["P"], beta_n, seed=r["seed"] + 1)
+        p = cfg.sample_momenta(r["n_mols"], r["P"], beta_n, seed=r["seed"] + 2)
+        w = Water(r["n_mols"], r["P"], box, T_K=r["T_K"], dtfs=0.25, eps_ewald=1e-6)
+        w.set_seed(r["seed"])
+        try:
+            w.set_state(q, p)
+        except Exception:
+            print("replica", len(ws), r, q.shape, bool(np.isfinite(q).all()),
+                  w.n_beads, w.n_dof, file=sys.stderr, flush=True)
+            raise
+        ws.append(w)
+    b = Batch(ws, True)
+    b.run(a.warmup)
+    b.sync()
+    t0 = time.perf_counter()
+    b.run(a.steps)
+    b.sync()
+    el = time.perf_counter() - t0
+    status = [w.scalars()["status"] for w in ws]
+    b.close()
+    bead_mol = sum(r["n_mols"] * r["P"] for r in reps)
+    res = {
+        "metric": "replica steps/s, 64 SC-PIMD replicas (batched)",
+        "value": a.steps * len(reps) / el,
+        "unit": "replica-steps/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": 1e3 * el / a.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded lattice starts, free-ring-polymer beads)",
+        "config": {"workload": "batched replicas: N in {64..512} H2O, P in {8..64}, T in 250-350 K",
+                   "replicas": len(reps), "molecule_bead_total": bead_mol,
+                   "parallelism": "single GPU"},
+        "molecule_bead_steps_per_s": bead_mol * a.steps / el,
+        "status_max": int(max(status)),
+    }
+    for w in ws:
+        w.close()
+    print(json.dumps(res), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "batch"])
+    ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -107,6 +175,10 @@ def main():
               "(each replica or bead set runs on one device); run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "batch":
+        run_batch(a)
+        return
 
     from qtip4pf_water_amd import Water
 

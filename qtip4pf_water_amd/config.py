@@ -125,6 +125,22 @@ def lattice_water_box(n_side, L_angs, seed=0, jitter_bohr=0.0):
     return np.ascontiguousarray(q), tuple(float(x) for x in Ls)
 
 
+def water_box(n_mols, L_angs, seed=0, jitter_bohr=0.0):
+    """n_mols rigid waters in a cubic box of edge L_angs: a full cubic
+    lattice when n_mols is a cube, otherwise n_mols sites of the smallest
+    cubic lattice that holds them, chosen at random (seeded).  Returns the
+    reference-layout positions and the box edge in bohr."""
+    n_side = 1
+    while n_side ** 3 < n_mols:
+        n_side += 1
+    q, box = lattice_water_box(n_side, L_angs, seed=seed, jitter_bohr=jitter_bohr)
+    if n_side ** 3 == n_mols:
+        return q, box
+    keep = np.sort(np.random.default_rng(seed + 7919).permutation(n_side ** 3)[:n_mols])
+    q = q.reshape(-1, 9)[keep].reshape(-1)
+    return np.ascontiguousarray(q), box
+
+
 def replicate_beads(q0: np.ndarray, n_beads: int) -> np.ndarray:
     """Every bead a copy of bead 0 (initialise.cpp:128-132)."""
     return np.ascontiguousarray(np.tile(q0, n_beads))
