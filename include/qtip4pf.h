@@ -136,6 +136,38 @@ int qtp_get_kcv(qtp_ctx *ctx, const double *q, const double *fphys, double *kcv)
 int qtp_get_conserved(qtp_ctx *ctx, const double *q, const double *p, const double *mass,
                       double energy, double *cons);
 
+/* ---- fast-CMD splined pair potential (forces.cpp:634-691, 791-882;
+ * update.cpp:16-75, 139-269) ------------------------------------------------
+ * qtp_set_splines loads the reference's three spline files' contents (values
+ * of the O-O, O-H and H-H pair potentials at start + j*step, j = 0..n-1;
+ * simulation.cpp:36-116 reads them with start 0, step 0.01 bohr) and fits
+ * them as Boost.Math's cubic_b_spline does.  Every atom pair of a type,
+ * intramolecular pairs included, within start + step*(n-1) adds the spline
+ * (minimum image).  After it the resident mode uses the splines until
+ * qtp_clear_splines; both require qtp_set_state again.  The *_wspline calls
+ * take the same arguments as their *_nospline counterparts (the reference
+ * passes the spline objects and squared ranges; here they are the loaded
+ * tables) and fail with QTP_ERR_STATE when no tables are loaded. */
+int qtp_set_splines(qtp_ctx *ctx, const double *oo, int n_oo, const double *oh, int n_oh,
+                    const double *hh, int n_hh, double start, double step);
+int qtp_clear_splines(qtp_ctx *ctx);
+/* forces.cpp:634-691 get_forces_wspline */
+int qtp_get_forces_wspline(qtp_ctx *ctx, const double *q, double *F, const double *charge,
+                           const int *point, const int *list, int bead, double *energy);
+/* forces.cpp:791-882 force_eval_sc_wspline */
+int qtp_force_eval_sc_wspline(qtp_ctx *ctx, const double *q, double *F_sc,
+                              const double *charge, const double *mass, const int *point,
+                              const int *list, double *sc_pot, double *fphys, double *energy);
+/* update.cpp:139-269 update_sc_wspline */
+int qtp_update_sc_wspline(qtp_ctx *ctx, double *q, double *p, const double *mass, double *F,
+                          double *fphys, const double *charge, int *point, int *list,
+                          double *disp, const double *dev, int is_nvt, double *heat,
+                          double *sc_pot, double *energy);
+/* update.cpp:16-75 update_trajectory_class_wspline */
+int qtp_update_trajectory_class_wspline(qtp_ctx *ctx, double *q, double *p, const double *mass,
+                                        double *F, const double *charge, int *point, int *list,
+                                        double *disp, double *energy);
+
 /* Host-side draws in the reference's order from a std::mt19937 owned by the
  * library (for callers without a C++ rng; the C++ adapter uses the caller's). */
 typedef struct qtp_host_rng qtp_host_rng;

@@ -32,6 +32,7 @@ class OqSys(C.Structure):
         ("eps", C.c_double), ("acc", C.c_double), ("rho", C.c_double),
         ("kwald_prefactor", C.c_double),
         ("sd_ox", C.c_double), ("sd_h", C.c_double),
+        ("use_spline", C.c_int),
     ]
 
 
@@ -102,12 +103,45 @@ def lib():
                                    C.c_int, _dp, _dp, _dp, C.c_int, _ip]
         L.oq_update_class.restype = C.c_double
         L.oq_update_class.argtypes = [C.POINTER(OqSys), _dp, _dp, _dp, _dp, _dp, _ip, _ip, _dp, _ip]
+        L.oq_spline_set.restype = C.c_int
+        L.oq_spline_set.argtypes = [_dp, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double,
+                                    C.c_double]
+        L.oq_spline_fit.restype = C.c_int
+        L.oq_spline_fit.argtypes = [_dp, C.c_int, C.c_double, _dp]
+        L.oq_spline_eval.restype = C.c_double
+        L.oq_spline_eval.argtypes = [_dp, C.c_int, C.c_double, C.c_double, C.c_double, _dp]
         L.oq_get_kcv.restype = C.c_double
         L.oq_get_kcv.argtypes = [C.POINTER(OqSys), _dp, _dp]
         L.oq_get_conserved.restype = C.c_double
         L.oq_get_conserved.argtypes = [C.POINTER(OqSys), _dp, _dp, _dp, C.c_double]
         _lib = L
     return _lib
+
+
+def spline_set(oo, oh, hh, a=0.0, h=0.01) -> None:
+    """Load the O-O, O-H, H-H spline tables (the reference's spline files:
+    values at a + j h) for force calls of oracles with use_spline set."""
+    t = [np.ascontiguousarray(x, dtype=np.float64) for x in (oo, oh, hh)]
+    rc = lib().oq_spline_set(_ptr(t[0]), t[0].size, _ptr(t[1]), t[1].size, _ptr(t[2]), t[2].size,
+                             float(a), float(h))
+    if rc != 0:
+        raise ValueError("spline tables need at least 5 values each")
+
+
+def spline_fit(f, h):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    beta = np.zeros(f.size + 2)
+    if lib().oq_spline_fit(_ptr(f), f.size, float(h), _ptr(beta)) != 0:
+        raise ValueError("spline fit needs at least 5 values")
+    return beta
+
+
+def spline_eval(beta, n, a, h, x):
+    """(value, derivative) of a fitted spline."""
+    d = C.c_double()
+    v = lib().oq_spline_eval(_ptr(np.ascontiguousarray(beta, dtype=np.float64)), int(n), float(a),
+                             float(h), float(x), C.byref(d))
+    return v, d.value
 
 
 def consts() -> dict:
@@ -130,6 +164,14 @@ class Oracle:
         self.charge = np.zeros(self.n_atoms)
         self.mass = np.zeros(self.n_dof)
         lib().oq_charges_masses(C.byref(s), _ptr(self.charge), _ptr(self.mass))
+
+    @property
+    def use_spline(self) -> bool:
+        return bool(self.sys.use_spline)
+
+    @use_spline.setter
+    def use_spline(self, on: bool):
+        self.sys.use_spline = 1 if on else 0
 
     def ewald(self) -> dict:
         e = OqEwald()

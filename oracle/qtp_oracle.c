@@ -551,10 +551,151 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
     return energy;
 }
 
+/* ---------------------------------------------------------------------------
+ * Splined pair potentials (forces.cpp:452-466, 634-691).
+ * ------------------------------------------------------------------------- */
+static struct {
+    int ready;
+    int n[3];
+    double a, h, cutsq[3];
+    double *beta[3];
+} SPL;
+
+/* Uniform cubic B-spline interpolation of f[0..n-1] at x_j = a + j h with
+ * clamped ends, s'(a) and s'(b) from the 5-point one-sided differences
+ * (-25 f0 + 48 f1 - 36 f2 + 16 f3 - 3 f4) / 12h and its mirror: the
+ * published construction of Boost.Math's cubic_b_spline (no version is
+ * pinned by the reference).  Interpolation rows c_{j-1} + 4 c_j + c_{j+1} =
+ * 6 f_j with c_{-1} = c_1 - 2h s'(a) and c_n = c_{n-2} + 2h s'(b); the
+ * tridiagonal system is solved in long double. */
+int oq_spline_fit(const double *f, int n, double h, double *beta)
+{
+    if (n < 5 || !(h > 0.0)) return -1;
+    const long double hl = h;
+    const long double a1 = (-25.0L * f[0] + 48.0L * f[1] - 36.0L * f[2] + 16.0L * f[3] -
+                            3.0L * f[4]) / (12.0L * hl);
+    const long double b1 = (25.0L * f[n - 1] - 48.0L * f[n - 2] + 36.0L * f[n - 3] -
+                            16.0L * f[n - 4] + 3.0L * f[n - 5]) / (12.0L * hl);
+    long double *dg = (long double *)malloc(sizeof(long double) * n);
+    long double *up = (long double *)malloc(sizeof(long double) * n);
+    long double *lo = (long double *)malloc(sizeof(long double) * n);
+    long double *rhs = (long double *)malloc(sizeof(long double) * n);
+    for (int j = 0; j < n; ++j) {
+        dg[j] = 4.0L;
+        up[j] = 1.0L;
+        lo[j] = 1.0L;
+        rhs[j] = 6.0L * f[j];
+    }
+    up[0] = 2.0L;
+    rhs[0] += 2.0L * hl * a1;
+    lo[n - 1] = 2.0L;
+    rhs[n - 1] -= 2.0L * hl * b1;
+    /* Thomas algorithm */
+    for (int j = 1; j < n; ++j) {
+        const long double m = lo[j] / dg[j - 1];
+        dg[j] -= m * up[j - 1];
+        rhs[j] -= m * rhs[j - 1];
+    }
+    long double *c = (long double *)malloc(sizeof(long double) * (n + 2));
+    c[n] = rhs[n - 1] / dg[n - 1];
+    for (int j = n - 2; j >= 0; --j) c[j + 1] = (rhs[j] - up[j] * c[j + 2]) / dg[j];
+    c[0] = c[2] - 2.0L * hl * a1;
+    c[n + 1] = c[n - 1] + 2.0L * hl * b1;
+    for (int j = 0; j < n + 2; ++j) beta[j] = (double)c[j];
+    free(dg);
+    free(up);
+    free(lo);
+    free(rhs);
+    free(c);
+    return 0;
+}
+
+double oq_spline_eval(const double *beta, int n, double a, double h, double x, double *d)
+{
+    double t = (x - a) / h;
+    int k = (int)floor(t);
+    if (k > n - 2) k = n - 2;  /* x = b uses the last interval */
+    if (k < 0) k = 0;
+    const double u = t - k;
+    const double v = 1.0 - u;
+    /* cubic B-spline weights of c_{k-1}, c_k, c_{k+1}, c_{k+2} */
+    const double w0 = v * v * v / 6.0;
+    const double w1 = (3.0 * u * u * u - 6.0 * u * u + 4.0) / 6.0;
+    const double w2 = (-3.0 * u * u * u + 3.0 * u * u + 3.0 * u + 1.0) / 6.0;
+    const double w3 = u * u * u / 6.0;
+    const double *c = beta + k; /* c[0] = c_{k-1} */
+    if (d) {
+        const double d0 = -0.5 * v * v;
+        const double d1 = 1.5 * u * u - 2.0 * u;
+        const double d2 = -1.5 * u * u + u + 0.5;
+        const double d3 = 0.5 * u * u;
+        *d = (c[0] * d0 + c[1] * d1 + c[2] * d2 + c[3] * d3) / h;
+    }
+    return c[0] * w0 + c[1] * w1 + c[2] * w2 + c[3] * w3;
+}
+
+void oq_spline_clear(void)
+{
+    for (int t = 0; t < 3; ++t) {
+        free(SPL.beta[t]);
+        SPL.beta[t] = NULL;
+    }
+    SPL.ready = 0;
+}
+
+int oq_spline_set(const double *oo, int n_oo, const double *oh, int n_oh, const double *hh,
+                  int n_hh, double a, double h)
+{
+    const double *tab[3] = {oo, oh, hh};
+    const int nn[3] = {n_oo, n_oh, n_hh};
+    oq_spline_clear();
+    for (int t = 0; t < 3; ++t) {
+        if (nn[t] < 5) return -1;
+        SPL.beta[t] = (double *)malloc(sizeof(double) * (nn[t] + 2));
+        if (oq_spline_fit(tab[t], nn[t], h, SPL.beta[t]) != 0) return -1;
+        SPL.n[t] = nn[t];
+        /* simulation.cpp:57-58: cut = start + step (n - 1), squared */
+        const double cut = a + h * (nn[t] - 1);
+        SPL.cutsq[t] = cut * cut;
+    }
+    SPL.a = a;
+    SPL.h = h;
+    SPL.ready = 1;
+    return 0;
+}
+
+/* forces.cpp:643-689 then get_spline :452-466 for each pair in range */
+static double spline_pairs(const oq_sys *s, const double *q, double *F)
+{
+    double energy = 0.0, r[3];
+    for (int i = 0; i < s->n_atoms; ++i) {
+        for (int j = i + 1; j < s->n_atoms; ++j) {
+            for (int d = 0; d < 3; ++d) r[d] = q[3 * i + d] - q[3 * j + d];
+            min_image(s, r);
+            const double rsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            const int t = (i % 3 == 0) ? ((j % 3 == 0) ? 0 : 1) : ((j % 3 == 0) ? 1 : 2);
+            if (rsq < SPL.cutsq[t]) {
+                const double rij = sqrt(rsq);
+                double fp;
+                energy += oq_spline_eval(SPL.beta[t], SPL.n[t], SPL.a, SPL.h, rij, &fp);
+                fp *= 1.0 / rij;
+                for (int d = 0; d < 3; ++d) {
+                    F[3 * i + d] -= fp * r[d];
+                    F[3 * j + d] += fp * r[d];
+                }
+            }
+        }
+    }
+    return energy;
+}
+
 double oq_get_forces(const oq_sys *s, const double *q, double *F, const double *charge,
                      const int *point, const int *list, int bead)
 {
-    return oq_get_forces_parts(s, q, F, charge, point, list, bead, NULL);
+    double e = oq_get_forces_parts(s, q, F, charge, point, list, bead, NULL);
+    if (s->use_spline && SPL.ready)
+        e += spline_pairs(s, q + (size_t)bead * s->n_dof, F + (size_t)bead * s->n_dof);
+    return e;
 }
 
 /* forces.cpp:698-789.  The reference reduces tot_energy and vop over an OMP

@@ -36,6 +36,8 @@ typedef struct {
     double eps, acc, rho, kwald_prefactor;
     /* momentum sampling widths (water.cpp:138-139) */
     double sd_ox, sd_h;
+    /* 1: force calls add the splined pair potential (get_forces_wspline) */
+    int use_spline;
 } oq_sys;
 
 /* Ewald parameters as derived inside every force call (forces.cpp:574-584). */
@@ -67,6 +69,21 @@ double oq_lj_oxygen(const double r[3], double rsq, double *F, int i, int j);
 double oq_intra_morse_bond(const double r[3], double roh, double *F, int i, int bond_step);
 double oq_intra_harm_angle(const double r1[3], const double r2[3], double roh1, double roh2,
                            double *F, int i);
+
+/* Splined pair potentials of the fast-CMD path (forces.cpp:452-466,
+ * 634-691; tables read by simulation.cpp:36-116).  Each table is the
+ * reference's spline file: f[j] = V(a + j h), j = 0..n-1, fitted as Boost.Math's
+ * cubic_b_spline does (uniform cubic B-spline interpolation with endpoint
+ * derivatives estimated by the 5-point one-sided difference, O(h^4)).
+ * oq_spline_set keeps the O-O, O-H and H-H fits for every force call made
+ * with sys->use_spline; n >= 5 each.  Returns 0, or -1 on bad sizes. */
+int oq_spline_set(const double *oo, int n_oo, const double *oh, int n_oh, const double *hh,
+                  int n_hh, double a, double h);
+void oq_spline_clear(void);
+/* The fit alone: beta[0..n+1] are the B-spline coefficients c_{-1..n}. */
+int oq_spline_fit(const double *f, int n, double h, double *beta);
+/* Value (and derivative if d != NULL) of a fitted spline at x in [a, a+(n-1)h]. */
+double oq_spline_eval(const double *beta, int n, double a, double h, double x, double *d);
 
 /* Single-bead force evaluation (forces.cpp:468-630).  Pointers are the full
  * reference arrays; only the slice of `bead` is read/written. */

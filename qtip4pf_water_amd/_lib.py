@@ -70,6 +70,14 @@ SIGNATURES = [
     ("qtp_update_trajectory_class_nospline", C.c_int, [_vp, dp, dp, dp, dp, dp, ip, ip, dp, dp]),
     ("qtp_get_kcv", C.c_int, [_vp, dp, dp, dp]),
     ("qtp_get_conserved", C.c_int, [_vp, dp, dp, dp, C.c_double, dp]),
+    ("qtp_set_splines", C.c_int, [_vp, dp, C.c_int, dp, C.c_int, dp, C.c_int, C.c_double,
+                                  C.c_double]),
+    ("qtp_clear_splines", C.c_int, [_vp]),
+    ("qtp_get_forces_wspline", C.c_int, [_vp, dp, dp, dp, ip, ip, C.c_int, dp]),
+    ("qtp_force_eval_sc_wspline", C.c_int, [_vp, dp, dp, dp, dp, ip, ip, dp, dp, dp]),
+    ("qtp_update_sc_wspline", C.c_int,
+     [_vp, dp, dp, dp, dp, dp, dp, ip, ip, dp, dp, C.c_int, dp, dp, dp]),
+    ("qtp_update_trajectory_class_wspline", C.c_int, [_vp, dp, dp, dp, dp, dp, ip, ip, dp, dp]),
     ("qtp_host_rng_create", C.c_int, [C.c_uint32, C.POINTER(_vp)]),
     ("qtp_host_rng_destroy", None, [_vp]),
     ("qtp_host_rng_draw_pile", C.c_int, [_vp, C.c_int, C.c_int, dp]),

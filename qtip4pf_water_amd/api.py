@@ -179,6 +179,56 @@ class Water:
             ptr(_f64(fphys)) if fphys is not None else None, C.byref(e)))
         return e.value, sc.value
 
+    # ---------------------------------------------- splined pair potential
+    def set_splines(self, oo, oh, hh, start=0.0, step=0.01):
+        """Load the fast-CMD spline tables (values at start + j*step) for the
+        *_wspline calls and the resident mode (set_state must follow)."""
+        t = [np.ascontiguousarray(x, dtype=np.float64) for x in (oo, oh, hh)]
+        self._chk(self._L.qtp_set_splines(self._h, ptr(t[0]), t[0].size, ptr(t[1]), t[1].size,
+                                          ptr(t[2]), t[2].size, float(start), float(step)))
+
+    def clear_splines(self):
+        self._chk(self._L.qtp_clear_splines(self._h))
+
+    def get_forces_wspline(self, q, F, charge, point, lst, bead=0) -> float:
+        e = C.c_double()
+        self._chk(self._L.qtp_get_forces_wspline(self._h, ptr(_f64(q)), ptr(_f64(F)),
+                                                 ptr(_f64(charge)) if charge is not None else None,
+                                                 ptr(_i32(point), ip), ptr(_i32(lst), ip),
+                                                 int(bead), C.byref(e)))
+        return e.value
+
+    def force_eval_sc_wspline(self, q, F_sc, charge, mass, point, lst, fphys=None):
+        e, sc = C.c_double(), C.c_double()
+        self._chk(self._L.qtp_force_eval_sc_wspline(
+            self._h, ptr(_f64(q)), ptr(_f64(F_sc)),
+            ptr(_f64(charge)) if charge is not None else None,
+            ptr(_f64(mass)) if mass is not None else None,
+            ptr(_i32(point), ip), ptr(_i32(lst), ip), C.byref(sc),
+            ptr(_f64(fphys)) if fphys is not None else None, C.byref(e)))
+        return e.value, sc.value
+
+    def update_sc_wspline(self, q, p, mass, F, fphys, charge, point, lst, disp, dev, is_nvt, heat):
+        h, sc, e = C.c_double(heat), C.c_double(), C.c_double()
+        self._chk(self._L.qtp_update_sc_wspline(
+            self._h, ptr(_f64(q)), ptr(_f64(p)),
+            ptr(_f64(mass)) if mass is not None else None, ptr(_f64(F)),
+            ptr(_f64(fphys)) if fphys is not None else None,
+            ptr(_f64(charge)) if charge is not None else None,
+            ptr(_i32(point), ip), ptr(_i32(lst), ip), ptr(_f64(disp)),
+            ptr(_f64(dev)) if dev is not None else None, 1 if is_nvt else 0,
+            C.byref(h), C.byref(sc), C.byref(e)))
+        return e.value, h.value, sc.value
+
+    def update_trajectory_class_wspline(self, q, p, mass, F, charge, point, lst, disp) -> float:
+        e = C.c_double()
+        self._chk(self._L.qtp_update_trajectory_class_wspline(
+            self._h, ptr(_f64(q)), ptr(_f64(p)),
+            ptr(_f64(mass)) if mass is not None else None, ptr(_f64(F)),
+            ptr(_f64(charge)) if charge is not None else None,
+            ptr(_i32(point), ip), ptr(_i32(lst), ip), ptr(_f64(disp)), C.byref(e)))
+        return e.value
+
     def neighbour_list(self, q, point, lst, disp, bead=0):
         self._chk(self._L.qtp_neighbour_list(self._h, ptr(_f64(q)), ptr(_i32(point), ip),
                                              ptr(_i32(lst), ip),

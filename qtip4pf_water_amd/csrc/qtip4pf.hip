@@ -97,7 +97,12 @@ struct qtp_ctx {
     int ntile = 0;       // real-space tiles per evaluation
     int nmb = 0;         // LJ/intra blocks per evaluation
     int AC = 64, nac = 0, ngc = 0, ngr = 0;
-    int ep_stride = 0, ep_lj = 0, ep_real = 0, ep_k = 0;
+    int ep_stride = 0, ep_lj = 0, ep_real = 0, ep_k = 0, ep_spl = 0;
+    // splined pair potentials (fast-CMD path): tables loaded, used by the
+    // resident steps, used by the current call
+    bool spl_loaded = false, spl_resident = false, spl_on = false;
+    SplTab spl{};
+    double *spl_coef = nullptr, *fspl = nullptr;
     int ptile = 0;       // propagation tiles (64 planar dofs each)
     size_t kforce_lds = 0, ksum_lds = 0, prop_lds = 0;
 
@@ -367,6 +372,7 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));
 
     CHK(dalloc(ctx, &ctx->flj, (size_t)E * N3));
+    CHK(dalloc(ctx, &ctx->fspl, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->Ve, E));
     CHK(dalloc(ctx, &ctx->epart, (size_t)E * ctx->ep_stride));
@@ -433,6 +439,19 @@ static int upload_constants(qtp_ctx *ctx)
 // ---------------------------------------------------------------------------
 // force phase: E single-bead evaluations (forces.cpp:468-630)
 // ---------------------------------------------------------------------------
+// energy partials summed per evaluation: the spline blocks only when used
+static int ep_live(const qtp_ctx *ctx) { return ctx->spl_on ? ctx->ep_stride : ctx->ep_spl; }
+
+// The splined potential is part of a force call only through ctx->spl_on,
+// which every entry point sets for its duration.
+struct SplineScope {
+    qtp_ctx *ctx;
+    bool prev;
+    SplineScope(qtp_ctx *c, bool on) : ctx(c), prev(c->spl_on) { c->spl_on = on; }
+    ~SplineScope() { ctx->spl_on = prev; }
+};
+
+
 // vout (optional): per-evaluation virial W_ab = sum_i r_i (x) F_i in the
 // minimum-image pair form (6 components xx yy zz xy xz yz), at fixed Ewald
 // parameters.
@@ -476,6 +495,17 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
                 pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, ctx->flist, ctx->fcap, lmode, lbase,
                 ctx->box, ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride, ctx->ep_lj, vp);
     }
+    if (ctx->spl_on) {
+        TimeScope ts(ctx, "spline");
+        if (vout)
+            k_spline<true><<<E * ctx->nblk, 256, 0, sk>>>(
+                pos, N3, ctx->n_atoms, ctx->NAP, ctx->nblk, ctx->box, ctx->spl, ctx->spl_coef,
+                ctx->fspl, ctx->epart, ctx->ep_stride, ctx->ep_spl, vp);
+        else
+            k_spline<false><<<E * ctx->nblk, 256, 0, sk>>>(
+                pos, N3, ctx->n_atoms, ctx->NAP, ctx->nblk, ctx->box, ctx->spl, ctx->spl_coef,
+                ctx->fspl, ctx->epart, ctx->ep_stride, ctx->ep_spl, vp);
+    }
     if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
     {
         TimeScope ts(ctx, "realspace");
@@ -496,13 +526,13 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
         k_kforce<<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
             pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
-            ctx->rpart, ctx->nblk, ctx->flj, fout);
+            ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout);
     }
     if (eout) {
         TimeScope ts(ctx, "esum");
-        k_esum<<<E, 64, 0, s>>>(E, ctx->epart, ctx->ep_stride, ctx->eself, eout);
+        k_esum<<<E, 64, 0, s>>>(E, ctx->epart, ctx->ep_stride, ep_live(ctx), ctx->eself, eout);
     }
-    if (vout) k_vsum<<<E, 64, 0, s>>>(ctx->vpart, ctx->ep_stride, vout);
+    if (vout) k_vsum<<<E, 64, 0, s>>>(ctx->vpart, ctx->ep_stride, ep_live(ctx), vout);
     HIPCHK(hipGetLastError());
     return QTP_OK;
 }
@@ -528,7 +558,7 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
     SCArgs a = sc_args(ctx);
     {
         TimeScope ts(ctx, "sc1");
-        k_sc1<<<P, SC_THREADS, 0, s>>>(a, ctx->q, ctx->fph, ctx->epart, ctx->ep_stride, ctx->eself,
+        k_sc1<<<P, SC_THREADS, 0, s>>>(a, ctx->q, ctx->fph, ctx->epart, ctx->ep_stride, ep_live(ctx), ctx->eself,
                                 ctx->Ve, ctx->m_atom, ctx->F, ctx->qpm, ctx->scal);
     }
     CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr));
@@ -635,7 +665,7 @@ static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr)
     {
         TimeScope ts(ctx, "vv_b");
         k_vv_b<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->drv.halfdt, ctx->fph, ctx->epart,
-                                  ctx->ep_stride, ctx->eself, ctx->F, ctx->p, ctx->out, ctr);
+                                  ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p, ctx->out, ctr);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;
@@ -907,7 +937,8 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->ep_lj = 0;
     ctx->ep_real = ctx->nmb;
     ctx->ep_k = ctx->nmb + ctx->ntile;
-    ctx->ep_stride = ctx->nmb + ctx->ntile + ctx->ngr;
+    ctx->ep_spl = ctx->nmb + ctx->ntile + ctx->ngr;
+    ctx->ep_stride = ctx->ep_spl + ctx->nblk;
     ctx->ptile = (3 * ctx->NAP + 63) / 64;
     ctx->prop_lds = 0;
     *out = ctx;
@@ -956,7 +987,7 @@ void qtp_destroy(qtp_ctx *ctx)
     void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
                     ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
                     ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart,
-                    ctx->spart, ctx->sprime, ctx->flj, ctx->fpm, ctx->Ve, ctx->epart,
+                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs};
@@ -988,7 +1019,7 @@ int qtp_get_stream(const qtp_ctx *ctx, void **stream)
 
 // ---- reference-shaped calls --------------------------------------------------
 
-int qtp_get_forces_nospline(qtp_ctx *ctx, const double *q, double *F, const double *charge,
+static int get_forces_impl(qtp_ctx *ctx, const double *q, double *F, const double *charge,
                             const int *point, const int *list, int bead, double *energy)
 {
     if (!ctx || !q || !F || !point || !list || bead < 0 || bead >= ctx->P) return QTP_ERR_ARG;
@@ -1013,6 +1044,7 @@ int qtp_get_forces_virial(qtp_ctx *ctx, const double *q, double *F, const double
 {
     if (!ctx || !q || !F || !point || !list || !virial || bead < 0 || bead >= ctx->P)
         return QTP_ERR_ARG;
+    SplineScope sp(ctx, false);
     CHK(check_finite(ctx, q + (long)bead * ctx->n_dof, ctx->n_dof, "positions"));
     CHK(upload_charges_masses(ctx, charge, nullptr));
     CHK(upload_planar(ctx, ctx->qpm, q + (long)bead * ctx->n_dof, 1));
@@ -1028,7 +1060,7 @@ int qtp_get_forces_virial(qtp_ctx *ctx, const double *q, double *F, const double
     return QTP_OK;
 }
 
-int qtp_force_eval_sc_nospline(qtp_ctx *ctx, const double *q, double *F_sc, const double *charge,
+static int force_eval_sc_impl(qtp_ctx *ctx, const double *q, double *F_sc, const double *charge,
                                const double *mass, const int *point, const int *list,
                                double *sc_pot, double *fphys, double *energy)
 {
@@ -1144,7 +1176,7 @@ int qtp_pile_g(qtp_ctx *ctx, double *p_nm, const double *mass, const double *dev
     return QTP_OK;
 }
 
-int qtp_update_sc_nospline(qtp_ctx *ctx, double *q, double *p, const double *mass, double *F,
+static int update_sc_impl(qtp_ctx *ctx, double *q, double *p, const double *mass, double *F,
                            double *fphys, const double *charge, int *point, int *list,
                            double *disp, const double *dev, int is_nvt, double *heat,
                            double *sc_pot, double *energy)
@@ -1192,7 +1224,7 @@ int qtp_update_sc_nospline(qtp_ctx *ctx, double *q, double *p, const double *mas
     return QTP_OK;
 }
 
-int qtp_update_trajectory_class_nospline(qtp_ctx *ctx, double *q, double *p, const double *mass,
+static int update_class_impl(qtp_ctx *ctx, double *q, double *p, const double *mass,
                                          double *F, const double *charge, int *point, int *list,
                                          double *disp, double *energy)
 {
@@ -1272,6 +1304,168 @@ int qtp_get_conserved(qtp_ctx *ctx, const double *q, const double *p, const doub
     return QTP_OK;
 }
 
+// ---- splined pair potentials (fast-CMD path) ------------------------------------
+
+// Uniform cubic B-spline interpolation of f[0..n-1] at a + j h, clamped with
+// endpoint slopes from the 5-point one-sided differences (O(h^4)): the
+// construction of Boost.Math's cubic_b_spline, which the reference builds
+// from its spline files (simulation.cpp:60, 84, 111).  Rows
+// c_{j-1} + 4 c_j + c_{j+1} = 6 f_j, c_{-1} = c_1 - 2h s'(a),
+// c_n = c_{n-2} + 2h s'(b); tridiagonal solve in long double.
+static void spline_fit(const double *f, int n, double h, double *c)
+{
+    const long double hl = h;
+    const long double sa = (-25.0L * f[0] + 48.0L * f[1] - 36.0L * f[2] + 16.0L * f[3] -
+                            3.0L * f[4]) / (12.0L * hl);
+    const long double sb = (25.0L * f[n - 1] - 48.0L * f[n - 2] + 36.0L * f[n - 3] -
+                            16.0L * f[n - 4] + 3.0L * f[n - 5]) / (12.0L * hl);
+    std::vector<long double> dg(n, 4.0L), up(n, 1.0L), lo(n, 1.0L), r(n), x(n);
+    for (int j = 0; j < n; ++j) r[j] = 6.0L * f[j];
+    up[0] = 2.0L;
+    r[0] += 2.0L * hl * sa;
+    lo[n - 1] = 2.0L;
+    r[n - 1] -= 2.0L * hl * sb;
+    for (int j = 1; j < n; ++j) {
+        const long double m = lo[j] / dg[j - 1];
+        dg[j] -= m * up[j - 1];
+        r[j] -= m * r[j - 1];
+    }
+    x[n - 1] = r[n - 1] / dg[n - 1];
+    for (int j = n - 2; j >= 0; --j) x[j] = (r[j] - up[j] * x[j + 1]) / dg[j];
+    for (int j = 0; j < n; ++j) c[j + 1] = (double)x[j];
+    c[0] = (double)(x[1] - 2.0L * hl * sa);
+    c[n + 1] = (double)(x[n - 2] + 2.0L * hl * sb);
+}
+
+int qtp_set_splines(qtp_ctx *ctx, const double *oo, int n_oo, const double *oh, int n_oh,
+                    const double *hh, int n_hh, double start, double step)
+{
+    if (!ctx || !oo || !oh || !hh || n_oo < 5 || n_oh < 5 || n_hh < 5 || !(step > 0.0) ||
+        !std::isfinite(start))
+        return QTP_ERR_ARG;
+    const double *tab[3] = {oo, oh, hh};
+    const int nn[3] = {n_oo, n_oh, n_hh};
+    for (int t = 0; t < 3; ++t) CHK(check_finite(ctx, tab[t], nn[t], "spline table"));
+    std::vector<double> coef;
+    SplTab st{};
+    st.a = start;
+    st.h = step;
+    st.inv_h = 1.0 / step;
+    for (int t = 0; t < 3; ++t) {
+        st.n[t] = nn[t];
+        st.off[t] = (int)coef.size();
+        coef.resize(coef.size() + nn[t] + 2);
+        spline_fit(tab[t], nn[t], step, coef.data() + st.off[t]);
+        // simulation.cpp:57-58: the table ends at start + step (n - 1)
+        const double cut = start + step * (nn[t] - 1);
+        st.cutsq[t] = cut * cut;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->spl_coef) hipFree(ctx->spl_coef);
+    ctx->spl_coef = nullptr;
+    CHK(dalloc(ctx, &ctx->spl_coef, coef.size()));
+    HIPCHK(hipMemcpy(ctx->spl_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice));
+    ctx->spl = st;
+    ctx->spl_loaded = true;
+    ctx->spl_resident = true;
+    invalidate_graphs(ctx);
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+int qtp_clear_splines(qtp_ctx *ctx)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    ctx->spl_resident = false;
+    invalidate_graphs(ctx);
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+static int need_splines(qtp_ctx *ctx)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    if (!ctx->spl_loaded) {
+        ctx->err = "no spline tables loaded (qtp_set_splines)";
+        return QTP_ERR_STATE;
+    }
+    return QTP_OK;
+}
+
+int qtp_get_forces_nospline(qtp_ctx *ctx, const double *q, double *F, const double *charge,
+                            const int *point, const int *list, int bead, double *energy)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    SplineScope sp(ctx, false);
+    return get_forces_impl(ctx, q, F, charge, point, list, bead, energy);
+}
+
+int qtp_get_forces_wspline(qtp_ctx *ctx, const double *q, double *F, const double *charge,
+                           const int *point, const int *list, int bead, double *energy)
+{
+    CHK(need_splines(ctx));
+    SplineScope sp(ctx, true);
+    return get_forces_impl(ctx, q, F, charge, point, list, bead, energy);
+}
+
+int qtp_force_eval_sc_nospline(qtp_ctx *ctx, const double *q, double *F_sc, const double *charge,
+                               const double *mass, const int *point, const int *list,
+                               double *sc_pot, double *fphys, double *energy)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    SplineScope sp(ctx, false);
+    return force_eval_sc_impl(ctx, q, F_sc, charge, mass, point, list, sc_pot, fphys, energy);
+}
+
+int qtp_force_eval_sc_wspline(qtp_ctx *ctx, const double *q, double *F_sc, const double *charge,
+                              const double *mass, const int *point, const int *list,
+                              double *sc_pot, double *fphys, double *energy)
+{
+    CHK(need_splines(ctx));
+    SplineScope sp(ctx, true);
+    return force_eval_sc_impl(ctx, q, F_sc, charge, mass, point, list, sc_pot, fphys, energy);
+}
+
+int qtp_update_sc_nospline(qtp_ctx *ctx, double *q, double *p, const double *mass, double *F,
+                           double *fphys, const double *charge, int *point, int *list,
+                           double *disp, const double *dev, int is_nvt, double *heat,
+                           double *sc_pot, double *energy)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    SplineScope sp(ctx, false);
+    return update_sc_impl(ctx, q, p, mass, F, fphys, charge, point, list, disp, dev, is_nvt, heat,
+                          sc_pot, energy);
+}
+
+int qtp_update_sc_wspline(qtp_ctx *ctx, double *q, double *p, const double *mass, double *F,
+                          double *fphys, const double *charge, int *point, int *list,
+                          double *disp, const double *dev, int is_nvt, double *heat,
+                          double *sc_pot, double *energy)
+{
+    CHK(need_splines(ctx));
+    SplineScope sp(ctx, true);
+    return update_sc_impl(ctx, q, p, mass, F, fphys, charge, point, list, disp, dev, is_nvt, heat,
+                          sc_pot, energy);
+}
+
+int qtp_update_trajectory_class_nospline(qtp_ctx *ctx, double *q, double *p, const double *mass,
+                                         double *F, const double *charge, int *point, int *list,
+                                         double *disp, double *energy)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    SplineScope sp(ctx, false);
+    return update_class_impl(ctx, q, p, mass, F, charge, point, list, disp, energy);
+}
+
+int qtp_update_trajectory_class_wspline(qtp_ctx *ctx, double *q, double *p, const double *mass,
+                                        double *F, const double *charge, int *point, int *list,
+                                        double *disp, double *energy)
+{
+    CHK(need_splines(ctx));
+    SplineScope sp(ctx, true);
+    return update_class_impl(ctx, q, p, mass, F, charge, point, list, disp, energy);
+}
+
 // ---- host rng --------------------------------------------------------------
 
 int qtp_host_rng_create(uint32_t seed, qtp_host_rng **out)
@@ -1312,6 +1506,7 @@ int qtp_set_state(qtp_ctx *ctx, const double *q, const double *p, const double *
                   const double *mass)
 {
     if (!ctx || !q || !p) return QTP_ERR_ARG;
+    SplineScope sp(ctx, ctx->spl_resident);
     const int P = ctx->P;
     CHK(check_finite(ctx, q, (long)P * ctx->n_dof, "positions"));
     CHK(check_finite(ctx, p, (long)P * ctx->n_dof, "momenta"));
@@ -1378,6 +1573,7 @@ int qtp_get_virial(qtp_ctx *ctx, double *virial, double *energy)
         ctx->err = "qtp_get_virial: no resident state (call qtp_set_state)";
         return QTP_ERR_STATE;
     }
+    SplineScope sp(ctx, ctx->spl_resident);
     const int P = ctx->P;
     // physical forces go to scratch: the resident force arrays are left as they are
     CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fpm, ctx->Ve, ctx->vout));
@@ -1393,6 +1589,7 @@ int qtp_get_virial(qtp_ctx *ctx, double *virial, double *energy)
 
 static int step_once(qtp_ctx *ctx, int nvt)
 {
+    SplineScope sp(ctx, ctx->spl_resident);
     if (ctx->P >= 2) return run_sc_step(ctx, nvt, 0, ctx->step_ctr);
     return run_class_step(ctx, ctx->step_ctr);
 }

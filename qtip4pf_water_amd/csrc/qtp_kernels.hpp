@@ -113,7 +113,8 @@ __device__ inline double philox_normal(uint64_t seed, uint64_t step, uint32_t st
 // (forces.cpp:468-630); eval e reads positions pos + e*pstride and bead
 // lst_bead(e)'s Verlet list.  Energies are left as per-block partials
 // epart[e][0 .. ep_stride) (LJ+intra blocks, real-space tiles, k-space
-// blocks); eval_energy() sums them in that fixed order.
+// blocks, spline blocks); eval_energy() sums the first ep_n of them (the
+// spline blocks only when the splined potential is on) in that fixed order.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ int lst_bead(int mode, int base, int e)
 {
@@ -143,9 +144,9 @@ __device__ __forceinline__ double wave_sum_array(const double *__restrict__ p, i
 // Energy of evaluation e: its partials (LJ+intra blocks, real-space tiles,
 // k-space blocks) minus the self term.  Called by a whole wavefront.
 __device__ __forceinline__ double eval_energy(const double *__restrict__ epart, int e, int ep_stride,
-                                              double eself)
+                                              int ep_n, double eself)
 {
-    return wave_sum_array(epart + (long)e * ep_stride, ep_stride) - eself;
+    return wave_sum_array(epart + (long)e * ep_stride, ep_n) - eself;
 }
 
 // erfc(x) = exp(-x^2) erfcx(x) with erfcx from the P8/Q8 rational of
@@ -534,13 +535,14 @@ inline size_t kforce_lds_bytes(int K)
 // KF_CHUNK at a time; the real-space slot partials are loaded at kernel
 // start (wave w takes slots w, w + KF_WAVES, ...).  Epilogue: waves 0-2 (one
 // per dimension) sum the wave partials, apply the M-site chain rule
-// (forces.cpp:613-619) and add the LJ/intramolecular forces.
+// (forces.cpp:613-619) and add the LJ/intramolecular forces (and the
+// splined pair forces when fspl is given).
 // grid: E * ceil(n_atoms / 63), 64 * KF_WAVES threads, kforce_lds_bytes(K).
 __global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
     int NAP, double gamma, double half1mg, Ewald ew, int G, const double *__restrict__ krec,
     const double *__restrict__ rpart, int nblk, const double *__restrict__ flj,
-    double *__restrict__ fout)
+    const double *__restrict__ fspl, double *__restrict__ fout)
 {
     extern __shared__ double smem[];
     const int K = ew.K;
@@ -564,7 +566,10 @@ __global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
             fy += rp[NAP];
             fz += rp[2 * NAP];
         }
-        if (w < 3) fl = flj[((long)e * 3 + w) * NAP + atom];
+        if (w < 3) {
+            fl = flj[((long)e * 3 + w) * NAP + atom];
+            if (fspl) fl += fspl[((long)e * 3 + w) * NAP + atom];
+        }
     }
     if (threadIdx.x < 192) {
         const int d = threadIdx.x >> 6;
@@ -806,14 +811,128 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
     }
 }
 
+// Splined pair potentials of the fast-CMD path (forces.cpp:634-691,
+// get_spline :452-466): every atom pair (O-O, O-H, H-H by the atoms'
+// slots; intramolecular pairs included) within its table's range, minimum
+// image.  Tables are uniform cubic B-splines: coefficients c_{-1..n} of type
+// t at coef[off[t] ..], knots a + j h.  Lane <-> atom i of a 64-atom block;
+// each lane sums its pairs with every other atom (both orders, no reaction
+// writes), the block's 4 waves taking every 4th partner block; energy and
+// virial halve the double count.  Writes the full spline force of each atom
+// to fspl and the block's energy to epart slot ep_off + I.
+// grid: E * nblk, 256 threads.
+struct SplTab {
+    double a, h, inv_h;
+    double cutsq[3];
+    int n[3], off[3];
+};
+__device__ __forceinline__ double spline_eval(const double *__restrict__ c0, int n, double a,
+                                              double inv_h, double x, double &dv)
+{
+    const double t = (x - a) * inv_h;
+    int k = (int)floor(t);
+    k = max(0, min(k, n - 2));
+    const double u = t - k, v = 1.0 - u;
+    const double *c = c0 + k;  // c[0] = c_{k-1}
+    const double u2 = u * u, u3 = u2 * u;
+    const double w0 = v * v * v, w1 = 3.0 * u3 - 6.0 * u2 + 4.0, w2 = -3.0 * u3 + 3.0 * u2 + 3.0 * u + 1.0;
+    dv = (c[0] * (-0.5 * v * v) + c[1] * (1.5 * u2 - 2.0 * u) + c[2] * (-1.5 * u2 + u + 0.5) +
+          c[3] * (0.5 * u2)) * inv_h;
+    return (c[0] * w0 + c[1] * w1 + c[2] * w2 + c[3] * u3) * (1.0 / 6.0);
+}
+template <bool VIR>
+__global__ __launch_bounds__(256) void k_spline(const double *__restrict__ pos, long pstride,
+                                                int n_atoms, int NAP, int nblk, Box box, SplTab st,
+                                                const double *__restrict__ coef,
+                                                double *__restrict__ fspl,
+                                                double *__restrict__ epart, int ep_stride,
+                                                int ep_off, double *__restrict__ vpart)
+{
+    const int e = blockIdx.x / nblk, I = blockIdx.x - e * nblk;
+    const int l = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const double *X = pos + (long)e * pstride;
+    __shared__ double sx[4][64], sy[4][64], sz[4][64];
+    __shared__ double red[4][3][64];
+    __shared__ double ewv[4], wv[4][6];
+    const int i = I * 64 + l;
+    const bool vi = i < n_atoms;
+    const double xi = vi ? X[i] : 0.0, yi = vi ? X[NAP + i] : 0.0, zi = vi ? X[2 * NAP + i] : 0.0;
+    const int oi = (i % 3 == 0);
+    double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
+    double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    const int rounds = (nblk + 3) / 4;
+    for (int rd = 0; rd < rounds; ++rd) {
+        const int J = rd * 4 + w;
+        const int jj = J * 64 + l;
+        const bool vj = J < nblk && jj < n_atoms;
+        sx[w][l] = vj ? X[jj] : 0.0;
+        sy[w][l] = vj ? X[NAP + jj] : 0.0;
+        sz[w][l] = vj ? X[2 * NAP + jj] : 0.0;
+        __syncthreads();
+        if (J < nblk) {
+            const int cmax = min(64, n_atoms - J * 64);
+            for (int c = 0; c < cmax; ++c) {
+                const int j = J * 64 + c;
+                const double dx = mimg_exact(xi - sx[w][c], box.L[0]);
+                const double dy = mimg_exact(yi - sy[w][c], box.L[1]);
+                const double dz = mimg_exact(zi - sz[w][c], box.L[2]);
+                const double rsq = dx * dx + dy * dy + dz * dz;
+                const int oj = (j % 3 == 0);
+                const int t = oi ? (oj ? 0 : 1) : (oj ? 1 : 2);
+                if (vi && j != i && rsq < st.cutsq[t]) {
+                    const double r = sqrt(rsq);
+                    double dv;
+                    const double v = spline_eval(coef + st.off[t], st.n[t], st.a, st.inv_h, r, dv);
+                    const double fp = dv * (1.0 / r);
+                    en += 0.5 * v;
+                    fx -= fp * dx;
+                    fy -= fp * dy;
+                    fz -= fp * dz;
+                    if (VIR) {
+                        const double hp = -0.5 * fp;
+                        v6[0] += hp * dx * dx;
+                        v6[1] += hp * dy * dy;
+                        v6[2] += hp * dz * dz;
+                        v6[3] += hp * dx * dy;
+                        v6[4] += hp * dx * dz;
+                        v6[5] += hp * dy * dz;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    red[w][0][l] = fx;
+    red[w][1][l] = fy;
+    red[w][2][l] = fz;
+    en = wave_sum(en);
+    if (l == 0) ewv[w] = en;
+    if (VIR) {
+        for (int c = 0; c < 6; ++c) {
+            const double v = wave_sum(v6[c]);
+            if (l == 0) wv[w][c] = v;
+        }
+    }
+    __syncthreads();
+    if (w < 3) {
+        const double v = red[0][w][l] + red[1][w][l] + red[2][w][l] + red[3][w][l];
+        fspl[((long)e * 3 + w) * NAP + i] = vi ? v : 0.0;
+    } else if (l == 0) {
+        epart[(long)e * ep_stride + ep_off + I] = ewv[0] + ewv[1] + ewv[2] + ewv[3];
+    }
+    if (VIR && w == 3 && l < 6)
+        vpart[((long)e * 6 + l) * ep_stride + ep_off + I] = wv[0][l] + wv[1][l] + wv[2][l] + wv[3][l];
+}
+
 // Virial of E evaluations: the 6 components' partials summed per evaluation.
 // grid: E, 64 threads.   out[e*6 + c], c = xx, yy, zz, xy, xz, yz
 __global__ __launch_bounds__(64) void k_vsum(const double *__restrict__ vpart, int ep_stride,
-                                             double *__restrict__ out)
+                                             int ep_n, double *__restrict__ out)
 {
     const int e = blockIdx.x;
     for (int c = 0; c < 6; ++c) {
-        const double v = wave_sum_array(vpart + ((long)e * 6 + c) * ep_stride, ep_stride);
+        const double v = wave_sum_array(vpart + ((long)e * 6 + c) * ep_stride, ep_n);
         if (threadIdx.x == 0) out[(long)e * 6 + c] = v;
     }
 }
@@ -821,10 +940,10 @@ __global__ __launch_bounds__(64) void k_vsum(const double *__restrict__ vpart, i
 // Energies of E evaluations (ordered sum of the partials minus the self term).
 // grid: 1 block of 64 threads per 64 evaluations.
 __global__ __launch_bounds__(64) void k_esum(int E, const double *__restrict__ epart, int ep_stride,
-                                             double eself, double *__restrict__ eout)
+                                             int ep_n, double eself, double *__restrict__ eout)
 {
     const int e = blockIdx.x;
-    const double v = eval_energy(epart, e, ep_stride, eself);
+    const double v = eval_energy(epart, e, ep_stride, ep_n, eself);
     if (threadIdx.x == 0) eout[e] = v;
 }
 
@@ -848,7 +967,8 @@ constexpr int SC_THREADS = 1024;
 __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__restrict__ q,
                                                     const double *__restrict__ fph,
                                                     const double *__restrict__ epart, int ep_stride,
-                                                    double eself, double *__restrict__ Vout,
+                                                    int ep_n, double eself,
+                                                    double *__restrict__ Vout,
                                                     const double *__restrict__ m_atom,
                                                     double *__restrict__ Fsc,
                                                     double *__restrict__ qpm,
@@ -860,7 +980,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
     const int b = blockIdx.x;
     const long o = (long)b * 3 * a.NAP;
     if (threadIdx.x < 64) {
-        const double v = eval_energy(epart, b, ep_stride, eself);
+        const double v = eval_energy(epart, b, ep_stride, ep_n, eself);
         if (threadIdx.x == 0) {
             s_V = v;
             Vout[b] = v;
@@ -1586,7 +1706,7 @@ __global__ __launch_bounds__(256) void k_vv_a(int n_atoms, int NAP, int NMP, dou
 __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfdt,
                                               const double *__restrict__ fe,
                                               const double *__restrict__ epart, int ep_stride,
-                                              double eself,
+                                              int ep_n, double eself,
                                               double *__restrict__ F, double *__restrict__ p,
                                               double *__restrict__ out,
                                               unsigned long long *__restrict__ step_ctr)
@@ -1599,7 +1719,7 @@ __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfd
         if (at < n_atoms) p[f] += halfdt * v;
     }
     if (blockIdx.x == 0 && threadIdx.x < 64) {
-        const double v = eval_energy(epart, 0, ep_stride, eself);
+        const double v = eval_energy(epart, 0, ep_stride, ep_n, eself);
         if (threadIdx.x == 0) {
             out[0] = v;
             out[1] = 0.0;
