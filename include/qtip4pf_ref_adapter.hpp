@@ -13,7 +13,11 @@
 // then link with -lqtip4pf.  Each function below has the reference's exact
 // signature (default arguments stay in the reference headers) and argument
 // meaning; the process globals of water.cpp:29-38 configure a lazily created
-// context on first use.  The spline (fast-CMD) variants are not provided.
+// context on first use.  The spline (fast-CMD) variants are defined when
+// Boost.Math's cubic_b_spline header is available (the reference needs it
+// for them anyway): the tables are read back from the caller's spline
+// objects at their knots (start 0, step 0.01 bohr, simulation.cpp:40-41,
+// 66-67, 93-94) and loaded once per set of spline objects.
 //
 // Random numbers: pile_g and update_sc_nospline(is_nvt) draw from the
 // caller's std::mt19937 with libstdc++'s normal_distribution and
@@ -54,6 +58,47 @@ double update_sc_nospline(double position_array[], double momentum_array[],
 double get_kcv(const double position_array[], const double force_array[]);
 double get_conserved(double position_array[], double momentum_array[], const double mass_array[],
                      double energy);
+
+#if defined(__has_include)
+#if __has_include(<boost/math/interpolators/cubic_b_spline.hpp>)
+#include <boost/math/interpolators/cubic_b_spline.hpp>
+#define QTIP4PF_ADAPTER_SPLINES 1
+// forces.h:18, 22; update.h:4, 6
+double get_forces_wspline(const double position_array[], double force_array[],
+                          const double charge_array[], const int point[], const int list[],
+                          const boost::math::cubic_b_spline<double> &oo_spline,
+                          const boost::math::cubic_b_spline<double> &oh_spline,
+                          const boost::math::cubic_b_spline<double> &hh_spline,
+                          const double oo_spline_cutsq, const double oh_spline_cutsq,
+                          const double hh_spline_cutsq, int bead);
+double force_eval_sc_wspline(const double position_array[], double force_array[],
+                             const double charge_array[], const double mass_array[],
+                             const int point[], const int list[],
+                             const boost::math::cubic_b_spline<double> &oo_spline,
+                             const boost::math::cubic_b_spline<double> &oh_spline,
+                             const boost::math::cubic_b_spline<double> &hh_spline,
+                             const double oo_spline_cutsq, const double oh_spline_cutsq,
+                             const double hh_spline_cutsq, double &sc_pot, double fphys[]);
+double update_trajectory_class_wspline(double position_array[], double momentum_array[],
+                                       const double mass_array[], double force_array[],
+                                       double charge_array[], int point[], int list[],
+                                       double disp_array[],
+                                       const boost::math::cubic_b_spline<double> &oo_spline,
+                                       const boost::math::cubic_b_spline<double> &oh_spline,
+                                       const boost::math::cubic_b_spline<double> &hh_spline,
+                                       const double oo_spline_cutsq, const double oh_spline_cutsq,
+                                       const double hh_spline_cutsq);
+double update_sc_wspline(double position_array[], double momentum_array[],
+                         const double mass_array[], double force_array[], double fphys[],
+                         double charge_array[], int point[], int list[], double disp_array[],
+                         std::mt19937 &rng, bool is_nvt, double &heat, double &sc_pot,
+                         const boost::math::cubic_b_spline<double> &oo_spline,
+                         const boost::math::cubic_b_spline<double> &oh_spline,
+                         const boost::math::cubic_b_spline<double> &hh_spline,
+                         const double oo_spline_cutsq, const double oh_spline_cutsq,
+                         const double hh_spline_cutsq);
+#endif
+#endif
 
 #ifdef QTIP4PF_ADAPTER_IMPLEMENTATION
 
@@ -225,6 +270,126 @@ double get_conserved(double position_array[], double momentum_array[], const dou
                         "get_conserved");
     return v;
 }
+
+#ifdef QTIP4PF_ADAPTER_SPLINES
+#include <cmath>
+
+namespace qtp_adapter {
+
+// simulation.cpp:40-41 (and the O-H, H-H copies): every table starts at 0
+// with a 0.01 bohr step; its range end is sqrt(cutsq).
+constexpr double spline_start = 0.0, spline_step = 0.01;
+
+inline void load_splines(const boost::math::cubic_b_spline<double> *sp[3], const double cutsq[3])
+{
+    static const void *last[3] = {nullptr, nullptr, nullptr};
+    static double last_cut[3] = {-1.0, -1.0, -1.0};
+    bool same = true;
+    for (int t = 0; t < 3; ++t) same = same && last[t] == sp[t] && last_cut[t] == cutsq[t];
+    if (same) return;
+    std::vector<double> tab[3];
+    for (int t = 0; t < 3; ++t) {
+        const long n = std::lround((std::sqrt(cutsq[t]) - spline_start) / spline_step) + 1;
+        tab[t].resize(n > 0 ? n : 0);
+        for (long j = 0; j < n; ++j) tab[t][j] = (*sp[t])(spline_start + j * spline_step);
+    }
+    report(qtp_set_splines(ctx(), tab[0].data(), (int)tab[0].size(), tab[1].data(),
+                           (int)tab[1].size(), tab[2].data(), (int)tab[2].size(), spline_start,
+                           spline_step),
+           "qtp_set_splines");
+    for (int t = 0; t < 3; ++t) {
+        last[t] = sp[t];
+        last_cut[t] = cutsq[t];
+    }
+}
+
+}  // namespace qtp_adapter
+
+#define QTP_ADAPTER_LOAD_SPLINES                                                               \
+    do {                                                                                      \
+        const boost::math::cubic_b_spline<double> *sp_[3] = {&oo_spline, &oh_spline, &hh_spline}; \
+        const double cs_[3] = {oo_spline_cutsq, oh_spline_cutsq, hh_spline_cutsq};             \
+        qtp_adapter::load_splines(sp_, cs_);                                                  \
+    } while (0)
+
+double get_forces_wspline(const double position_array[], double force_array[],
+                          const double charge_array[], const int point[], const int list[],
+                          const boost::math::cubic_b_spline<double> &oo_spline,
+                          const boost::math::cubic_b_spline<double> &oh_spline,
+                          const boost::math::cubic_b_spline<double> &hh_spline,
+                          const double oo_spline_cutsq, const double oh_spline_cutsq,
+                          const double hh_spline_cutsq, int bead)
+{
+    QTP_ADAPTER_LOAD_SPLINES;
+    double e = 0.0;
+    qtp_adapter::report(qtp_get_forces_wspline(qtp_adapter::ctx(), position_array, force_array,
+                                               charge_array, point, list, bead, &e),
+                        "get_forces_wspline");
+    return e;
+}
+
+double force_eval_sc_wspline(const double position_array[], double force_array[],
+                             const double charge_array[], const double mass_array[],
+                             const int point[], const int list[],
+                             const boost::math::cubic_b_spline<double> &oo_spline,
+                             const boost::math::cubic_b_spline<double> &oh_spline,
+                             const boost::math::cubic_b_spline<double> &hh_spline,
+                             const double oo_spline_cutsq, const double oh_spline_cutsq,
+                             const double hh_spline_cutsq, double &sc_pot, double fphys[])
+{
+    QTP_ADAPTER_LOAD_SPLINES;
+    double e = 0.0;
+    qtp_adapter::report(qtp_force_eval_sc_wspline(qtp_adapter::ctx(), position_array, force_array,
+                                                  charge_array, mass_array, point, list, &sc_pot,
+                                                  fphys, &e),
+                        "force_eval_sc_wspline");
+    return e;
+}
+
+double update_trajectory_class_wspline(double position_array[], double momentum_array[],
+                                       const double mass_array[], double force_array[],
+                                       double charge_array[], int point[], int list[],
+                                       double disp_array[],
+                                       const boost::math::cubic_b_spline<double> &oo_spline,
+                                       const boost::math::cubic_b_spline<double> &oh_spline,
+                                       const boost::math::cubic_b_spline<double> &hh_spline,
+                                       const double oo_spline_cutsq, const double oh_spline_cutsq,
+                                       const double hh_spline_cutsq)
+{
+    QTP_ADAPTER_LOAD_SPLINES;
+    double e = 0.0;
+    qtp_adapter::report(qtp_update_trajectory_class_wspline(qtp_adapter::ctx(), position_array,
+                                                            momentum_array, mass_array,
+                                                            force_array, charge_array, point,
+                                                            list, disp_array, &e),
+                        "update_trajectory_class_wspline");
+    return e;
+}
+
+double update_sc_wspline(double position_array[], double momentum_array[],
+                         const double mass_array[], double force_array[], double fphys[],
+                         double charge_array[], int point[], int list[], double disp_array[],
+                         std::mt19937 &rng, bool is_nvt, double &heat, double &sc_pot,
+                         const boost::math::cubic_b_spline<double> &oo_spline,
+                         const boost::math::cubic_b_spline<double> &oh_spline,
+                         const boost::math::cubic_b_spline<double> &hh_spline,
+                         const double oo_spline_cutsq, const double oh_spline_cutsq,
+                         const double hh_spline_cutsq)
+{
+    QTP_ADAPTER_LOAD_SPLINES;
+    std::vector<double> dev;
+    if (is_nvt) qtp_adapter::draw_pile(rng, dev);
+    double e = 0.0;
+    qtp_adapter::report(qtp_update_sc_wspline(qtp_adapter::ctx(), position_array, momentum_array,
+                                              mass_array, force_array, fphys, charge_array, point,
+                                              list, disp_array, is_nvt ? dev.data() : nullptr,
+                                              is_nvt ? 1 : 0, &heat, &sc_pot, &e),
+                        "update_sc_wspline");
+    return e;
+}
+
+#undef QTP_ADAPTER_LOAD_SPLINES
+#endif  // QTIP4PF_ADAPTER_SPLINES
 
 #endif  // QTIP4PF_ADAPTER_IMPLEMENTATION
 #endif  // QTIP4PF_REF_ADAPTER_HPP
