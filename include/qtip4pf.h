@@ -223,6 +223,23 @@ void qtp_batch_destroy(qtp_batch *b);
  * [3] list rebuilds so far, [4] steps so far, [5] status flags. */
 int qtp_get_scalars(qtp_ctx *ctx, double out[6]);
 
+/* Observables of the resident state, computed on the device (no state
+ * download): out[0] = K_cv (path_integrals.cpp:29-48), out[1] = ring-polymer
+ * conserved quantity (get_conserved, path_integrals.cpp:52-81) of the SC
+ * potential, out[2] = SC potential, out[3] = sc_pot.  For n_beads == 1:
+ * kinetic energy, V + KE, V, 0.  The spring term is summed in bead space,
+ * (1/2) m w_P^2 sum_b (q_{b+1} - q_b)^2, which the orthonormal transform
+ * makes equal to the reference's normal-mode sum (the reference's truncated
+ * pi in w_k moves it by ~1e-13 relative). */
+int qtp_get_observables(qtp_ctx *ctx, double out[4]);
+/* Advance n_steps resident steps recording the observables of qtp_get_observables
+ * on the device after every step whose counter is a multiple of `every`;
+ * trace[4 * i + c] receives sample i (at most n_steps / every + 1 samples,
+ * *n_samples set).  One host synchronisation at the end (the reference
+ * driver's per-sample download, simulation.cpp:951-960, goes away). */
+int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *trace,
+                     int *n_samples);
+
 /* Per-kernel timing of the last qtp_run (HIP events around each launch of the
  * named kernel class on the context's stream); for measurement only. */
 int qtp_enable_timing(qtp_ctx *ctx, int on);

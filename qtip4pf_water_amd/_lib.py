@@ -88,6 +88,8 @@ SIGNATURES = [
     ("qtp_set_seed", C.c_int, [_vp, C.c_uint64]),
     ("qtp_run", C.c_int, [_vp, C.c_int, C.c_int]),
     ("qtp_sync", C.c_int, [_vp]),
+    ("qtp_get_observables", C.c_int, [_vp, dp]),
+    ("qtp_run_observed", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, dp, ip]),
     ("qtp_batch_create", C.c_int, [C.POINTER(_vp), ip, C.c_int, C.POINTER(_vp)]),
     ("qtp_batch_run", C.c_int, [_vp, C.c_int]),
     ("qtp_batch_sync", C.c_int, [_vp]),

@@ -314,6 +314,22 @@ class Water:
     def sync(self):
         self._chk(self._L.qtp_sync(self._h))
 
+    def observables(self) -> dict:
+        """K_cv, conserved quantity, SC potential and sc_pot of the resident
+        state, computed on the device."""
+        o = np.zeros(4)
+        self._chk(self._L.qtp_get_observables(self._h, ptr(o)))
+        return {"kcv": o[0], "conserved": o[1], "energy": o[2], "sc_pot": o[3]}
+
+    def run_observed(self, n_steps: int, is_nvt: bool = False, every: int = 1):
+        """Resident steps with observables recorded every `every` steps on the
+        device; returns an (n_samples, 4) array (kcv, conserved, energy, sc_pot)."""
+        tr = np.zeros(4 * (int(n_steps) // int(every) + 1))
+        n = C.c_int()
+        self._chk(self._L.qtp_run_observed(self._h, int(n_steps), 1 if is_nvt else 0, int(every),
+                                           ptr(tr), C.byref(n)))
+        return tr[:4 * n.value].reshape(-1, 4)
+
     def scalars(self) -> dict:
         o = np.zeros(6)
         self._chk(self._L.qtp_get_scalars(self._h, ptr(o)))

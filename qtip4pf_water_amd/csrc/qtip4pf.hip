@@ -134,6 +134,11 @@ struct qtp_ctx {
     std::vector<double> h_mass_atom, h_charge;
     bool state_set = false;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // [is_nvt]
+    // resident observables: result, sample trace, and the graphs that record it
+    double *obs_out = nullptr, *trace = nullptr;
+    int *ntrace = nullptr;
+    int trace_cap = 0, obs_every = 0;
+    hipGraphExec_t graph_obs[2] = {nullptr, nullptr};
     bool timing = false;
     std::map<std::string, Timer> timers;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
@@ -386,6 +391,8 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->ppart, 2 * (size_t)ctx->ptile + 2));
     CHK(dalloc(ctx, &ctx->hpart, (size_t)ctx->P * ctx->ptile + 1));
     CHK(dalloc(ctx, &ctx->obs, (size_t)ctx->ptile * 4 + 64));
+    CHK(dalloc(ctx, &ctx->obs_out, 4));
+    CHK(dalloc(ctx, &ctx->ntrace, 1));
     ctx->dev_cap = 3 + (long)(P > 1 ? P - 1 : 0) * ctx->n_dof;
     CHK(dalloc(ctx, &ctx->dev, (size_t)ctx->dev_cap));
     return QTP_OK;
@@ -848,6 +855,11 @@ static void invalidate_graphs(qtp_ctx *ctx)
             hipGraphExecDestroy(g);
             g = nullptr;
         }
+    for (auto &g : ctx->graph_obs)
+        if (g) {
+            hipGraphExecDestroy(g);
+            g = nullptr;
+        }
 }
 
 // ===========================================================================
@@ -990,7 +1002,7 @@ void qtp_destroy(qtp_ctx *ctx)
                     ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
-                    ctx->ppart, ctx->hpart, ctx->dev, ctx->obs};
+                    ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->s2) hipStreamSynchronize(ctx->s2);
@@ -1722,6 +1734,105 @@ void qtp_batch_destroy(qtp_batch *b)
     if (!b) return;
     for (qtp_ctx *c : b->ctx) hipStreamSynchronize(c->stream);
     delete b;
+}
+
+// ---- resident observables ---------------------------------------------------
+
+// Observables of the resident state into dst (every == 0) or, on steps whose
+// counter is a multiple of every, into the next trace row.
+static int launch_obs(qtp_ctx *ctx, int every, double *dst)
+{
+    const int nb = (3 * ctx->NAP + 255) / 256;
+    k_obs_part<<<nb, 256, 0, ctx->stream>>>(ctx->P, ctx->n_atoms, ctx->NAP, ctx->drv.omega_psq,
+                                             ctx->q, ctx->p, ctx->fph, ctx->m_atom, ctx->obs);
+    k_obs_fin<<<1, 64, 0, ctx->stream>>>(ctx->P, nb, 0.5 * ctx->n_dof / ctx->drv.beta, ctx->obs,
+                                          ctx->out, ctx->step_ctr, every, ctx->trace_cap,
+                                          ctx->ntrace, dst);
+    HIPCHK(hipGetLastError());
+    return QTP_OK;
+}
+
+int qtp_get_observables(qtp_ctx *ctx, double out[4])
+{
+    if (!ctx || !out) return QTP_ERR_ARG;
+    if (!ctx->state_set) {
+        ctx->err = "qtp_get_observables: no resident state (call qtp_set_state)";
+        return QTP_ERR_STATE;
+    }
+    CHK(launch_obs(ctx, 0, ctx->obs_out));
+    HIPCHK(hipMemcpyAsync(out, ctx->obs_out, 4 * sizeof(double), hipMemcpyDeviceToHost,
+                          ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    return read_status(ctx);
+}
+
+int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *trace,
+                     int *n_samples)
+{
+    if (!ctx || n_steps < 0 || every < 1 || !trace) return QTP_ERR_ARG;
+    if (!ctx->state_set) {
+        ctx->err = "qtp_run_observed: no resident state (call qtp_set_state)";
+        return QTP_ERR_STATE;
+    }
+    const int nvt = (is_nvt && ctx->P >= 2) ? 1 : 0;
+    const int need = n_steps / every + 1;
+    if (need > ctx->trace_cap) {
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        invalidate_graphs(ctx);
+        if (ctx->trace) hipFree(ctx->trace);
+        ctx->trace = nullptr;
+        ctx->trace_cap = 0;
+        CHK(dalloc(ctx, &ctx->trace, (size_t)4 * need));
+        ctx->trace_cap = need;
+    }
+    if (every != ctx->obs_every) {
+        for (auto &g : ctx->graph_obs)
+            if (g) {
+                hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+        ctx->obs_every = every;
+    }
+    HIPCHK(hipMemsetAsync(ctx->ntrace, 0, sizeof(int), ctx->stream));
+    if (ctx->timing) {
+        for (int i = 0; i < n_steps; ++i) {
+            CHK(step_once(ctx, nvt));
+            CHK(launch_obs(ctx, every, ctx->trace));
+        }
+        CHK(flush_timing(ctx));
+    } else {
+        if (!ctx->graph_obs[nvt]) {
+            HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+            int st = step_once(ctx, nvt);
+            if (st == QTP_OK) st = launch_obs(ctx, every, ctx->trace);
+            hipGraph_t g = nullptr;
+            const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+            if (st != QTP_OK) {
+                if (g) hipGraphDestroy(g);
+                return st;
+            }
+            if (e != hipSuccess) {
+                ctx->err = std::string("graph capture: ") + hipGetErrorString(e);
+                return QTP_ERR_HIP;
+            }
+            const hipError_t ei = hipGraphInstantiate(&ctx->graph_obs[nvt], g, nullptr, nullptr, 0);
+            hipGraphDestroy(g);
+            if (ei != hipSuccess) {
+                ctx->graph_obs[nvt] = nullptr;
+                ctx->err = std::string("graph instantiate: ") + hipGetErrorString(ei);
+                return QTP_ERR_HIP;
+            }
+        }
+        for (int i = 0; i < n_steps; ++i) HIPCHK(hipGraphLaunch(ctx->graph_obs[nvt], ctx->stream));
+    }
+    int n = 0;
+    HIPCHK(hipMemcpyAsync(&n, ctx->ntrace, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    n = std::min(n, need);
+    if (n > 0)
+        HIPCHK(hipMemcpy(trace, ctx->trace, (size_t)4 * n * sizeof(double), hipMemcpyDeviceToHost));
+    if (n_samples) *n_samples = n;
+    return read_status(ctx);
 }
 
 int qtp_sync(qtp_ctx *ctx)

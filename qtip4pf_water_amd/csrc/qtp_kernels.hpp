@@ -1733,6 +1733,88 @@ __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfd
 // ---------------------------------------------------------------------------
 // get_kcv: 0.5 n_dof / beta + (1/P) sum_{even b} sum_j -(q_bj - qc_j) f_bj.
 // grid: ceil(3 NAP / 256); out: partials per block.
+// Observables of the resident state, per 256-dof block: the centroid-virial
+// sum -(q_b - qbar).f_b over even beads (path_integrals.cpp:29-48; centroid
+// divided by float(P) as :21), the kinetic energy sum p^2/2m and the ring
+// polymer spring energy (1/2) m w_P^2 sum_b (q_{b+1} - q_b)^2, which equals
+// get_conserved's normal-mode sum (1/2) m w_k^2 q~_k^2 (path_integrals.cpp:
+// 63-76) because the transform is orthonormal and w_k = 2 w_P sin(k pi / P).
+// part[blk], part[nb + blk], part[2 nb + blk].   grid: nb, 256 threads.
+__global__ __launch_bounds__(256) void k_obs_part(int P, int n_atoms, int NAP, double omega_psq,
+                                                  const double *__restrict__ q,
+                                                  const double *__restrict__ p,
+                                                  const double *__restrict__ f,
+                                                  const double *__restrict__ m_atom,
+                                                  double *__restrict__ part)
+{
+    __shared__ double sh[16];
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    double kc = 0.0, ke = 0.0, sp = 0.0;
+    if (t < 3 * NAP && (t % NAP) < n_atoms) {
+        const long N3 = 3L * NAP;
+        const double m = m_atom[t % NAP];
+        double qc = 0.0;
+        for (int b = 0; b < P; ++b) qc += q[b * N3 + t];
+        qc /= (double)(float)P;
+        for (int b = 0; b < P; ++b) {
+            const double qb = q[b * N3 + t], pb = p[b * N3 + t];
+            if ((b & 1) == 0) kc -= (qb - qc) * f[b * N3 + t];
+            ke += 0.5 * pb * pb / m;
+            if (P > 1) {
+                const double dq = q[((b + 1) % P) * N3 + t] - qb;
+                sp += 0.5 * m * omega_psq * dq * dq;
+            }
+        }
+    }
+    kc = block_sum(kc, sh);
+    ke = block_sum(ke, sh);
+    sp = block_sum(sp, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = kc;
+        part[gridDim.x + blockIdx.x] = ke;
+        part[2 * gridDim.x + blockIdx.x] = sp;
+    }
+}
+
+// Reduce k_obs_part's partials into {K_cv, conserved, energy, sc_pot} (P > 1)
+// or {kinetic energy, V + KE, V, 0} (P = 1).  With a trace, only on steps
+// whose counter is a multiple of `every`, into trace[n][4], n counted in
+// *ntrace (host-cleared before a run), at most cap samples.
+// grid: 1, 64 threads.
+__global__ __launch_bounds__(64) void k_obs_fin(int P, int nb, double half_ndof_over_beta,
+                                                const double *__restrict__ part,
+                                                const double *__restrict__ out,
+                                                const unsigned long long *__restrict__ step_ctr,
+                                                int every, int cap, int *__restrict__ ntrace,
+                                                double *__restrict__ dst)
+{
+    if (every > 0 && (step_ctr[0] % (unsigned long long)every) != 0ull) return;
+    const double kc = wave_sum_array(part, nb);
+    const double ke = wave_sum_array(part + nb, nb);
+    const double sp = wave_sum_array(part + 2 * nb, nb);
+    if (threadIdx.x == 0) {
+        double *o = dst;
+        if (every > 0) {
+            const int n = ntrace[0];
+            if (n >= cap) return;
+            o = dst + 4L * n;
+            ntrace[0] = n + 1;
+        }
+        const double e = out[0];
+        if (P > 1) {
+            o[0] = half_ndof_over_beta + kc / P;
+            o[1] = e + ke + sp;
+            o[2] = e;
+            o[3] = out[1];
+        } else {
+            o[0] = ke;
+            o[1] = e + ke;
+            o[2] = e;
+            o[3] = 0.0;
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_kcv(int P, int n_atoms, int NAP,
                                              const double *__restrict__ q,
                                              const double *__restrict__ f,
