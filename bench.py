@@ -36,6 +36,22 @@ KERNELS = ["realspace", "ksum", "sreduce", "kforce", "ljintra", "sites", "finali
            "sc2", "prop_a", "prop_b", "check", "nlist"]
 
 
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/pmc_traffic_sc_pimd_216x32.txt: FETCH_SIZE and WRITE_SIZE passes
+    of this workload, calibrated against a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_sc_pimd_216x32.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("TRAFFIC_BYTES_PER_LAUNCH"):
+                    kv = dict(t.split("=") for t in line.split()[1:])
+                    return float(kv[kernel]) if kernel in kv else None
+    except OSError:
+        return None
+    return None
+
+
 def workload(name):
     if name == "c2":
         return dict(n_side=6, L_angs=18.64, P=32, T_K=298.0, dtfs=0.25, nvt=True,
@@ -159,6 +175,24 @@ def run_batch(a):
     print(json.dumps(res), flush=True)
 
 
+def copy_calibration(mib):
+    """A plain device-to-device copy of `mib` MiB (the runtime's copy kernel),
+    three times: under rocprofv3 --pmc its FETCH_SIZE / WRITE_SIZE per
+    dispatch calibrate the counters against a known byte count."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    n = int(mib) << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(n)) or hip.hipMalloc(C.byref(b), C.c_size_t(n)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(n))
+    for _ in range(3):
+        hip.hipMemcpy(b, a, C.c_size_t(n), 3)   # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +203,8 @@ def main():
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calib-mib", type=int, default=0,
+                    help="after the run, copy this many MiB device-to-device (counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build shards nothing across GPUs "
@@ -250,7 +286,7 @@ def main():
             "peak": PEAK_FP64_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP64_TFLOPS,
-            "traffic": None,
+            "traffic": measured_traffic(dom) if a.workload == "c2" else None,
             "flops_per_launch": fl[dom] * evals_per_launch,
             "us_per_launch": kt[dom]["us_avg"],
         },
@@ -259,6 +295,8 @@ def main():
                   "status": sc["status"]},
     }
     w.close()
+    if a.copy_calib_mib > 0:
+        copy_calibration(a.copy_calib_mib)
     if not a.no_cpu:
         res["cpu_baseline"] = cpu_baseline(wl, q, p, box, n_mols, a.cpu_seconds)
         res["speedup_vs_cpu"] = sps / res["cpu_baseline"]["value"]
