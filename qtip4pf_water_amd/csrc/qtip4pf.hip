@@ -635,7 +635,7 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
 {
     hipStream_t s = ctx->stream;
     PropArgs A = prop_args(ctx, nvt, host_dev);
-    const dim3 grid(ctx->ptile, ctx->P);
+    const int grid = ctx->ptile * ctx->P;
     {
         TimeScope ts(ctx, "prop_a");
         k_prop_fwd<<<grid, 64, 0, s>>>(A, ctx->Cm, ctx->mode, ctx->q, ctx->p, ctx->F, ctx->m_atom,

@@ -47,6 +47,17 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // Block-wide sum (blockDim multiple of 64, <= 1024); result valid in thread 0.
+// Workgroups are dispatched to the 8 XCDs (each with its own L2) round
+// robin: workgroup b runs on XCD b % 8.  This maps b to a logical index so
+// that each XCD gets one contiguous eighth of the logical range: blocks that
+// share data (same evaluation, same dof tile) then share an L2.  Identity
+// when the grid is not a multiple of 8.
+__device__ __forceinline__ int xcd_remap(int b, int n)
+{
+    if (n & 7) return b;
+    return (b & 7) * (n >> 3) + (b >> 3);
+}
+
 __device__ __forceinline__ double block_sum(double v, double *sh)
 {
     v = wave_sum(v);
@@ -201,7 +212,8 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
                                                    double *__restrict__ epart, int ep_stride,
                                                    int ep_off, double *__restrict__ vpart)
 {
-    const int e = blockIdx.x / ntile, t = blockIdx.x - e * ntile;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int e = bid / ntile, t = bid - e * ntile;
     const int I = tiles[t].x, J = tiles[t].y;
     const int l = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -552,7 +564,8 @@ __global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
     const size_t redn = (size_t)KF_WAVES * 3 * 64;
     double *ef = CF + (cfn > redn ? cfn : redn);  // [3][64]
     const int nab = (n_atoms + KF_SITES - 1) / KF_SITES;
-    const int e = blockIdx.x / nab, ab = blockIdx.x - e * nab;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int e = bid / nab, ab = bid - e * nab;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int atom = ab * KF_SITES + lane;
@@ -682,7 +695,8 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
     constexpr int MPB = 256 / LJ_LPM;
     double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int nmb = (n_mols + MPB - 1) / MPB;
-    const int e = blockIdx.x / nmb, mb = blockIdx.x - e * nmb;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int e = bid / nmb, mb = bid - e * nmb;
     const int gi = threadIdx.x / LJ_LPM, t = threadIdx.x % LJ_LPM;
     const int i = mb * MPB + gi;
     const double *P = pos + (long)e * pstride;
@@ -1107,7 +1121,7 @@ __device__ __forceinline__ void cayley_half(double &uq, double &up, double m, do
 // planar dofs; modes k >= 1 also get their Langevin step and second half
 // step here (they do not wait for the centroid).  One wavefront per
 // (dof tile, mode); the transform rows are wave-uniform (scalar loads).
-// grid: (ntile, P), 64 threads.
+// grid: ntile * P (tile-major, XCD-remapped), 64 threads.
 // part[tile] = centroid KE after the first half step (tile partial);
 // part[ntile + tile] = chi-square partial (device random numbers);
 // hpart[k * ntile + tile] = KE before - KE after of mode k's Langevin step.
@@ -1124,7 +1138,8 @@ __global__ __launch_bounds__(64) void k_prop_fwd(PropArgs A, const double *__res
                                                  double *__restrict__ hpart,
                                                  const unsigned long long *__restrict__ step_ctr)
 {
-    const int P = A.P, k = blockIdx.y, tile = blockIdx.x, lane = threadIdx.x;
+    const int P = A.P, lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = lb / P, k = lb - tile * P, lane = threadIdx.x;
     const long N3 = 3L * A.NAP;
     const int f = tile * 64 + lane;
     const int at = f % A.NAP, d = f / A.NAP;
@@ -1182,7 +1197,7 @@ __global__ __launch_bounds__(64) void k_prop_fwd(PropArgs A, const double *__res
 
 // Centroid rescale (Bussi, tau -> 0), its second half step, inverse
 // transform to bead b and displacement tracking.  One wavefront per
-// (dof tile, bead).   grid: (ntile, P), 64 threads.
+// (dof tile, bead).   grid: ntile * P (tile-major, XCD-remapped), 64 threads.
 // hpart[tile] (mode-0 slot) = - centroid KE after the rescale (bead-0 waves).
 __global__ __launch_bounds__(64) void k_prop_inv(PropArgs A, const double *__restrict__ Cm,
                                                  const double4 *__restrict__ mode,
@@ -1196,7 +1211,8 @@ __global__ __launch_bounds__(64) void k_prop_inv(PropArgs A, const double *__res
                                                  double *__restrict__ hpart,
                                                  const unsigned long long *__restrict__ step_ctr)
 {
-    const int P = A.P, b = blockIdx.y, tile = blockIdx.x, lane = threadIdx.x;
+    const int P = A.P, lb = xcd_remap(blockIdx.x, gridDim.x);
+    const int tile = lb / P, b = lb - tile * P, lane = threadIdx.x;
     const long N3 = 3L * A.NAP;
     const int f = tile * 64 + lane;
     const int at = f % A.NAP, d = f / A.NAP;
