@@ -578,20 +578,34 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
     return QTP_OK;
 }
 
-static int run_lists(qtp_ctx *ctx, int force_all)
+// with_heat: also reduce a thermostatted step's heat (folded into the fused
+// list kernel; a kernel of its own on the multi-kernel path)
+static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
 {
     hipStream_t s = ctx->stream;
     const int P = ctx->P;
+    HeatArgs heat{};
+    if (with_heat) {
+        heat.part = ctx->ppart;
+        heat.hpart = ctx->hpart;
+        heat.heat_acc = ctx->heat;
+        heat.heat_step = ctx->heat_step;
+        heat.ntile = ctx->ptile;
+        heat.nh = ctx->P * ctx->ptile;
+    }
     if (ctx->n_mols <= NL_MAXMOL) {
         TimeScope ts(ctx, "nlist");
         k_nlist_bead<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box,
                                         RC.r_skin_sq, RC.r_skin - RC.r_cut, force_all,
                                         ctx->list_size, ctx->fcap, ctx->disp, ctx->flag, ctx->nreb,
                                         ctx->point, ctx->fpoint, ctx->list, ctx->flist,
-                                        ctx->status);
+                                        ctx->status, heat);
         HIPCHK(hipGetLastError());
         return QTP_OK;
     }
+    if (with_heat)
+        k_heat<<<1, 64, 0, s>>>(heat.ntile, heat.nh, heat.part, heat.hpart, heat.heat_acc,
+                                heat.heat_step);
     {
         TimeScope ts(ctx, "check");
         k_check<<<P, 256, 0, s>>>(ctx->n_mols, ctx->NMP, ctx->disp, RC.r_skin - RC.r_cut, force_all,
@@ -648,11 +662,8 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
                                        ctx->ppart, ctx->dev, ctx->q, ctx->p, ctx->disp, ctx->NMP,
                                        ctx->hpart, ctr);
     }
-    if (nvt)
-        k_heat<<<1, 64, 0, s>>>(ctx->ptile, ctx->P * ctx->ptile, ctx->ppart, ctx->hpart, ctx->heat,
-                                ctx->heat_step);
     HIPCHK(hipGetLastError());
-    CHK(run_lists(ctx, 0));
+    CHK(run_lists(ctx, 0, nvt != 0));
     CHK(run_sc_forces(ctx, 1, ctr));
     return QTP_OK;
 }

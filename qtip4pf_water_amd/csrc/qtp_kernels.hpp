@@ -412,6 +412,7 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
     if (g >= G) return;
     const int4 gr = groups[g];
     double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0, s4 = 0.0, s5 = 0.0, s6 = 0.0, s7 = 0.0;
+#pragma unroll 4
     for (int a = 0; a < AC; ++a) {
         const double2 Xv = T[(a * 3 + 0) * K + gr.x];
         const double2 Yv = T[(a * 3 + 1) * K + gr.y];
@@ -1548,6 +1549,14 @@ __global__ __launch_bounds__(256) void k_nl_fill(int n_mols, int NAP, const doub
 // per wavefront with ballot + prefix-popcount compaction; the distance test
 // is bit-for-bit neighbour.cpp:42-53.   grid: P.
 constexpr int NL_MAXMOL = 4096;
+// Optional heat sum of a thermostatted step (k_heat's work), done by wave 0
+// of block 0 here so that it costs no launch of its own.
+struct HeatArgs {
+    const double *part;  // centroid KE partials (ntile) then chi partials
+    const double *hpart; // per-(mode, tile) heat partials (nh)
+    double *heat_acc, *heat_step;
+    int ntile, nh;
+};
 __global__ __launch_bounds__(1024) void k_nlist_bead(int n_mols, int NAP, int NMP,
                                                      const double *__restrict__ q, Box box,
                                                      double r_skin_sq, double skin, int force_all,
@@ -1559,12 +1568,21 @@ __global__ __launch_bounds__(1024) void k_nlist_bead(int n_mols, int NAP, int NM
                                                      int *__restrict__ fpoint,
                                                      int *__restrict__ list,
                                                      int *__restrict__ flist,
-                                                     int *__restrict__ status)
+                                                     int *__restrict__ status, HeatArgs heat)
 {
     __shared__ double s1[1024], s2[1024];
     __shared__ int ch[NL_MAXMOL], cf[NL_MAXMOL];
     __shared__ int s_flag, s_tot[2];
     const int b = blockIdx.x, tid = threadIdx.x;
+    if (heat.part && b == 0 && tid < 64) {
+        const double ke = wave_sum_array(heat.part, heat.ntile);
+        const double h = wave_sum_array(heat.hpart, heat.nh);
+        if (tid == 0) {
+            const double ret = ke + h;
+            heat.heat_step[0] = ret;
+            heat.heat_acc[0] -= ret;
+        }
+    }
     double *D = disp + (long)b * 3 * NMP;
     // --- two largest O displacements
     double m1 = 0.0, m2 = 0.0;
