@@ -175,6 +175,37 @@ int qtp_host_rng_create(uint32_t seed, qtp_host_rng **out);
 void qtp_host_rng_destroy(qtp_host_rng *r);
 int qtp_host_rng_draw_pile(qtp_host_rng *r, int n_atoms, int n_beads, double *dev);
 
+/* ---- starts and position files (host only) ------------------------------ */
+
+/* initialise.cpp:20-98 with the box of water.cpp:90-97, 117: 4*nx*ny*nz
+ * molecules on an fcc lattice at density rho_inp (g cm^-3), every bead a
+ * copy of the first.  pos: n_beads * 36*nx*ny*nz doubles (reference layout);
+ * boxl (may be NULL): the box edges in bohr. */
+int qtp_initialise_fcc(int nx, int ny, int nz, double rho_inp, int n_beads, double *pos,
+                       double *boxl);
+/* initialise.cpp:189-541 with water.cpp:101-109, 117: 8*nx*ny*nz molecules
+ * of proton-disordered hexagonal ice on the orthorhombic cell, satisfying the
+ * ice rules with zero net dipole, drawn from rng in the reference's order
+ * (initialise_orth(rng, outfile)).  max_outer > 0 bounds the number of fresh
+ * hydrogen assignments (the reference loops until one has zero dipole;
+ * QTP_ERR_STATE when the bound is hit); n_outer (may be NULL) returns how
+ * many were made. */
+int qtp_initialise_orth(qtp_host_rng *rng, int nx, int ny, int nz, double rho_inp, int n_beads,
+                        long max_outer, double *pos, double *boxl, long *n_outer);
+/* simulation.cpp:826-871: Maxwell-Boltzmann momenta of every bead at the
+ * bead temperature n_beads*T_K (an O and an H normal_distribution drawn from
+ * rng in the reference's order), minus the centre-of-mass momentum. */
+int qtp_host_rng_draw_momenta(qtp_host_rng *rng, int n_mols, int n_beads, double T_K,
+                              double *p);
+/* initialise.cpp:107-135 file format (one atom per line, "x y z" in bohr).
+ * A file of n_atoms lines is copied to every bead, as the reference does; a
+ * file of n_beads*n_atoms lines (water.cpp:226-238, final_positions.txt)
+ * keeps every bead, which the reference's reader does not.  beads_read (may
+ * be NULL): 1 or n_beads. */
+int qtp_read_positions(const char *path, int n_mols, int n_beads, double *pos, int *beads_read);
+/* Every bead, one atom per line, 17 significant digits (read back exactly). */
+int qtp_write_positions(const char *path, int n_mols, int n_beads, const double *pos);
+
 /* ---- resident mode ------------------------------------------------------ */
 
 /* Upload q and p (reference layout), charges and masses (NULL: qTIP4P/F
@@ -191,6 +222,17 @@ int qtp_get_lists(qtp_ctx *ctx, int *point, int *list);
  * virial[b*6 + c]; energy[b] = V of bead b if energy is not NULL.  No
  * counterpart in the reference.  Leaves the resident state untouched. */
 int qtp_get_virial(qtp_ctx *ctx, double *virial, double *energy);
+
+/* Lossless checkpoint of the resident state (the reference restarts from
+ * positions only, collapsing the beads: initialise.cpp:107-135).  Saves q,
+ * p, displacements, half lists, rebuild counts, accumulated heat, the
+ * thermostat seed and step counter; qtp_load_checkpoint (in place of
+ * qtp_set_state; charges/masses as there) restores them into a context of
+ * the same system and recomputes the forces, so the run continues exactly
+ * (bit for bit) as it would have without the stop. */
+int qtp_save_checkpoint(qtp_ctx *ctx, const char *path);
+int qtp_load_checkpoint(qtp_ctx *ctx, const char *path, const double *charge,
+                        const double *mass);
 
 /* Thermostat random numbers for resident NVT steps: Philox4x32-10 streams
  * keyed by seed and the device step counter, Box-Muller normals; the centroid

@@ -274,6 +274,11 @@ def ref():
         L.ref_bussi.restype = C.c_double
         L.ref_bussi.argtypes = [_dp, _dp]
         L.ref_draw_pile_deviates.argtypes = [_dp]
+        L.ref_set_lattice.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, _dp]
+        L.ref_initialise.argtypes = [C.c_int, _dp]
+        L.ref_initialise_fromfile.argtypes = [C.c_char_p, _dp]
+        L.ref_draw_momenta.argtypes = [_dp]
+        L.ref_set_beta_n.argtypes = [C.c_double]
         _ref = L
     return _ref
 
@@ -282,3 +287,20 @@ def ref_set_from(o: Oracle) -> None:
     s = o.sys
     b = np.array([s.boxl[0], s.boxl[1], s.boxl[2]])
     ref().ref_set_globals(s.n_mols, s.n_beads, _ptr(b), s.list_size, s.beta, s.beta_n, s.dt)
+
+
+def ref_lattice(orth: bool, nx: int, ny: int, nz: int, rho: float, n_beads: int = 1,
+                seed: int | None = None):
+    """The reference's own initialise_fcc / initialise_orth
+    (initialise.cpp:20-98, :334-541) with water.cpp's lattice globals; orth
+    draws from the harness mt19937 seeded with `seed`.  Returns (positions,
+    box)."""
+    R = ref()
+    box = np.zeros(3)
+    R.ref_set_lattice(1 if orth else 0, nx, ny, nz, float(rho), n_beads, _ptr(box))
+    if seed is not None:
+        R.ref_seed(seed)
+    n = n_beads * (72 if orth else 36) * nx * ny * nz
+    out = np.zeros(n)
+    R.ref_initialise(1 if orth else 0, _ptr(out))
+    return out, tuple(box)

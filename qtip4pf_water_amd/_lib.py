@@ -81,9 +81,17 @@ SIGNATURES = [
     ("qtp_host_rng_create", C.c_int, [C.c_uint32, C.POINTER(_vp)]),
     ("qtp_host_rng_destroy", None, [_vp]),
     ("qtp_host_rng_draw_pile", C.c_int, [_vp, C.c_int, C.c_int, dp]),
+    ("qtp_host_rng_draw_momenta", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, dp]),
+    ("qtp_initialise_fcc", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_int, dp, dp]),
+    ("qtp_initialise_orth", C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                      C.c_long, dp, dp, C.POINTER(C.c_long)]),
+    ("qtp_read_positions", C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, C.POINTER(C.c_int)]),
+    ("qtp_write_positions", C.c_int, [C.c_char_p, C.c_int, C.c_int, dp]),
     ("qtp_set_state", C.c_int, [_vp, dp, dp, dp, dp]),
     ("qtp_get_state", C.c_int, [_vp, dp, dp, dp, dp, dp]),
     ("qtp_get_lists", C.c_int, [_vp, ip, ip]),
+    ("qtp_save_checkpoint", C.c_int, [_vp, C.c_char_p]),
+    ("qtp_load_checkpoint", C.c_int, [_vp, C.c_char_p, dp, dp]),
     ("qtp_get_virial", C.c_int, [_vp, dp, dp]),
     ("qtp_set_seed", C.c_int, [_vp, C.c_uint64]),
     ("qtp_run", C.c_int, [_vp, C.c_int, C.c_int]),

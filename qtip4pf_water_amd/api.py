@@ -20,6 +20,7 @@ draws them; `HostRng` draws them from std::mt19937 with libstdc++.
 from __future__ import annotations
 
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -64,6 +65,30 @@ class HostRng:
             raise QtpError(rc, "host rng draw")
         return out
 
+    def draw_momenta(self, n_mols: int, n_beads: int, T_K: float) -> np.ndarray:
+        """Initial momenta of every bead (simulation.cpp:826-871)."""
+        out = np.zeros(n_beads * 9 * n_mols)
+        rc = self._L.qtp_host_rng_draw_momenta(self._h, int(n_mols), int(n_beads), float(T_K),
+                                               ptr(out))
+        if rc != 0:
+            raise QtpError(rc, "host rng draw_momenta")
+        return out
+
+    def initialise_orth(self, nx: int, ny: int, nz: int, rho_inp: float = 0.92,
+                        n_beads: int = 1, max_outer: int = 0):
+        """Proton-disordered ice, ice rules and zero dipole, drawn from this
+        generator (initialise.cpp:334-541).  Returns (positions, box, number
+        of hydrogen assignments tried)."""
+        pos = np.zeros(n_beads * 72 * nx * ny * nz)
+        box = np.zeros(3)
+        n_outer = C.c_long()
+        rc = self._L.qtp_initialise_orth(self._h, int(nx), int(ny), int(nz), float(rho_inp),
+                                         int(n_beads), int(max_outer), ptr(pos), ptr(box),
+                                         C.byref(n_outer))
+        if rc != 0:
+            raise QtpError(rc, "initialise_orth")
+        return pos, tuple(box), n_outer.value
+
     def __del__(self):
         try:
             if self._h:
@@ -81,6 +106,38 @@ def philox4x32_10(ctr, key):
     if rc != 0:
         raise QtpError(rc, "philox")
     return o
+
+
+def initialise_fcc(nx: int, ny: int, nz: int, rho_inp: float = 1.0, n_beads: int = 1):
+    """Molecules on an fcc lattice (initialise.cpp:20-98); returns (positions,
+    box in bohr) for 4*nx*ny*nz molecules, every bead a copy."""
+    pos = np.zeros(n_beads * 36 * nx * ny * nz)
+    box = np.zeros(3)
+    rc = _lib.load().qtp_initialise_fcc(int(nx), int(ny), int(nz), float(rho_inp), int(n_beads),
+                                        ptr(pos), ptr(box))
+    if rc != 0:
+        raise QtpError(rc, "initialise_fcc")
+    return pos, tuple(box)
+
+
+def read_positions(path, n_mols: int, n_beads: int):
+    """Positions in the reference's file format (initialise.cpp:107-135);
+    returns (positions, beads read: 1 or n_beads)."""
+    pos = np.zeros(n_beads * 9 * n_mols)
+    nb = C.c_int()
+    rc = _lib.load().qtp_read_positions(os.fsencode(path), int(n_mols), int(n_beads), ptr(pos),
+                                        C.byref(nb))
+    if rc != 0:
+        raise QtpError(rc, f"read_positions: {path}")
+    return pos, nb.value
+
+
+def write_positions(path, pos, n_mols: int, n_beads: int):
+    """Every bead, one atom per line, exact on reading back."""
+    rc = _lib.load().qtp_write_positions(os.fsencode(path), int(n_mols), int(n_beads),
+                                         ptr(_f64(pos)))
+    if rc != 0:
+        raise QtpError(rc, f"write_positions: {path}")
 
 
 class Water:
@@ -304,6 +361,14 @@ class Water:
         v, e = np.zeros(6 * self.n_beads), np.zeros(self.n_beads)
         self._chk(self._L.qtp_get_virial(self._h, ptr(v), ptr(e)))
         return np.stack([voigt_to_matrix(v[6 * b:6 * b + 6]) for b in range(self.n_beads)]), e
+
+    def save_checkpoint(self, path):
+        self._chk(self._L.qtp_save_checkpoint(self._h, os.fsencode(path)))
+
+    def load_checkpoint(self, path, charge=None, mass=None):
+        self._chk(self._L.qtp_load_checkpoint(self._h, os.fsencode(path),
+                                              ptr(_f64(charge)) if charge is not None else None,
+                                              ptr(_f64(mass)) if mass is not None else None))
 
     def set_seed(self, seed: int):
         self._chk(self._L.qtp_set_seed(self._h, C.c_uint64(seed)))

@@ -12,6 +12,7 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 SRC = os.path.join(PKG, "csrc", "qtip4pf.hip")
+SETUP_SRC = os.path.join(PKG, "csrc", "qtp_setup.cpp")  # host-only
 OUT = os.path.join(PKG, "libqtip4pf.so")
 ARCH = os.environ.get("QTP_OFFLOAD_ARCH", "gfx950")
 
@@ -41,7 +42,7 @@ def build(verbose: bool = False) -> str:
     deps = [SRC, os.path.join(PKG, "csrc", "qtp_kernels.hpp"),
             os.path.join(ROOT, "include", "qtip4pf.h")]
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC, SETUP_SRC]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -16,56 +16,12 @@
 
 #include "../../include/qtip4pf.h"
 #include "qtp_kernels.hpp"
+#include "qtp_host.hpp"
 
 using namespace qtp;
 
 namespace {
 
-// qtip4pfparams.h:43-133, evaluated in the reference's order (pi truncated, :45)
-struct RefConsts {
-    double pi, angs2bohr, me, kb, Na, eh, kcal2au, mo, mh, K2au, au2fs, fs2au;
-    double r_cut, r_cut_sq, r_skin, r_skin_sq;
-    double gamma_vs, epslj, sigmalj, sigmalj6, sigmalj12, e_cut;
-    double roheq, ar, ar2, ar3, ar4fac, Dr, theta_eq, ktheta, qo, qh;
-    RefConsts()
-    {
-        pi = 3.14159265359;
-        angs2bohr = 1.0 / 0.529177210903;
-        me = 9.10938356e-31;
-        kb = 1.38064852e-23;
-        Na = 6.0221409e23;
-        eh = 4.3597447222071e-18;
-        kcal2au = 1.5936e-3;
-        const double mhgmol = 1.0080, mogmol = 15.9994;
-        mo = 0.001 * mogmol / (Na * me);
-        mh = 0.001 * mhgmol / (Na * me);
-        K2au = kb / eh;
-        const double hbar_SI = 1.054571817e-34;
-        au2fs = hbar_SI * 1e15 / eh;
-        fs2au = 1.0 / au2fs;
-        r_cut = 9.0 * angs2bohr;
-        r_cut_sq = r_cut * r_cut;
-        r_skin = 1.30 * r_cut;
-        r_skin_sq = r_skin * r_skin;
-        gamma_vs = 0.73612;
-        epslj = 0.1852 * kcal2au;
-        sigmalj = 3.1589 * angs2bohr;
-        sigmalj6 = std::pow(sigmalj, 6.0);
-        sigmalj12 = sigmalj6 * sigmalj6;
-        e_cut = 4.0 * epslj * (std::pow((sigmalj / r_cut), 12.0) - std::pow((sigmalj / r_cut), 6.0));
-        roheq = 0.9419 * angs2bohr;
-        ar = 2.287 / angs2bohr;
-        ar2 = ar * ar;
-        ar3 = ar2 * ar;
-        ar4fac = 7.0 * ar3 * ar / 12.0;
-        Dr = 116.09 * kcal2au;
-        theta_eq = 107.4 * pi / 180.0;
-        ktheta = 87.85 * kcal2au;
-        qo = -1.1128;
-        qh = -0.5 * qo;
-    }
-};
-const RefConsts RC;
 
 struct Timer {
     double ms = 0.0;
@@ -73,10 +29,6 @@ struct Timer {
 };
 
 }  // namespace
-
-struct qtp_host_rng {
-    std::mt19937 gen;
-};
 
 struct qtp_ctx {
     qtp_params prm{};
@@ -1569,6 +1521,126 @@ int qtp_get_state(qtp_ctx *ctx, double *q, double *p, double *F, double *fphys, 
     if (F) CHK(download_planar(ctx, ctx->F, P, F));
     if (fphys) CHK(download_planar(ctx, ctx->fph, P, fphys));
     if (disp) CHK(download_disp(ctx, disp, 0, P));
+    return QTP_OK;
+}
+
+// ---- checkpoint / restart ------------------------------------------------------
+
+// File layout (little endian): "QTPCKPT1", int32 version (1), int32 n_mols,
+// n_beads, list_size; double boxl[3], T_K, dtfs; uint64 seed, step counter;
+// double heat; then in the reference layout: q, p (n_beads * n_dof),
+// disp (n_beads * 3 * n_mols), point (n_beads * n_mols int32),
+// list (n_beads * list_size int32), rebuild counts (n_beads doubles).
+static const char QTP_CKPT_MAGIC[8] = {'Q', 'T', 'P', 'C', 'K', 'P', 'T', '1'};
+
+int qtp_save_checkpoint(qtp_ctx *ctx, const char *path)
+{
+    if (!ctx || !path) return QTP_ERR_ARG;
+    if (!ctx->state_set) {
+        ctx->err = "qtp_save_checkpoint: no resident state";
+        return QTP_ERR_STATE;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    const int P = ctx->P;
+    const size_t nd = (size_t)P * ctx->n_dof, nm = (size_t)P * 3 * ctx->n_mols;
+    std::vector<double> q(nd), p(nd), disp(nm), nreb(P);
+    std::vector<int> point((size_t)P * ctx->n_mols), list((size_t)P * ctx->list_size);
+    CHK(qtp_get_state(ctx, q.data(), p.data(), nullptr, nullptr, disp.data()));
+    CHK(qtp_get_lists(ctx, point.data(), list.data()));
+    unsigned long long ctr = 0;
+    double heat = 0.0;
+    HIPCHK(hipMemcpy(&ctr, ctx->step_ctr, sizeof(ctr), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(&heat, ctx->heat, sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(nreb.data(), ctx->nreb, P * sizeof(double), hipMemcpyDeviceToHost));
+    FILE *f = std::fopen(path, "wb");
+    if (!f) {
+        ctx->err = std::string("cannot open ") + path;
+        return QTP_ERR_ARG;
+    }
+    const int32_t hdr[4] = {1, ctx->n_mols, P, ctx->list_size};
+    const double par[5] = {ctx->drv.boxl[0], ctx->drv.boxl[1], ctx->drv.boxl[2], ctx->prm.T_K,
+                           ctx->prm.dtfs};
+    const uint64_t u[2] = {ctx->seed, ctr};
+    bool ok = std::fwrite(QTP_CKPT_MAGIC, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 &&
+              std::fwrite(par, 8, 5, f) == 5 && std::fwrite(u, 8, 2, f) == 2 &&
+              std::fwrite(&heat, 8, 1, f) == 1 && std::fwrite(q.data(), 8, nd, f) == nd &&
+              std::fwrite(p.data(), 8, nd, f) == nd && std::fwrite(disp.data(), 8, nm, f) == nm &&
+              std::fwrite(point.data(), 4, point.size(), f) == point.size() &&
+              std::fwrite(list.data(), 4, list.size(), f) == list.size() &&
+              std::fwrite(nreb.data(), 8, P, f) == (size_t)P;
+    ok = (std::fclose(f) == 0) && ok;
+    if (!ok) {
+        ctx->err = std::string("write failed: ") + path;
+        return QTP_ERR_ARG;
+    }
+    return QTP_OK;
+}
+
+int qtp_load_checkpoint(qtp_ctx *ctx, const char *path, const double *charge, const double *mass)
+{
+    if (!ctx || !path) return QTP_ERR_ARG;
+    FILE *f = std::fopen(path, "rb");
+    if (!f) {
+        ctx->err = std::string("cannot open ") + path;
+        return QTP_ERR_ARG;
+    }
+    const int P = ctx->P;
+    const size_t nd = (size_t)P * ctx->n_dof, nm = (size_t)P * 3 * ctx->n_mols;
+    char magic[8];
+    int32_t hdr[4];
+    double par[5], heat = 0.0;
+    uint64_t u[2];
+    std::vector<double> q(nd), p(nd), disp(nm), nreb(P);
+    std::vector<int> point((size_t)P * ctx->n_mols), list((size_t)P * ctx->list_size);
+    bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, QTP_CKPT_MAGIC, 8) == 0 &&
+              std::fread(hdr, 4, 4, f) == 4 && hdr[0] == 1 && hdr[1] == ctx->n_mols &&
+              hdr[2] == P && hdr[3] == ctx->list_size && std::fread(par, 8, 5, f) == 5 &&
+              std::fread(u, 8, 2, f) == 2 && std::fread(&heat, 8, 1, f) == 1 &&
+              std::fread(q.data(), 8, nd, f) == nd && std::fread(p.data(), 8, nd, f) == nd &&
+              std::fread(disp.data(), 8, nm, f) == nm &&
+              std::fread(point.data(), 4, point.size(), f) == point.size() &&
+              std::fread(list.data(), 4, list.size(), f) == list.size() &&
+              std::fread(nreb.data(), 8, P, f) == (size_t)P;
+    std::fclose(f);
+    if (!ok) {
+        ctx->err = std::string("not a checkpoint of this system: ") + path;
+        return QTP_ERR_ARG;
+    }
+    for (int k = 0; k < 3; ++k)
+        if (par[k] != ctx->drv.boxl[k]) {
+            ctx->err = "checkpoint box differs from the context's";
+            return QTP_ERR_ARG;
+        }
+    SplineScope sp(ctx, ctx->spl_resident);
+    CHK(check_finite(ctx, q.data(), (long)nd, "positions"));
+    CHK(check_finite(ctx, p.data(), (long)nd, "momenta"));
+    invalidate_graphs(ctx);
+    CHK(upload_charges_masses(ctx, charge, mass));
+    CHK(upload_planar(ctx, ctx->q, q.data(), P));
+    CHK(upload_planar(ctx, ctx->p, p.data(), P));
+    CHK(upload_disp(ctx, disp.data(), 0, P));
+    // the saved half lists, and the full lists expanded from them: each row in
+    // ascending partner order, as the list kernels write it, so the force
+    // sums run in the same order as before the stop
+    CHK(upload_lists(ctx, point.data(), list.data(), 0, P, 0));
+    HIPCHK(hipMemset(ctx->status, 0, sizeof(int)));
+    if (P >= 2) {
+        CHK(run_sc_forces(ctx, 0, nullptr));
+    } else {
+        CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, ctx->Ve));
+        HIPCHK(hipMemcpyAsync(ctx->F, ctx->fph, 3 * ctx->NAP * sizeof(double),
+                              hipMemcpyDeviceToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->out, ctx->Ve, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    CHK(flush_timing(ctx));
+    CHK(read_status(ctx));
+    HIPCHK(hipMemcpy(ctx->heat, &heat, sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->nreb, nreb.data(), P * sizeof(double), hipMemcpyHostToDevice));
+    const unsigned long long ctr = u[1];
+    HIPCHK(hipMemcpy(ctx->step_ctr, &ctr, sizeof(ctr), hipMemcpyHostToDevice));
+    ctx->seed = u[0];
+    ctx->state_set = true;
     return QTP_OK;
 }
 
