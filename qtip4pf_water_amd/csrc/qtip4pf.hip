@@ -898,7 +898,7 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->G = (int)g.size();
     ctx->drv.n_kgroups = ctx->G;
     ctx->ngc = (ctx->G + 255) / 256;
-    ctx->ngr = (ctx->G + 255) / 256;
+    ctx->ngr = (ctx->G + SR_GPB - 1) / SR_GPB;
     const int K = ctx->ew.K;
     ctx->AC = std::max(1, std::min(64, (int)(48 * 1024 / (3 * K * 16))));
     ctx->nac = (ctx->n_atoms + ctx->AC - 1) / ctx->AC;

@@ -436,11 +436,14 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 // Sum the partial structure factors, energy per group (forces.cpp:274, 315,
 // 364, 428) and the force record of k_kforce from S' = 2 c S of the variants
 // the reference evaluates (gr.w bit v set; others zero).
-// grid: E * ceil(G/256).
+// Eight lanes per group, lane v <-> component v of the 8 partial sums, so the
+// chunk partials are read as contiguous 64-byte rows; the chunks are summed
+// in ascending order.  grid: E * ceil(G / SR_GPB), 256 threads.
 // With vpart set, also the reciprocal-space virial tensor
 //   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 kappa^2)))
 // (E_k = c |S(k)|^2 per sign variant; the strain derivative of kwald's sum at
 // fixed Ewald parameters).
+constexpr int SR_GPB = 32;  // groups per block
 __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spart, int nac, int G,
                                                  const int4 *__restrict__ groups,
                                                  const double *__restrict__ gcf,
@@ -450,69 +453,70 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
                                                  double q4, double *__restrict__ vpart)
 {
     __shared__ double sh[16];
-    const int nb = (G + 255) / 256;
+    const int nb = (G + SR_GPB - 1) / SR_GPB;
     const int e = blockIdx.x / nb, gb = blockIdx.x - e * nb;
-    const int g = gb * 256 + threadIdx.x;
-    double en = 0.0;
+    const int g = gb * SR_GPB + (threadIdx.x >> 3);
+    const int v = threadIdx.x & 7;
+    const bool live = g < G;
+    double sv = 0.0, cf = 0.0;
+    if (live) {
+        const double *p = spart + ((long)e * nac * G + g) * 8 + v;
+        const long cs = (long)G * 8;
+        int ac = 0;
+        // eight chunk loads in flight, summed in ascending chunk order
+        for (; ac + 8 <= nac; ac += 8) {
+            double t[8];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) t[j] = p[(ac + j) * cs];
+#pragma unroll
+            for (int j = 0; j < 8; ++j) sv += t[j];
+        }
+        for (; ac < nac; ++ac) sv += p[ac * cs];
+        if (!((groups[g].w >> (v >> 1)) & 1)) sv = 0.0;
+        cf = gcf[g];
+    }
+    double en = cf * sv * sv;
     double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    if (g < G) {
-        double s[8];
-        for (int v = 0; v < 8; ++v) s[v] = 0.0;
-        for (int ac = 0; ac < nac; ++ac) {
-            const double *p = spart + (((long)e * nac + ac) * G + g) * 8;
-            for (int v = 0; v < 8; ++v) s[v] += p[v];
-        }
-        const int mask = groups[g].w;
-        const double cf = gcf[g];
-        double sq = 0.0;
-        for (int v = 0; v < 4; ++v) {
-            if ((mask >> v) & 1) {
-                sq += s[2 * v] * s[2 * v];
-                sq += s[2 * v + 1] * s[2 * v + 1];
-            } else {
-                s[2 * v] = 0.0;
-                s[2 * v + 1] = 0.0;
-            }
-        }
-        en = cf * sq;
-        if (vpart) {
+    // the four values of this lane's side: S0r, S0i, S1r, S1i (times 2c)
+    const int base = (threadIdx.x & 63) & ~3;
+    const double c2 = 2.0 * cf;
+    const double S0r = c2 * __shfl(sv, base + 0, 64), S0i = c2 * __shfl(sv, base + 1, 64);
+    const double S1r = c2 * __shfl(sv, base + 2, 64), S1i = c2 * __shfl(sv, base + 3, 64);
+    if (vpart) {
+        const double pair = __shfl_xor(sv, 1, 64);
+        if (live && !(v & 1)) {
             const double4 rk = grk[g];
             const double k2 = rk.x * rk.x + rk.y * rk.y + rk.z * rk.z;
             const double bb = 2.0 * (1.0 / k2 + q4);
-            for (int v = 0; v < 4; ++v) {
-                const double Ev = cf * (s[2 * v] * s[2 * v] + s[2 * v + 1] * s[2 * v + 1]);
-                const double ky = (v < 2) ? rk.y : -rk.y, kz = (v & 1) ? -rk.z : rk.z;
-                v6[0] += Ev * (1.0 - bb * rk.x * rk.x);
-                v6[1] += Ev * (1.0 - bb * ky * ky);
-                v6[2] += Ev * (1.0 - bb * kz * kz);
-                v6[3] -= Ev * bb * rk.x * ky;
-                v6[4] -= Ev * bb * rk.x * kz;
-                v6[5] -= Ev * bb * ky * kz;
-            }
+            const int var = v >> 1;
+            const double Ev = cf * (sv * sv + pair * pair);
+            const double ky = (var < 2) ? rk.y : -rk.y, kz = (var & 1) ? -rk.z : rk.z;
+            v6[0] = Ev * (1.0 - bb * rk.x * rk.x);
+            v6[1] = Ev * (1.0 - bb * ky * ky);
+            v6[2] = Ev * (1.0 - bb * kz * kz);
+            v6[3] = -Ev * bb * rk.x * ky;
+            v6[4] = -Ev * bb * rk.x * kz;
+            v6[5] = -Ev * bb * ky * kz;
         }
-        // force record of k_kforce (9 doubles): with S_v = 2c S(k_v) per variant,
-        // A side (variants 0, 1): c1 = S0i+S1i, c2 = S1r-S0r, c3 = -(S0r+S1r),
-        // c4 = S1i-S0i; B side (variants 2, 3) likewise from S2, S3; then the
-        // group's (kx, ky, kz) packed into the ninth slot
-        const double c2 = 2.0 * cf;
+    }
+    if (live) {
+        // force record of k_kforce (9 doubles): A side (variants 0, 1):
+        // c1 = S0i+S1i, c2 = S1r-S0r, c3 = -(S0r+S1r), c4 = S1i-S0i; B side
+        // (variants 2, 3) likewise; then the group's (kx, ky, kz) packed
         double *o = sprime + ((long)e * G + g) * KF_REC;
-        for (int side = 0; side < 2; ++side) {
-            const double S0r = c2 * s[4 * side], S0i = c2 * s[4 * side + 1];
-            const double S1r = c2 * s[4 * side + 2], S1i = c2 * s[4 * side + 3];
-            o[4 * side + 0] = S0i + S1i;
-            o[4 * side + 1] = S1r - S0r;
-            o[4 * side + 2] = -(S0r + S1r);
-            o[4 * side + 3] = S1i - S0i;
+        const int k = v & 3;
+        o[v] = k == 0 ? S0i + S1i : k == 1 ? S1r - S0r : k == 2 ? -(S0r + S1r) : S1i - S0i;
+        if (v == 0) {
+            const int4 gg = groups[g];
+            o[8] = __longlong_as_double((long long)(gg.x | (gg.y << 10) | (gg.z << 20)));
         }
-        const int4 gg = groups[g];
-        o[8] = __longlong_as_double((long long)(gg.x | (gg.y << 10) | (gg.z << 20)));
     }
     en = block_sum(en, sh);
     if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + gb] = en;
     if (vpart) {
         for (int c = 0; c < 6; ++c) {
-            const double v = block_sum(v6[c], sh);
-            if (threadIdx.x == 0) vpart[((long)e * 6 + c) * ep_stride + ep_off + gb] = v;
+            const double vs = block_sum(v6[c], sh);
+            if (threadIdx.x == 0) vpart[((long)e * 6 + c) * ep_stride + ep_off + gb] = vs;
         }
     }
 }
