@@ -12,6 +12,7 @@ HBM, replayed as one HIP graph per step.
     python bench.py [--steps K] [--warmup W] [--gpus 1]
     python bench.py --workload batch [--replicas 64] [--steps K]   # batched replicas
     python bench.py --workload c1                                  # classical 216 NVE
+    python bench.py --workload c5 --steps 20 --warmup 3            # 4096 H2O x 32 beads
 
 Prints one JSON line.  Needs the built library (python -m qtip4pf_water_amd.build).
 """
@@ -59,6 +60,9 @@ def workload(name):
     if name == "c1":
         return dict(n_side=6, L_angs=18.64, P=1, T_K=298.0, dtfs=0.25, nvt=False,
                     label="classical NVE 216 H2O, velocity Verlet, dt 0.25 fs")
+    if name == "c5":
+        return dict(n_side=16, L_angs=49.72, P=32, T_K=298.0, dtfs=0.25, nvt=True,
+                    label="SC-PIMD NVT 4096 H2O x 32 beads (PILE-G), 298 K, dt 0.25 fs")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -198,7 +202,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "batch"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "c1", "c5", "batch"])
     ap.add_argument("--replicas", type=int, default=64)
     ap.add_argument("--timing-steps", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -257,7 +261,8 @@ def main():
     step_flops = 2 * wl["P"] * fl["total"] if wl["P"] > 1 else fl["total"]
 
     res = {
-        "metric": "PIMD steps/s 216 H2O x 32 beads" if wl["P"] > 1 else "MD steps/s 216 H2O",
+        "metric": (f"PIMD steps/s {n_mols} H2O x {wl['P']} beads" if wl["P"] > 1
+                   else f"MD steps/s {n_mols} H2O"),
         "value": sps,
         "unit": "steps/s",
         "n_gpus": 1,

@@ -545,7 +545,7 @@ static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
         heat.ntile = ctx->ptile;
         heat.nh = ctx->P * ctx->ptile;
     }
-    if (ctx->n_mols <= NL_MAXMOL) {
+    if (ctx->n_mols <= NL_FUSED_MAXMOL) {
         TimeScope ts(ctx, "nlist");
         k_nlist_bead<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box,
                                         RC.r_skin_sq, RC.r_skin - RC.r_cut, force_all,
@@ -556,7 +556,7 @@ static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
         return QTP_OK;
     }
     if (with_heat)
-        k_heat<<<1, 64, 0, s>>>(heat.ntile, heat.nh, heat.part, heat.hpart, heat.heat_acc,
+        k_heat<<<1, 1024, 0, s>>>(heat.ntile, heat.nh, heat.part, heat.hpart, heat.heat_acc,
                                 heat.heat_step);
     {
         TimeScope ts(ctx, "check");
@@ -1139,7 +1139,7 @@ int qtp_pile_g(qtp_ctx *ctx, double *p_nm, const double *mass, const double *dev
                                               ctx->drv.beta_n, ctx->mode, ctx->m_atom,
                                               ctx->sqm_atom, ctx->ppart, ctx->dev, ctx->np,
                                               ctx->hpart);
-        k_heat<<<1, 64, 0, ctx->stream>>>(nt, nt, ctx->ppart, ctx->hpart, nullptr, ctx->heat_step);
+        k_heat<<<1, 1024, 0, ctx->stream>>>(nt, nt, ctx->ppart, ctx->hpart, nullptr, ctx->heat_step);
     }
     HIPCHK(hipGetLastError());
     double h = 0.0;

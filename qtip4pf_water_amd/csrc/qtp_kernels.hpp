@@ -184,6 +184,46 @@ __device__ __forceinline__ double erfc_via_erfcx(double x, double ex)
     return ex * y;
 }
 
+// 1/sqrt(x) for finite x > 0: the hardware estimate (relative error below
+// 2^-22) and one third-order Newton correction y (1 + e/2 + 3e^2/8) with
+// e = 1 - x y^2, which leaves about one ulp.
+__device__ __forceinline__ double rsqrt_pos(double x)
+{
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-(x * y), y, 1.0);
+    return fma(y * e, fma(0.375, e, 0.5), y);
+}
+
+// e^x for -708 <= x <= 0 (no overflow, no subnormal results in the range the
+// real-space kernel uses, x >= -(kappa r_c)^2 >= -38.5): Cody-Waite reduction
+// x = n ln2 + r, |r| <= ln2/2, with ln2 split as hi (its first 32 significant
+// bits, so n hi is exact) + lo; e^r by its Taylor series to degree 13
+// (truncation below 5e-18); then 2^n.  Under one ulp over [-40, 0].
+__device__ __forceinline__ double exp_nonpos(double x)
+{
+    constexpr double L2E = 1.4426950408889634;          // log2(e)
+    constexpr double LN2_HI = 0.6931471803691238;       // ln 2 to 32 bits
+    constexpr double LN2_LO = 1.9082149292705877e-10;   // ln 2 - LN2_HI
+    const double n = rint(x * L2E);
+    double r = fma(-n, LN2_HI, x);
+    r = fma(-n, LN2_LO, r);
+    double p = 1.0 / 6227020800.0;  // 1/13!
+    p = fma(p, r, 1.0 / 479001600.0);
+    p = fma(p, r, 1.0 / 39916800.0);
+    p = fma(p, r, 1.0 / 3628800.0);
+    p = fma(p, r, 1.0 / 362880.0);
+    p = fma(p, r, 1.0 / 40320.0);
+    p = fma(p, r, 1.0 / 5040.0);
+    p = fma(p, r, 1.0 / 720.0);
+    p = fma(p, r, 1.0 / 120.0);
+    p = fma(p, r, 1.0 / 24.0);
+    p = fma(p, r, 1.0 / 6.0);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)n);
+}
+
 // Real-space Ewald over all site pairs i<j within r_c (forces.cpp:586-608,
 // rwald :141-177).  One 256-thread block per 64x64 tile (I<=J) of the site
 // pair matrix; wave w takes the 16 j-columns 16w..16w+15 against all 64 i.
@@ -278,6 +318,7 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
     }
     __syncthreads();
     // pass 2: lane l takes queue entries [l*S, l*S + S)
+    const bool near = J - I <= 1;
     const int S = (total + 63) >> 6;
     const int q0 = min(l * S, total), q1 = min(q0 + S, total);
     double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
@@ -301,10 +342,12 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
         const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
         const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
         const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-        const double rinv = rsqrt(rsq);
-        const double ex = exp(-ew.kappa2 * rsq);
+        const double rinv = rsqrt_pos(rsq);
+        const double ex = exp_nonpos(-ew.kappa2 * rsq);
         double ec = erfc_via_erfcx(ew.kappa * (rsq * rinv), ex);
-        if ((I * 64 + il) / 3 == (J * 64 + jl) / 3) ec -= 1.0;  // same molecule: -erf (forces.cpp:158-166)
+        // same molecule: -erf (forces.cpp:158-166); a molecule's three sites
+        // lie in one 64-site block or two consecutive ones
+        if (near && (I * 64 + il) / 3 == (J * 64 + jl) / 3) ec -= 1.0;
         const double qq = a.w * b.w;
         const double ep = qq * rinv * ec;
         const double pf = (ep + qq * ew.pref2k * ex) * (rinv * rinv);
@@ -1259,13 +1302,17 @@ __global__ __launch_bounds__(64) void k_prop_inv(PropArgs A, const double *__res
 
 // The thermostat's heat for one step: pile_g's return value (KE before -
 // KE after, thermostat.cpp:86-129); heat_acc -= it (update.cpp:325).
-__global__ __launch_bounds__(64) void k_heat(int ntile, int nh, const double *__restrict__ part,
-                                             const double *__restrict__ hpart,
-                                             double *__restrict__ heat_acc,
-                                             double *__restrict__ heat_step)
+__global__ __launch_bounds__(1024) void k_heat(int ntile, int nh, const double *__restrict__ part,
+                                               const double *__restrict__ hpart,
+                                               double *__restrict__ heat_acc,
+                                               double *__restrict__ heat_step)
 {
-    const double ke = wave_sum_array(part, ntile);
-    const double h = wave_sum_array(hpart, nh);
+    __shared__ double sh[16];
+    double ke = 0.0, h = 0.0;
+    for (int i = threadIdx.x; i < ntile; i += blockDim.x) ke += part[i];
+    for (int i = threadIdx.x; i < nh; i += blockDim.x) h += hpart[i];
+    ke = block_sum(ke, sh);
+    h = block_sum(h, sh);
     if (threadIdx.x == 0) {
         const double ret = ke + h;
         if (heat_step) heat_step[0] = ret;
@@ -1553,6 +1600,11 @@ __global__ __launch_bounds__(256) void k_nl_fill(int n_mols, int NAP, const doub
 // per wavefront with ballot + prefix-popcount compaction; the distance test
 // is bit-for-bit neighbour.cpp:42-53.   grid: P.
 constexpr int NL_MAXMOL = 4096;
+// Above this size the rebuild runs on the four-kernel path (k_check,
+// k_nl_count, k_nl_scan, k_nl_fill: a wavefront per molecule row instead of
+// one block per bead), whose rebuild steps are far shorter at thousands of
+// molecules; below it the single fused launch costs less per step.
+constexpr int NL_FUSED_MAXMOL = 1024;
 // Optional heat sum of a thermostatted step (k_heat's work), done by wave 0
 // of block 0 here so that it costs no launch of its own.
 struct HeatArgs {

@@ -1,0 +1,109 @@
+"""The large single system: 4096 H2O in a 49.72 A box (k_max = 14 and
+r_c = 14.8 A with the reference's parameter derivation, forces.cpp:573-584),
+against the oracle on the same seeded inputs, at the tolerances of the
+smaller systems.  The oracle's single-bead force evaluation at this size
+takes a few seconds, so each check uses one or two beads; the 32-bead run
+checks that the resident step runs at full size."""
+import sys
+
+import numpy as np
+import pytest
+
+from qtip4pf_water_amd import config as cfg
+
+pytestmark = pytest.mark.gpu
+
+N_SIDE, L_ANGS = 16, 49.72
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def e_self_scale(o):
+    """|Ewald self energy| (forces.cpp:622-624), the largest single term of V.
+    At 4096 waters V (~10 Eh) is the difference of Ewald terms of ~600 Eh, and
+    one running sum over ~8M real-space pair terms (the reference's and the
+    oracle's order) can round by ~1e-11 Eh by itself, so energies are held to
+    1e-12 of this scale instead of 1e-12 of V."""
+    return o.ewald()["prefactor"] * 0.5 * float(np.sum(o.charge ** 2))
+
+
+def rel_atom_err(F, Fr):
+    d = np.linalg.norm((F - Fr).reshape(-1, 3), axis=1)
+    n = np.linalg.norm(Fr.reshape(-1, 3), axis=1)
+    return float(np.max(d / np.maximum(n, 1e-300)))
+
+
+def _system(P, seed):
+    q0, box = cfg.lattice_water_box(N_SIDE, L_ANGS, seed=seed, jitter_bohr=0.05)
+    q = cfg.replicate_beads(q0, P)
+    if P > 1:
+        q = q + np.random.default_rng(seed + 100).normal(0, 0.03, q.shape)
+    return np.ascontiguousarray(q), box
+
+
+def test_large_neighbour_list_and_forces(oracle_mod):
+    from qtip4pf_water_amd import Water
+    n = N_SIDE ** 3
+    say("large get_forces_nospline", n, "mols")
+    q, box = _system(1, 41)
+    o = oracle_mod.Oracle(n, 1, box)
+    pr, lr, dr = o.new_lists()
+    o.neighbour_list(q, pr, lr, dr, 0)
+    Fr = np.zeros(o.n_dof)
+    er = o.get_forces(q, Fr, pr, lr, 0)
+    with Water(n, 1, box) as w:
+        assert w.derived["k_max"] == o.ewald()["k_max"] == 14
+        pg, lg, dg = w.new_lists()
+        w.neighbour_list(q, pg, lg, dg, 0)
+        assert np.array_equal(pg, pr) and np.array_equal(lg, lr) and np.array_equal(dg, dr)
+        F = np.zeros(o.n_dof)
+        e = w.get_forces_nospline(q, F, o.charge, pg, lg, 0)
+    assert rel_atom_err(F, Fr) < 1e-10
+    assert abs(e - er) <= 1e-12 * e_self_scale(o)
+
+
+def test_large_resident_sc_forces(oracle_mod):
+    """Resident state at P = 2: lists built on the device and the SC forces of
+    qtp_set_state against force_eval_sc_nospline of the oracle."""
+    from qtip4pf_water_amd import Water
+    n, P = N_SIDE ** 3, 2
+    say("large resident SC forces", n, "mols P", P)
+    q, box = _system(P, 42)
+    o = oracle_mod.Oracle(n, P, box)
+    point, lst, _ = o.all_lists(q)
+    Fr, fr = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+    er, scr = o.force_eval_sc(q, Fr, fr, point, lst)
+    with Water(n, P, box) as w:
+        w.set_state(q, np.zeros_like(q))
+        _, _, F, f, _ = w.get_state()
+        s = w.scalars()
+        pg, lg = w.get_lists()
+    assert np.array_equal(pg, point) and np.array_equal(lg, lst)
+    assert rel_atom_err(f, fr) < 1e-10
+    assert rel_atom_err(F[: o.n_dof], Fr[: o.n_dof]) < 1e-10   # even bead
+    assert rel_atom_err(F[o.n_dof:], Fr[o.n_dof:]) < 1e-9      # odd bead (FD Hessian term)
+    assert abs(s["energy"] - er) <= 1e-12 * e_self_scale(o)
+    assert abs(s["sc_pot"] - scr) <= 1e-12 * e_self_scale(o)
+
+
+def test_large_32_bead_steps():
+    """4096 x 32 resident NVT steps: runs, stays finite, reports no overflow."""
+    from qtip4pf_water_amd import Water
+    n, P = N_SIDE ** 3, 32
+    say("large resident run", n, "mols P", P)
+    q0, box = cfg.lattice_water_box(N_SIDE, L_ANGS, seed=43, jitter_bohr=0.05)
+    beta_n = 1.0 / (cfg.K2AU * 298.0) / P
+    q = cfg.free_ring_polymer_beads(q0, P, beta_n, seed=44)
+    p = cfg.sample_momenta(n, P, beta_n, seed=45)
+    with Water(n, P, box) as w:
+        w.set_seed(5)
+        w.set_state(q, p)
+        w.run(3, is_nvt=True)
+        s = w.scalars()
+        q1, p1, F1, _, _ = w.get_state()
+    assert s["status"] == 0 and s["steps"] == 3
+    assert np.all(np.isfinite(q1)) and np.all(np.isfinite(p1)) and np.all(np.isfinite(F1))
+    assert np.isfinite(s["energy"])
