@@ -56,7 +56,7 @@ struct qtp_ctx {
     SplTab spl{};
     double *spl_coef = nullptr, *fspl = nullptr;
     int ptile = 0;       // propagation tiles (64 planar dofs each)
-    size_t kforce_lds = 0, ksum_lds = 0, prop_lds = 0;
+    size_t kforce_lds = 0, kforce_lds_split = 0, ksum_lds = 0, prop_lds = 0;
 
     // device state (planar SoA)
     double *q = nullptr, *p = nullptr, *F = nullptr, *fph = nullptr, *disp = nullptr;
@@ -67,6 +67,7 @@ struct qtp_ctx {
     // force-phase scratch (E <= max(P,1))
     double *qpm = nullptr, *vpart = nullptr, *vout = nullptr, *rpart = nullptr, *spart = nullptr, *sprime = nullptr;
     double *flj = nullptr, *fpm = nullptr, *Ve = nullptr, *epart = nullptr;
+    double *fk2 = nullptr;  // second half of split k-space forces (SC phases)
     double *scal = nullptr, *out = nullptr, *heat = nullptr, *heat_step = nullptr;
     unsigned long long *step_ctr = nullptr;
     // constants
@@ -331,6 +332,7 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->flj, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fspl, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
+    CHK(dalloc(ctx, &ctx->fk2, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->Ve, E));
     CHK(dalloc(ctx, &ctx->epart, (size_t)E * ctx->ep_stride));
     CHK(dalloc(ctx, &ctx->scal, (size_t)E * 4));
@@ -414,8 +416,10 @@ struct SplineScope {
 // vout (optional): per-evaluation virial W_ab = sum_i r_i (x) F_i in the
 // minimum-image pair form (6 components xx yy zz xy xz yz), at fixed Ewald
 // parameters.
+// fout2: when given, k-space forces are split in two halves (k_kforce's split
+// form) and the second half goes there; the caller adds it.
 static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbase, double *fout,
-                     double *eout, double *vout = nullptr)
+                     double *eout, double *vout = nullptr, double *fout2 = nullptr)
 {
     hipStream_t s = ctx->stream;
     const long N3 = 3L * ctx->NAP;
@@ -483,9 +487,14 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     {
         TimeScope ts(ctx, "kforce");
         const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
-        k_kforce<<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
-            pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
-            ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout);
+        if (fout2)
+            k_kforce<KF_SPLIT_WAVES, true><<<2 * E * nab, 64 * KF_SPLIT_WAVES, ctx->kforce_lds_split, s>>>(
+                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
+                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, fout2);
+        else
+            k_kforce<KF_WAVES, false><<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
+                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
+                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, nullptr);
     }
     if (eout) {
         TimeScope ts(ctx, "esum");
@@ -513,17 +522,17 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
 {
     hipStream_t s = ctx->stream;
     const int P = ctx->P;
-    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fph, nullptr));
+    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fph, nullptr, nullptr, ctx->fk2));
     SCArgs a = sc_args(ctx);
     {
         TimeScope ts(ctx, "sc1");
-        k_sc1<<<P, SC_THREADS, 0, s>>>(a, ctx->q, ctx->fph, ctx->epart, ctx->ep_stride, ep_live(ctx), ctx->eself,
+        k_sc1<<<P, SC_THREADS, 0, s>>>(a, ctx->q, ctx->fph, ctx->fk2, ctx->epart, ctx->ep_stride, ep_live(ctx), ctx->eself,
                                 ctx->Ve, ctx->m_atom, ctx->F, ctx->qpm, ctx->scal);
     }
-    CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr));
+    CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr, nullptr, ctx->fk2));
     {
         TimeScope ts(ctx, "sc2");
-        k_sc2<<<P, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
+        k_sc2<<<P, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, ctx->fk2, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
                                 ctr);
     }
     HIPCHK(hipGetLastError());
@@ -904,6 +913,7 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->nac = (ctx->n_atoms + ctx->AC - 1) / ctx->AC;
     ctx->ksum_lds = (size_t)ctx->AC * 3 * K * sizeof(double2);
     ctx->kforce_lds = kforce_lds_bytes(K);
+    ctx->kforce_lds_split = kforce_lds_bytes(K, KF_SPLIT_WAVES);
     std::vector<int2> tl;
     for (int I = 0; I < ctx->nblk; ++I)
         for (int J = I; J < ctx->nblk; ++J) tl.push_back(make_int2(I, J));
@@ -928,8 +938,13 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     }
     if (ctx->kforce_lds > 64 * 1024 || ctx->ksum_lds > 64 * 1024) {
         if (ctx->kforce_lds > 160 * 1024 ||
-            hipFuncSetAttribute((const void *)k_kforce, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                (int)ctx->kforce_lds) != hipSuccess) {
+            hipFuncSetAttribute((const void *)k_kforce<KF_WAVES, false>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)ctx->kforce_lds) != hipSuccess ||
+            (ctx->kforce_lds_split > 64 * 1024 &&
+             hipFuncSetAttribute((const void *)k_kforce<KF_SPLIT_WAVES, true>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)ctx->kforce_lds_split) != hipSuccess)) {
             ctx->err = "LDS request above the device limit (k_max too large)";
             return QTP_ERR_ARG;
         }
@@ -962,7 +977,7 @@ void qtp_destroy(qtp_ctx *ctx)
     void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
                     ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
                     ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart,
-                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->Ve, ctx->epart,
+                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace};

@@ -571,18 +571,23 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
 // k-groups; S', k and the group table are read with wave-uniform addresses
 // (scalar loads).   grid: E * ceil(n_atoms/63), 512 threads.
 constexpr int KF_WAVES = 12;   // waves per block: two blocks fit one CU
+// Split form (the resident SC phases): each 63-site block is two blocks of
+// KF_SPLIT_WAVES waves, one per half of the k-groups; the second half's
+// force goes to a second buffer that the SC kernels add.  Three such blocks
+// fit one CU, so the grid spreads over the CUs more evenly.
+constexpr int KF_SPLIT_WAVES = 6;
 constexpr int KF_SITES = 63;
 constexpr int KF_CHUNK = 48;   // k-group records staged per wave and round
 // LDS bytes of k_kforce for a K-entry factor table
-inline size_t kforce_lds_bytes(int K)
+inline size_t kforce_lds_bytes(int K, int W = KF_WAVES)
 {
-    const size_t cf = (size_t)KF_WAVES * KF_CHUNK * KF_REC;
-    const size_t red = (size_t)KF_WAVES * 3 * 64;
+    const size_t cf = (size_t)W * KF_CHUNK * KF_REC;
+    const size_t red = (size_t)W * 3 * 64;
     return (size_t)3 * K * 64 * 16 + (cf > red ? cf : red) * 8 + (size_t)3 * 64 * 8;
 }
 // Reciprocal-space forces and the final per-atom forces of E evaluations.
-// Block = 63 sites (21 whole molecules) x KF_WAVES waves; lanes <-> sites,
-// wave w takes a contiguous part of the k-groups.  Groups in build order
+// Block = 63 sites (21 whole molecules) x W waves; lanes <-> sites, wave w
+// takes a contiguous part of the k-groups.  Groups in build order
 // form columns of fixed (kx, ky) and rising kz.  For site j the group's
 // force is minus the k-weighted imaginary parts of S'_v* e^{i k_v r_j} over
 // its sign variants; with A = X Y, B = X Y* (per column) and Z the z factor
@@ -598,12 +603,17 @@ inline size_t kforce_lds_bytes(int K)
 // (forces.cpp:613-619) and add the LJ/intramolecular forces (and the
 // splined pair forces when fspl is given).
 // grid: E * ceil(n_atoms / 63), 64 * KF_WAVES threads, kforce_lds_bytes(K).
-__global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
+// SPLIT: grid 2 * E * nab; blocks [0, E nab) take the first half of the
+// groups and everything else, writing fout; blocks [E nab, 2 E nab) take the
+// second half of the groups only, writing fout2.
+template <int W, bool SPLIT>
+__global__ __launch_bounds__(64 * W) void k_kforce(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
     int NAP, double gamma, double half1mg, Ewald ew, int G, const double *__restrict__ krec,
     const double *__restrict__ rpart, int nblk, const double *__restrict__ flj,
-    const double *__restrict__ fspl, double *__restrict__ fout)
+    const double *__restrict__ fspl, double *__restrict__ fout, double *__restrict__ fout2)
 {
+    constexpr int KF_WAVES = W;
     extern __shared__ double smem[];
     const int K = ew.K;
     double2 *T = (double2 *)smem;           // [3][K][64]
@@ -612,7 +622,9 @@ __global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
     const size_t redn = (size_t)KF_WAVES * 3 * 64;
     double *ef = CF + (cfn > redn ? cfn : redn);  // [3][64]
     const int nab = (n_atoms + KF_SITES - 1) / KF_SITES;
-    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int half = SPLIT ? (bid >= (int)(gridDim.x >> 1) ? 1 : 0) : 0;
+    if (SPLIT) bid -= half * (int)(gridDim.x >> 1);
     const int e = bid / nab, ab = bid - e * nab;
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -620,7 +632,7 @@ __global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
     const bool va = lane < KF_SITES && atom < n_atoms;
     const double *X = pos + (long)e * pstride;
     double fx = 0.0, fy = 0.0, fz = 0.0, fl = 0.0;
-    if (va) {
+    if (va && half == 0) {
         for (int sl = w; sl < nblk; sl += KF_WAVES) {
             const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + atom;
             fx += rp[0];
@@ -640,8 +652,9 @@ __global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
         else
             for (int m = 0; m < K; ++m) T[(d * K + m) * 64 + lane] = make_double2(0.0, 0.0);
     }
-    const int per = (G + KF_WAVES - 1) / KF_WAVES;
-    const int g0 = w * per, g1 = min(G, g0 + per);
+    constexpr int VW = SPLIT ? 2 * KF_WAVES : KF_WAVES;  // group shares
+    const int per = (G + VW - 1) / VW;
+    const int g0 = (half * KF_WAVES + w) * per, g1 = min(G, g0 + per);
     const int rounds = (per + KF_CHUNK - 1) / KF_CHUNK;
     double *cw = CF + (size_t)w * KF_CHUNK * KF_REC;
     const double *R = krec + (long)e * G * KF_REC;
@@ -719,7 +732,7 @@ __global__ __launch_bounds__(64 * KF_WAVES) void k_kforce(
         const int site = lane % 3;  // 63 = 21 molecules: lane%3 == atom%3
         const double v = site == 0 ? fl + gamma * ef[w * 64 + lane]
                                    : fl + (ef[w * 64 + lane] + half1mg * ef[w * 64 + lane - site]);
-        fout[((long)e * 3 + w) * NAP + atom] = v;
+        (half ? fout2 : fout)[((long)e * 3 + w) * NAP + atom] = v;
     }
 }
 
@@ -1026,8 +1039,11 @@ struct SCArgs {
 // the finite-difference step eps / sqrt(sum (f/m)^2 / (n_atoms P)) and the
 // displaced positions q +- step f/m of the second force phase.
 constexpr int SC_THREADS = 1024;
+// f2 (may be null): the second half of split k-space forces, added into fph
+// first (the sum stays in fph for the later kernels).
 __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__restrict__ q,
-                                                    const double *__restrict__ fph,
+                                                    double *__restrict__ fph,
+                                                    const double *__restrict__ f2,
                                                     const double *__restrict__ epart, int ep_stride,
                                                     int ep_n, double eself,
                                                     double *__restrict__ Vout,
@@ -1041,6 +1057,10 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
     __shared__ double s_V;
     const int b = blockIdx.x;
     const long o = (long)b * 3 * a.NAP;
+    if (f2) {
+        for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) fph[o + t] += f2[o + t];
+        __syncthreads();
+    }
     if (threadIdx.x < 64) {
         const double v = eval_energy(epart, b, ep_stride, ep_n, eself);
         if (threadIdx.x == 0) {
@@ -1100,8 +1120,11 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
 // Second half (forces.cpp:776-789): odd beads F_sc = 4f/3 - (f+ - f-) / (9 w^2
 // step); the half-step kick p += dt/2 F_sc when requested; block 0 sums the
 // SC potential and sc_pot in bead order.   grid: P, SC_THREADS threads.
+// f2 (may be null): the second half of the split k-space forces of the
+// phase-2 evaluations, added to fpm here.
 __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__restrict__ fph,
                                                     const double *__restrict__ fpm,
+                                                    const double *__restrict__ f2,
                                                     const double *__restrict__ scal,
                                                     double *__restrict__ Fsc, double *__restrict__ p,
                                                     int kick, double *__restrict__ out,
@@ -1112,11 +1135,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__re
     if (b & 1) {
         const int k = b >> 1;
         const double fscale = scal[b * 4 + 3];
-        const double *fp = fpm + (long)(2 * k) * 3 * a.NAP;
-        const double *fm = fpm + (long)(2 * k + 1) * 3 * a.NAP;
+        const long op = (long)(2 * k) * 3 * a.NAP, om = (long)(2 * k + 1) * 3 * a.NAP;
         for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) {
             double F = 4.0 * fph[o + t] / 3.0;
-            F -= fscale * (fp[t] - fm[t]);
+            const double fp = f2 ? fpm[op + t] + f2[op + t] : fpm[op + t];
+            const double fm = f2 ? fpm[om + t] + f2[om + t] : fpm[om + t];
+            F -= fscale * (fp - fm);
             Fsc[o + t] = F;
             if (kick) p[o + t] += a.halfdt * F;
         }
