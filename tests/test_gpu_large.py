@@ -107,3 +107,38 @@ def test_large_32_bead_steps():
     assert s["status"] == 0 and s["steps"] == 3
     assert np.all(np.isfinite(q1)) and np.all(np.isfinite(p1)) and np.all(np.isfinite(F1))
     assert np.isfinite(s["energy"])
+
+
+def test_large_checkpoint_continues_bit_for_bit(tmp_path):
+    """A restart from a checkpoint continues exactly at this size too (the
+    four-kernel list path, k_max = 14, 196 real-space blocks per bead)."""
+    from qtip4pf_water_amd import Water
+    n, P = N_SIDE ** 3, 8  # (P = 2 from a free-ring-polymer start is unstable at any size)
+    say("large checkpoint", n, "mols P", P)
+    q0, box = cfg.lattice_water_box(N_SIDE, L_ANGS, seed=46, jitter_bohr=0.05)
+    beta_n = 1.0 / (cfg.K2AU * 298.0) / P
+    q = cfg.free_ring_polymer_beads(q0, P, beta_n, seed=47)
+    p = cfg.sample_momenta(n, P, beta_n, seed=48)
+
+    def snap(w):
+        qq, pp, F, f, d = w.get_state()
+        return qq, pp, F, f, d, w.scalars()["heat"]
+
+    with Water(n, P, box) as w:
+        w.set_seed(9)
+        w.set_state(q, p)
+        w.run(12, is_nvt=True)
+        ref = snap(w)
+    path = tmp_path / "large.qtp"
+    with Water(n, P, box) as w:
+        w.set_seed(9)
+        w.set_state(q, p)
+        w.run(5, is_nvt=True)
+        w.save_checkpoint(path)
+    with Water(n, P, box) as w:
+        w.load_checkpoint(path)
+        w.run(7, is_nvt=True)
+        got = snap(w)
+    assert np.all(np.isfinite(ref[0])) and np.all(np.isfinite(ref[2]))
+    for a, b in zip(got, ref):
+        assert np.array_equal(np.asarray(a), np.asarray(b))
