@@ -57,6 +57,10 @@ struct qtp_ctx {
     double *spl_coef = nullptr, *fspl = nullptr;
     int ptile = 0;       // propagation tiles (64 planar dofs each)
     size_t kforce_lds = 0, kforce_lds_split = 0, ksum_lds = 0, prop_lds = 0;
+    int kf_chunk = KF_CHUNK, kf_chunk_split = KF_CHUNK;  // k_kforce record staging
+    // split k_kforce in the SC phases when its P * ceil(n_atoms/63) blocks
+    // would not fill two per CU (small systems); large ones have blocks enough
+    bool kf_split = false;
 
     // device state (planar SoA)
     double *q = nullptr, *p = nullptr, *F = nullptr, *fph = nullptr, *disp = nullptr;
@@ -490,11 +494,13 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         if (fout2)
             k_kforce<KF_SPLIT_WAVES, true><<<2 * E * nab, 64 * KF_SPLIT_WAVES, ctx->kforce_lds_split, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
-                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, fout2);
+                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, fout2,
+                ctx->kf_chunk_split);
         else
             k_kforce<KF_WAVES, false><<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
-                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, nullptr);
+                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, nullptr,
+                ctx->kf_chunk);
     }
     if (eout) {
         TimeScope ts(ctx, "esum");
@@ -522,17 +528,18 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
 {
     hipStream_t s = ctx->stream;
     const int P = ctx->P;
-    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fph, nullptr, nullptr, ctx->fk2));
+    double *f2 = ctx->kf_split ? ctx->fk2 : nullptr;
+    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fph, nullptr, nullptr, f2));
     SCArgs a = sc_args(ctx);
     {
         TimeScope ts(ctx, "sc1");
-        k_sc1<<<P, SC_THREADS, 0, s>>>(a, ctx->q, ctx->fph, ctx->fk2, ctx->epart, ctx->ep_stride, ep_live(ctx), ctx->eself,
+        k_sc1<<<P, SC_THREADS, 0, s>>>(a, ctx->q, ctx->fph, f2, ctx->epart, ctx->ep_stride, ep_live(ctx), ctx->eself,
                                 ctx->Ve, ctx->m_atom, ctx->F, ctx->qpm, ctx->scal);
     }
-    CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr, nullptr, ctx->fk2));
+    CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr, nullptr, f2));
     {
         TimeScope ts(ctx, "sc2");
-        k_sc2<<<P, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, ctx->fk2, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
+        k_sc2<<<P, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, f2, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
                                 ctr);
     }
     HIPCHK(hipGetLastError());
@@ -912,8 +919,12 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->AC = std::max(1, std::min(64, (int)(48 * 1024 / (3 * K * 16))));
     ctx->nac = (ctx->n_atoms + ctx->AC - 1) / ctx->AC;
     ctx->ksum_lds = (size_t)ctx->AC * 3 * K * sizeof(double2);
-    ctx->kforce_lds = kforce_lds_bytes(K);
-    ctx->kforce_lds_split = kforce_lds_bytes(K, KF_SPLIT_WAVES);
+    // two 12-wave blocks or three 6-wave (split) blocks per CU
+    ctx->kf_chunk = kforce_chunk(K, KF_WAVES, 2);
+    ctx->kf_chunk_split = kforce_chunk(K, KF_SPLIT_WAVES, 3);
+    ctx->kforce_lds = kforce_lds_bytes(K, KF_WAVES, ctx->kf_chunk);
+    ctx->kf_split = (long)ctx->P * ((ctx->n_atoms + KF_SITES - 1) / KF_SITES) <= 512;
+    ctx->kforce_lds_split = kforce_lds_bytes(K, KF_SPLIT_WAVES, ctx->kf_chunk_split);
     std::vector<int2> tl;
     for (int I = 0; I < ctx->nblk; ++I)
         for (int J = I; J < ctx->nblk; ++J) tl.push_back(make_int2(I, J));

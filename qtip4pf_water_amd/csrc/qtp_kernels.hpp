@@ -576,12 +576,23 @@ constexpr int KF_WAVES = 12;   // waves per block: two blocks fit one CU
 // force goes to a second buffer that the SC kernels add.  Three such blocks
 // fit one CU, so the grid spreads over the CUs more evenly.
 constexpr int KF_SPLIT_WAVES = 6;
+
 constexpr int KF_SITES = 63;
-constexpr int KF_CHUNK = 48;   // k-group records staged per wave and round
-// LDS bytes of k_kforce for a K-entry factor table
-inline size_t kforce_lds_bytes(int K, int W = KF_WAVES)
+constexpr int KF_CHUNK = 48;   // most k-group records staged per wave and round
+// Records staged per wave and round for W-wave blocks and K-entry tables: as
+// many as keep `per_cu` blocks within one CU's 160 KB of LDS, at most
+// KF_CHUNK (large k_max leaves less room next to the factor tables); the
+// area also holds the W x 3 x 64 reduction, which sets its floor.
+inline int kforce_chunk(int K, int W, int per_cu)
 {
-    const size_t cf = (size_t)W * KF_CHUNK * KF_REC;
+    const long room = 160L * 1024 / per_cu - 3L * K * 64 * 16 - 3L * 64 * 8;
+    const long c = std::max(room, (long)W * 3 * 64 * 8) / ((long)W * KF_REC * 8);
+    return (int)std::max(8L, std::min((long)KF_CHUNK, c));
+}
+// LDS bytes of k_kforce for a K-entry factor table
+inline size_t kforce_lds_bytes(int K, int W = KF_WAVES, int chunk = KF_CHUNK)
+{
+    const size_t cf = (size_t)W * chunk * KF_REC;
     const size_t red = (size_t)W * 3 * 64;
     return (size_t)3 * K * 64 * 16 + (cf > red ? cf : red) * 8 + (size_t)3 * 64 * 8;
 }
@@ -597,7 +608,7 @@ inline size_t kforce_lds_bytes(int K, int W = KF_WAVES)
 // and p23, m23 likewise with B and c5..c8.  A column accumulates the four
 // Z-linear sums per side over its groups and applies A, B, kx, ky once.
 // Group records (k_sreduce) are staged per wave in LDS with vector loads,
-// KF_CHUNK at a time; the real-space slot partials are loaded at kernel
+// chunk (<= KF_CHUNK) at a time; the real-space slot partials are loaded at kernel
 // start (wave w takes slots w, w + KF_WAVES, ...).  Epilogue: waves 0-2 (one
 // per dimension) sum the wave partials, apply the M-site chain rule
 // (forces.cpp:613-619) and add the LJ/intramolecular forces (and the
@@ -611,14 +622,15 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
     int NAP, double gamma, double half1mg, Ewald ew, int G, const double *__restrict__ krec,
     const double *__restrict__ rpart, int nblk, const double *__restrict__ flj,
-    const double *__restrict__ fspl, double *__restrict__ fout, double *__restrict__ fout2)
+    const double *__restrict__ fspl, double *__restrict__ fout, double *__restrict__ fout2,
+    int chunk)
 {
     constexpr int KF_WAVES = W;
     extern __shared__ double smem[];
     const int K = ew.K;
     double2 *T = (double2 *)smem;           // [3][K][64]
-    double *CF = smem + 2 * 3 * K * 64;     // [KF_WAVES][KF_CHUNK][KF_REC]; later red [W][3][64]
-    const size_t cfn = (size_t)KF_WAVES * KF_CHUNK * KF_REC;
+    double *CF = smem + 2 * 3 * K * 64;     // [KF_WAVES][chunk][KF_REC]; later red [W][3][64]
+    const size_t cfn = (size_t)KF_WAVES * chunk * KF_REC;
     const size_t redn = (size_t)KF_WAVES * 3 * 64;
     double *ef = CF + (cfn > redn ? cfn : redn);  // [3][64]
     const int nab = (n_atoms + KF_SITES - 1) / KF_SITES;
@@ -655,15 +667,15 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
     constexpr int VW = SPLIT ? 2 * KF_WAVES : KF_WAVES;  // group shares
     const int per = (G + VW - 1) / VW;
     const int g0 = (half * KF_WAVES + w) * per, g1 = min(G, g0 + per);
-    const int rounds = (per + KF_CHUNK - 1) / KF_CHUNK;
-    double *cw = CF + (size_t)w * KF_CHUNK * KF_REC;
+    const int rounds = (per + chunk - 1) / chunk;
+    double *cw = CF + (size_t)w * chunk * KF_REC;
     const double *R = krec + (long)e * G * KF_REC;
     double Ar = 0.0, Ai = 0.0, Br = 0.0, Bi = 0.0, kxr = 0.0, kyr = 0.0;
     double u1 = 0.0, u2 = 0.0, v1 = 0.0, v2 = 0.0, u3 = 0.0, u4 = 0.0, v3 = 0.0, v4 = 0.0;
     int pkxy = -1;
     for (int rd = 0; rd < rounds; ++rd) {
-        const int c0 = g0 + rd * KF_CHUNK;
-        const int nc = max(0, min(KF_CHUNK, g1 - c0));
+        const int c0 = g0 + rd * chunk;
+        const int nc = max(0, min(chunk, g1 - c0));
         for (int t = lane; t < nc * KF_REC; t += 64) cw[t] = R[(long)c0 * KF_REC + t];
         __syncthreads();  // tables (first round) and this round's records
         for (int i = 0; i < nc; ++i) {
