@@ -344,3 +344,38 @@ def test_reference_adapter_driver(oracle_mod, tmp_path, P, nvt):
         assert abs(eg[s] - er) <= 1e-12 * e_self_scale(o) * P, s
     assert np.max(np.abs(qg - qr)) <= 1e-12 * np.max(np.abs(qr))
     assert np.max(np.abs(pg - pr)) <= 1e-10 * np.max(np.abs(pr))
+
+
+def test_orthorhombic_ice_box(W, oracle_mod):
+    """Non-cubic box: proton-disordered ice from the reference's own start
+    (initialise_orth, 4 x 3 x 3 orthorhombic cells, 288 molecules), forces,
+    energies and lists against the oracle."""
+    from qtip4pf_water_amd import HostRng
+    say("orthorhombic ice 288 mols")
+    P = 4
+    q, box, _ = HostRng(2024).initialise_orth(4, 3, 3, 0.92, n_beads=P)
+    assert len({round(b, 6) for b in box}) == 3        # three different edges
+    q = q + np.random.default_rng(5).normal(0.0, 0.03, q.shape)
+    n = 288
+    o = oracle_mod.Oracle(n, P, box)
+    point, lst, disp = o.all_lists(q)
+    with W(n, P, box) as w:
+        assert w.derived["k_max"] == o.ewald()["k_max"]
+        assert w.derived["rcut"] == pytest.approx(o.ewald()["rcut"], rel=1e-15)
+        pg, lg, dg = w.new_lists()
+        for b in range(P):
+            w.neighbour_list(q, pg, lg, dg, b)
+        assert np.array_equal(pg, point) and np.array_equal(lg, lst)
+        for b in (0, 3):
+            Fr, F = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+            er = o.get_forces(q, Fr, point, lst, b)
+            e = w.get_forces_nospline(q, F, o.charge, point, lst, b)
+            sl = slice(b * o.n_dof, (b + 1) * o.n_dof)
+            assert rel_atom_err(F[sl], Fr[sl]) < 1e-10
+            assert abs(e - er) <= 1e-12 * e_self_scale(o)
+        Fr, fr = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+        er, scr = o.force_eval_sc(q, Fr, fr, point, lst)
+        F, f = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+        e, sc = w.force_eval_sc_nospline(q, F, o.charge, o.mass, point, lst, fphys=f)
+    assert rel_atom_err(f, fr) < 1e-10
+    assert abs(e - er) <= 1e-12 * e_self_scale(o)
