@@ -539,7 +539,7 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
     CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr, nullptr, f2));
     {
         TimeScope ts(ctx, "sc2");
-        k_sc2<<<P, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, f2, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
+        k_sc2<<<P + 1, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, f2, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
                                 ctr);
     }
     HIPCHK(hipGetLastError());
@@ -563,7 +563,7 @@ static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
     }
     if (ctx->n_mols <= NL_FUSED_MAXMOL) {
         TimeScope ts(ctx, "nlist");
-        k_nlist_bead<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box,
+        k_nlist_bead<<<P + (with_heat ? 1 : 0), 1024, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box,
                                         RC.r_skin_sq, RC.r_skin - RC.r_cut, force_all,
                                         ctx->list_size, ctx->fcap, ctx->disp, ctx->flag, ctx->nreb,
                                         ctx->point, ctx->fpoint, ctx->list, ctx->flist,

@@ -1130,8 +1130,9 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
     }
 }
 // Second half (forces.cpp:776-789): odd beads F_sc = 4f/3 - (f+ - f-) / (9 w^2
-// step); the half-step kick p += dt/2 F_sc when requested; block 0 sums the
-// SC potential and sc_pot in bead order.   grid: P, SC_THREADS threads.
+// step); the half-step kick p += dt/2 F_sc when requested; an extra last
+// block sums the SC potential and sc_pot in bead order (beside the others).
+// grid: P + 1, SC_THREADS threads.
 // f2 (may be null): the second half of the split k-space forces of the
 // phase-2 evaluations, added to fpm here.
 __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__restrict__ fph,
@@ -1144,6 +1145,25 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__re
 {
     const int b = blockIdx.x;
     const long o = (long)b * 3 * a.NAP;
+    if (b == a.P) {
+        if (threadIdx.x == 0) {
+            double tot = 0.0, vop = 0.0;
+            for (int bb = 0; bb < a.P; ++bb) {
+                if ((bb & 1) == 0) {
+                    tot += scal[bb * 4 + 0];
+                    vop += scal[bb * 4 + 1];
+                } else {
+                    tot += scal[bb * 4 + 0];
+                    tot += scal[bb * 4 + 2];
+                }
+            }
+            tot /= 3.0;
+            out[0] = tot;
+            out[1] = 2.0 * vop / a.P;
+            if (step_ctr) step_ctr[0] += 1ull;
+        }
+        return;
+    }
     if (b & 1) {
         const int k = b >> 1;
         const double fscale = scal[b * 4 + 3];
@@ -1158,22 +1178,6 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__re
         }
     } else if (kick) {
         for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) p[o + t] += a.halfdt * Fsc[o + t];
-    }
-    if (b == 0 && threadIdx.x == 0) {
-        double tot = 0.0, vop = 0.0;
-        for (int bb = 0; bb < a.P; ++bb) {
-            if ((bb & 1) == 0) {
-                tot += scal[bb * 4 + 0];
-                vop += scal[bb * 4 + 1];
-            } else {
-                tot += scal[bb * 4 + 0];
-                tot += scal[bb * 4 + 2];
-            }
-        }
-        tot /= 3.0;
-        out[0] = tot;
-        out[1] = 2.0 * vop / a.P;
-        if (step_ctr) step_ctr[0] += 1ull;
     }
 }
 
@@ -1635,6 +1639,16 @@ __global__ __launch_bounds__(256) void k_nl_fill(int n_mols, int NAP, const doub
 // bead.  Row counts live in LDS (n_mols <= NL_MAXMOL), rows are handled one
 // per wavefront with ballot + prefix-popcount compaction; the distance test
 // is bit-for-bit neighbour.cpp:42-53.   grid: P.
+// (a1, a2) <- the two largest of {a1 >= a2} and {b1 >= b2}
+__device__ __forceinline__ void top2_merge(double &a1, double &a2, double b1, double b2)
+{
+    if (a1 >= b1) {
+        a2 = a2 >= b1 ? a2 : b1;
+    } else {
+        a2 = b2 >= a1 ? b2 : a1;
+        a1 = b1;
+    }
+}
 constexpr int NL_MAXMOL = 4096;
 // Above this size the rebuild runs on the four-kernel path (k_check,
 // k_nl_count, k_nl_scan, k_nl_fill: a wavefront per molecule row instead of
@@ -1662,18 +1676,21 @@ __global__ __launch_bounds__(1024) void k_nlist_bead(int n_mols, int NAP, int NM
                                                      int *__restrict__ flist,
                                                      int *__restrict__ status, HeatArgs heat)
 {
-    __shared__ double s1[1024], s2[1024];
+    __shared__ double s1[16], s2[16];
     __shared__ int ch[NL_MAXMOL], cf[NL_MAXMOL];
     __shared__ int s_flag, s_tot[2];
     const int b = blockIdx.x, tid = threadIdx.x;
-    if (heat.part && b == 0 && tid < 64) {
-        const double ke = wave_sum_array(heat.part, heat.ntile);
-        const double h = wave_sum_array(heat.hpart, heat.nh);
-        if (tid == 0) {
-            const double ret = ke + h;
-            heat.heat_step[0] = ret;
-            heat.heat_acc[0] -= ret;
+    if (heat.part && b == (int)gridDim.x - 1) {  // the extra last block: the heat sum only
+        if (tid < 64) {
+            const double ke = wave_sum_array(heat.part, heat.ntile);
+            const double h = wave_sum_array(heat.hpart, heat.nh);
+            if (tid == 0) {
+                const double ret = ke + h;
+                heat.heat_step[0] = ret;
+                heat.heat_acc[0] -= ret;
+            }
         }
+        return;
     }
     double *D = disp + (long)b * 3 * NMP;
     // --- two largest O displacements
@@ -1688,27 +1705,20 @@ __global__ __launch_bounds__(1024) void k_nlist_bead(int n_mols, int NAP, int NM
             m2 = d2;
         }
     }
-    s1[tid] = m1;
-    s2[tid] = m2;
-    __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if (tid < o) {
-            const double a1 = s1[tid], a2 = s2[tid], b1 = s1[tid + o], b2 = s2[tid + o];
-            double t1, t2;
-            if (a1 >= b1) {
-                t1 = a1;
-                t2 = a2 >= b1 ? a2 : b1;
-            } else {
-                t1 = b1;
-                t2 = b2 >= a1 ? b2 : a1;
-            }
-            s1[tid] = t1;
-            s2[tid] = t2;
-        }
-        __syncthreads();
+    // top two of the block: within each wave by shuffles, then over the 16 waves
+    for (int o = 32; o > 0; o >>= 1) {
+        const double b1 = __shfl_xor(m1, o, 64), b2 = __shfl_xor(m2, o, 64);
+        top2_merge(m1, m2, b1, b2);
     }
+    if ((tid & 63) == 0) {
+        s1[tid >> 6] = m1;
+        s2[tid >> 6] = m2;
+    }
+    __syncthreads();
     if (tid == 0) {
-        const int f = force_all || (sqrt(s1[0]) + sqrt(s2[0]) > skin);
+        double t1 = s1[0], t2 = s2[0];
+        for (int ww = 1; ww < 16; ++ww) top2_merge(t1, t2, s1[ww], s2[ww]);
+        const int f = force_all || (sqrt(t1) + sqrt(t2) > skin);
         s_flag = f;
         flag[b] = f;
         if (f && nreb) nreb[b] += 1.0;
