@@ -567,9 +567,8 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
 // Reciprocal-space forces (forces.cpp:276-279, 319-324, 368-373, 434-442),
 // plus the real-space partial slots, the M-site chain rule (forces.cpp:613-619)
 // and the LJ+intra forces: the physical force of every site of a block of
-// 21 molecules (63 sites, lane <-> site; lane 63 idle).  The 8 waves split the
-// k-groups; S', k and the group table are read with wave-uniform addresses
-// (scalar loads).   grid: E * ceil(n_atoms/63), 512 threads.
+// 21 molecules (63 sites, lane <-> site; lane 63 idle), the waves splitting
+// the k-groups (details at k_kforce).
 constexpr int KF_WAVES = 12;   // waves per block: two blocks fit one CU
 // Split form (the resident SC phases): each 63-site block is two blocks of
 // KF_SPLIT_WAVES waves, one per half of the k-groups; the second half's
