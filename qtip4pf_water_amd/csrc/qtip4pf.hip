@@ -7,6 +7,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -61,6 +62,11 @@ struct qtp_ctx {
     // split k_kforce in the SC phases when its P * ceil(n_atoms/63) blocks
     // would not fill two per CU (small systems); large ones have blocks enough
     bool kf_split = false;
+    // k-space forces on the matrix cores (k_kforce_mf) for k_max <= 15; the
+    // record kernel above that, or when QTP_KFORCE=vector is set at create
+    bool kf_mf = false;
+    int mf_Mp = 0, mf_nks = 0, mf_nrt = 0;
+    size_t kforce_mf_lds = 0;
 
     // device state (planar SoA)
     double *q = nullptr, *p = nullptr, *F = nullptr, *fph = nullptr, *disp = nullptr;
@@ -72,6 +78,9 @@ struct qtp_ctx {
     double *qpm = nullptr, *vpart = nullptr, *vout = nullptr, *rpart = nullptr, *spart = nullptr, *sprime = nullptr;
     double *flj = nullptr, *fpm = nullptr, *Ve = nullptr, *epart = nullptr;
     double *fk2 = nullptr;  // second half of split k-space forces (SC phases)
+    double *wm = nullptr;   // k_kforce_mf coefficient matrices [E][4 nks][Mp]
+    int *gcol = nullptr;    // column of each k-group in the matrix rows
+    int4 *mf_rt = nullptr;  // k_kforce_mf row-tile table
     double *scal = nullptr, *out = nullptr, *heat = nullptr, *heat_step = nullptr;
     unsigned long long *step_ctr = nullptr;
     // constants
@@ -288,6 +297,60 @@ static void build_groups(qtp_ctx *ctx, std::vector<int4> &g, std::vector<double>
     }
 }
 
+// Matrix layout of k_kforce_mf.  The groups come in columns of fixed
+// (kx, ky) and rising kz (build_groups' order); the columns are sorted by
+// their largest kz (descending, stable) and paired into 16-row tiles, each
+// taking ceil((2 kz_max + 2) / 4) k-steps of 4.  The tile table lists the
+// tiles in the order the MF_WAVES waves of a block take them (wave w reads
+// entries w, w + MF_WAVES, ...): round q hands the sorted tiles
+// q MF_WAVES ... out to the waves in turn, reversed in odd rounds, which
+// evens out the k-steps per wave.  Returns false when a (kx, ky) pair
+// appears in two runs (then the record kernel is used).
+static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
+                            std::vector<int4> &rt, int &Mp, int &nks)
+{
+    struct Col {
+        int kx, ky, kzmax;
+    };
+    std::vector<Col> cols;
+    std::vector<int> gc(g.size());
+    std::map<int, int> seen;
+    for (size_t i = 0; i < g.size(); ++i) {
+        if (cols.empty() || cols.back().kx != g[i].x || cols.back().ky != g[i].y) {
+            if (!seen.emplace(g[i].x | (g[i].y << 16), (int)cols.size()).second) return false;
+            cols.push_back({g[i].x, g[i].y, g[i].z});
+        } else {
+            cols.back().kzmax = std::max(cols.back().kzmax, g[i].z);
+        }
+        gc[i] = (int)cols.size() - 1;
+    }
+    const int nc = (int)cols.size();
+    std::vector<int> ord(nc), rank(nc);
+    for (int i = 0; i < nc; ++i) ord[i] = i;
+    std::stable_sort(ord.begin(), ord.end(),
+                     [&](int a, int b) { return cols[a].kzmax > cols[b].kzmax; });
+    for (int i = 0; i < nc; ++i) rank[ord[i]] = i;
+    gcol.resize(g.size());
+    for (size_t i = 0; i < g.size(); ++i) gcol[i] = rank[gc[i]];
+    const int ntl = (nc + 1) / 2;
+    Mp = 16 * ntl;
+    nks = 0;
+    const int rounds = (ntl + MF_WAVES - 1) / MF_WAVES;
+    rt.assign((size_t)rounds * MF_WAVES, make_int4(0, 0, 0, 0));
+    for (int t = 0; t < ntl; ++t) {
+        const Col &c0 = cols[ord[2 * t]];
+        int4 x;
+        x.x = c0.kx | (c0.ky << 16);
+        x.y = 2 * t + 1 < nc ? (cols[ord[2 * t + 1]].kx | (cols[ord[2 * t + 1]].ky << 16)) : 0;
+        x.z = (2 * c0.kzmax + 2 + 3) / 4;  // the first column has the larger kz_max
+        x.w = 16 * t;
+        nks = std::max(nks, x.z);
+        const int q = t / MF_WAVES, p = t - q * MF_WAVES;
+        rt[(size_t)q * MF_WAVES + ((q & 1) ? MF_WAVES - 1 - p : p)] = x;
+    }
+    return true;
+}
+
 // Orthonormal real normal-mode matrix (FFTW R2HC halfcomplex layout with the
 // scalings of fourier.cpp:22-66), entries from long double with exact pi.
 static void build_nm(int P, std::vector<double> &C)
@@ -337,6 +400,7 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->fspl, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fk2, (size_t)E * N3));
+    if (ctx->kf_mf) CHK(dalloc(ctx, &ctx->wm, (size_t)E * 4 * ctx->mf_nks * ctx->mf_Mp));
     CHK(dalloc(ctx, &ctx->Ve, E));
     CHK(dalloc(ctx, &ctx->epart, (size_t)E * ctx->ep_stride));
     CHK(dalloc(ctx, &ctx->scal, (size_t)E * 4));
@@ -449,7 +513,9 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         TimeScope ts(ctx, "sreduce");
         k_sreduce<<<E * ctx->ngr, 256, 0, sk>>>(ctx->spart, ctx->nac, ctx->G, ctx->groups, ctx->gcf,
                                                  ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
-                                                 ctx->grk, 0.25 / ctx->ew.kappa2, vp);
+                                                 ctx->grk, 0.25 / ctx->ew.kappa2, vp, ctx->wm,
+                                                 ctx->mf_Mp, 4 * ctx->mf_nks, ctx->gcol,
+                                                 ctx->ew.bk[2]);
     }
     {
         TimeScope ts(ctx, "ljintra");
@@ -491,7 +557,13 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     {
         TimeScope ts(ctx, "kforce");
         const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
-        if (fout2)
+        if (ctx->kf_mf)
+            k_kforce_mf<<<E * ((ctx->n_atoms + MF_SITES - 1) / MF_SITES), 64 * MF_WAVES,
+                          ctx->kforce_mf_lds, s>>>(
+                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
+                ctx->mf_nks, ctx->mf_rt, ctx->mf_nrt, ctx->rpart, ctx->nblk, ctx->flj,
+                ctx->spl_on ? ctx->fspl : nullptr, fout);
+        else if (fout2)
             k_kforce<KF_SPLIT_WAVES, true><<<2 * E * nab, 64 * KF_SPLIT_WAVES, ctx->kforce_lds_split, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
                 ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, fout2,
@@ -925,6 +997,19 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->kforce_lds = kforce_lds_bytes(K, KF_WAVES, ctx->kf_chunk);
     ctx->kf_split = (long)ctx->P * ((ctx->n_atoms + KF_SITES - 1) / KF_SITES) <= 512;
     ctx->kforce_lds_split = kforce_lds_bytes(K, KF_SPLIT_WAVES, ctx->kf_chunk_split);
+    std::vector<int> gcol;
+    std::vector<int4> mf_rt;
+    {
+        const char *kv = std::getenv("QTP_KFORCE");
+        const bool want_mf = !(kv && std::strcmp(kv, "vector") == 0);
+        ctx->kf_mf = want_mf && 2 * K <= 4 * MF_MAXKS && ctx->G > 0 &&
+                     build_mf_layout(g, gcol, mf_rt, ctx->mf_Mp, ctx->mf_nks);
+        if (ctx->kf_mf) {
+            ctx->mf_nrt = (int)mf_rt.size();
+            ctx->kforce_mf_lds = kforce_mf_lds_bytes(K, ctx->mf_nks);
+            ctx->kf_split = false;
+        }
+    }
     std::vector<int2> tl;
     for (int I = 0; I < ctx->nblk; ++I)
         for (int J = I; J < ctx->nblk; ++J) tl.push_back(make_int2(I, J));
@@ -965,8 +1050,16 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
         (st = dalloc(ctx, &ctx->grk, ctx->G)) || (st = dalloc(ctx, &ctx->tiles, ctx->ntile)) ||
         (st = dalloc(ctx, &ctx->Cm, (size_t)ctx->P * ctx->P)) ||
         (st = dalloc(ctx, &ctx->mode, ctx->P)) ||
+        (ctx->kf_mf && ((st = dalloc(ctx, &ctx->gcol, ctx->G)) ||
+                        (st = dalloc(ctx, &ctx->mf_rt, ctx->mf_nrt)))) ||
         (st = alloc_all(ctx)))
         return st;
+    if (ctx->kf_mf &&
+        (hipMemcpy(ctx->gcol, gcol.data(), gcol.size() * sizeof(int), hipMemcpyHostToDevice) ||
+         hipMemcpy(ctx->mf_rt, mf_rt.data(), mf_rt.size() * sizeof(int4), hipMemcpyHostToDevice))) {
+        ctx->err = "constant upload failed";
+        return QTP_ERR_HIP;
+    }
     if (hipMemcpy(ctx->groups, g.data(), g.size() * sizeof(int4), hipMemcpyHostToDevice) ||
         hipMemcpy(ctx->gcf, cf.data(), cf.size() * sizeof(double), hipMemcpyHostToDevice) ||
         hipMemcpy(ctx->grk, rk.data(), rk.size() * sizeof(double4), hipMemcpyHostToDevice) ||
@@ -988,7 +1081,7 @@ void qtp_destroy(qtp_ctx *ctx)
     void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
                     ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
                     ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart,
-                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->Ve, ctx->epart,
+                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->wm, ctx->gcol, ctx->mf_rt, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace};

@@ -486,6 +486,12 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 //   W_ab = sum_k E_k (delta_ab - 2 k_a k_b (1/k^2 + 1/(4 kappa^2)))
 // (E_k = c |S(k)|^2 per sign variant; the strain derivative of kwald's sum at
 // fixed Ewald parameters).
+// With wm set, the force coefficients go to k_kforce_mf's matrix instead of
+// the records: W[n][8 col + s] (n = 2 kz + 0/1 for the Re/Im z factor; s =
+// u1 u2 v1 v2 u3 u4 v3 v4, the Z-linear sums of k_kforce; col = gcol[g]).
+// Each coefficient c_v enters twice: as the u weight of its own component and,
+// times +-kz b_z, as the v weight of the other part of the same variant pair.
+// Only these entries are ever written; the rest of W stays zero.
 constexpr int SR_GPB = 32;  // groups per block
 __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spart, int nac, int G,
                                                  const int4 *__restrict__ groups,
@@ -493,7 +499,9 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
                                                  double *__restrict__ sprime,
                                                  double *__restrict__ epart, int ep_stride,
                                                  int ep_off, const double4 *__restrict__ grk,
-                                                 double q4, double *__restrict__ vpart)
+                                                 double q4, double *__restrict__ vpart,
+                                                 double *__restrict__ wm, int Mp, int NN,
+                                                 const int *__restrict__ gcol, double bkz)
 {
     __shared__ double sh[16];
     const int nb = (G + SR_GPB - 1) / SR_GPB;
@@ -543,15 +551,27 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
         }
     }
     if (live) {
-        // force record of k_kforce (9 doubles): A side (variants 0, 1):
-        // c1 = S0i+S1i, c2 = S1r-S0r, c3 = -(S0r+S1r), c4 = S1i-S0i; B side
-        // (variants 2, 3) likewise; then the group's (kx, ky, kz) packed
-        double *o = sprime + ((long)e * G + g) * KF_REC;
+        // force coefficients: A side (variants 0, 1): c1 = S0i+S1i,
+        // c2 = S1r-S0r, c3 = -(S0r+S1r), c4 = S1i-S0i; B side (variants 2, 3)
+        // likewise
         const int k = v & 3;
-        o[v] = k == 0 ? S0i + S1i : k == 1 ? S1r - S0r : k == 2 ? -(S0r + S1r) : S1i - S0i;
-        if (v == 0) {
-            const int4 gg = groups[g];
-            o[8] = __longlong_as_double((long long)(gg.x | (gg.y << 10) | (gg.z << 20)));
+        const double cv = k == 0 ? S0i + S1i : k == 1 ? S1r - S0r : k == 2 ? -(S0r + S1r) : S1i - S0i;
+        if (wm) {
+            const int kz = groups[g].z;
+            const int lo = v & 1, hi = (v >> 1) & 1;
+            const int row = 8 * gcol[g] + (v >> 2) * 4;
+            const double kzr = bkz * kz;
+            double *W = wm + (long)e * NN * Mp;
+            W[(long)(2 * kz + lo) * Mp + row + hi] = cv;
+            W[(long)(2 * kz + 1 - lo) * Mp + row + 3 - hi] = (lo == hi ? -kzr : kzr) * cv;
+        } else {
+            // record of k_kforce (9 doubles): c1..c8, then (kx, ky, kz) packed
+            double *o = sprime + ((long)e * G + g) * KF_REC;
+            o[v] = cv;
+            if (v == 0) {
+                const int4 gg = groups[g];
+                o[8] = __longlong_as_double((long long)(gg.x | (gg.y << 10) | (gg.z << 20)));
+            }
         }
     }
     en = block_sum(en, sh);
@@ -744,6 +764,178 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
         const double v = site == 0 ? fl + gamma * ef[w * 64 + lane]
                                    : fl + (ef[w * 64 + lane] + half1mg * ef[w * 64 + lane - site]);
         (half ? fout2 : fout)[((long)e * 3 + w) * NAP + atom] = v;
+    }
+}
+
+// Reciprocal-space forces on FP64 matrix cores (v_mfma_f64_16x16x4f64), the
+// same sums as k_kforce (forces.cpp:276-279, 319-324, 368-373, 434-442).
+// Per evaluation, the Z-linear sums of every column and every site form one
+// product  D[8 col + s][site] = sum_n W[8 col + s][n] Zt[n][site]  with
+// Zt[2 kz][j], Zt[2 kz + 1][j] = Re, Im e^{i kz b_z z_j} and W written by
+// k_sreduce.  Columns are sorted by their largest kz, so a 16-row tile (two
+// columns) takes only the k-steps of 4 that its columns reach (tile table
+// rt: packed (kx, ky) of both columns, k-steps, first row; in the order the
+// waves take them, k-steps balanced over the waves; k-steps 0 = no tile).
+// The epilogue applies each column's A = XY, B = XY* and kx, ky per site:
+// with lane group g = lane / 16 holding rows 8 c' + g and 8 c' + g + 4 of
+// both columns of a tile (the 16x16 result layout), g = 0, 1 give the x and
+// y forces of (Ar u1 + Br u3) and (Ai u2 + Bi u4), g = 2, 3 the z force of
+// (Ar v1 + Br v3) and (Ai v2 + Bi v4).
+// Block = 48 sites (16 whole molecules: three 16-site tiles) x MF_WAVES waves;
+// each wave reads the z factors of the three site tiles (the B operands) from
+// LDS and takes every MF_WAVES-th row tile.  Epilogue as k_kforce: wave
+// partials summed in wave order, real-space slot partials, M-site chain rule
+// (forces.cpp:613-619), LJ/intra (and spline) forces.
+// grid: E * ceil(n_atoms / 48), 64 * MF_WAVES threads, kforce_mf_lds_bytes.
+constexpr int MF_SITES = 48;
+constexpr int MF_WAVES = 8;
+constexpr int MF_MAXKS = 8;  // k-steps per tile (A operands in registers): 2 K <= 32 (k_max <= 15)
+typedef double mf_v4d __attribute__((ext_vector_type(4)));
+
+inline size_t kforce_mf_lds_bytes(int K, int nks)
+{
+    return (size_t)2 * K * MF_SITES * 16 + (size_t)4 * nks * MF_SITES * 8 +
+           (size_t)2 * MF_WAVES * 3 * MF_SITES * 8 + (size_t)3 * MF_SITES * 8;
+}
+
+__global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
+    const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
+    int NAP, double gamma, double half1mg, Ewald ew, const double *__restrict__ wm, int Mp,
+    int nks, const int4 *__restrict__ rt, int nrt, const double *__restrict__ rpart, int nblk,
+    const double *__restrict__ flj, const double *__restrict__ fspl, double *__restrict__ fout)
+{
+    extern __shared__ double smem[];
+    const int K = ew.K, NN = 4 * nks;
+    double2 *T = (double2 *)smem;                    // [2][K][48] x (charge folded), y factors
+    double *Zt = smem + 4 * K * MF_SITES;            // [NN][48]
+    double *red = Zt + NN * MF_SITES;                // [W][3][48] k-space wave partials
+    double *rsm = red + MF_WAVES * 3 * MF_SITES;     // [W][3][48] real-space slot partials
+    double *ef = rsm + MF_WAVES * 3 * MF_SITES;      // [3][48]
+    const int nbm = (n_atoms + MF_SITES - 1) / MF_SITES;
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int e = bid / nbm, ab = bid - e * nbm;
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int a0 = ab * MF_SITES;
+    const double *X = pos + (long)e * pstride;
+    double rx = 0.0, ry = 0.0, rz = 0.0;
+    if (lane < MF_SITES && a0 + lane < n_atoms) {
+        for (int sl = w; sl < nblk; sl += MF_WAVES) {
+            const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + a0 + lane;
+            rx += rp[0];
+            ry += rp[NAP];
+            rz += rp[2 * NAP];
+        }
+    }
+    if (threadIdx.x < 3 * MF_SITES) {
+        // factor tables by the recurrence of kw_table (forces.cpp:212-245)
+        const int d = threadIdx.x / MF_SITES, j = threadIdx.x - d * MF_SITES;
+        const int atom = a0 + j;
+        const bool va = atom < n_atoms;
+        double sn = 0.0, cs = 0.0, qf = 0.0;
+        if (va) {
+            qf = d == 0 ? charge[atom] : 1.0;
+            sincos(ew.bk[d] * site_coord(X, NAP, atom, d, gamma, half1mg), &sn, &cs);
+        }
+        double cm = 1.0, sm = 0.0;
+        for (int m = 0; m < K; ++m) {
+            if (m > 0) {
+                const double cn = cs * cm - sn * sm;
+                const double snn = sn * cm + cs * sm;
+                cm = cn;
+                sm = snn;
+            }
+            const double c = va ? cm * qf : 0.0, s = va ? sm * qf : 0.0;
+            if (d < 2) {
+                T[(d * K + m) * MF_SITES + j] = make_double2(c, s);
+            } else {
+                Zt[(2 * m) * MF_SITES + j] = c;
+                Zt[(2 * m + 1) * MF_SITES + j] = s;
+            }
+        }
+        if (d == 2)
+            for (int n = 2 * K; n < NN; ++n) Zt[n * MF_SITES + j] = 0.0;
+    }
+    __syncthreads();
+    const int j = lane & 15, gq = lane >> 4;
+    const double *Zl = Zt + gq * MF_SITES + j;  // B operand of k-step t, site tile st: Zl[4t*48 + 16st]
+    const bool odd = gq & 1;
+    const double sg = odd ? 1.0 : -1.0;
+    const bool gxy = gq < 2;
+    double acc1[3] = {0.0, 0.0, 0.0}, acc2[3] = {0.0, 0.0, 0.0};
+    const double *We = wm + (long)e * NN * Mp;
+    for (int i = w; i < nrt; i += MF_WAVES) {
+        const int4 r = rt[i];
+        if (r.z == 0) continue;
+        double af[MF_MAXKS];
+#pragma unroll
+        for (int t = 0; t < MF_MAXKS; ++t)
+            af[t] = t < r.z ? We[(long)(4 * t + gq) * Mp + r.w + j] : 0.0;
+        mf_v4d d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0;
+#pragma unroll
+        for (int t = 0; t < MF_MAXKS; ++t) {
+            if (t < r.z) {
+                const double *zb = Zl + 4 * t * MF_SITES;
+                d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], zb[0], d0, 0, 0, 0);
+                d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], zb[16], d1, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], zb[32], d2, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int ck = h ? r.y : r.x;
+            const int kx = ck & 0xffff, ky = ck >> 16;
+            const double cx = gxy ? ew.bk[0] * kx : 1.0, cy = gxy ? ew.bk[1] * ky : 0.0;
+#pragma unroll
+            for (int st = 0; st < 3; ++st) {
+                const mf_v4d &D = st == 0 ? d0 : st == 1 ? d1 : d2;
+                const double2 Xv = T[kx * MF_SITES + 16 * st + j];
+                const double2 Yv = T[(K + ky) * MF_SITES + 16 * st + j];
+                const double s1 = Xv.x * (odd ? Yv.y : Yv.x), s2 = Xv.y * (odd ? Yv.x : Yv.y);
+                const double a = fma(sg, s2, s1), b = fma(-sg, s1, s2);
+                const double ta = a * D[2 * h], tb = b * D[2 * h + 1];
+                acc1[st] = fma(-cx, ta + tb, acc1[st]);
+                acc2[st] = fma(-cy, ta - tb, acc2[st]);
+            }
+        }
+    }
+#pragma unroll
+    for (int st = 0; st < 3; ++st) {
+        acc1[st] += __shfl_xor(acc1[st], 16, 64);
+        acc2[st] += __shfl_xor(acc2[st], 16, 64);
+        if (gq == 0) {
+            red[(w * 3 + 0) * MF_SITES + 16 * st + j] = acc1[st];
+            red[(w * 3 + 1) * MF_SITES + 16 * st + j] = acc2[st];
+        } else if (gq == 2) {
+            red[(w * 3 + 2) * MF_SITES + 16 * st + j] = acc1[st];
+        }
+    }
+    if (lane < MF_SITES) {
+        rsm[(w * 3 + 0) * MF_SITES + lane] = rx;
+        rsm[(w * 3 + 1) * MF_SITES + lane] = ry;
+        rsm[(w * 3 + 2) * MF_SITES + lane] = rz;
+    }
+    __syncthreads();
+    const int d = threadIdx.x / MF_SITES, sj = threadIdx.x - d * MF_SITES;
+    if (threadIdx.x < 3 * MF_SITES) {
+        double vr = 0.0, vk = 0.0;
+#pragma unroll
+        for (int ww = 0; ww < MF_WAVES; ++ww) {
+            vr += rsm[(ww * 3 + d) * MF_SITES + sj];
+            vk += red[(ww * 3 + d) * MF_SITES + sj];
+        }
+        ef[d * MF_SITES + sj] = vr + vk;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3 * MF_SITES && a0 + sj < n_atoms) {
+        const int atom = a0 + sj;
+        const long o = ((long)e * 3 + d) * NAP + atom;
+        double fl = flj[o];
+        if (fspl) fl += fspl[o];
+        const int site = sj % 3;  // 48 sites = 16 molecules: sj%3 == atom%3
+        const double v = site == 0 ? fl + gamma * ef[d * MF_SITES + sj]
+                                   : fl + (ef[d * MF_SITES + sj] + half1mg * ef[d * MF_SITES + sj - site]);
+        fout[o] = v;
     }
 }
 
