@@ -62,11 +62,12 @@ struct qtp_ctx {
     // split k_kforce in the SC phases when its P * ceil(n_atoms/63) blocks
     // would not fill two per CU (small systems); large ones have blocks enough
     bool kf_split = false;
-    // k-space forces on the matrix cores (k_kforce_mf) for k_max <= 15; the
-    // record kernel above that, or when QTP_KFORCE=vector is set at create
+    // k-space sums and forces on the matrix cores (k_ksum_mf, k_kforce_mf)
+    // for k_max <= 15; the vector kernels above that, or when
+    // QTP_KSPACE=vector is set at create
     bool kf_mf = false;
-    int mf_Mp = 0, mf_nks = 0, mf_nrt = 0;
-    size_t kforce_mf_lds = 0;
+    int mf_Mp = 0, mf_nks = 0, mf_nrt = 0, ks_nunit = 0;
+    size_t kforce_mf_lds = 0, ksum_mf_lds = 0;
 
     // device state (planar SoA)
     double *q = nullptr, *p = nullptr, *F = nullptr, *fph = nullptr, *disp = nullptr;
@@ -81,6 +82,8 @@ struct qtp_ctx {
     double *wm = nullptr;   // k_kforce_mf coefficient matrices [E][4 nks][Mp]
     int *gcol = nullptr;    // column of each k-group in the matrix rows
     int4 *mf_rt = nullptr;  // k_kforce_mf row-tile table
+    int4 *ks_col = nullptr; // k_ksum_mf column table
+    int2 *ks_unit = nullptr;
     double *scal = nullptr, *out = nullptr, *heat = nullptr, *heat_step = nullptr;
     unsigned long long *step_ctr = nullptr;
     // constants
@@ -305,12 +308,17 @@ static void build_groups(qtp_ctx *ctx, std::vector<int4> &g, std::vector<double>
 // entries w, w + MF_WAVES, ...): round q hands the sorted tiles
 // q MF_WAVES ... out to the waves in turn, reversed in odd rounds, which
 // evens out the k-steps per wave.  Returns false when a (kx, ky) pair
-// appears in two runs (then the record kernel is used).
+// appears in two runs or a column's kz are not consecutive (then the vector
+// kernels are used).
+// k_ksum_mf: kcol per sorted column (kx | ky << 16, first group, first kz,
+// last kz), padded to a multiple of four with empty columns; units (tile of
+// four columns, n-tile) for every n = 2 kz + 0/1 that a tile's columns reach.
 static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
-                            std::vector<int4> &rt, int &Mp, int &nks)
+                            std::vector<int4> &rt, int &Mp, int &nks, std::vector<int4> &kcol,
+                            std::vector<int2> &kunit)
 {
     struct Col {
-        int kx, ky, kzmax;
+        int kx, ky, kzmax, g0, kz0;
     };
     std::vector<Col> cols;
     std::vector<int> gc(g.size());
@@ -318,9 +326,10 @@ static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
     for (size_t i = 0; i < g.size(); ++i) {
         if (cols.empty() || cols.back().kx != g[i].x || cols.back().ky != g[i].y) {
             if (!seen.emplace(g[i].x | (g[i].y << 16), (int)cols.size()).second) return false;
-            cols.push_back({g[i].x, g[i].y, g[i].z});
+            cols.push_back({g[i].x, g[i].y, g[i].z, (int)i, g[i].z});
         } else {
-            cols.back().kzmax = std::max(cols.back().kzmax, g[i].z);
+            if (g[i].z != cols.back().kzmax + 1) return false;
+            cols.back().kzmax = g[i].z;
         }
         gc[i] = (int)cols.size() - 1;
     }
@@ -348,6 +357,15 @@ static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
         const int q = t / MF_WAVES, p = t - q * MF_WAVES;
         rt[(size_t)q * MF_WAVES + ((q & 1) ? MF_WAVES - 1 - p : p)] = x;
     }
+    const int nq = (nc + 3) / 4;
+    kcol.assign((size_t)4 * nq, make_int4(0, 0, 1, 0));
+    for (int i = 0; i < nc; ++i) {
+        const Col &c = cols[ord[i]];
+        kcol[i] = make_int4(c.kx | (c.ky << 16), c.g0, c.kz0, c.kzmax);
+    }
+    kunit.clear();
+    for (int t = 0; t < nq; ++t)
+        for (int u = 0; u <= (2 * kcol[4 * t].w + 1) / 16; ++u) kunit.push_back(make_int2(t, u));
     return true;
 }
 
@@ -505,9 +523,14 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     }
     {
         TimeScope ts(ctx, "ksum");
-        k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, sk>>>(
-            pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->G,
-            ctx->AC, ctx->ngc, ctx->nac, ctx->spart);
+        if (ctx->kf_mf)
+            k_ksum_mf<<<E * ctx->nac, 64 * KS_WAVES, ctx->ksum_mf_lds, sk>>>(
+                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->ks_col,
+                ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, ctx->spart);
+        else
+            k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, sk>>>(
+                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->G,
+                ctx->AC, ctx->ngc, ctx->nac, ctx->spart);
     }
     {
         TimeScope ts(ctx, "sreduce");
@@ -515,7 +538,7 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
                                                  ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
                                                  ctx->grk, 0.25 / ctx->ew.kappa2, vp, ctx->wm,
                                                  ctx->mf_Mp, 4 * ctx->mf_nks, ctx->gcol,
-                                                 ctx->ew.bk[2]);
+                                                 ctx->ew.bk[2], ctx->kf_mf ? 1 : 0);
     }
     {
         TimeScope ts(ctx, "ljintra");
@@ -998,15 +1021,18 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->kf_split = (long)ctx->P * ((ctx->n_atoms + KF_SITES - 1) / KF_SITES) <= 512;
     ctx->kforce_lds_split = kforce_lds_bytes(K, KF_SPLIT_WAVES, ctx->kf_chunk_split);
     std::vector<int> gcol;
-    std::vector<int4> mf_rt;
+    std::vector<int4> mf_rt, ks_col;
+    std::vector<int2> ks_unit;
     {
-        const char *kv = std::getenv("QTP_KFORCE");
+        const char *kv = std::getenv("QTP_KSPACE");
         const bool want_mf = !(kv && std::strcmp(kv, "vector") == 0);
-        ctx->kf_mf = want_mf && 2 * K <= 4 * MF_MAXKS && ctx->G > 0 &&
-                     build_mf_layout(g, gcol, mf_rt, ctx->mf_Mp, ctx->mf_nks);
+        ctx->kf_mf = want_mf && 2 * K <= 4 * MF_MAXKS && ctx->G > 0 && ctx->AC % 4 == 0 &&
+                     build_mf_layout(g, gcol, mf_rt, ctx->mf_Mp, ctx->mf_nks, ks_col, ks_unit);
         if (ctx->kf_mf) {
             ctx->mf_nrt = (int)mf_rt.size();
+            ctx->ks_nunit = (int)ks_unit.size();
             ctx->kforce_mf_lds = kforce_mf_lds_bytes(K, ctx->mf_nks);
+            ctx->ksum_mf_lds = ksum_mf_lds_bytes(K, ctx->AC);
             ctx->kf_split = false;
         }
     }
@@ -1051,12 +1077,16 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
         (st = dalloc(ctx, &ctx->Cm, (size_t)ctx->P * ctx->P)) ||
         (st = dalloc(ctx, &ctx->mode, ctx->P)) ||
         (ctx->kf_mf && ((st = dalloc(ctx, &ctx->gcol, ctx->G)) ||
-                        (st = dalloc(ctx, &ctx->mf_rt, ctx->mf_nrt)))) ||
+                        (st = dalloc(ctx, &ctx->mf_rt, ctx->mf_nrt)) ||
+                        (st = dalloc(ctx, &ctx->ks_col, ks_col.size())) ||
+                        (st = dalloc(ctx, &ctx->ks_unit, ks_unit.size())))) ||
         (st = alloc_all(ctx)))
         return st;
     if (ctx->kf_mf &&
         (hipMemcpy(ctx->gcol, gcol.data(), gcol.size() * sizeof(int), hipMemcpyHostToDevice) ||
-         hipMemcpy(ctx->mf_rt, mf_rt.data(), mf_rt.size() * sizeof(int4), hipMemcpyHostToDevice))) {
+         hipMemcpy(ctx->mf_rt, mf_rt.data(), mf_rt.size() * sizeof(int4), hipMemcpyHostToDevice) ||
+         hipMemcpy(ctx->ks_col, ks_col.data(), ks_col.size() * sizeof(int4), hipMemcpyHostToDevice) ||
+         hipMemcpy(ctx->ks_unit, ks_unit.data(), ks_unit.size() * sizeof(int2), hipMemcpyHostToDevice))) {
         ctx->err = "constant upload failed";
         return QTP_ERR_HIP;
     }
@@ -1081,7 +1111,7 @@ void qtp_destroy(qtp_ctx *ctx)
     void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
                     ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
                     ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart,
-                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->wm, ctx->gcol, ctx->mf_rt, ctx->Ve, ctx->epart,
+                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->wm, ctx->gcol, ctx->mf_rt, ctx->ks_col, ctx->ks_unit, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace};

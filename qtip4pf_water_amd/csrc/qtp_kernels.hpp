@@ -400,6 +400,10 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
     }
 }
 
+// accumulator of v_mfma_f64_16x16x4f64: D[i][j] sits in lane j + 16 (i % 4),
+// element i / 4; operands A[i][k] from lane i + 16 k, B[k][j] from lane j + 16 k
+typedef double mf_v4d __attribute__((ext_vector_type(4)));
+
 // Per-site factor table e^{i m b_d r_d}, m = 0..K-1, by the cos/sin recurrence
 // of forces.cpp:212-245 (charge folded into the x factor, :241-244).
 template <int STRIDE>
@@ -476,6 +480,98 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
     o[4] = s4; o[5] = s5; o[6] = s6; o[7] = s7;
 }
 
+// The partial structure factors of k_ksum on FP64 matrix cores.  Per chunk
+// of AC sites and per column (kx, ky), the products
+//   Q[4 col + p][n] = sum_j F_p(j) Zt[n][j],  F = (Xr Yr, Xi Yi, Xr Yi, Xi Yr)
+//   at site j,  Zt[2 kz], Zt[2 kz + 1] = Re, Im Z(kz)
+// form one matrix product over the sites; k_sreduce combines them into the
+// eight sums of each group (A = X Y, B = X Y*: Ar = F0 - F1, Ai = F2 + F3,
+// Br = F0 + F1, Bi = F3 - F2).  A unit = 16 rows (four columns in
+// k_kforce_mf's sorted order) x 16 values of n (n-tile 0 or 1); each lane
+// forms its A operand F_p with one product of two table entries.  Output per
+// group and chunk (same place as k_ksum's sums): Re/Im Z times F0..F3, in
+// that order.
+// kcol[c] = (kx | ky << 16, first group, first kz, last kz) per sorted
+// column (padding columns: first kz > last kz).
+// grid: E * nac, 64 * KS_WAVES threads, ksum_mf_lds_bytes(K, AC).
+constexpr int KS_WAVES = 8;
+// LDS rows of AC + 1 entries: the B operands of a quarter-wave (16 rows n at
+// one site) and the A factors of a tile's columns then fall in different banks
+inline size_t ksum_mf_lds_bytes(int K, int AC) { return ((size_t)4 * K + 32) * (AC + 1) * 8; }
+
+__global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
+    const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
+    int NAP, double gamma, double half1mg, Ewald ew, const int4 *__restrict__ kcol,
+    const int2 *__restrict__ kunit, int nunit, int G, int AC, int nac, double *__restrict__ spart)
+{
+    extern __shared__ double smem[];
+    const int K = ew.K;
+    const int RS = AC + 1;           // LDS row stride
+    double2 *T = (double2 *)smem;    // [2][K][RS] x (charge folded), y factors
+    double *Zt = smem + 4 * K * RS;  // [32][RS]
+    const int e = blockIdx.x / nac, ac = blockIdx.x - e * nac;
+    const double *X = pos + (long)e * pstride;
+    for (int t = threadIdx.x; t < 3 * AC; t += blockDim.x) {
+        // factor tables by the recurrence of kw_table (forces.cpp:212-245)
+        const int d = t / AC, j = t - d * AC;
+        const int atom = ac * AC + j;
+        const bool va = atom < n_atoms;
+        double sn = 0.0, cs = 0.0, qf = 0.0;
+        if (va) {
+            qf = d == 0 ? charge[atom] : 1.0;
+            sincos(ew.bk[d] * site_coord(X, NAP, atom, d, gamma, half1mg), &sn, &cs);
+        }
+        double cm = 1.0, sm = 0.0;
+        for (int m = 0; m < K; ++m) {
+            if (m > 0) {
+                const double cn = cs * cm - sn * sm;
+                const double snn = sn * cm + cs * sm;
+                cm = cn;
+                sm = snn;
+            }
+            const double c = va ? cm * qf : 0.0, s = va ? sm * qf : 0.0;
+            if (d < 2) {
+                T[(d * K + m) * RS + j] = make_double2(c, s);
+            } else {
+                Zt[(2 * m) * RS + j] = c;
+                Zt[(2 * m + 1) * RS + j] = s;
+            }
+        }
+        if (d == 2)
+            for (int n = 2 * K; n < 32; ++n) Zt[n * RS + j] = 0.0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int j16 = lane & 15, gq = lane >> 4;
+    // A row j16 = (column j16 / 4, product p = lane % 4): F_p = X_a Y_b with
+    // a = Re for p = 0, 2, b = Re for p = 0, 3 (Im otherwise)
+    const int pp = lane & 3;
+    const int xa = (pp == 1 || pp == 3) ? 1 : 0, yb = (pp == 1 || pp == 2) ? 1 : 0;
+    const double *Td = (const double *)T;
+    double *out = spart + ((long)e * nac + ac) * G * 8;
+    for (int u = w; u < nunit; u += KS_WAVES) {
+        const int2 un = kunit[u];
+        const int ck = kcol[4 * un.x + (j16 >> 2)].x;
+        const int kx = ck & 0xffff, ky = ck >> 16;
+        const double *Xr = Td + 2 * (kx * RS + gq) + xa;  // site 4 s + gq
+        const double *Yr = Td + 2 * ((K + ky) * RS + gq) + yb;
+        const double *Zr = Zt + (16 * un.y + j16) * RS + gq;
+        mf_v4d acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int s4 = 0; s4 < AC; s4 += 4)
+            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xr[2 * s4] * Yr[2 * s4], Zr[s4], acc, 0, 0, 0);
+        // lane: n = 16 n-tile + j16, product gq, column 4 tile + reg
+        const int n = 16 * un.y + j16, kz = n >> 1;
+#pragma unroll
+        for (int reg = 0; reg < 4; ++reg) {
+            const int4 cr = kcol[4 * un.x + reg];
+            if (kz >= cr.z && kz <= cr.w)
+                out[(long)(cr.y + kz - cr.z) * 8 + gq * 2 + (n & 1)] = acc[reg];
+        }
+    }
+}
+
 // Sum the partial structure factors, energy per group (forces.cpp:274, 315,
 // 364, 428) and the force record of k_kforce from S' = 2 c S of the variants
 // the reference evaluates (gr.w bit v set; others zero).
@@ -501,7 +597,8 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
                                                  int ep_off, const double4 *__restrict__ grk,
                                                  double q4, double *__restrict__ vpart,
                                                  double *__restrict__ wm, int Mp, int NN,
-                                                 const int *__restrict__ gcol, double bkz)
+                                                 const int *__restrict__ gcol, double bkz,
+                                                 int pform)
 {
     __shared__ double sh[16];
     const int nb = (G + SR_GPB - 1) / SR_GPB;
@@ -523,6 +620,28 @@ __global__ __launch_bounds__(256) void k_sreduce(const double *__restrict__ spar
             for (int j = 0; j < 8; ++j) sv += t[j];
         }
         for (; ac < nac; ++ac) sv += p[ac * cs];
+    }
+    if (pform) {
+        // k_ksum_mf's sums q = (Re Z, Im Z) x (Xr Yr, Xi Yi, Xr Yi, Xi Yr) ->
+        // the sums of k_ksum: Re/Im of A Z, A Z*, then B Z, B Z*
+        const int base = threadIdx.x & 63 & ~7;
+        double q[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) q[i] = __shfl(sv, base + i, 64);
+        const double ArZr = q[0] - q[2], AiZi = q[5] + q[7], AiZr = q[4] + q[6], ArZi = q[1] - q[3];
+        const double BrZr = q[0] + q[2], BiZi = q[7] - q[5], BiZr = q[6] - q[4], BrZi = q[1] + q[3];
+        switch (v) {
+        case 0: sv = ArZr - AiZi; break;
+        case 1: sv = AiZr + ArZi; break;
+        case 2: sv = ArZr + AiZi; break;
+        case 3: sv = AiZr - ArZi; break;
+        case 4: sv = BrZr - BiZi; break;
+        case 5: sv = BiZr + BrZi; break;
+        case 6: sv = BrZr + BiZi; break;
+        default: sv = BiZr - BrZi; break;
+        }
+    }
+    if (live) {
         if (!((groups[g].w >> (v >> 1)) & 1)) sv = 0.0;
         cf = gcf[g];
     }
@@ -790,12 +909,36 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
 constexpr int MF_SITES = 48;
 constexpr int MF_WAVES = 8;
 constexpr int MF_MAXKS = 8;  // k-steps per tile (A operands in registers): 2 K <= 32 (k_max <= 15)
-typedef double mf_v4d __attribute__((ext_vector_type(4)));
 
 inline size_t kforce_mf_lds_bytes(int K, int nks)
 {
     return (size_t)2 * K * MF_SITES * 16 + (size_t)4 * nks * MF_SITES * 8 +
            (size_t)2 * MF_WAVES * 3 * MF_SITES * 8 + (size_t)3 * MF_SITES * 8;
+}
+
+// One 16-row tile of k_kforce_mf over three site tiles, NT k-steps: the A
+// operands (W rows, global), then the B operands (z factors, LDS), then the
+// 3 NT products, as straight-line code so that the loads overlap.
+template <int NT>
+__device__ __forceinline__ void mf_tile(const double *__restrict__ Wr, long Mp4,
+                                        const double *__restrict__ Zl, mf_v4d &d0, mf_v4d &d1,
+                                        mf_v4d &d2)
+{
+    double a[NT], b0[NT], b1[NT], b2[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) a[t] = Wr[t * Mp4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        b0[t] = Zl[4 * t * MF_SITES];
+        b1[t] = Zl[4 * t * MF_SITES + 16];
+        b2[t] = Zl[4 * t * MF_SITES + 32];
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+        d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b0[t], d0, 0, 0, 0);
+        d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b1[t], d1, 0, 0, 0);
+        d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t], b2[t], d2, 0, 0, 0);
+    }
 }
 
 __global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
@@ -867,19 +1010,18 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
     for (int i = w; i < nrt; i += MF_WAVES) {
         const int4 r = rt[i];
         if (r.z == 0) continue;
-        double af[MF_MAXKS];
-#pragma unroll
-        for (int t = 0; t < MF_MAXKS; ++t)
-            af[t] = t < r.z ? We[(long)(4 * t + gq) * Mp + r.w + j] : 0.0;
         mf_v4d d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0;
-#pragma unroll
-        for (int t = 0; t < MF_MAXKS; ++t) {
-            if (t < r.z) {
-                const double *zb = Zl + 4 * t * MF_SITES;
-                d0 = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], zb[0], d0, 0, 0, 0);
-                d1 = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], zb[16], d1, 0, 0, 0);
-                d2 = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], zb[32], d2, 0, 0, 0);
-            }
+        const double *Wr = We + (long)gq * Mp + r.w + j;
+        const long Mp4 = 4L * Mp;
+        switch (r.z) {  // 1 <= r.z <= MF_MAXKS (host table)
+        case 1: mf_tile<1>(Wr, Mp4, Zl, d0, d1, d2); break;
+        case 2: mf_tile<2>(Wr, Mp4, Zl, d0, d1, d2); break;
+        case 3: mf_tile<3>(Wr, Mp4, Zl, d0, d1, d2); break;
+        case 4: mf_tile<4>(Wr, Mp4, Zl, d0, d1, d2); break;
+        case 5: mf_tile<5>(Wr, Mp4, Zl, d0, d1, d2); break;
+        case 6: mf_tile<6>(Wr, Mp4, Zl, d0, d1, d2); break;
+        case 7: mf_tile<7>(Wr, Mp4, Zl, d0, d1, d2); break;
+        default: mf_tile<8>(Wr, Mp4, Zl, d0, d1, d2); break;
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
