@@ -66,7 +66,7 @@ struct qtp_ctx {
     // for k_max <= 15; the vector kernels above that, or when
     // QTP_KSPACE=vector is set at create
     bool kf_mf = false;
-    int mf_Mp = 0, mf_nks = 0, mf_nrt = 0, ks_nunit = 0;
+    int mf_Mp = 0, mf_nks = 0, mf_nrt = 0, ks_nunit = 0, ks_ncol4 = 0;
     size_t kforce_mf_lds = 0, ksum_mf_lds = 0;
 
     // device state (planar SoA)
@@ -526,7 +526,7 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         if (ctx->kf_mf)
             k_ksum_mf<<<E * ctx->nac, 64 * KS_WAVES, ctx->ksum_mf_lds, sk>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->ks_col,
-                ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, ctx->spart);
+                ctx->ks_ncol4, ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, ctx->spart);
         else
             k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, sk>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->G,
@@ -1032,7 +1032,8 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
             ctx->mf_nrt = (int)mf_rt.size();
             ctx->ks_nunit = (int)ks_unit.size();
             ctx->kforce_mf_lds = kforce_mf_lds_bytes(K, ctx->mf_nks);
-            ctx->ksum_mf_lds = ksum_mf_lds_bytes(K, ctx->AC);
+            ctx->ks_ncol4 = (int)ks_col.size();
+            ctx->ksum_mf_lds = ksum_mf_lds_bytes(K, ctx->AC, ctx->ks_ncol4, ctx->ks_nunit);
             ctx->kf_split = false;
         }
     }

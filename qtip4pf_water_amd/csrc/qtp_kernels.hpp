@@ -496,12 +496,16 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 // grid: E * nac, 64 * KS_WAVES threads, ksum_mf_lds_bytes(K, AC).
 constexpr int KS_WAVES = 8;
 // LDS rows of AC + 1 entries: the B operands of a quarter-wave (16 rows n at
-// one site) and the A factors of a tile's columns then fall in different banks
-inline size_t ksum_mf_lds_bytes(int K, int AC) { return ((size_t)4 * K + 32) * (AC + 1) * 8; }
+// one site) and the A factors of a tile's columns then fall in different banks.
+// The column and unit tables are staged in LDS behind the factor tables.
+inline size_t ksum_mf_lds_bytes(int K, int AC, int ncol4, int nunit)
+{
+    return ((size_t)4 * K + 32) * (AC + 1) * 8 + (size_t)ncol4 * 16 + (size_t)nunit * 8;
+}
 
 __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
-    int NAP, double gamma, double half1mg, Ewald ew, const int4 *__restrict__ kcol,
+    int NAP, double gamma, double half1mg, Ewald ew, const int4 *__restrict__ kcol, int ncol4,
     const int2 *__restrict__ kunit, int nunit, int G, int AC, int nac, double *__restrict__ spart)
 {
     extern __shared__ double smem[];
@@ -509,8 +513,12 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     const int RS = AC + 1;           // LDS row stride
     double2 *T = (double2 *)smem;    // [2][K][RS] x (charge folded), y factors
     double *Zt = smem + 4 * K * RS;  // [32][RS]
+    int4 *sc = (int4 *)(Zt + 32 * RS);  // [ncol4] column table
+    int2 *su = (int2 *)(sc + ncol4);    // [nunit] unit table
     const int e = blockIdx.x / nac, ac = blockIdx.x - e * nac;
     const double *X = pos + (long)e * pstride;
+    for (int t = threadIdx.x; t < ncol4; t += blockDim.x) sc[t] = kcol[t];
+    for (int t = threadIdx.x; t < nunit; t += blockDim.x) su[t] = kunit[t];
     for (int t = threadIdx.x; t < 3 * AC; t += blockDim.x) {
         // factor tables by the recurrence of kw_table (forces.cpp:212-245)
         const int d = t / AC, j = t - d * AC;
@@ -551,8 +559,8 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     const double *Td = (const double *)T;
     double *out = spart + ((long)e * nac + ac) * G * 8;
     for (int u = w; u < nunit; u += KS_WAVES) {
-        const int2 un = kunit[u];
-        const int ck = kcol[4 * un.x + (j16 >> 2)].x;
+        const int2 un = su[u];
+        const int ck = sc[4 * un.x + (j16 >> 2)].x;
         const int kx = ck & 0xffff, ky = ck >> 16;
         const double *Xr = Td + 2 * (kx * RS + gq) + xa;  // site 4 s + gq
         const double *Yr = Td + 2 * ((K + ky) * RS + gq) + yb;
@@ -565,7 +573,7 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
         const int n = 16 * un.y + j16, kz = n >> 1;
 #pragma unroll
         for (int reg = 0; reg < 4; ++reg) {
-            const int4 cr = kcol[4 * un.x + reg];
+            const int4 cr = sc[4 * un.x + reg];
             if (kz >= cr.z && kz <= cr.w)
                 out[(long)(cr.y + kz - cr.z) * 8 + gq * 2 + (n & 1)] = acc[reg];
         }

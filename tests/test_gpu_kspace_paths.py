@@ -1,0 +1,76 @@
+"""The two implementations of the reciprocal-space sums and forces: the
+matrix-core kernels (k_ksum_mf / k_kforce_mf, used for k_max <= 15) and the
+vector kernels (k_ksum / k_kforce, used above that or with QTP_KSPACE=vector
+set when a context is created).  Both against the oracle at the parity
+tolerances, and against each other (both FP64; they differ only in summation
+order)."""
+import sys
+
+import numpy as np
+import pytest
+
+from qtip4pf_water_amd import config as cfg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def rel_atom_err(F, Fr):
+    d = np.linalg.norm((F - Fr).reshape(-1, 3), axis=1)
+    n = np.linalg.norm(Fr.reshape(-1, 3), axis=1)
+    return float(np.max(d / np.maximum(n, 1e-300)))
+
+
+def system(n_side, L_angs, P, seed):
+    q0, box = cfg.lattice_water_box(n_side, L_angs, seed=seed, jitter_bohr=0.05)
+    q = cfg.replicate_beads(q0, P)
+    if P > 1:
+        q = q + np.random.default_rng(seed + 100).normal(0, 0.03, q.shape)
+    return np.ascontiguousarray(q), box
+
+
+def _forces(monkeypatch, path, n, P, box, q, o, point, lst):
+    from qtip4pf_water_amd import Water
+    if path == "vector":
+        monkeypatch.setenv("QTP_KSPACE", "vector")
+    else:
+        monkeypatch.delenv("QTP_KSPACE", raising=False)
+    with Water(n, P, box) as w:
+        F, f = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+        e, sc = w.force_eval_sc_nospline(q, F, o.charge, o.mass, point, lst, fphys=f)
+        F1 = np.zeros(P * o.n_dof)
+        e1 = w.get_forces_nospline(q, F1, o.charge, point, lst, 0)
+    return F, f, e, sc, F1[: o.n_dof], e1
+
+
+@pytest.mark.parametrize("n_side,L,P,seed", [(6, 18.64, 4, 51), (3, 9.4, 2, 52)])
+def test_both_kspace_paths_match_oracle_and_each_other(monkeypatch, oracle_mod, n_side, L, P, seed):
+    n = n_side ** 3
+    say("k-space paths", n, "mols P", P)
+    q, box = system(n_side, L, P, seed)
+    o = oracle_mod.Oracle(n, P, box)
+    point, lst, _ = o.all_lists(q)
+    Fr, fr = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+    er, scr = o.force_eval_sc(q, Fr, fr, point, lst)
+    F1r = np.zeros(P * o.n_dof)
+    e1r = o.get_forces(q, F1r, point, lst, 0)
+    got = {}
+    for path in ("matrix", "vector"):
+        F, f, e, sc, F1, e1 = _forces(monkeypatch, path, n, P, box, q, o, point, lst)
+        assert rel_atom_err(f, fr) < 1e-10, path
+        assert rel_atom_err(F1, F1r[: o.n_dof]) < 1e-10, path
+        assert abs(e - er) <= 1e-12 * abs(er), path
+        assert abs(sc - scr) <= 1e-12 * abs(scr), path
+        assert abs(e1 - e1r) <= 1e-12 * abs(e1r), path
+        for b in range(P):
+            sl = slice(b * o.n_dof, (b + 1) * o.n_dof)
+            assert rel_atom_err(F[sl], Fr[sl]) < (1e-10 if b % 2 == 0 else 1e-9), (path, b)
+        got[path] = (f, F1, e1)
+    # the two paths: same sums in another order
+    assert rel_atom_err(got["matrix"][0], got["vector"][0]) < 1e-12
+    assert rel_atom_err(got["matrix"][1], got["vector"][1]) < 1e-12
+    assert abs(got["matrix"][2] - got["vector"][2]) <= 1e-13 * abs(got["vector"][2])
