@@ -324,6 +324,7 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
     double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
     double vxx = 0.0, vyy = 0.0, vzz = 0.0, vxy = 0.0, vxz = 0.0, vyz = 0.0;
     int icur = -1;
+    double4 a = make_double4(0.0, 0.0, 0.0, 0.0);
     for (int q = q0; q < q1; ++q) {
         const int ent = qe[w][q];
         const int il = ent & 63, jl = c0 + (ent >> 6);
@@ -335,8 +336,8 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
             }
             fx = fy = fz = 0.0;
             icur = il;
+            a = sI[il];
         }
-        const double4 a = sI[il];
         const double4 b = sJ[jl];
         const double dx = mimg_fast(a.x - b.x, box.L[0], box.inv[0]);
         const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
@@ -495,12 +496,12 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 // column (padding columns: first kz > last kz).
 // grid: E * nac, 64 * KS_WAVES threads, ksum_mf_lds_bytes(K, AC).
 constexpr int KS_WAVES = 8;
-// LDS rows of AC + 1 entries: the B operands of a quarter-wave (16 rows n at
+// LDS rows of AC + 2 entries: the B operands of a quarter-wave (16 rows n at
 // one site) and the A factors of a tile's columns then fall in different banks.
 // The column and unit tables are staged in LDS behind the factor tables.
 inline size_t ksum_mf_lds_bytes(int K, int AC, int ncol4, int nunit)
 {
-    return ((size_t)4 * K + 32) * (AC + 1) * 8 + (size_t)ncol4 * 16 + (size_t)nunit * 8;
+    return ((size_t)4 * K + 32) * (AC + 2) * 8 + (size_t)ncol4 * 16 + (size_t)nunit * 8;
 }
 
 __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
@@ -510,7 +511,7 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
 {
     extern __shared__ double smem[];
     const int K = ew.K;
-    const int RS = AC + 1;           // LDS row stride
+    const int RS = AC + 2;           // LDS row stride
     double2 *T = (double2 *)smem;    // [2][K][RS] x (charge folded), y factors
     double *Zt = smem + 4 * K * RS;  // [32][RS]
     int4 *sc = (int4 *)(Zt + 32 * RS);  // [ncol4] column table
