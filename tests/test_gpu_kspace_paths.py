@@ -74,3 +74,31 @@ def test_both_kspace_paths_match_oracle_and_each_other(monkeypatch, oracle_mod, 
     assert rel_atom_err(got["matrix"][0], got["vector"][0]) < 1e-12
     assert rel_atom_err(got["matrix"][1], got["vector"][1]) < 1e-12
     assert abs(got["matrix"][2] - got["vector"][2]) <= 1e-13 * abs(got["vector"][2])
+
+
+def test_largest_matrix_core_k_max(monkeypatch, oracle_mod):
+    """k_max = 15, the largest the matrix-core kernels take (2 K = 32 z-factor
+    entries, eight k-steps): 5,832 H2O at liquid density, one bead."""
+    from qtip4pf_water_amd import Water
+    n_side, L = 18, 55.94
+    n = n_side ** 3
+    say("k_max 15 box", n, "mols")
+    q, box = system(n_side, L, 1, 53)
+    o = oracle_mod.Oracle(n, 1, box)
+    assert o.ewald()["k_max"] == 15
+    point, lst, _ = o.all_lists(q)
+    Fr = np.zeros(o.n_dof)
+    er = o.get_forces(q, Fr, point, lst, 0)
+    scale = o.ewald()["prefactor"] * 0.5 * float(np.sum(o.charge ** 2))
+    for path in ("matrix", "vector"):
+        if path == "vector":
+            monkeypatch.setenv("QTP_KSPACE", "vector")
+        else:
+            monkeypatch.delenv("QTP_KSPACE", raising=False)
+        with Water(n, 1, box) as w:
+            assert w.derived["k_max"] == 15
+            F = np.zeros(o.n_dof)
+            e = w.get_forces_nospline(q, F, o.charge, point, lst, 0)
+        assert rel_atom_err(F, Fr) < 1e-10, path
+        # energies at this size: 1e-12 of the self-energy scale (as for 4096)
+        assert abs(e - er) <= 1e-12 * scale, path
