@@ -412,12 +412,12 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->vout, (size_t)E * 6));
     CHK(dalloc(ctx, &ctx->rpart, (size_t)E * ctx->nblk * N3));
     CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));
-    CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));
+    if (!ctx->kf_mf) CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));
 
     CHK(dalloc(ctx, &ctx->flj, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fspl, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
-    CHK(dalloc(ctx, &ctx->fk2, (size_t)E * N3));
+    if (ctx->kf_split) CHK(dalloc(ctx, &ctx->fk2, (size_t)E * N3));
     if (ctx->kf_mf) CHK(dalloc(ctx, &ctx->wm, (size_t)E * 4 * ctx->mf_nks * ctx->mf_Mp));
     CHK(dalloc(ctx, &ctx->Ve, E));
     CHK(dalloc(ctx, &ctx->epart, (size_t)E * ctx->ep_stride));
