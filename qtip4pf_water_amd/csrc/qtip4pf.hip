@@ -565,16 +565,24 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
     {
         TimeScope ts(ctx, "realspace");
-        if (vout)
-            k_realspace<true><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles,
-                ctx->ntile, ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real,
-                vp);
-        else
-            k_realspace<false><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles,
-                ctx->ntile, ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real,
-                vp);
+        // coefficients in SGPRs (more waves per SIMD) for large grids, where
+        // that measured faster (4096 x 32: 3.42 against 3.60 ms); literals
+        // for small ones (216 x 32: no gain, and the second stream's kernels
+        // lost more than the launch gained)
+        const bool carg = (long)E * ctx->ntile >= RS_CARG_BLOCKS;
+        const RsCoef cf = rs_coef();
+#define QTP_RS_LAUNCH(VIR, CARG)                                                                   \
+    k_realspace<VIR, CARG><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(                               \
+        pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles, ctx->ntile,     \
+        ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real, vp, cf)
+        if (vout) {
+            if (carg) QTP_RS_LAUNCH(true, true);
+            else QTP_RS_LAUNCH(true, false);
+        } else {
+            if (carg) QTP_RS_LAUNCH(false, true);
+            else QTP_RS_LAUNCH(false, false);
+        }
+#undef QTP_RS_LAUNCH
     }
     if (fork) HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
     {

@@ -160,21 +160,47 @@ __device__ __forceinline__ double eval_energy(const double *__restrict__ epart, 
     return wave_sum_array(epart + (long)e * ep_stride, ep_n) - eself;
 }
 
+// Coefficients of the real-space special functions: the erfcx rational and
+// the e^r Taylor series (e[k] = 1/k!).  As literals (LitCoef) they sit in
+// VGPRs across the pair loop; passed as a kernel argument (RsCoef) they are
+// loaded into SGPRs, which frees about 40 VGPRs (more waves per SIMD).
+struct RsCoef {
+    double p[QTP_ERFCX_DEG + 1], q[QTP_ERFCX_DEG + 1], e[14];
+};
+struct LitCoef {
+    static constexpr double p[QTP_ERFCX_DEG + 1] = {QTP_ERFCX_P};
+    static constexpr double q[QTP_ERFCX_DEG + 1] = {QTP_ERFCX_Q};
+    static constexpr double e[14] = {1.0, 1.0, 0.5, 1.0 / 6.0, 1.0 / 24.0, 1.0 / 120.0,
+                                     1.0 / 720.0, 1.0 / 5040.0, 1.0 / 40320.0, 1.0 / 362880.0,
+                                     1.0 / 3628800.0, 1.0 / 39916800.0, 1.0 / 479001600.0,
+                                     1.0 / 6227020800.0};
+};
+// the argument form, filled on the host with the same values
+inline RsCoef rs_coef()
+{
+    RsCoef c;
+    for (int k = 0; k <= QTP_ERFCX_DEG; ++k) {
+        c.p[k] = LitCoef::p[k];
+        c.q[k] = LitCoef::q[k];
+    }
+    for (int k = 0; k < 14; ++k) c.e[k] = LitCoef::e[k];
+    return c;
+}
+
 // erfc(x) = exp(-x^2) erfcx(x) with erfcx from the P8/Q8 rational of
 // qtp_erfcx.h (relative error < 9e-16 on [0, 6.2]; the host guarantees
 // kappa * r_c <= 6.2, so every in-cutoff pair is in range).  ex = exp(-x^2)
 // is passed in (the force needs it too).  The quotient uses the hardware
 // reciprocal refined by two Newton steps and one residual correction
 // (Q >= 1 on the range: no scaling cases).
-__device__ __forceinline__ double erfc_via_erfcx(double x, double ex)
+template <class C>
+__device__ __forceinline__ double erfc_via_erfcx(double x, double ex, const C &c)
 {
-    constexpr double P[QTP_ERFCX_DEG + 1] = {QTP_ERFCX_P};
-    constexpr double Q[QTP_ERFCX_DEG + 1] = {QTP_ERFCX_Q};
-    double p = P[QTP_ERFCX_DEG], q = Q[QTP_ERFCX_DEG];
+    double p = c.p[QTP_ERFCX_DEG], q = c.q[QTP_ERFCX_DEG];
 #pragma unroll
     for (int k = QTP_ERFCX_DEG - 1; k >= 0; --k) {
-        p = fma(p, x, P[k]);
-        q = fma(q, x, Q[k]);
+        p = fma(p, x, c.p[k]);
+        q = fma(q, x, c.q[k]);
     }
     double r = __builtin_amdgcn_rcp(q);
     r = fma(fma(-q, r, 1.0), r, r);
@@ -199,7 +225,8 @@ __device__ __forceinline__ double rsqrt_pos(double x)
 // x = n ln2 + r, |r| <= ln2/2, with ln2 split as hi (its first 32 significant
 // bits, so n hi is exact) + lo; e^r by its Taylor series to degree 13
 // (truncation below 5e-18); then 2^n.  Under one ulp over [-40, 0].
-__device__ __forceinline__ double exp_nonpos(double x)
+template <class C>
+__device__ __forceinline__ double exp_nonpos(double x, const C &c)
 {
     constexpr double L2E = 1.4426950408889634;          // log2(e)
     constexpr double LN2_HI = 0.6931471803691238;       // ln 2 to 32 bits
@@ -207,20 +234,9 @@ __device__ __forceinline__ double exp_nonpos(double x)
     const double n = rint(x * L2E);
     double r = fma(-n, LN2_HI, x);
     r = fma(-n, LN2_LO, r);
-    double p = 1.0 / 6227020800.0;  // 1/13!
-    p = fma(p, r, 1.0 / 479001600.0);
-    p = fma(p, r, 1.0 / 39916800.0);
-    p = fma(p, r, 1.0 / 3628800.0);
-    p = fma(p, r, 1.0 / 362880.0);
-    p = fma(p, r, 1.0 / 40320.0);
-    p = fma(p, r, 1.0 / 5040.0);
-    p = fma(p, r, 1.0 / 720.0);
-    p = fma(p, r, 1.0 / 120.0);
-    p = fma(p, r, 1.0 / 24.0);
-    p = fma(p, r, 1.0 / 6.0);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
+    double p = c.e[13];
+#pragma unroll
+    for (int k = 12; k >= 0; --k) p = fma(p, r, c.e[k]);
     return ldexp(p, (int)n);
 }
 
@@ -241,8 +257,11 @@ __device__ __forceinline__ double exp_nonpos(double x)
 // grid: E * ntile, 256 threads.
 // VIR: also accumulate the pair virial r_ij (x) f_ij (6 components) into
 // vpart[(e*6 + c)*ep_stride + ep_off + tile].
+// CARG: the special functions' coefficients come from the argument cf (SGPRs)
+// instead of literals (VGPRs); the host picks it for large grids.
 constexpr int RS_WAVES = 4;
-template <bool VIR>
+constexpr long RS_CARG_BLOCKS = 16384;  // grids at least this large use CARG
+template <bool VIR, bool CARG>
 __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ pos, long pstride,
                                                    const double *__restrict__ charge, int n_atoms,
                                                    int NAP, int nblk, double gamma, double half1mg,
@@ -250,7 +269,8 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
                                                    Box box, Ewald ew,
                                                    double *__restrict__ rpart,
                                                    double *__restrict__ epart, int ep_stride,
-                                                   int ep_off, double *__restrict__ vpart)
+                                                   int ep_off, double *__restrict__ vpart,
+                                                   const RsCoef cf)
 {
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int e = bid / ntile, t = bid - e * ntile;
@@ -344,8 +364,15 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
         const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
         const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
         const double rinv = rsqrt_pos(rsq);
-        const double ex = exp_nonpos(-ew.kappa2 * rsq);
-        double ec = erfc_via_erfcx(ew.kappa * (rsq * rinv), ex);
+        double ex, ec;
+        if constexpr (CARG) {
+            ex = exp_nonpos(-ew.kappa2 * rsq, cf);
+            ec = erfc_via_erfcx(ew.kappa * (rsq * rinv), ex, cf);
+        } else {
+            constexpr LitCoef lc{};
+            ex = exp_nonpos(-ew.kappa2 * rsq, lc);
+            ec = erfc_via_erfcx(ew.kappa * (rsq * rinv), ex, lc);
+        }
         // same molecule: -erf (forces.cpp:158-166); a molecule's three sites
         // lie in one 64-site block or two consecutive ones
         if (near && (I * 64 + il) / 3 == (J * 64 + jl) / 3) ec -= 1.0;
