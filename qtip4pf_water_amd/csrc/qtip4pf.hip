@@ -49,6 +49,7 @@ struct qtp_ctx {
     int G = 0;           // k groups
     int ntile = 0;       // real-space tiles per evaluation
     int nmb = 0;         // LJ/intra blocks per evaluation
+    int lj_lpm = 16;     // lanes per molecule in k_ljintra
     int AC = 64, nac = 0, ngc = 0, ngr = 0;
     int ep_stride = 0, ep_lj = 0, ep_real = 0, ep_k = 0, ep_spl = 0;
     // splined pair potentials (fast-CMD path): tables loaded, used by the
@@ -542,14 +543,19 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     }
     {
         TimeScope ts(ctx, "ljintra");
-        if (vout)
-            k_ljintra<true><<<E * ctx->nmb, 256, 0, sk>>>(
-                pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, ctx->flist, ctx->fcap, lmode, lbase,
-                ctx->box, ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride, ctx->ep_lj, vp);
-        else
-            k_ljintra<false><<<E * ctx->nmb, 256, 0, sk>>>(
-                pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, ctx->flist, ctx->fcap, lmode, lbase,
-                ctx->box, ctx->ff, ctx->flj, ctx->epart, ctx->ep_stride, ctx->ep_lj, vp);
+#define QTP_LJ_LAUNCH(VIR, LPM)                                                                    \
+    k_ljintra<VIR, LPM><<<E * ctx->nmb, 256, 0, sk>>>(pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, \
+                                                      ctx->flist, ctx->fcap, lmode, lbase,         \
+                                                      ctx->box, ctx->ff, ctx->flj, ctx->epart,     \
+                                                      ctx->ep_stride, ctx->ep_lj, vp)
+        if (ctx->lj_lpm == 64) {
+            if (vout) QTP_LJ_LAUNCH(true, 64);
+            else QTP_LJ_LAUNCH(false, 64);
+        } else {
+            if (vout) QTP_LJ_LAUNCH(true, 16);
+            else QTP_LJ_LAUNCH(false, 16);
+        }
+#undef QTP_LJ_LAUNCH
     }
     if (ctx->spl_on) {
         TimeScope ts(ctx, "spline");
@@ -1049,7 +1055,9 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     for (int I = 0; I < ctx->nblk; ++I)
         for (int J = I; J < ctx->nblk; ++J) tl.push_back(make_int2(I, J));
     ctx->ntile = (int)tl.size();
-    ctx->nmb = (ctx->n_mols + (256 / LJ_LPM) - 1) / (256 / LJ_LPM);
+    // a whole wave per molecule when one launch covers few molecules
+    ctx->lj_lpm = (long)std::max(ctx->P, 1) * ctx->n_mols <= 2048 ? 64 : 16;
+    ctx->nmb = (ctx->n_mols + (256 / ctx->lj_lpm) - 1) / (256 / ctx->lj_lpm);
     ctx->ep_lj = 0;
     ctx->ep_real = ctx->nmb;
     ctx->ep_k = ctx->nmb + ctx->ntile;

@@ -1118,12 +1118,14 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
 }
 
 // LJ over the full O-O list (forces.cpp:489-515, lj_oxygen :25-59) and the
-// intramolecular terms (forces.cpp:519-554, :65-138).  16 lanes per molecule
-// share its list row; lane 0 of the group adds the intramolecular terms.
-// Each LJ pair is met from both sides; its energy is halved.
-// grid: E * ceil(n_mols/16), 256 threads.
-constexpr int LJ_LPM = 16;
-template <bool VIR>
+// intramolecular terms (forces.cpp:519-554, :65-138).  LPM lanes per molecule
+// share its list row (16, or a whole wave when the launch is small: one
+// evaluation of a few hundred molecules leaves most CUs idle otherwise, and
+// each lane's walk of the row is a chain of dependent loads); lane 0 of the
+// group adds the intramolecular terms.  Each LJ pair is met from both sides;
+// its energy is halved.
+// grid: E * ceil(n_mols / (256 / LPM)), 256 threads.
+template <bool VIR, int LPM>
 __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos, long pstride,
                                                  int n_mols, int NAP,
                                                  const int *__restrict__ fpoint,
@@ -1134,12 +1136,12 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
                                                  int ep_off, double *__restrict__ vpart)
 {
     __shared__ double sh[16];
-    constexpr int MPB = 256 / LJ_LPM;
+    constexpr int MPB = 256 / LPM;
     double v6[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     const int nmb = (n_mols + MPB - 1) / MPB;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int e = bid / nmb, mb = bid - e * nmb;
-    const int gi = threadIdx.x / LJ_LPM, t = threadIdx.x % LJ_LPM;
+    const int gi = threadIdx.x / LPM, t = threadIdx.x % LPM;
     const int i = mb * MPB + gi;
     const double *P = pos + (long)e * pstride;
     double fo[3] = {0.0, 0.0, 0.0}, elj = 0.0, en = 0.0;
@@ -1152,7 +1154,7 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
         const int *fl = flist + (long)b * fcap;
         // an overflowed list (status flag set by the list kernels) is never read past capacity
         const int k0 = (int)min((long)fp[i], fcap), k1 = (int)min((long)fp[i + 1], fcap);
-        for (int k = k0 + t; k < k1; k += LJ_LPM) {
+        for (int k = k0 + t; k < k1; k += LPM) {
             const int aj = 3 * fl[k];
             double r[3];
             for (int d = 0; d < 3; ++d)
@@ -1179,7 +1181,7 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
             }
         }
     }
-    for (int off = LJ_LPM / 2; off > 0; off >>= 1) {
+    for (int off = LPM / 2; off > 0; off >>= 1) {
         for (int d = 0; d < 3; ++d) fo[d] += __shfl_xor(fo[d], off, 64);
         elj += __shfl_xor(elj, off, 64);
     }
