@@ -524,11 +524,14 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     }
     {
         TimeScope ts(ctx, "ksum");
-        if (ctx->kf_mf)
-            k_ksum_mf<<<E * ctx->nac, 64 * KS_WAVES, ctx->ksum_mf_lds, sk>>>(
+        if (ctx->kf_mf) {
+            // few chunks in the launch: several blocks per chunk
+            const int usplit = E * ctx->nac < 128 ? std::min(4, std::max(1, ctx->ks_nunit / KS_WAVES)) : 1;
+            k_ksum_mf<<<E * ctx->nac * usplit, 64 * KS_WAVES, ctx->ksum_mf_lds, sk>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->ks_col,
-                ctx->ks_ncol4, ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, ctx->spart);
-        else
+                ctx->ks_ncol4, ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, usplit,
+                ctx->spart);
+        } else
             k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, sk>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->G,
                 ctx->AC, ctx->ngc, ctx->nac, ctx->spart);

@@ -520,8 +520,10 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 // group and chunk (same place as k_ksum's sums): Re/Im Z times F0..F3, in
 // that order.
 // kcol[c] = (kx | ky << 16, first group, first kz, last kz) per sorted
-// column (padding columns: first kz > last kz).
-// grid: E * nac, 64 * KS_WAVES threads, ksum_mf_lds_bytes(K, AC).
+// column (padding columns: first kz > last kz).  usplit blocks share a chunk
+// (each builds its tables and takes 1/usplit of the units) when a launch has
+// few chunks.
+// grid: E * nac * usplit, 64 * KS_WAVES threads, ksum_mf_lds_bytes(K, AC).
 constexpr int KS_WAVES = 8;
 // LDS rows of AC + 2 entries: the B operands of a quarter-wave (16 rows n at
 // one site) and the A factors of a tile's columns then fall in different banks.
@@ -534,7 +536,8 @@ inline size_t ksum_mf_lds_bytes(int K, int AC, int ncol4, int nunit)
 __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
     int NAP, double gamma, double half1mg, Ewald ew, const int4 *__restrict__ kcol, int ncol4,
-    const int2 *__restrict__ kunit, int nunit, int G, int AC, int nac, double *__restrict__ spart)
+    const int2 *__restrict__ kunit, int nunit, int G, int AC, int nac, int usplit,
+    double *__restrict__ spart)
 {
     extern __shared__ double smem[];
     const int K = ew.K;
@@ -543,7 +546,10 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     double *Zt = smem + 4 * K * RS;  // [32][RS]
     int4 *sc = (int4 *)(Zt + 32 * RS);  // [ncol4] column table
     int2 *su = (int2 *)(sc + ncol4);    // [nunit] unit table
-    const int e = blockIdx.x / nac, ac = blockIdx.x - e * nac;
+    const int per_e = nac * usplit;
+    const int e = blockIdx.x / per_e, r = blockIdx.x - e * per_e;
+    const int ac = r / usplit, us = r - ac * usplit;
+    const int u0 = us * nunit / usplit, u1 = (us + 1) * nunit / usplit;
     const double *X = pos + (long)e * pstride;
     for (int t = threadIdx.x; t < ncol4; t += blockDim.x) sc[t] = kcol[t];
     for (int t = threadIdx.x; t < nunit; t += blockDim.x) su[t] = kunit[t];
@@ -586,7 +592,7 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     const int xa = (pp == 1 || pp == 3) ? 1 : 0, yb = (pp == 1 || pp == 2) ? 1 : 0;
     const double *Td = (const double *)T;
     double *out = spart + ((long)e * nac + ac) * G * 8;
-    for (int u = w; u < nunit; u += KS_WAVES) {
+    for (int u = u0 + w; u < u1; u += KS_WAVES) {
         const int2 un = su[u];
         const int ck = sc[4 * un.x + (j16 >> 2)].x;
         const int kx = ck & 0xffff, ky = ck >> 16;
