@@ -67,8 +67,8 @@ struct qtp_ctx {
     // for k_max <= 15; the vector kernels above that, or when
     // QTP_KSPACE=vector is set at create
     bool kf_mf = false;
-    int mf_Mp = 0, mf_nks = 0, mf_nrt = 0, ks_nunit = 0, ks_ncol4 = 0;
-    size_t kforce_mf_lds = 0, ksum_mf_lds = 0;
+    int mf_Mp = 0, mf_nks = 0, mf_nrt = 0, mf_nrt_small = 0, ks_nunit = 0, ks_ncol4 = 0;
+    size_t kforce_mf_lds = 0, kforce_mf_lds_small = 0, ksum_mf_lds = 0;
 
     // device state (planar SoA)
     double *q = nullptr, *p = nullptr, *F = nullptr, *fph = nullptr, *disp = nullptr;
@@ -82,7 +82,8 @@ struct qtp_ctx {
     double *fk2 = nullptr;  // second half of split k-space forces (SC phases)
     double *wm = nullptr;   // k_kforce_mf coefficient matrices [E][4 nks][Mp]
     int *gcol = nullptr;    // column of each k-group in the matrix rows
-    int4 *mf_rt = nullptr;  // k_kforce_mf row-tile table
+    int4 *mf_rt = nullptr;  // k_kforce_mf row-tile table (MF_WAVES waves)
+    int4 *mf_rt_small = nullptr;  // and for MF_WAVES_SMALL
     int4 *ks_col = nullptr; // k_ksum_mf column table
     int2 *ks_unit = nullptr;
     double *scal = nullptr, *out = nullptr, *heat = nullptr, *heat_step = nullptr;
@@ -304,18 +305,15 @@ static void build_groups(qtp_ctx *ctx, std::vector<int4> &g, std::vector<double>
 // Matrix layout of k_kforce_mf.  The groups come in columns of fixed
 // (kx, ky) and rising kz (build_groups' order); the columns are sorted by
 // their largest kz (descending, stable) and paired into 16-row tiles, each
-// taking ceil((2 kz_max + 2) / 4) k-steps of 4.  The tile table lists the
-// tiles in the order the MF_WAVES waves of a block take them (wave w reads
-// entries w, w + MF_WAVES, ...): round q hands the sorted tiles
-// q MF_WAVES ... out to the waves in turn, reversed in odd rounds, which
-// evens out the k-steps per wave.  Returns false when a (kx, ky) pair
+// taking ceil((2 kz_max + 2) / 4) k-steps of 4 (tiles: in that sorted
+// order; mf_tile_order arranges them for a block of W waves).  Returns false when a (kx, ky) pair
 // appears in two runs or a column's kz are not consecutive (then the vector
 // kernels are used).
 // k_ksum_mf: kcol per sorted column (kx | ky << 16, first group, first kz,
 // last kz), padded to a multiple of four with empty columns; units (tile of
 // four columns, n-tile) for every n = 2 kz + 0/1 that a tile's columns reach.
 static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
-                            std::vector<int4> &rt, int &Mp, int &nks, std::vector<int4> &kcol,
+                            std::vector<int4> &tiles, int &Mp, int &nks, std::vector<int4> &kcol,
                             std::vector<int2> &kunit)
 {
     struct Col {
@@ -345,8 +343,7 @@ static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
     const int ntl = (nc + 1) / 2;
     Mp = 16 * ntl;
     nks = 0;
-    const int rounds = (ntl + MF_WAVES - 1) / MF_WAVES;
-    rt.assign((size_t)rounds * MF_WAVES, make_int4(0, 0, 0, 0));
+    tiles.assign(ntl, make_int4(0, 0, 0, 0));
     for (int t = 0; t < ntl; ++t) {
         const Col &c0 = cols[ord[2 * t]];
         int4 x;
@@ -355,8 +352,7 @@ static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
         x.z = (2 * c0.kzmax + 2 + 3) / 4;  // the first column has the larger kz_max
         x.w = 16 * t;
         nks = std::max(nks, x.z);
-        const int q = t / MF_WAVES, p = t - q * MF_WAVES;
-        rt[(size_t)q * MF_WAVES + ((q & 1) ? MF_WAVES - 1 - p : p)] = x;
+        tiles[t] = x;
     }
     const int nq = (nc + 3) / 4;
     kcol.assign((size_t)4 * nq, make_int4(0, 0, 1, 0));
@@ -368,6 +364,22 @@ static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
     for (int t = 0; t < nq; ++t)
         for (int u = 0; u <= (2 * kcol[4 * t].w + 1) / 16; ++u) kunit.push_back(make_int2(t, u));
     return true;
+}
+
+// The tile table of k_kforce_mf for blocks of W waves: wave w reads entries
+// w, w + W, ...; round q hands the sorted tiles q W ... out to the waves in
+// turn, reversed in odd rounds, which evens out the k-steps per wave (empty
+// entries: k-steps 0).
+static std::vector<int4> mf_tile_order(const std::vector<int4> &tiles, int W)
+{
+    const int ntl = (int)tiles.size();
+    const int rounds = (ntl + W - 1) / W;
+    std::vector<int4> rt((size_t)rounds * W, make_int4(0, 0, 0, 0));
+    for (int t = 0; t < ntl; ++t) {
+        const int q = t / W, p = t - q * W;
+        rt[(size_t)q * W + ((q & 1) ? W - 1 - p : p)] = tiles[t];
+    }
+    return rt;
 }
 
 // Orthonormal real normal-mode matrix (FFTW R2HC halfcomplex layout with the
@@ -597,9 +609,15 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     {
         TimeScope ts(ctx, "kforce");
         const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
-        if (ctx->kf_mf)
-            k_kforce_mf<<<E * ((ctx->n_atoms + MF_SITES - 1) / MF_SITES), 64 * MF_WAVES,
-                          ctx->kforce_mf_lds, s>>>(
+        const int nbm = E * ((ctx->n_atoms + MF_SITES - 1) / MF_SITES);
+        // few blocks: twice the waves per block (when its LDS fits the default 64 KB)
+        if (ctx->kf_mf && nbm < MF_SMALL_BLOCKS && ctx->kforce_mf_lds_small <= 64 * 1024)
+            k_kforce_mf<MF_WAVES_SMALL><<<nbm, 64 * MF_WAVES_SMALL, ctx->kforce_mf_lds_small, s>>>(
+                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
+                ctx->mf_nks, ctx->mf_rt_small, ctx->mf_nrt_small, ctx->rpart, ctx->nblk, ctx->flj,
+                ctx->spl_on ? ctx->fspl : nullptr, fout);
+        else if (ctx->kf_mf)
+            k_kforce_mf<MF_WAVES><<<nbm, 64 * MF_WAVES, ctx->kforce_mf_lds, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
                 ctx->mf_nks, ctx->mf_rt, ctx->mf_nrt, ctx->rpart, ctx->nblk, ctx->flj,
                 ctx->spl_on ? ctx->fspl : nullptr, fout);
@@ -1038,17 +1056,21 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->kf_split = (long)ctx->P * ((ctx->n_atoms + KF_SITES - 1) / KF_SITES) <= 512;
     ctx->kforce_lds_split = kforce_lds_bytes(K, KF_SPLIT_WAVES, ctx->kf_chunk_split);
     std::vector<int> gcol;
-    std::vector<int4> mf_rt, ks_col;
+    std::vector<int4> mf_tiles, mf_rt, mf_rt_small, ks_col;
     std::vector<int2> ks_unit;
     {
         const char *kv = std::getenv("QTP_KSPACE");
         const bool want_mf = !(kv && std::strcmp(kv, "vector") == 0);
         ctx->kf_mf = want_mf && 2 * K <= 4 * MF_MAXKS && ctx->G > 0 && ctx->AC % 4 == 0 &&
-                     build_mf_layout(g, gcol, mf_rt, ctx->mf_Mp, ctx->mf_nks, ks_col, ks_unit);
+                     build_mf_layout(g, gcol, mf_tiles, ctx->mf_Mp, ctx->mf_nks, ks_col, ks_unit);
         if (ctx->kf_mf) {
+            mf_rt = mf_tile_order(mf_tiles, MF_WAVES);
+            mf_rt_small = mf_tile_order(mf_tiles, MF_WAVES_SMALL);
             ctx->mf_nrt = (int)mf_rt.size();
+            ctx->mf_nrt_small = (int)mf_rt_small.size();
             ctx->ks_nunit = (int)ks_unit.size();
-            ctx->kforce_mf_lds = kforce_mf_lds_bytes(K, ctx->mf_nks);
+            ctx->kforce_mf_lds = kforce_mf_lds_bytes(K, ctx->mf_nks, MF_WAVES);
+            ctx->kforce_mf_lds_small = kforce_mf_lds_bytes(K, ctx->mf_nks, MF_WAVES_SMALL);
             ctx->ks_ncol4 = (int)ks_col.size();
             ctx->ksum_mf_lds = ksum_mf_lds_bytes(K, ctx->AC, ctx->ks_ncol4, ctx->ks_nunit);
             ctx->kf_split = false;
@@ -1098,6 +1120,7 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
         (st = dalloc(ctx, &ctx->mode, ctx->P)) ||
         (ctx->kf_mf && ((st = dalloc(ctx, &ctx->gcol, ctx->G)) ||
                         (st = dalloc(ctx, &ctx->mf_rt, ctx->mf_nrt)) ||
+                        (st = dalloc(ctx, &ctx->mf_rt_small, ctx->mf_nrt_small)) ||
                         (st = dalloc(ctx, &ctx->ks_col, ks_col.size())) ||
                         (st = dalloc(ctx, &ctx->ks_unit, ks_unit.size())))) ||
         (st = alloc_all(ctx)))
@@ -1105,6 +1128,8 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     if (ctx->kf_mf &&
         (hipMemcpy(ctx->gcol, gcol.data(), gcol.size() * sizeof(int), hipMemcpyHostToDevice) ||
          hipMemcpy(ctx->mf_rt, mf_rt.data(), mf_rt.size() * sizeof(int4), hipMemcpyHostToDevice) ||
+         hipMemcpy(ctx->mf_rt_small, mf_rt_small.data(), mf_rt_small.size() * sizeof(int4),
+                   hipMemcpyHostToDevice) ||
          hipMemcpy(ctx->ks_col, ks_col.data(), ks_col.size() * sizeof(int4), hipMemcpyHostToDevice) ||
          hipMemcpy(ctx->ks_unit, ks_unit.data(), ks_unit.size() * sizeof(int2), hipMemcpyHostToDevice))) {
         ctx->err = "constant upload failed";
@@ -1131,7 +1156,7 @@ void qtp_destroy(qtp_ctx *ctx)
     void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
                     ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
                     ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart,
-                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->wm, ctx->gcol, ctx->mf_rt, ctx->ks_col, ctx->ks_unit, ctx->Ve, ctx->epart,
+                    ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->wm, ctx->gcol, ctx->mf_rt, ctx->mf_rt_small, ctx->ks_col, ctx->ks_unit, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace};

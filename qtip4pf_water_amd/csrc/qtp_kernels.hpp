@@ -942,20 +942,23 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
 // both columns of a tile (the 16x16 result layout), g = 0, 1 give the x and
 // y forces of (Ar u1 + Br u3) and (Ai u2 + Bi u4), g = 2, 3 the z force of
 // (Ar v1 + Br v3) and (Ai v2 + Bi v4).
-// Block = 48 sites (16 whole molecules: three 16-site tiles) x MF_WAVES waves;
-// each wave reads the z factors of the three site tiles (the B operands) from
-// LDS and takes every MF_WAVES-th row tile.  Epilogue as k_kforce: wave
+// Block = 48 sites (16 whole molecules: three 16-site tiles) x W waves (8,
+// or 16 when a launch has few site blocks); each wave reads the z factors of
+// the three site tiles (the B operands) from LDS and takes every W-th row
+// tile (the tile table is ordered for W).  Epilogue as k_kforce: wave
 // partials summed in wave order, real-space slot partials, M-site chain rule
 // (forces.cpp:613-619), LJ/intra (and spline) forces.
-// grid: E * ceil(n_atoms / 48), 64 * MF_WAVES threads, kforce_mf_lds_bytes.
+// grid: E * ceil(n_atoms / 48), 64 * W threads, kforce_mf_lds_bytes(K, nks, W).
 constexpr int MF_SITES = 48;
-constexpr int MF_WAVES = 8;
+constexpr int MF_WAVES = 8;       // waves per block of large launches
+constexpr int MF_WAVES_SMALL = 16;  // and of launches of fewer than MF_SMALL_BLOCKS blocks
+constexpr int MF_SMALL_BLOCKS = 64;
 constexpr int MF_MAXKS = 8;  // k-steps per tile (A operands in registers): 2 K <= 32 (k_max <= 15)
 
-inline size_t kforce_mf_lds_bytes(int K, int nks)
+inline size_t kforce_mf_lds_bytes(int K, int nks, int W)
 {
     return (size_t)2 * K * MF_SITES * 16 + (size_t)4 * nks * MF_SITES * 8 +
-           (size_t)2 * MF_WAVES * 3 * MF_SITES * 8 + (size_t)3 * MF_SITES * 8;
+           (size_t)2 * W * 3 * MF_SITES * 8 + (size_t)3 * MF_SITES * 8;
 }
 
 // One 16-row tile of k_kforce_mf over three site tiles, NT k-steps: the A
@@ -983,7 +986,8 @@ __device__ __forceinline__ void mf_tile(const double *__restrict__ Wr, long Mp4,
     }
 }
 
-__global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
+template <int W>
+__global__ __launch_bounds__(64 * W) void k_kforce_mf(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
     int NAP, double gamma, double half1mg, Ewald ew, const double *__restrict__ wm, int Mp,
     int nks, const int4 *__restrict__ rt, int nrt, const double *__restrict__ rpart, int nblk,
@@ -994,8 +998,8 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
     double2 *T = (double2 *)smem;                    // [2][K][48] x (charge folded), y factors
     double *Zt = smem + 4 * K * MF_SITES;            // [NN][48]
     double *red = Zt + NN * MF_SITES;                // [W][3][48] k-space wave partials
-    double *rsm = red + MF_WAVES * 3 * MF_SITES;     // [W][3][48] real-space slot partials
-    double *ef = rsm + MF_WAVES * 3 * MF_SITES;      // [3][48]
+    double *rsm = red + W * 3 * MF_SITES;     // [W][3][48] real-space slot partials
+    double *ef = rsm + W * 3 * MF_SITES;      // [3][48]
     const int nbm = (n_atoms + MF_SITES - 1) / MF_SITES;
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int e = bid / nbm, ab = bid - e * nbm;
@@ -1005,7 +1009,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
     const double *X = pos + (long)e * pstride;
     double rx = 0.0, ry = 0.0, rz = 0.0;
     if (lane < MF_SITES && a0 + lane < n_atoms) {
-        for (int sl = w; sl < nblk; sl += MF_WAVES) {
+        for (int sl = w; sl < nblk; sl += W) {
             const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + a0 + lane;
             rx += rp[0];
             ry += rp[NAP];
@@ -1049,7 +1053,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
     const bool gxy = gq < 2;
     double acc1[3] = {0.0, 0.0, 0.0}, acc2[3] = {0.0, 0.0, 0.0};
     const double *We = wm + (long)e * NN * Mp;
-    for (int i = w; i < nrt; i += MF_WAVES) {
+    for (int i = w; i < nrt; i += W) {
         const int4 r = rt[i];
         if (r.z == 0) continue;
         mf_v4d d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0;
@@ -1104,7 +1108,7 @@ __global__ __launch_bounds__(64 * MF_WAVES) void k_kforce_mf(
     if (threadIdx.x < 3 * MF_SITES) {
         double vr = 0.0, vk = 0.0;
 #pragma unroll
-        for (int ww = 0; ww < MF_WAVES; ++ww) {
+        for (int ww = 0; ww < W; ++ww) {
             vr += rsm[(ww * 3 + d) * MF_SITES + sj];
             vk += red[(ww * 3 + d) * MF_SITES + sj];
         }
