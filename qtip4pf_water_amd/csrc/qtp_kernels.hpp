@@ -523,7 +523,7 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 // column (padding columns: first kz > last kz).  usplit blocks share a chunk
 // (each builds its tables and takes 1/usplit of the units) when a launch has
 // few chunks.
-// grid: E * nac * usplit, 64 * KS_WAVES threads, ksum_mf_lds_bytes(K, AC).
+// grid: E * nac * usplit, 64 * KS_WAVES threads, ksum_mf_lds_bytes(K, AC, ncol4, nunit).
 constexpr int KS_WAVES = 8;
 // LDS rows of AC + 2 entries: the B operands of a quarter-wave (16 rows n at
 // one site) and the A factors of a tile's columns then fall in different banks.
