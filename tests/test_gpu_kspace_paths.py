@@ -102,3 +102,39 @@ def test_largest_matrix_core_k_max(monkeypatch, oracle_mod):
         assert rel_atom_err(F, Fr) < 1e-10, path
         # energies at this size: 1e-12 of the self-energy scale (as for 4096)
         assert abs(e - er) <= 1e-12 * scale, path
+
+
+def test_nve_conservation_both_paths(monkeypatch):
+    """216 x 32 SC-PIMD NVE, 4000 steps from one start on each k-space path:
+    the conserved quantity's largest excursion of the matrix-core path is no
+    worse than the vector path's (the oracle is too slow for this length at
+    this size; the vector path is the one the shorter oracle comparisons of
+    test_gpu_drift.py also hold), and both stay small against the kinetic
+    energy."""
+    from qtip4pf_water_amd import Water
+    n_side, L, P = 6, 18.64, 32
+    n = n_side ** 3
+    say("NVE conservation, both paths", n, "mols P", P)
+    q0, box = cfg.lattice_water_box(n_side, L, seed=61, jitter_bohr=0.05)
+    beta_n = 1.0 / (cfg.K2AU * 298.0) / P
+    q = cfg.free_ring_polymer_beads(q0, P, beta_n, seed=62)
+    p = cfg.sample_momenta(n, P, beta_n, seed=63)
+    exc = {}
+    for path in ("matrix", "vector"):
+        if path == "vector":
+            monkeypatch.setenv("QTP_KSPACE", "vector")
+        else:
+            monkeypatch.delenv("QTP_KSPACE", raising=False)
+        with Water(n, P, box) as w:
+            w.set_state(q, p)
+            c0 = w.observables()["conserved"]
+            tr = w.run_observed(4000, is_nvt=False, every=50)
+            assert w.scalars()["status"] == 0
+        e = np.concatenate([[c0], tr[:, 1]])
+        assert np.all(np.isfinite(e)), path
+        exc[path] = float(np.max(np.abs(e - e[0])))
+    _, mass = cfg.charges_masses(n)
+    ke = 0.5 * float(np.sum(p * p / np.tile(mass, P)))
+    say(f"excursions: matrix {exc['matrix']:.3e} vector {exc['vector']:.3e} Eh (KE {ke:.3e})")
+    assert exc["matrix"] <= 1.5 * exc["vector"] + 1e-6 * ke
+    assert exc["matrix"] < 1e-2 * ke
