@@ -39,8 +39,6 @@ def hipcc() -> str:
 
 
 def build(verbose: bool = False) -> str:
-    deps = [SRC, os.path.join(PKG, "csrc", "qtp_kernels.hpp"),
-            os.path.join(ROOT, "include", "qtip4pf.h")]
     tmp = OUT + ".tmp"
     cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC, SETUP_SRC]
     if verbose:
@@ -50,7 +48,6 @@ def build(verbose: bool = False) -> str:
         sys.stderr.write(r.stdout + r.stderr)
         raise RuntimeError("hipcc failed")
     os.replace(tmp, OUT)
-    del deps
     return OUT
 
 

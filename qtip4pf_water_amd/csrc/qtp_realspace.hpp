@@ -1,0 +1,194 @@
+// qtp_realspace.hpp -- real-space Ewald (forces.cpp:141-177, 586-608).
+#pragma once
+#include "qtp_common.hpp"
+
+namespace qtp {
+// Real-space Ewald over all site pairs i<j within r_c (forces.cpp:586-608,
+// rwald :141-177).  One 256-thread block per 64x64 tile (I<=J) of the site
+// pair matrix; wave w takes the 16 j-columns 16w..16w+15 against all 64 i.
+// Pass 1 (lane <-> i) marks each lane's in-cutoff columns (16-bit mask).
+// The wave then lays its in-cutoff pairs out in an LDS queue, lane-major
+// (offsets by a wave prefix sum of the mask popcounts), and pass 2 gives
+// every lane the same number of consecutive queue entries: lanes stay busy
+// however unevenly the pairs fall on the rows.  Consecutive entries mostly
+// share i, so a lane keeps its i-force in registers and adds it to the
+// wave's LDS row when i changes; j reactions go to the LDS column array with
+// ds_add_f64.  All LDS adds of a row or column come from one wavefront, in
+// program order and fixed lane order, so the result is reproducible.
+// Partial site forces go to slot J (block I's sites) and slot I (block J's
+// sites): every (site, slot) has exactly one writer.
+// grid: E * ntile, 256 threads.
+// VIR: also accumulate the pair virial r_ij (x) f_ij (6 components) into
+// vpart[(e*6 + c)*ep_stride + ep_off + tile].
+// CARG: the special functions' coefficients come from the argument cf (SGPRs)
+// instead of literals (VGPRs); the host picks it for large grids.
+constexpr int RS_WAVES = 4;
+constexpr long RS_CARG_BLOCKS = 16384;  // grids at least this large use CARG
+template <bool VIR, bool CARG>
+__global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ pos, long pstride,
+                                                   const double *__restrict__ charge, int n_atoms,
+                                                   int NAP, int nblk, double gamma, double half1mg,
+                                                   const int2 *__restrict__ tiles, int ntile,
+                                                   Box box, Ewald ew,
+                                                   double *__restrict__ rpart,
+                                                   double *__restrict__ epart, int ep_stride,
+                                                   int ep_off, double *__restrict__ vpart,
+                                                   const RsCoef cf)
+{
+    const int bid = xcd_remap(blockIdx.x, gridDim.x);
+    const int e = bid / ntile, t = bid - e * ntile;
+    const int I = tiles[t].x, J = tiles[t].y;
+    const int l = threadIdx.x & 63;
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const double *X = pos + (long)e * pstride;
+    __shared__ double4 sI[64], sJ[64];  // x, y, z, charge
+    __shared__ double fj[3][64];
+    __shared__ double fi[RS_WAVES][3][64];
+    __shared__ unsigned short qe[RS_WAVES][64 * 16];
+    __shared__ double ewv[RS_WAVES];
+    __shared__ double wv[RS_WAVES][6];
+    if (w < 2) {
+        const int s = (w == 0 ? J : I) * 64 + l;
+        double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (s < n_atoms)
+            v = make_double4(site_coord(X, NAP, s, 0, gamma, half1mg),
+                             site_coord(X, NAP, s, 1, gamma, half1mg),
+                             site_coord(X, NAP, s, 2, gamma, half1mg), charge[s]);
+        (w == 0 ? sJ : sI)[l] = v;
+        if (w == 0) {
+            fj[0][l] = 0.0;
+            fj[1][l] = 0.0;
+            fj[2][l] = 0.0;
+        }
+    }
+    fi[w][0][l] = 0.0;
+    fi[w][1][l] = 0.0;
+    fi[w][2][l] = 0.0;
+    __syncthreads();
+    const int c0 = 16 * w;
+    const int i = I * 64 + l;
+    // pass 1: in-cutoff columns of row i (j > i; j < n_atoms)
+    unsigned int mask = 0u;
+    if (i < n_atoms) {
+        const double4 a = sI[l];
+        for (int c = 0; c < 16; ++c) {
+            const int jl = c0 + c;
+            const int j = J * 64 + jl;
+            const double4 b = sJ[jl];
+            const double dx = mimg_fast(a.x - b.x, box.L[0], box.inv[0]);
+            const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
+            const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
+            const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+            if (j > i && j < n_atoms && rsq < ew.rcut_sq) mask |= 1u << c;
+        }
+    }
+    // the wave's pair queue, lane-major
+    const int pc = __popc(mask);
+    int incl = pc;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o, 64);
+        if (l >= o) incl += v;
+    }
+    const int total = __shfl(incl, 63, 64);
+    {
+        int pos_q = incl - pc;
+        unsigned int m = mask;
+        while (m) {
+            const int c = __ffs(m) - 1;
+            m &= m - 1u;
+            qe[w][pos_q++] = (unsigned short)(l | (c << 6));
+        }
+    }
+    __syncthreads();
+    // pass 2: lane l takes queue entries [l*S, l*S + S)
+    const bool near = J - I <= 1;
+    const int S = (total + 63) >> 6;
+    const int q0 = min(l * S, total), q1 = min(q0 + S, total);
+    double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
+    double vxx = 0.0, vyy = 0.0, vzz = 0.0, vxy = 0.0, vxz = 0.0, vyz = 0.0;
+    int icur = -1;
+    double4 a = make_double4(0.0, 0.0, 0.0, 0.0);
+    for (int q = q0; q < q1; ++q) {
+        const int ent = qe[w][q];
+        const int il = ent & 63, jl = c0 + (ent >> 6);
+        if (il != icur) {
+            if (icur >= 0) {
+                atomicAdd(&fi[w][0][icur], fx);
+                atomicAdd(&fi[w][1][icur], fy);
+                atomicAdd(&fi[w][2][icur], fz);
+            }
+            fx = fy = fz = 0.0;
+            icur = il;
+            a = sI[il];
+        }
+        const double4 b = sJ[jl];
+        const double dx = mimg_fast(a.x - b.x, box.L[0], box.inv[0]);
+        const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
+        const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
+        const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+        const double rinv = rsqrt_pos(rsq);
+        double ex, ec;
+        if constexpr (CARG) {
+            ex = exp_nonpos(-ew.kappa2 * rsq, cf);
+            ec = erfc_via_erfcx(ew.kappa * (rsq * rinv), ex, cf);
+        } else {
+            constexpr LitCoef lc{};
+            ex = exp_nonpos(-ew.kappa2 * rsq, lc);
+            ec = erfc_via_erfcx(ew.kappa * (rsq * rinv), ex, lc);
+        }
+        // same molecule: -erf (forces.cpp:158-166); a molecule's three sites
+        // lie in one 64-site block or two consecutive ones
+        if (near && (I * 64 + il) / 3 == (J * 64 + jl) / 3) ec -= 1.0;
+        const double qq = a.w * b.w;
+        const double ep = qq * rinv * ec;
+        const double pf = (ep + qq * ew.pref2k * ex) * (rinv * rinv);
+        en += ep;
+        fx = fma(pf, dx, fx);
+        fy = fma(pf, dy, fy);
+        fz = fma(pf, dz, fz);
+        if (VIR) {
+            vxx = fma(pf * dx, dx, vxx);
+            vyy = fma(pf * dy, dy, vyy);
+            vzz = fma(pf * dz, dz, vzz);
+            vxy = fma(pf * dx, dy, vxy);
+            vxz = fma(pf * dx, dz, vxz);
+            vyz = fma(pf * dy, dz, vyz);
+        }
+        atomicAdd(&fj[0][jl], -pf * dx);
+        atomicAdd(&fj[1][jl], -pf * dy);
+        atomicAdd(&fj[2][jl], -pf * dz);
+    }
+    if (icur >= 0) {
+        atomicAdd(&fi[w][0][icur], fx);
+        atomicAdd(&fi[w][1][icur], fy);
+        atomicAdd(&fi[w][2][icur], fz);
+    }
+    en = wave_sum(en);
+    if (l == 0) ewv[w] = en;
+    if (VIR) {
+        const double v6[6] = {vxx, vyy, vzz, vxy, vxz, vyz};
+        for (int cc = 0; cc < 6; ++cc) {
+            const double v = wave_sum(v6[cc]);
+            if (l == 0) wv[w][cc] = v;
+        }
+    }
+    __syncthreads();
+    if (VIR && w == 0 && l < 6)
+        vpart[((long)e * 6 + l) * ep_stride + ep_off + t] = wv[0][l] + wv[1][l] + wv[2][l] + wv[3][l];
+    const long base_e = (long)e * nblk * 3 * NAP;
+    if (w < 3) {
+        // wave d writes component d of both slot entries of this tile
+        const int d = w;
+        const double si = fi[0][d][l] + fi[1][d][l] + fi[2][d][l] + fi[3][d][l];
+        if (I == J) {
+            rpart[base_e + (long)I * 3 * NAP + d * NAP + i] = si + fj[d][l];
+        } else {
+            rpart[base_e + (long)J * 3 * NAP + d * NAP + i] = si;                 // slot J
+            rpart[base_e + (long)I * 3 * NAP + d * NAP + J * 64 + l] = fj[d][l];  // slot I
+        }
+    } else if (l == 0) {
+        epart[(long)e * ep_stride + ep_off + t] = ewv[0] + ewv[1] + ewv[2] + ewv[3];
+    }
+}
+
+}  // namespace qtp
