@@ -92,3 +92,32 @@ def test_sc_pimd_nve_drift_no_worse_than_reference(oracle_mod):
     ke = 0.5 * np.sum(p * p / np.tile(o.mass, P))
     say(f"SC-PIMD excursion gpu {dg:.3e} ref {dr:.3e} Eh (KE {ke:.3e})")
     assert dg <= 1.5 * dr + 1e-6 * ke
+
+
+def test_long_nvt_run_stays_bounded():
+    """216 x 32 SC-PIMD NVT, 20,000 steps (5 ps) of the benchmark workload's
+    start: no list overflow, lists rebuilt as molecules move, and the
+    observables stay finite and within the band of their own early values
+    (K_cv within 10 %, the potential energy within 10 Eh of the SC sum)."""
+    from qtip4pf_water_amd import Water
+    n_side, L, P = 6, 18.64, 32
+    n = n_side ** 3
+    say("long NVT run", n, "mols P", P, "20000 steps")
+    q0, box = cfg.lattice_water_box(n_side, L, seed=2024)
+    beta_n = 1.0 / (cfg.K2AU * 298.0) / P
+    q = cfg.free_ring_polymer_beads(q0, P, beta_n, seed=2025)
+    p = cfg.sample_momenta(n, P, beta_n, seed=2026)
+    with Water(n, P, box) as w:
+        w.set_seed(77)
+        w.set_state(q, p)
+        w.run(2000, is_nvt=True)  # leave the lattice start
+        tr = w.run_observed(18000, is_nvt=True, every=1000)
+        s = w.scalars()
+    assert s["status"] == 0 and s["steps"] == 20000
+    assert s["rebuilds"] > 100
+    assert np.all(np.isfinite(tr))
+    kcv, energy = tr[:, 0], tr[:, 2]
+    say(f"K_cv {kcv.min():.4f}..{kcv.max():.4f} Eh, V {energy.min():.3f}..{energy.max():.3f} Eh, "
+        f"{s['rebuilds']:.0f} rebuilds")
+    assert np.all(np.abs(kcv / kcv[0] - 1.0) < 0.1)
+    assert np.all(np.abs(energy - energy[0]) < 10.0)
