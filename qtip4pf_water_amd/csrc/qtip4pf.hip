@@ -3,6 +3,8 @@
 // One context = one system (the reference's process globals, water.cpp:29-38)
 // with its device buffers, one HIP stream and (resident mode) one captured
 // HIP graph per step kind.  See include/qtip4pf.h for the entry points.
+// The context and its set-up and launch sequences are in qtp_host_ctx.hpp,
+// qtp_host_setup.hpp and qtp_host_run.hpp; this file holds the C ABI.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,955 +23,9 @@
 
 using namespace qtp;
 
-namespace {
-
-
-struct Timer {
-    double ms = 0.0;
-    int n = 0;
-};
-
-}  // namespace
-
-struct qtp_ctx {
-    qtp_params prm{};
-    qtp_derived drv{};
-    std::string err;
-    hipStream_t stream = nullptr;
-    // second stream for the k-space half of a force phase (fork/join by events)
-    hipStream_t s2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
-
-    int P = 0, n_mols = 0, n_atoms = 0, n_dof = 0, NAP = 0, NMP = 0, nblk = 0, list_size = 0;
-    long fcap = 0;
-    Box box{};
-    FF ff{};
-    Ewald ew{};
-    double eself = 0.0;  // sum_i 0.5 * (2 kappa/sqrt(pi)) q_i^2 (forces.cpp:622-624)
-    int G = 0;           // k groups
-    int ntile = 0;       // real-space tiles per evaluation
-    int nmb = 0;         // LJ/intra blocks per evaluation
-    int lj_lpm = 16;     // lanes per molecule in k_ljintra
-    int AC = 64, nac = 0, ngc = 0, ngr = 0;
-    int ep_stride = 0, ep_lj = 0, ep_real = 0, ep_k = 0, ep_spl = 0;
-    // splined pair potentials (fast-CMD path): tables loaded, used by the
-    // resident steps, used by the current call
-    bool spl_loaded = false, spl_resident = false, spl_on = false;
-    SplTab spl{};
-    double *spl_coef = nullptr, *fspl = nullptr;
-    int ptile = 0;       // propagation tiles (64 planar dofs each)
-    size_t kforce_lds = 0, kforce_lds_split = 0, ksum_lds = 0, prop_lds = 0;
-    int kf_chunk = KF_CHUNK, kf_chunk_split = KF_CHUNK;  // k_kforce record staging
-    // split k_kforce in the SC phases when its P * ceil(n_atoms/63) blocks
-    // would not fill two per CU (small systems); large ones have blocks enough
-    bool kf_split = false;
-    // k-space sums and forces on the matrix cores (k_ksum_mf, k_kforce_mf)
-    // for k_max <= 15; the vector kernels above that, or when
-    // QTP_KSPACE=vector is set at create
-    bool kf_mf = false;
-    int mf_Mp = 0, mf_nks = 0, mf_nrt = 0, mf_nrt_small = 0, ks_nunit = 0, ks_ncol4 = 0;
-    size_t kforce_mf_lds = 0, kforce_mf_lds_small = 0, ksum_mf_lds = 0;
-
-    // device state (planar SoA)
-    double *q = nullptr, *p = nullptr, *F = nullptr, *fph = nullptr, *disp = nullptr;
-    double *m_atom = nullptr, *sqm_atom = nullptr, *charge = nullptr;
-    int *point = nullptr, *list = nullptr, *fpoint = nullptr, *flist = nullptr, *cnt = nullptr;
-    int *flag = nullptr, *status = nullptr;
-    double *nreb = nullptr;
-    // force-phase scratch (E <= max(P,1))
-    double *qpm = nullptr, *vpart = nullptr, *vout = nullptr, *rpart = nullptr, *spart = nullptr, *sprime = nullptr;
-    double *flj = nullptr, *fpm = nullptr, *Ve = nullptr, *epart = nullptr;
-    double *fk2 = nullptr;  // second half of split k-space forces (SC phases)
-    double *wm = nullptr;   // k_kforce_mf coefficient matrices [E][4 nks][Mp]
-    int *gcol = nullptr;    // column of each k-group in the matrix rows
-    int4 *mf_rt = nullptr;  // k_kforce_mf row-tile table (MF_WAVES waves)
-    int4 *mf_rt_small = nullptr;  // and for MF_WAVES_SMALL
-    int4 *ks_col = nullptr; // k_ksum_mf column table
-    int2 *ks_unit = nullptr;
-    double *scal = nullptr, *out = nullptr, *heat = nullptr, *heat_step = nullptr;
-    unsigned long long *step_ctr = nullptr;
-    // constants
-    int4 *groups = nullptr;
-    double *gcf = nullptr;
-    double4 *grk = nullptr;
-    int2 *tiles = nullptr;
-    double *Cm = nullptr;
-    double4 *mode = nullptr;
-    // propagation scratch
-    double *nq = nullptr, *np = nullptr, *ppart = nullptr, *hpart = nullptr, *dev = nullptr;
-    double *obs = nullptr;
-    long dev_cap = 0;
-    uint64_t seed = 0;
-
-    // host copies
-    std::vector<double> h_mass_atom, h_charge;
-    bool state_set = false;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // [is_nvt]
-    // resident observables: result, sample trace, and the graphs that record it
-    double *obs_out = nullptr, *trace = nullptr;
-    int *ntrace = nullptr;
-    int trace_cap = 0, obs_every = 0;
-    hipGraphExec_t graph_obs[2] = {nullptr, nullptr};
-    bool timing = false;
-    std::map<std::string, Timer> timers;
-    std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;
-    std::vector<hipEvent_t> evpool;
-    size_t evnext = 0;
-};
-
-// ---------------------------------------------------------------------------
-#define HIPCHK(x)                                                                    \
-    do {                                                                             \
-        hipError_t e_ = (x);                                                         \
-        if (e_ != hipSuccess) {                                                      \
-            ctx->err = std::string(#x) + ": " + hipGetErrorString(e_);               \
-            return QTP_ERR_HIP;                                                      \
-        }                                                                            \
-    } while (0)
-
-#define CHK(x)                                                                       \
-    do {                                                                             \
-        int s_ = (x);                                                                \
-        if (s_ != QTP_OK) return s_;                                                 \
-    } while (0)
-
-template <class T>
-static int dalloc(qtp_ctx *ctx, T **p, size_t n)
-{
-    HIPCHK(hipMalloc((void **)p, std::max<size_t>(n, 1) * sizeof(T)));
-    HIPCHK(hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)));
-    return QTP_OK;
-}
-
-// Timed launch bracket (only when timing is enabled; never inside a graph).
-static hipEvent_t ev_get(qtp_ctx *ctx)
-{
-    if (ctx->evnext >= ctx->evpool.size()) {
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        ctx->evpool.push_back(e);
-    }
-    return ctx->evpool[ctx->evnext++];
-}
-
-struct TimeScope {
-    qtp_ctx *ctx;
-    const char *name;
-    hipEvent_t a = nullptr, b = nullptr;
-    TimeScope(qtp_ctx *c, const char *n) : ctx(c), name(n)
-    {
-        if (ctx->timing) {
-            a = ev_get(ctx);
-            b = ev_get(ctx);
-            if (a) hipEventRecord(a, ctx->stream);
-        }
-    }
-    ~TimeScope()
-    {
-        if (ctx->timing && a && b) {
-            hipEventRecord(b, ctx->stream);
-            ctx->pending.push_back({name, {a, b}});
-        }
-    }
-};
-
-static int flush_timing(qtp_ctx *ctx)
-{
-    if (ctx->pending.empty()) return QTP_OK;
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    for (auto &pe : ctx->pending) {
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
-        Timer &t = ctx->timers[pe.first];
-        t.ms += ms;
-        t.n += 1;
-    }
-    ctx->pending.clear();
-    ctx->evnext = 0;
-    return QTP_OK;
-}
-
-// ---------------------------------------------------------------------------
-// set-up
-// ---------------------------------------------------------------------------
-static void derive(qtp_ctx *ctx)
-{
-    const qtp_params &P = ctx->prm;
-    qtp_derived &d = ctx->drv;
-    d.n_mols = P.n_mols;
-    d.n_atoms = 3 * P.n_mols;
-    d.n_dof = 3 * d.n_atoms;
-    d.n_beads = P.n_beads;
-    for (int k = 0; k < 3; ++k) d.boxl[k] = P.boxl[k];
-    d.vol = P.boxl[0] * P.boxl[1] * P.boxl[2];
-    const double rho_ox = P.n_mols / d.vol;
-    d.list_size = P.list_size > 0
-                      ? P.list_size
-                      : (int)std::round(4.0 * RC.r_skin * RC.r_skin * RC.r_skin * P.n_mols * rho_ox);
-    // water.cpp:76-83, 138-157
-    const double T_red = RC.K2au * P.T_K;
-    d.beta = 1.0 / T_red;
-    d.piovern = RC.pi / P.n_beads;
-    d.beta_n = d.beta / P.n_beads;
-    d.omega_p = 1.0 / (d.beta_n * 1.0);
-    d.omega_psq = d.omega_p * d.omega_p;
-    d.sd_ox = std::sqrt(RC.mo / d.beta_n);
-    d.sd_h = std::sqrt(RC.mh / d.beta_n);
-    d.dt = P.dtfs * RC.fs2au;
-    d.halfdt = 0.5 * d.dt;
-    d.epsilonsc = P.epsilon_sc;
-    d.acc = std::sqrt(-std::log(P.eps_ewald));
-    d.rho = d.n_atoms / d.vol;
-    const double min_xy = std::min(P.boxl[0], P.boxl[1]);
-    d.min_box = std::min(min_xy, P.boxl[2]);
-    d.kwald_prefactor = 2.0 * RC.pi / d.vol;
-    const double max_xy = std::max(P.boxl[0], P.boxl[1]);
-    d.max_box = std::max(max_xy, P.boxl[2]);
-    // forces.cpp:574-584
-    if (P.ewald_kappa > 0.0) {
-        d.kappa = P.ewald_kappa;
-        d.rcut = P.ewald_rcut;
-        d.k_cut = P.ewald_kcut;
-        d.k_max = P.ewald_kmax;
-    } else {
-        double kappa = 2.6 * std::pow(d.rho / d.vol, 1.0 / 6.0);
-        double rcut = d.acc / kappa;
-        if (rcut > 0.5 * d.min_box) {
-            rcut = 0.5 * d.min_box;
-            kappa = d.acc / rcut;
-        }
-        d.kappa = kappa;
-        d.rcut = rcut;
-        d.k_cut = 2.0 * d.acc * kappa;
-        d.k_max = (int)(d.k_cut * d.max_box / (2.0 * RC.pi));
-    }
-}
-
-// k-vector groups in the reference's inclusion rules (forces.cpp:251-445):
-// kx-only: 0 < k^2 < k_cut^2; (kx,0,kz): kz loop breaks at k^2 > k_cut^2;
-// (kx,ky,0): ky loop breaks likewise; (kx,ky,kz) inside an unbroken ky.
-// Coefficient c = the reference's exp_fac (kx-only: kwald_prefactor * exp_fac).
-static void build_groups(qtp_ctx *ctx, std::vector<int4> &g, std::vector<double> &cf,
-                         std::vector<double4> &rk)
-{
-    const qtp_derived &d = ctx->drv;
-    const int K = d.k_max + 1;
-    const double kc2 = d.k_cut * d.k_cut;
-    const double q4 = 0.25 / (d.kappa * d.kappa);
-    double bk[3];
-    for (int k = 0; k < 3; ++k) bk[k] = 2.0 * RC.pi / d.boxl[k];
-    const double kp = d.kwald_prefactor;
-    for (int kx = 0; kx < K; ++kx) {
-        const double r0 = bk[0] * kx;
-        double ksq = r0 * r0;
-        if (ksq > 0.0 && ksq < kc2) {
-            double ef0 = std::exp(-ksq * q4) / ksq;
-            if (kx != 0) ef0 *= 2.0;
-            g.push_back(make_int4(kx, 0, 0, 1));
-            cf.push_back(kp * ef0);
-            rk.push_back(make_double4(r0, 0.0, 0.0, 0.0));
-        }
-        for (int kz = 1; kz < K; ++kz) {
-            const double r2 = bk[2] * kz;
-            ksq = r0 * r0 + r2 * r2;
-            if (ksq > kc2) break;
-            double ef = kp * std::exp(-ksq * q4) / ksq;
-            if (kx != 0) ef *= 2.0;
-            g.push_back(make_int4(kx, 0, kz, 3));
-            cf.push_back(ef);
-            rk.push_back(make_double4(r0, 0.0, r2, 0.0));
-        }
-        for (int ky = 1; ky < K; ++ky) {
-            const double r1 = bk[1] * ky;
-            ksq = r0 * r0 + r1 * r1;
-            if (ksq > kc2) break;
-            double ef = kp * std::exp(-ksq * q4) / ksq;
-            if (kx != 0) ef *= 2.0;
-            g.push_back(make_int4(kx, ky, 0, 5));
-            cf.push_back(ef);
-            rk.push_back(make_double4(r0, r1, 0.0, 0.0));
-            for (int kz = 1; kz < K; ++kz) {
-                const double r2 = bk[2] * kz;
-                ksq = r0 * r0 + r1 * r1 + r2 * r2;
-                if (ksq > kc2) break;
-                double ef3 = kp * std::exp(-ksq * q4) / ksq;
-                if (kx != 0) ef3 *= 2.0;
-                g.push_back(make_int4(kx, ky, kz, 15));
-                cf.push_back(ef3);
-                rk.push_back(make_double4(r0, r1, r2, 0.0));
-            }
-        }
-    }
-}
-
-// Matrix layout of k_kforce_mf.  The groups come in columns of fixed
-// (kx, ky) and rising kz (build_groups' order); the columns are sorted by
-// their largest kz (descending, stable) and paired into 16-row tiles, each
-// taking ceil((2 kz_max + 2) / 4) k-steps of 4 (tiles: in that sorted
-// order; mf_tile_order arranges them for a block of W waves).  Returns false when a (kx, ky) pair
-// appears in two runs or a column's kz are not consecutive (then the vector
-// kernels are used).
-// k_ksum_mf: kcol per sorted column (kx | ky << 16, first group, first kz,
-// last kz), padded to a multiple of four with empty columns; units (tile of
-// four columns, n-tile) for every n = 2 kz + 0/1 that a tile's columns reach.
-static bool build_mf_layout(const std::vector<int4> &g, std::vector<int> &gcol,
-                            std::vector<int4> &tiles, int &Mp, int &nks, std::vector<int4> &kcol,
-                            std::vector<int2> &kunit)
-{
-    struct Col {
-        int kx, ky, kzmax, g0, kz0;
-    };
-    std::vector<Col> cols;
-    std::vector<int> gc(g.size());
-    std::map<int, int> seen;
-    for (size_t i = 0; i < g.size(); ++i) {
-        if (cols.empty() || cols.back().kx != g[i].x || cols.back().ky != g[i].y) {
-            if (!seen.emplace(g[i].x | (g[i].y << 16), (int)cols.size()).second) return false;
-            cols.push_back({g[i].x, g[i].y, g[i].z, (int)i, g[i].z});
-        } else {
-            if (g[i].z != cols.back().kzmax + 1) return false;
-            cols.back().kzmax = g[i].z;
-        }
-        gc[i] = (int)cols.size() - 1;
-    }
-    const int nc = (int)cols.size();
-    std::vector<int> ord(nc), rank(nc);
-    for (int i = 0; i < nc; ++i) ord[i] = i;
-    std::stable_sort(ord.begin(), ord.end(),
-                     [&](int a, int b) { return cols[a].kzmax > cols[b].kzmax; });
-    for (int i = 0; i < nc; ++i) rank[ord[i]] = i;
-    gcol.resize(g.size());
-    for (size_t i = 0; i < g.size(); ++i) gcol[i] = rank[gc[i]];
-    const int ntl = (nc + 1) / 2;
-    Mp = 16 * ntl;
-    nks = 0;
-    tiles.assign(ntl, make_int4(0, 0, 0, 0));
-    for (int t = 0; t < ntl; ++t) {
-        const Col &c0 = cols[ord[2 * t]];
-        int4 x;
-        x.x = c0.kx | (c0.ky << 16);
-        x.y = 2 * t + 1 < nc ? (cols[ord[2 * t + 1]].kx | (cols[ord[2 * t + 1]].ky << 16)) : 0;
-        x.z = (2 * c0.kzmax + 2 + 3) / 4;  // the first column has the larger kz_max
-        x.w = 16 * t;
-        nks = std::max(nks, x.z);
-        tiles[t] = x;
-    }
-    const int nq = (nc + 3) / 4;
-    kcol.assign((size_t)4 * nq, make_int4(0, 0, 1, 0));
-    for (int i = 0; i < nc; ++i) {
-        const Col &c = cols[ord[i]];
-        kcol[i] = make_int4(c.kx | (c.ky << 16), c.g0, c.kz0, c.kzmax);
-    }
-    kunit.clear();
-    for (int t = 0; t < nq; ++t)
-        for (int u = 0; u <= (2 * kcol[4 * t].w + 1) / 16; ++u) kunit.push_back(make_int2(t, u));
-    return true;
-}
-
-// The tile table of k_kforce_mf for blocks of W waves: wave w reads entries
-// w, w + W, ...; round q hands the sorted tiles q W ... out to the waves in
-// turn, reversed in odd rounds, which evens out the k-steps per wave (empty
-// entries: k-steps 0).
-static std::vector<int4> mf_tile_order(const std::vector<int4> &tiles, int W)
-{
-    const int ntl = (int)tiles.size();
-    const int rounds = (ntl + W - 1) / W;
-    std::vector<int4> rt((size_t)rounds * W, make_int4(0, 0, 0, 0));
-    for (int t = 0; t < ntl; ++t) {
-        const int q = t / W, p = t - q * W;
-        rt[(size_t)q * W + ((q & 1) ? W - 1 - p : p)] = tiles[t];
-    }
-    return rt;
-}
-
-// Orthonormal real normal-mode matrix (FFTW R2HC halfcomplex layout with the
-// scalings of fourier.cpp:22-66), entries from long double with exact pi.
-static void build_nm(int P, std::vector<double> &C)
-{
-    C.assign((size_t)P * P, 0.0);
-    const long double twopi = 6.283185307179586476925286766559005768L;
-    const long double s1 = sqrtl(1.0L / P), s2 = sqrtl(2.0L / P);
-    for (int b = 0; b < P; ++b) {
-        C[0 * P + b] = (double)s1;
-        for (int k = 1; k < P - k; ++k) {
-            const long double ang = twopi * (long double)(((long)k * b) % P) / P;
-            C[(size_t)k * P + b] = (double)(s2 * cosl(ang));
-            C[(size_t)(P - k) * P + b] = (double)(-s2 * sinl(ang));
-        }
-        if (P % 2 == 0) C[(size_t)(P / 2) * P + b] = (double)(s1 * ((b % 2) ? -1.0L : 1.0L));
-    }
-}
-
-static int alloc_all(qtp_ctx *ctx)
-{
-    const int P = ctx->P, E = std::max(P, 1);
-    const long N3 = 3L * ctx->NAP;
-    CHK(dalloc(ctx, &ctx->q, (size_t)P * N3));
-    CHK(dalloc(ctx, &ctx->p, (size_t)P * N3));
-    CHK(dalloc(ctx, &ctx->F, (size_t)P * N3));
-    CHK(dalloc(ctx, &ctx->fph, (size_t)P * N3));
-    CHK(dalloc(ctx, &ctx->disp, (size_t)P * 3 * ctx->NMP));
-    CHK(dalloc(ctx, &ctx->m_atom, ctx->NAP));
-    CHK(dalloc(ctx, &ctx->sqm_atom, ctx->NAP));
-    CHK(dalloc(ctx, &ctx->charge, ctx->NAP));
-    CHK(dalloc(ctx, &ctx->point, (size_t)P * ctx->n_mols));
-    CHK(dalloc(ctx, &ctx->list, (size_t)P * ctx->list_size));
-    CHK(dalloc(ctx, &ctx->fpoint, (size_t)P * (ctx->n_mols + 1)));
-    CHK(dalloc(ctx, &ctx->flist, (size_t)P * ctx->fcap));
-    CHK(dalloc(ctx, &ctx->cnt, (size_t)P * 2 * ctx->n_mols));
-    CHK(dalloc(ctx, &ctx->flag, P));
-    CHK(dalloc(ctx, &ctx->status, 1));
-    CHK(dalloc(ctx, &ctx->nreb, P));
-    CHK(dalloc(ctx, &ctx->qpm, (size_t)E * N3));
-    CHK(dalloc(ctx, &ctx->vpart, (size_t)E * 6 * ctx->ep_stride));
-    CHK(dalloc(ctx, &ctx->vout, (size_t)E * 6));
-    CHK(dalloc(ctx, &ctx->rpart, (size_t)E * ctx->nblk * N3));
-    CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));
-    if (!ctx->kf_mf) CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));
-
-    CHK(dalloc(ctx, &ctx->flj, (size_t)E * N3));
-    CHK(dalloc(ctx, &ctx->fspl, (size_t)E * N3));
-    CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
-    if (ctx->kf_split) CHK(dalloc(ctx, &ctx->fk2, (size_t)E * N3));
-    if (ctx->kf_mf) CHK(dalloc(ctx, &ctx->wm, (size_t)E * 4 * ctx->mf_nks * ctx->mf_Mp));
-    CHK(dalloc(ctx, &ctx->Ve, E));
-    CHK(dalloc(ctx, &ctx->epart, (size_t)E * ctx->ep_stride));
-    CHK(dalloc(ctx, &ctx->scal, (size_t)E * 4));
-    CHK(dalloc(ctx, &ctx->out, 8));
-    CHK(dalloc(ctx, &ctx->heat, 1));
-    CHK(dalloc(ctx, &ctx->heat_step, 1));
-    CHK(dalloc(ctx, &ctx->step_ctr, 1));
-    CHK(dalloc(ctx, &ctx->nq, (size_t)P * N3));
-    CHK(dalloc(ctx, &ctx->np, (size_t)P * N3));
-    CHK(dalloc(ctx, &ctx->ppart, 2 * (size_t)ctx->ptile + 2));
-    CHK(dalloc(ctx, &ctx->hpart, (size_t)ctx->P * ctx->ptile + 1));
-    CHK(dalloc(ctx, &ctx->obs, (size_t)ctx->ptile * 4 + 64));
-    CHK(dalloc(ctx, &ctx->obs_out, 4));
-    CHK(dalloc(ctx, &ctx->ntrace, 1));
-    ctx->dev_cap = 3 + (long)(P > 1 ? P - 1 : 0) * ctx->n_dof;
-    CHK(dalloc(ctx, &ctx->dev, (size_t)ctx->dev_cap));
-    return QTP_OK;
-}
-
-static int upload_constants(qtp_ctx *ctx)
-{
-    // masses and charges (simulation.cpp:811-822)
-    ctx->h_mass_atom.assign(ctx->NAP, 1.0);
-    ctx->h_charge.assign(ctx->NAP, 0.0);
-    std::vector<double> sqm(ctx->NAP, 0.0);
-    for (int i = 0; i < ctx->n_mols; ++i) {
-        ctx->h_mass_atom[3 * i] = RC.mo;
-        ctx->h_mass_atom[3 * i + 1] = RC.mh;
-        ctx->h_mass_atom[3 * i + 2] = RC.mh;
-        ctx->h_charge[3 * i] = RC.qo;
-        ctx->h_charge[3 * i + 1] = RC.qh;
-        ctx->h_charge[3 * i + 2] = RC.qh;
-    }
-    for (int a = 0; a < ctx->n_atoms; ++a) sqm[a] = std::sqrt(ctx->h_mass_atom[a] / ctx->drv.beta_n);
-    HIPCHK(hipMemcpy(ctx->m_atom, ctx->h_mass_atom.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->sqm_atom, sqm.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->charge, ctx->h_charge.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
-    // self energy constant in the reference's per-site order
-    const double pref = 2.0 * ctx->drv.kappa / std::sqrt(RC.pi);
-    double es = 0.0;
-    for (int a = 0; a < ctx->n_atoms; ++a) es += 0.5 * pref * ctx->h_charge[a] * ctx->h_charge[a];
-    ctx->eself = es;
-    // normal modes and per-mode propagator / thermostat constants
-    const int P = ctx->P;
-    std::vector<double> C;
-    build_nm(P, C);
-    HIPCHK(hipMemcpy(ctx->Cm, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice));
-    std::vector<double4> md(P);
-    const double pre = 2.0 / (1.0 * ctx->drv.beta_n), pre_sq = pre * pre;
-    const double tpre = 4.0 / (ctx->drv.beta_n * 1.0);
-    const double dt = ctx->drv.dt;
-    for (int k = 0; k < P; ++k) {
-        const double s = std::sin(k * ctx->drv.piovern);              // update.cpp:302
-        const double wsq = pre_sq * s * s;
-        const double scale = 1.0 / std::sqrt(1.0 + 0.25 * wsq * dt * dt);
-        const double gk = tpre * std::sin(k * RC.pi / P);              // thermostat.cpp:119
-        const double c1 = std::exp(-dt * gk);
-        const double c2 = std::sqrt(1.0 - c1 * c1);
-        md[k] = make_double4(scale, wsq, c1, c2);
-    }
-    HIPCHK(hipMemcpy(ctx->mode, md.data(), P * sizeof(double4), hipMemcpyHostToDevice));
-    return QTP_OK;
-}
-
-// ---------------------------------------------------------------------------
-// force phase: E single-bead evaluations (forces.cpp:468-630)
-// ---------------------------------------------------------------------------
-// energy partials summed per evaluation: the spline blocks only when used
-static int ep_live(const qtp_ctx *ctx) { return ctx->spl_on ? ctx->ep_stride : ctx->ep_spl; }
-
-// The splined potential is part of a force call only through ctx->spl_on,
-// which every entry point sets for its duration.
-struct SplineScope {
-    qtp_ctx *ctx;
-    bool prev;
-    SplineScope(qtp_ctx *c, bool on) : ctx(c), prev(c->spl_on) { c->spl_on = on; }
-    ~SplineScope() { ctx->spl_on = prev; }
-};
-
-
-// vout (optional): per-evaluation virial W_ab = sum_i r_i (x) F_i in the
-// minimum-image pair form (6 components xx yy zz xy xz yz), at fixed Ewald
-// parameters.
-// fout2: when given, k-space forces are split in two halves (k_kforce's split
-// form) and the second half goes there; the caller adds it.
-static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbase, double *fout,
-                     double *eout, double *vout = nullptr, double *fout2 = nullptr)
-{
-    hipStream_t s = ctx->stream;
-    const long N3 = 3L * ctx->NAP;
-    const double g = ctx->ff.gamma_vs, h = ctx->ff.half1mg;
-    double *vp = vout ? ctx->vpart : nullptr;
-    // The k-space sums (ksum, sreduce) and the LJ/intramolecular kernel do not
-    // depend on the real-space kernel: they run on the second stream, joined
-    // before kforce.  With per-kernel timing on, everything stays on one
-    // stream so each launch is timed alone.
-    const bool fork = !ctx->timing;
-    hipStream_t sk = fork ? ctx->s2 : s;
-    if (fork) {
-        HIPCHK(hipEventRecord(ctx->ev_fork, s));
-        HIPCHK(hipStreamWaitEvent(sk, ctx->ev_fork, 0));
-    }
-    {
-        TimeScope ts(ctx, "ksum");
-        if (ctx->kf_mf) {
-            // few chunks in the launch: several blocks per chunk
-            const int usplit = E * ctx->nac < 128 ? std::min(4, std::max(1, ctx->ks_nunit / KS_WAVES)) : 1;
-            k_ksum_mf<<<E * ctx->nac * usplit, 64 * KS_WAVES, ctx->ksum_mf_lds, sk>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->ks_col,
-                ctx->ks_ncol4, ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, usplit,
-                ctx->spart);
-        } else
-            k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, sk>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->groups, ctx->G,
-                ctx->AC, ctx->ngc, ctx->nac, ctx->spart);
-    }
-    {
-        TimeScope ts(ctx, "sreduce");
-        k_sreduce<<<E * ctx->ngr, 256, 0, sk>>>(ctx->spart, ctx->nac, ctx->G, ctx->groups, ctx->gcf,
-                                                 ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
-                                                 ctx->grk, 0.25 / ctx->ew.kappa2, vp, ctx->wm,
-                                                 ctx->mf_Mp, 4 * ctx->mf_nks, ctx->gcol,
-                                                 ctx->ew.bk[2], ctx->kf_mf ? 1 : 0);
-    }
-    {
-        TimeScope ts(ctx, "ljintra");
-#define QTP_LJ_LAUNCH(VIR, LPM)                                                                    \
-    k_ljintra<VIR, LPM><<<E * ctx->nmb, 256, 0, sk>>>(pos, N3, ctx->n_mols, ctx->NAP, ctx->fpoint, \
-                                                      ctx->flist, ctx->fcap, lmode, lbase,         \
-                                                      ctx->box, ctx->ff, ctx->flj, ctx->epart,     \
-                                                      ctx->ep_stride, ctx->ep_lj, vp)
-        if (ctx->lj_lpm == 64) {
-            if (vout) QTP_LJ_LAUNCH(true, 64);
-            else QTP_LJ_LAUNCH(false, 64);
-        } else {
-            if (vout) QTP_LJ_LAUNCH(true, 16);
-            else QTP_LJ_LAUNCH(false, 16);
-        }
-#undef QTP_LJ_LAUNCH
-    }
-    if (ctx->spl_on) {
-        TimeScope ts(ctx, "spline");
-        if (vout)
-            k_spline<true><<<E * ctx->nblk, 256, 0, sk>>>(
-                pos, N3, ctx->n_atoms, ctx->NAP, ctx->nblk, ctx->box, ctx->spl, ctx->spl_coef,
-                ctx->fspl, ctx->epart, ctx->ep_stride, ctx->ep_spl, vp);
-        else
-            k_spline<false><<<E * ctx->nblk, 256, 0, sk>>>(
-                pos, N3, ctx->n_atoms, ctx->NAP, ctx->nblk, ctx->box, ctx->spl, ctx->spl_coef,
-                ctx->fspl, ctx->epart, ctx->ep_stride, ctx->ep_spl, vp);
-    }
-    if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
-    {
-        TimeScope ts(ctx, "realspace");
-        // coefficients in SGPRs (more waves per SIMD) for large grids, where
-        // that measured faster (4096 x 32: 3.42 against 3.60 ms); literals
-        // for small ones (216 x 32: no gain, and the second stream's kernels
-        // lost more than the launch gained)
-        const bool carg = (long)E * ctx->ntile >= RS_CARG_BLOCKS;
-        const RsCoef cf = rs_coef();
-#define QTP_RS_LAUNCH(VIR, CARG)                                                                   \
-    k_realspace<VIR, CARG><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(                               \
-        pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles, ctx->ntile,     \
-        ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real, vp, cf)
-        if (vout) {
-            if (carg) QTP_RS_LAUNCH(true, true);
-            else QTP_RS_LAUNCH(true, false);
-        } else {
-            if (carg) QTP_RS_LAUNCH(false, true);
-            else QTP_RS_LAUNCH(false, false);
-        }
-#undef QTP_RS_LAUNCH
-    }
-    if (fork) HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
-    {
-        TimeScope ts(ctx, "kforce");
-        const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
-        const int nbm = E * ((ctx->n_atoms + MF_SITES - 1) / MF_SITES);
-        // few blocks: twice the waves per block (when its LDS fits the default 64 KB)
-        if (ctx->kf_mf && nbm < MF_SMALL_BLOCKS && ctx->kforce_mf_lds_small <= 64 * 1024)
-            k_kforce_mf<MF_WAVES_SMALL><<<nbm, 64 * MF_WAVES_SMALL, ctx->kforce_mf_lds_small, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
-                ctx->mf_nks, ctx->mf_rt_small, ctx->mf_nrt_small, ctx->rpart, ctx->nblk, ctx->flj,
-                ctx->spl_on ? ctx->fspl : nullptr, fout);
-        else if (ctx->kf_mf)
-            k_kforce_mf<MF_WAVES><<<nbm, 64 * MF_WAVES, ctx->kforce_mf_lds, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
-                ctx->mf_nks, ctx->mf_rt, ctx->mf_nrt, ctx->rpart, ctx->nblk, ctx->flj,
-                ctx->spl_on ? ctx->fspl : nullptr, fout);
-        else if (fout2)
-            k_kforce<KF_SPLIT_WAVES, true><<<2 * E * nab, 64 * KF_SPLIT_WAVES, ctx->kforce_lds_split, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
-                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, fout2,
-                ctx->kf_chunk_split);
-        else
-            k_kforce<KF_WAVES, false><<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
-                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, nullptr,
-                ctx->kf_chunk);
-    }
-    if (eout) {
-        TimeScope ts(ctx, "esum");
-        k_esum<<<E, 64, 0, s>>>(E, ctx->epart, ctx->ep_stride, ep_live(ctx), ctx->eself, eout);
-    }
-    if (vout) k_vsum<<<E, 64, 0, s>>>(ctx->vpart, ctx->ep_stride, ep_live(ctx), vout);
-    HIPCHK(hipGetLastError());
-    return QTP_OK;
-}
-
-static SCArgs sc_args(qtp_ctx *ctx)
-{
-    SCArgs a;
-    a.P = ctx->P;
-    a.n_atoms = ctx->n_atoms;
-    a.NAP = ctx->NAP;
-    a.omega_psq = ctx->drv.omega_psq;
-    a.epsilonsc = ctx->drv.epsilonsc;
-    a.halfdt = ctx->drv.halfdt;
-    return a;
-}
-
-// SC forces for all beads from ctx->q with the current lists (forces.cpp:698-789)
-static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
-{
-    hipStream_t s = ctx->stream;
-    const int P = ctx->P;
-    double *f2 = ctx->kf_split ? ctx->fk2 : nullptr;
-    CHK(run_phase(ctx, ctx->q, P, 0, 0, ctx->fph, nullptr, nullptr, f2));
-    SCArgs a = sc_args(ctx);
-    {
-        TimeScope ts(ctx, "sc1");
-        k_sc1<<<P, SC_THREADS, 0, s>>>(a, ctx->q, ctx->fph, f2, ctx->epart, ctx->ep_stride, ep_live(ctx), ctx->eself,
-                                ctx->Ve, ctx->m_atom, ctx->F, ctx->qpm, ctx->scal);
-    }
-    CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr, nullptr, f2));
-    {
-        TimeScope ts(ctx, "sc2");
-        k_sc2<<<P + 1, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, f2, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
-                                ctr);
-    }
-    HIPCHK(hipGetLastError());
-    return QTP_OK;
-}
-
-// with_heat: also reduce a thermostatted step's heat (folded into the fused
-// list kernel; a kernel of its own on the multi-kernel path)
-static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
-{
-    hipStream_t s = ctx->stream;
-    const int P = ctx->P;
-    HeatArgs heat{};
-    if (with_heat) {
-        heat.part = ctx->ppart;
-        heat.hpart = ctx->hpart;
-        heat.heat_acc = ctx->heat;
-        heat.heat_step = ctx->heat_step;
-        heat.ntile = ctx->ptile;
-        heat.nh = ctx->P * ctx->ptile;
-    }
-    if (ctx->n_mols <= NL_FUSED_MAXMOL) {
-        TimeScope ts(ctx, "nlist");
-        k_nlist_bead<<<P + (with_heat ? 1 : 0), 1024, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box,
-                                        RC.r_skin_sq, RC.r_skin - RC.r_cut, force_all,
-                                        ctx->list_size, ctx->fcap, ctx->disp, ctx->flag, ctx->nreb,
-                                        ctx->point, ctx->fpoint, ctx->list, ctx->flist,
-                                        ctx->status, heat);
-        HIPCHK(hipGetLastError());
-        return QTP_OK;
-    }
-    if (with_heat)
-        k_heat<<<1, 1024, 0, s>>>(heat.ntile, heat.nh, heat.part, heat.hpart, heat.heat_acc,
-                                heat.heat_step);
-    {
-        TimeScope ts(ctx, "check");
-        k_check<<<P, 256, 0, s>>>(ctx->n_mols, ctx->NMP, ctx->disp, RC.r_skin - RC.r_cut, force_all,
-                                  ctx->flag, ctx->nreb);
-    }
-    const int nrb = (ctx->n_mols + 3) / 4;
-    {
-        TimeScope ts(ctx, "nlist");
-        k_nl_count<<<P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box, RC.r_skin_sq,
-                                           ctx->flag, ctx->cnt);
-        k_nl_scan<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->list_size, ctx->fcap, ctx->flag, ctx->cnt,
-                                     ctx->point, ctx->fpoint, ctx->list, ctx->disp, ctx->NMP,
-                                     ctx->status);
-        k_nl_fill<<<P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box, RC.r_skin_sq,
-                                          ctx->list_size, ctx->fcap, ctx->flag, ctx->point,
-                                          ctx->fpoint, ctx->list, ctx->flist);
-    }
-    HIPCHK(hipGetLastError());
-    return QTP_OK;
-}
-
-static PropArgs prop_args(qtp_ctx *ctx, int nvt, int host_dev)
-{
-    PropArgs A;
-    A.P = ctx->P;
-    A.n_atoms = ctx->n_atoms;
-    A.NAP = ctx->NAP;
-    A.ntile = ctx->ptile;
-    A.halfdt = ctx->drv.halfdt;
-    A.dt = ctx->drv.dt;
-    A.beta_n = ctx->drv.beta_n;
-    A.nvt = nvt;
-    A.host_dev = host_dev;
-    A.n_dof = ctx->n_dof;
-    A.seed = ctx->seed;
-    return A;
-}
-
-// One SC step on the device state (update.cpp:271-401).
-static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *ctr)
-{
-    hipStream_t s = ctx->stream;
-    PropArgs A = prop_args(ctx, nvt, host_dev);
-    const int grid = ctx->ptile * ctx->P;
-    {
-        TimeScope ts(ctx, "prop_a");
-        k_prop_fwd<<<grid, 64, 0, s>>>(A, ctx->Cm, ctx->mode, ctx->q, ctx->p, ctx->F, ctx->m_atom,
-                                       ctx->sqm_atom, ctx->dev, ctx->nq, ctx->np, ctx->ppart,
-                                       ctx->hpart, ctr);
-    }
-    {
-        TimeScope ts(ctx, "prop_b");
-        k_prop_inv<<<grid, 64, 0, s>>>(A, ctx->Cm, ctx->mode, ctx->nq, ctx->np, ctx->m_atom,
-                                       ctx->ppart, ctx->dev, ctx->q, ctx->p, ctx->disp, ctx->NMP,
-                                       ctx->hpart, ctr);
-    }
-    HIPCHK(hipGetLastError());
-    CHK(run_lists(ctx, 0, nvt != 0));
-    CHK(run_sc_forces(ctx, 1, ctr));
-    return QTP_OK;
-}
-
-// One classical velocity-Verlet step (update.cpp:78-137).
-static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr)
-{
-    hipStream_t s = ctx->stream;
-    const int nb = (3 * ctx->NAP + 255) / 256;
-    {
-        TimeScope ts(ctx, "vv_a");
-        k_vv_a<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->NMP, ctx->drv.halfdt, ctx->drv.dt,
-                                  ctx->m_atom, ctx->F, ctx->q, ctx->p, ctx->disp);
-    }
-    CHK(run_lists(ctx, 0));
-    CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, nullptr));
-    {
-        TimeScope ts(ctx, "vv_b");
-        k_vv_b<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->drv.halfdt, ctx->fph, ctx->epart,
-                                  ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p, ctx->out, ctr);
-    }
-    HIPCHK(hipGetLastError());
-    return QTP_OK;
-}
-
-// ---------------------------------------------------------------------------
-// host <-> device layout conversion
-// ---------------------------------------------------------------------------
-static void to_planar(const qtp_ctx *ctx, const double *ref, int nb, std::vector<double> &out)
-{
-    const long N3 = 3L * ctx->NAP;
-    out.assign((size_t)nb * N3, 0.0);
-    for (int b = 0; b < nb; ++b)
-        for (int a = 0; a < ctx->n_atoms; ++a)
-            for (int d = 0; d < 3; ++d)
-                out[b * N3 + (long)d * ctx->NAP + a] = ref[(long)b * ctx->n_dof + 3 * a + d];
-}
-
-static void from_planar(const qtp_ctx *ctx, const std::vector<double> &pl, int nb, double *ref)
-{
-    const long N3 = 3L * ctx->NAP;
-    for (int b = 0; b < nb; ++b)
-        for (int a = 0; a < ctx->n_atoms; ++a)
-            for (int d = 0; d < 3; ++d)
-                ref[(long)b * ctx->n_dof + 3 * a + d] = pl[b * N3 + (long)d * ctx->NAP + a];
-}
-
-static int upload_planar(qtp_ctx *ctx, double *dst, const double *ref, int nb)
-{
-    std::vector<double> h;
-    to_planar(ctx, ref, nb, h);
-    HIPCHK(hipMemcpyAsync(dst, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
-                          ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    return QTP_OK;
-}
-
-static int download_planar(qtp_ctx *ctx, const double *src, int nb, double *ref)
-{
-    std::vector<double> h((size_t)nb * 3 * ctx->NAP);
-    HIPCHK(hipMemcpyAsync(h.data(), src, h.size() * sizeof(double), hipMemcpyDeviceToHost,
-                          ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
-    from_planar(ctx, h, nb, ref);
-    return QTP_OK;
-}
-
-// disp: reference [b][3*mol + d]  <->  device [b][d][NMP]
-static int upload_disp(qtp_ctx *ctx, const double *ref, int b0, int nb)
-{
-    std::vector<double> h((size_t)nb * 3 * ctx->NMP, 0.0);
-    for (int b = 0; b < nb; ++b)
-        for (int i = 0; i < ctx->n_mols; ++i)
-            for (int d = 0; d < 3; ++d)
-                h[((size_t)b * 3 + d) * ctx->NMP + i] = ref[(long)(b0 + b) * 3 * ctx->n_mols + 3 * i + d];
-    HIPCHK(hipMemcpy(ctx->disp + (size_t)b0 * 3 * ctx->NMP, h.data(), h.size() * sizeof(double),
-                     hipMemcpyHostToDevice));
-    return QTP_OK;
-}
-
-static int download_disp(qtp_ctx *ctx, double *ref, int b0, int nb)
-{
-    std::vector<double> h((size_t)nb * 3 * ctx->NMP);
-    HIPCHK(hipMemcpy(h.data(), ctx->disp + (size_t)b0 * 3 * ctx->NMP, h.size() * sizeof(double),
-                     hipMemcpyDeviceToHost));
-    for (int b = 0; b < nb; ++b)
-        for (int i = 0; i < ctx->n_mols; ++i)
-            for (int d = 0; d < 3; ++d)
-                ref[(long)(b0 + b) * 3 * ctx->n_mols + 3 * i + d] = h[((size_t)b * 3 + d) * ctx->NMP + i];
-    return QTP_OK;
-}
-
-// Caller's half lists (reference layout) -> device half + full lists.  Row
-// n_mols-1 is ignored as forces.cpp:489 does.
-static int upload_lists(qtp_ctx *ctx, const int *point, const int *list, int b0, int nb,
-                        int dev_b0)
-{
-    const int n = ctx->n_mols;
-    std::vector<int> fp((size_t)nb * (n + 1)), fl((size_t)nb * ctx->fcap, 0);
-    for (int bb = 0; bb < nb; ++bb) {
-        const int b = b0 + bb;
-        const int *pt = point + (long)b * n;
-        const int *ls = list + (long)b * ctx->list_size;
-        std::vector<std::vector<int>> rows(n);
-        for (int i = 0; i < n - 1; ++i) {
-            const int k0 = pt[i], k1 = pt[i + 1];
-            if (k0 < 0 || k1 > ctx->list_size || k1 < k0) {
-                ctx->err = "neighbour list: point[] out of range";
-                return QTP_ERR_ARG;
-            }
-            for (int k = k0; k < k1; ++k) {
-                const int j = ls[k];
-                if (j < 0 || j >= n) {
-                    ctx->err = "neighbour list: entry out of range";
-                    return QTP_ERR_ARG;
-                }
-                rows[j].push_back(i);
-            }
-        }
-        long pos = 0;
-        for (int i = 0; i < n; ++i) {
-            fp[(size_t)bb * (n + 1) + i] = (int)pos;
-            std::vector<int> r = rows[i];  // partners j < i (i in row j)
-            if (i < n - 1)
-                for (int k = pt[i]; k < pt[i + 1]; ++k) r.push_back(ls[k]);
-            for (int j : r) {
-                if (pos >= ctx->fcap) {
-                    ctx->err = "neighbour list: full list exceeds capacity";
-                    return QTP_ERR_LIST_OVERFLOW;
-                }
-                fl[(size_t)bb * ctx->fcap + pos++] = j;
-            }
-        }
-        fp[(size_t)bb * (n + 1) + n] = (int)pos;
-    }
-    HIPCHK(hipMemcpy(ctx->fpoint + (size_t)dev_b0 * (n + 1), fp.data(), fp.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->flist + (size_t)dev_b0 * ctx->fcap, fl.data(), fl.size() * sizeof(int),
-                     hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->point + (size_t)dev_b0 * n, point + (long)b0 * n, (size_t)nb * n * sizeof(int),
-                     hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(ctx->list + (size_t)dev_b0 * ctx->list_size, list + (long)b0 * ctx->list_size,
-                     (size_t)nb * ctx->list_size * sizeof(int), hipMemcpyHostToDevice));
-    return QTP_OK;
-}
-
-static int check_finite(qtp_ctx *ctx, const double *a, long n, const char *what)
-{
-    for (long i = 0; i < n; ++i)
-        if (!std::isfinite(a[i])) {
-            ctx->err = std::string(what) + " contains a non-finite value";
-            return QTP_ERR_NONFINITE;
-        }
-    return QTP_OK;
-}
-
-static int read_status(qtp_ctx *ctx)
-{
-    int st = 0;
-    HIPCHK(hipMemcpy(&st, ctx->status, sizeof(int), hipMemcpyDeviceToHost));
-    if (st & 1) {
-        ctx->err = "list size exceeded! (neighbour.cpp:56-58): increase list_size";
-        return QTP_ERR_LIST_OVERFLOW;
-    }
-    return QTP_OK;
-}
-
-static int upload_charges_masses(qtp_ctx *ctx, const double *charge, const double *mass)
-{
-    if (charge) {
-        std::vector<double> c(ctx->NAP, 0.0);
-        for (int a = 0; a < ctx->n_atoms; ++a) c[a] = charge[a];
-        if (c != ctx->h_charge) {
-            ctx->h_charge = c;
-            HIPCHK(hipMemcpy(ctx->charge, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
-            const double pref = 2.0 * ctx->drv.kappa / std::sqrt(RC.pi);
-            double es = 0.0;
-            for (int a = 0; a < ctx->n_atoms; ++a) es += 0.5 * pref * c[a] * c[a];
-            ctx->eself = es;
-        }
-    }
-    if (mass) {
-        std::vector<double> m(ctx->NAP, 1.0);
-        for (int a = 0; a < ctx->n_atoms; ++a) m[a] = mass[3 * a];
-        if (m != ctx->h_mass_atom) {
-            ctx->h_mass_atom = m;
-            std::vector<double> sq(ctx->NAP, 0.0);
-            for (int a = 0; a < ctx->n_atoms; ++a) sq[a] = std::sqrt(m[a] / ctx->drv.beta_n);
-            HIPCHK(hipMemcpy(ctx->m_atom, m.data(), m.size() * sizeof(double), hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(ctx->sqm_atom, sq.data(), sq.size() * sizeof(double), hipMemcpyHostToDevice));
-        }
-    }
-    return QTP_OK;
-}
-
-static void invalidate_graphs(qtp_ctx *ctx)
-{
-    for (auto &g : ctx->graph)
-        if (g) {
-            hipGraphExecDestroy(g);
-            g = nullptr;
-        }
-    for (auto &g : ctx->graph_obs)
-        if (g) {
-            hipGraphExecDestroy(g);
-            g = nullptr;
-        }
-}
+#include "qtp_host_ctx.hpp"
+#include "qtp_host_setup.hpp"
+#include "qtp_host_run.hpp"
 
 // ===========================================================================
 // C ABI
