@@ -1,5 +1,6 @@
-// qtp_host_ctx.hpp -- host side of the gfx950 qTIP4P/F step: the context (one system: buffers, streams, graphs, derived
-// parameters), error and allocation helpers, per-kernel timing.
+// qtp_host_ctx.hpp -- host side of the gfx950 qTIP4P/F step: the context
+// (one system: buffers, streams, graphs, derived parameters), error and
+// allocation helpers, per-kernel timing.
 // Included once, in order, by qtip4pf.hip (one translation unit).
 #pragma once
 

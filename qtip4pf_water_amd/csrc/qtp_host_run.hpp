@@ -1,5 +1,6 @@
-// qtp_host_run.hpp -- host side of the gfx950 qTIP4P/F step: launch sequences: force phases, SC forces, neighbour lists, the
-// SC and classical steps, host <-> device layout conversion.
+// qtp_host_run.hpp -- host side of the gfx950 qTIP4P/F step: launch
+// sequences (force phases, SC forces, neighbour lists, the SC and classical
+// steps) and host <-> device layout conversion.
 // Included once, in order, by qtip4pf.hip (one translation unit).
 #pragma once
 

@@ -1,5 +1,6 @@
-// qtp_host_setup.hpp -- host side of the gfx950 qTIP4P/F step: context set-up: the reference's derived parameters, k-groups
-// and the matrix-core k-space layout, the normal-mode matrix, allocation.
+// qtp_host_setup.hpp -- host side of the gfx950 qTIP4P/F step: context
+// set-up (the reference's derived parameters, k-groups and the matrix-core
+// k-space layout, the normal-mode matrix, allocation).
 // Included once, in order, by qtip4pf.hip (one translation unit).
 #pragma once
 
