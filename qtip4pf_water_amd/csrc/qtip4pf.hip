@@ -136,6 +136,8 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     for (int I = 0; I < ctx->nblk; ++I)
         for (int J = I; J < ctx->nblk; ++J) tl.push_back(make_int2(I, J));
     ctx->ntile = (int)tl.size();
+    ctx->rs_cull = std::min(ctx->box.L[0], std::min(ctx->box.L[1], ctx->box.L[2])) >=
+                   RS_CULL_MIN_BOX * std::sqrt(ctx->ew.rcut_sq);
     // a whole wave per molecule when one launch covers few molecules
     ctx->lj_lpm = (long)std::max(ctx->P, 1) * ctx->n_mols <= 2048 ? 64 : 16;
     ctx->nmb = (ctx->n_mols + (256 / ctx->lj_lpm) - 1) / (256 / ctx->lj_lpm);
@@ -211,7 +213,7 @@ void qtp_destroy(qtp_ctx *ctx)
     for (auto e : ctx->evpool) hipEventDestroy(e);
     void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
                     ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
-                    ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart,
+                    ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart, ctx->rs_sites, ctx->rs_box,
                     ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->wm, ctx->gcol, ctx->mf_rt, ctx->mf_rt_small, ctx->ks_col, ctx->ks_unit, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,

@@ -92,6 +92,11 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
                 ctx->fspl, ctx->epart, ctx->ep_stride, ctx->ep_spl, vp);
     }
     if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
+    if (ctx->rs_cull) {
+        TimeScope ts(ctx, "sites");
+        k_sites<<<E * ctx->nblk, 64, 0, s>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
+                                             ctx->nblk, g, h, ctx->rs_sites, ctx->rs_box);
+    }
     {
         TimeScope ts(ctx, "realspace");
         // coefficients in SGPRs (more waves per SIMD) for large grids, where
@@ -100,16 +105,20 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         // lost more than the launch gained)
         const bool carg = (long)E * ctx->ntile >= RS_CARG_BLOCKS;
         const RsCoef cf = rs_coef();
-#define QTP_RS_LAUNCH(VIR, CARG)                                                                   \
-    k_realspace<VIR, CARG><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(                               \
+#define QTP_RS_LAUNCH(VIR, CARG, CULL)                                                             \
+    k_realspace<VIR, CARG, CULL><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(                         \
         pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles, ctx->ntile,     \
-        ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real, vp, cf)
-        if (vout) {
-            if (carg) QTP_RS_LAUNCH(true, true);
-            else QTP_RS_LAUNCH(true, false);
+        ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real, vp, cf,         \
+        ctx->rs_sites, ctx->rs_box)
+        if (ctx->rs_cull) {
+            if (vout) QTP_RS_LAUNCH(true, true, true);
+            else QTP_RS_LAUNCH(false, true, true);
+        } else if (vout) {
+            if (carg) QTP_RS_LAUNCH(true, true, false);
+            else QTP_RS_LAUNCH(true, false, false);
         } else {
-            if (carg) QTP_RS_LAUNCH(false, true);
-            else QTP_RS_LAUNCH(false, false);
+            if (carg) QTP_RS_LAUNCH(false, true, false);
+            else QTP_RS_LAUNCH(false, false, false);
         }
 #undef QTP_RS_LAUNCH
     }

@@ -239,6 +239,10 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->vpart, (size_t)E * 6 * ctx->ep_stride));
     CHK(dalloc(ctx, &ctx->vout, (size_t)E * 6));
     CHK(dalloc(ctx, &ctx->rpart, (size_t)E * ctx->nblk * N3));
+    if (ctx->rs_cull) {
+        CHK(dalloc(ctx, &ctx->rs_sites, (size_t)E * ctx->nblk * 64));
+        CHK(dalloc(ctx, &ctx->rs_box, (size_t)E * ctx->nblk * 6));
+    }
     CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));
     if (!ctx->kf_mf) CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));
 
