@@ -92,3 +92,63 @@ def test_culled_forces_unwrapped_molecules(oracle_mod):
             assert rel_atom_err(F[sl], Fr[sl]) < (1e-10 if b % 2 == 0 else 1e-9), (name, b)
     assert rel_atom_err(got["unwrapped"][0], got["wrapped"][0]) < 1e-11
     assert rel_atom_err(got["unwrapped"][1], got["wrapped"][1]) < 1e-11
+
+
+def _pair_distances_near(sites, box, lo, hi):
+    """Minimum-image distances of all site pairs that fall in [lo, hi]."""
+    L = np.asarray(box)
+    out = []
+    for i in range(len(sites) - 1):
+        d = sites[i + 1:] - sites[i]
+        d -= L * np.round(d / L)
+        r = np.sqrt(np.einsum("ij,ij->i", d, d))
+        out.append(r[(r >= lo) & (r <= hi)])
+    return np.sort(np.concatenate(out))
+
+
+def test_culled_virial_matches_energy_differences(oracle_mod):
+    """The virial form of the culled kernel: 1000 H2O (box 2.66 r_c), the
+    diagonal of W = -dV/d(strain) against central differences of the GPU's
+    own energy in stretched boxes at fixed Ewald parameters.  Strain 1e-8,
+    and r_c moved into a gap of the site-pair distances at least 2e-7 r_c
+    wide on either side, so no pair crosses the cutoff; tolerance 1e-6 of the
+    largest component (the differences carry ~1e-13 Eh of rounding over
+    2e-8)."""
+    from qtip4pf_water_amd import Water
+    n_side, L, h = 10, 31.07, 1e-8
+    n = n_side ** 3
+    q0, box = cfg.lattice_water_box(n_side, L, seed=75, jitter_bohr=0.1)
+    g = oracle_mod.consts()["gamma_vs"]
+    x = q0.reshape(n, 3, 3)
+    sites = x.copy()
+    sites[:, 0] = g * x[:, 0] + 0.5 * (1.0 - g) * (x[:, 1] + x[:, 2])
+    with Water(n, 1, box) as w0:
+        d = w0.derived
+        ew = {"kappa": d["kappa"], "rcut": d["rcut"], "k_cut": d["k_cut"], "k_max": d["k_max"]}
+    r = _pair_distances_near(sites.reshape(-1, 3), box, 0.995 * ew["rcut"], ew["rcut"])
+    gaps = np.diff(r)
+    k = int(np.argmax(gaps))
+    assert gaps[k] > 4e-7 * ew["rcut"], gaps[k]
+    ew["rcut"] = 0.5 * (r[k] + r[k + 1])
+    assert box[0] >= 2.5 * ew["rcut"]
+    say("culled virial FD", n, "mols, r_c", ew["rcut"], "gap", gaps[k])
+
+    def energy_virial(q, b):
+        o = oracle_mod.Oracle(n, 1, b)
+        point, lst, _ = o.all_lists(q)
+        with Water(n, 1, b, ewald=ew) as w:
+            return w.get_forces_virial(q, np.zeros(o.n_dof), o.charge, point, lst, 0)
+
+    _, W = energy_virial(q0, box)
+    fd = np.zeros(3)
+    for a in range(3):
+        es = []
+        for sgn in (1.0, -1.0):
+            b2 = list(box)
+            b2[a] *= 1.0 + sgn * h
+            q2 = q0.reshape(-1, 3).copy()
+            q2[:, a] *= 1.0 + sgn * h
+            es.append(energy_virial(q2.ravel(), tuple(b2))[0])
+        fd[a] = -(es[0] - es[1]) / (2.0 * h)
+    say("W diag", np.diag(W), "FD", fd)
+    assert np.max(np.abs(np.diag(W) - fd)) <= 1e-6 * np.max(np.abs(W))
