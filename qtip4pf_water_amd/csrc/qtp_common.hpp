@@ -150,6 +150,10 @@ __device__ __forceinline__ double wave_sum_array(const double *__restrict__ p, i
 {
     const int lane = threadIdx.x & 63;
     double s = 0.0;
+    // unrolled so that sixteen loads are in flight; each lane still adds its
+    // partials in index order (the same result as the plain loop).  4096 x 32:
+    // k_sc1 172 -> 70 us, its wavefront summing ~21k partials per bead
+#pragma unroll 16
     for (int k = lane; k < n; k += 64) s += p[k];
     return wave_sum(s);
 }
