@@ -109,7 +109,7 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     k_realspace<VIR, CARG, CULL><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(                         \
         pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles, ctx->ntile,     \
         ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real, vp, cf,         \
-        ctx->rs_sites, ctx->rs_box)
+        ctx->rs_sites, ctx->rs_box, ctx->rs_far)
         if (ctx->rs_cull) {
             if (vout) QTP_RS_LAUNCH(true, true, true);
             else QTP_RS_LAUNCH(false, true, true);
@@ -125,6 +125,8 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     if (fork) HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
     {
         TimeScope ts(ctx, "kforce");
+        // tiles the CULL form skipped: their (zero) slots are not read
+        const unsigned char *rfar = ctx->rs_cull ? ctx->rs_far : nullptr;
         const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
         const int nbm = E * ((ctx->n_atoms + MF_SITES - 1) / MF_SITES);
         // few blocks: twice the waves per block (when its LDS fits the default 64 KB)
@@ -132,12 +134,12 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
             k_kforce_mf<MF_WAVES_SMALL><<<nbm, 64 * MF_WAVES_SMALL, ctx->kforce_mf_lds_small, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
                 ctx->mf_nks, ctx->mf_rt_small, ctx->mf_nrt_small, ctx->rpart, ctx->nblk, ctx->flj,
-                ctx->spl_on ? ctx->fspl : nullptr, fout);
+                ctx->spl_on ? ctx->fspl : nullptr, fout, rfar, ctx->ntile);
         else if (ctx->kf_mf)
             k_kforce_mf<MF_WAVES><<<nbm, 64 * MF_WAVES, ctx->kforce_mf_lds, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
                 ctx->mf_nks, ctx->mf_rt, ctx->mf_nrt, ctx->rpart, ctx->nblk, ctx->flj,
-                ctx->spl_on ? ctx->fspl : nullptr, fout);
+                ctx->spl_on ? ctx->fspl : nullptr, fout, rfar, ctx->ntile);
         else if (fout2)
             k_kforce<KF_SPLIT_WAVES, true><<<2 * E * nab, 64 * KF_SPLIT_WAVES, ctx->kforce_lds_split, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,

@@ -242,6 +242,7 @@ static int alloc_all(qtp_ctx *ctx)
     if (ctx->rs_cull) {
         CHK(dalloc(ctx, &ctx->rs_sites, (size_t)E * ctx->nblk * 64));
         CHK(dalloc(ctx, &ctx->rs_box, (size_t)E * ctx->nblk * 6));
+        CHK(dalloc(ctx, &ctx->rs_far, (size_t)E * ctx->ntile));
     }
     CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));
     if (!ctx->kf_mf) CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));

@@ -567,7 +567,8 @@ __global__ __launch_bounds__(64 * W) void k_kforce_mf(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
     int NAP, double gamma, double half1mg, Ewald ew, const double *__restrict__ wm, int Mp,
     int nks, const int4 *__restrict__ rt, int nrt, const double *__restrict__ rpart, int nblk,
-    const double *__restrict__ flj, const double *__restrict__ fspl, double *__restrict__ fout)
+    const double *__restrict__ flj, const double *__restrict__ fspl, double *__restrict__ fout,
+    const unsigned char *__restrict__ rfar, int ntile)
 {
     extern __shared__ double smem[];
     const int K = ew.K, NN = 4 * nks;
@@ -584,12 +585,39 @@ __global__ __launch_bounds__(64 * W) void k_kforce_mf(
     const int a0 = ab * MF_SITES;
     const double *X = pos + (long)e * pstride;
     double rx = 0.0, ry = 0.0, rz = 0.0;
+    // real-space slots w, w + W, ...: with culling (rfar), a slot whose tile
+    // was skipped for every 64-site block the wave's sites fall in holds
+    // zeros only and is not read (one mask per wave, built before the loop)
+    const int nsl = (nblk - w + W - 1) / W;
+    unsigned long long live = 0ull;
+    const bool masked = rfar && nsl <= 64;
+    if (masked) {
+        const unsigned char *far = rfar + (long)e * ntile;
+        const int B0 = a0 >> 6, B1 = (min(a0 + MF_SITES, n_atoms) - 1) >> 6;
+        bool lv = false;
+        if (lane < nsl) {
+            const int sl = w + W * lane;
+            lv = !far[tile_index(min(B0, sl), max(B0, sl), nblk)] ||
+                 !far[tile_index(min(B1, sl), max(B1, sl), nblk)];
+        }
+        live = __ballot(lv);
+    }
     if (lane < MF_SITES && a0 + lane < n_atoms) {
-        for (int sl = w; sl < nblk; sl += W) {
-            const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + a0 + lane;
-            rx += rp[0];
-            ry += rp[NAP];
-            rz += rp[2 * NAP];
+        if (masked) {
+            for (unsigned long long m = live; m; m &= m - 1ull) {
+                const int sl = w + W * (__ffsll((long long)m) - 1);
+                const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + a0 + lane;
+                rx += rp[0];
+                ry += rp[NAP];
+                rz += rp[2 * NAP];
+            }
+        } else {
+            for (int sl = w; sl < nblk; sl += W) {
+                const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + a0 + lane;
+                rx += rp[0];
+                ry += rp[NAP];
+                rz += rp[2 * NAP];
+            }
         }
     }
     if (threadIdx.x < 3 * MF_SITES) {

@@ -25,7 +25,8 @@ namespace qtp {
 // CULL: the sites come from k_sites (x, y, z, q per site and a bounding box
 // per 64-site block).  A tile whose two blocks' boxes are more than r_c apart
 // under every periodic image holds no pair within the cutoff: it writes zero
-// slots and returns before loading any site.  The same pairs are evaluated
+// slots, marks itself in rfar[e * ntile + tile] (k_kforce_mf skips slots so
+// marked) and returns before loading any site.  The same pairs are evaluated
 // either way.  The host picks it for boxes several r_c wide, where most tiles
 // are far (4096 H2O: 54% of the tiles, sites in molecule order).
 constexpr int RS_WAVES = 4;
@@ -46,6 +47,13 @@ __device__ __forceinline__ double periodic_gap(double a0, double a1, double b0, 
     f = fma(-L, floor(f * invL), f);
     const double back = L - f - (a1 - a0) - (b1 - b0);
     return fmax(0.0, fmin(f, back));
+}
+
+// Index of tile (I, J), I <= J, in the real-space tile list (I-major upper
+// triangle of the nblk x nblk block matrix).
+__device__ __forceinline__ int tile_index(int I, int J, int nblk)
+{
+    return I * nblk - (I * (I - 1)) / 2 + (J - I);
 }
 
 // Sites of each 64-site block as (x, y, z, q) (M from O, H, H; forces.cpp:
@@ -91,7 +99,8 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
                                                    int ep_off, double *__restrict__ vpart,
                                                    const RsCoef cf,
                                                    const double4 *__restrict__ sites,
-                                                   const double *__restrict__ bbox)
+                                                   const double *__restrict__ bbox,
+                                                   unsigned char *__restrict__ rfar)
 {
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int e = bid / ntile, t = bid - e * ntile;
@@ -99,16 +108,22 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
     const int l = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double *X = pos + (long)e * pstride;
-    if (CULL && I != J) {
-        const double *bI = bbox + ((long)e * nblk + I) * 6;
-        const double *bJ = bbox + ((long)e * nblk + J) * 6;
-        double gsq = 0.0;
-        for (int d = 0; d < 3; ++d) {
-            const double g = periodic_gap(bI[d], bI[3 + d], bJ[d], bJ[3 + d], box.L[d], box.inv[d]);
-            gsq = fma(g, g, gsq);
+    if (CULL) {
+        bool far = false;
+        if (I != J) {
+            const double *bI = bbox + ((long)e * nblk + I) * 6;
+            const double *bJ = bbox + ((long)e * nblk + J) * 6;
+            double gsq = 0.0;
+            for (int d = 0; d < 3; ++d) {
+                const double g =
+                    periodic_gap(bI[d], bI[3 + d], bJ[d], bJ[3 + d], box.L[d], box.inv[d]);
+                gsq = fma(g, g, gsq);
+            }
+            // the margin covers the rounding of the boxes and of rsq
+            far = gsq > ew.rcut_sq * (1.0 + 1e-9);
         }
-        // the margin covers the rounding of the boxes and of rsq
-        if (gsq > ew.rcut_sq * (1.0 + 1e-9)) {
+        if (threadIdx.x == 0) rfar[(long)e * ntile + t] = far ? 1 : 0;
+        if (far) {
             if (VIR && w == 0 && l < 6) vpart[((long)e * 6 + l) * ep_stride + ep_off + t] = 0.0;
             const long base_e = (long)e * nblk * 3 * NAP;
             if (w < 3) {
