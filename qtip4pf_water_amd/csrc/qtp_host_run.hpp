@@ -144,12 +144,12 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
             k_kforce<KF_SPLIT_WAVES, true><<<2 * E * nab, 64 * KF_SPLIT_WAVES, ctx->kforce_lds_split, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
                 ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, fout2,
-                ctx->kf_chunk_split);
+                ctx->kf_chunk_split, rfar, ctx->ntile);
         else
             k_kforce<KF_WAVES, false><<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
                 ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, nullptr,
-                ctx->kf_chunk);
+                ctx->kf_chunk, rfar, ctx->ntile);
     }
     if (eout) {
         TimeScope ts(ctx, "esum");

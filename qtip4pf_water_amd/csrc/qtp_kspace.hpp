@@ -379,7 +379,7 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
     int NAP, double gamma, double half1mg, Ewald ew, int G, const double *__restrict__ krec,
     const double *__restrict__ rpart, int nblk, const double *__restrict__ flj,
     const double *__restrict__ fspl, double *__restrict__ fout, double *__restrict__ fout2,
-    int chunk)
+    int chunk, const unsigned char *__restrict__ rfar, int ntile)
 {
     constexpr int KF_WAVES = W;
     extern __shared__ double smem[];
@@ -400,12 +400,39 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
     const bool va = lane < KF_SITES && atom < n_atoms;
     const double *X = pos + (long)e * pstride;
     double fx = 0.0, fy = 0.0, fz = 0.0, fl = 0.0;
+    // skipped tiles' slots (culling) are not read: one mask per wave, as in
+    // k_kforce_mf
+    const int nsl = (nblk - w + KF_WAVES - 1) / KF_WAVES;
+    const bool masked = rfar && half == 0 && nsl <= 64;
+    unsigned long long live = 0ull;
+    if (masked) {
+        const unsigned char *far = rfar + (long)e * ntile;
+        const int a0 = ab * KF_SITES;
+        const int B0 = a0 >> 6, B1 = (min(a0 + KF_SITES, n_atoms) - 1) >> 6;
+        bool lv = false;
+        if (lane < nsl) {
+            const int sl = w + KF_WAVES * lane;
+            lv = !far[tile_index(min(B0, sl), max(B0, sl), nblk)] ||
+                 !far[tile_index(min(B1, sl), max(B1, sl), nblk)];
+        }
+        live = __ballot(lv);
+    }
     if (va && half == 0) {
-        for (int sl = w; sl < nblk; sl += KF_WAVES) {
-            const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + atom;
-            fx += rp[0];
-            fy += rp[NAP];
-            fz += rp[2 * NAP];
+        if (masked) {
+            for (unsigned long long m = live; m; m &= m - 1ull) {
+                const int sl = w + KF_WAVES * (__ffsll((long long)m) - 1);
+                const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + atom;
+                fx += rp[0];
+                fy += rp[NAP];
+                fz += rp[2 * NAP];
+            }
+        } else {
+            for (int sl = w; sl < nblk; sl += KF_WAVES) {
+                const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + atom;
+                fx += rp[0];
+                fy += rp[NAP];
+                fz += rp[2 * NAP];
+            }
         }
         if (w < 3) {
             fl = flj[((long)e * 3 + w) * NAP + atom];
