@@ -57,7 +57,7 @@ __device__ __forceinline__ int tile_index(int I, int J, int nblk)
 }
 
 // Sites of each 64-site block as (x, y, z, q) (M from O, H, H; forces.cpp:
-// 480-486) and the block's bounding box (lo xyz, hi xyz), for the CULL form
+// 558-564) and the block's bounding box (lo xyz, hi xyz), for the CULL form
 // of k_realspace.  grid: E * nblk, 64 threads.
 __global__ __launch_bounds__(64) void k_sites(const double *__restrict__ pos, long pstride,
                                               const double *__restrict__ charge, int n_atoms,
