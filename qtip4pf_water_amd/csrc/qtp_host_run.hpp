@@ -188,8 +188,11 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
     CHK(run_phase(ctx, ctx->qpm, P, 1, 0, ctx->fpm, nullptr, nullptr, f2));
     {
         TimeScope ts(ctx, "sc2");
-        k_sc2<<<P + 1, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, f2, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
-                                ctr);
+        // at most four runs of SC_THREADS dofs per block (one block per bead
+        // up to 455 molecules)
+        const int nch = std::max(1, (3 * ctx->NAP + 4 * SC_THREADS - 1) / (4 * SC_THREADS));
+        k_sc2<<<P * nch + 1, SC_THREADS, 0, s>>>(a, ctx->fph, ctx->fpm, f2, ctx->scal, ctx->F, ctx->p, kick, ctx->out,
+                                ctr, nch);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;

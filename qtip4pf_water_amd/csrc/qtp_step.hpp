@@ -102,7 +102,8 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
 // Second half (forces.cpp:776-789): odd beads F_sc = 4f/3 - (f+ - f-) / (9 w^2
 // step); the half-step kick p += dt/2 F_sc when requested; an extra last
 // block sums the SC potential and sc_pot in bead order (beside the others).
-// grid: P + 1, SC_THREADS threads.
+// grid: P * nch + 1, SC_THREADS threads: nch blocks per bead, each taking
+// every nch-th run of SC_THREADS dofs (nch > 1 for large systems).
 // f2 (may be null): the second half of the split k-space forces of the
 // phase-2 evaluations, added to fpm here.
 __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__restrict__ fph,
@@ -111,10 +112,12 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__re
                                                     const double *__restrict__ scal,
                                                     double *__restrict__ Fsc, double *__restrict__ p,
                                                     int kick, double *__restrict__ out,
-                                                    unsigned long long *__restrict__ step_ctr)
+                                                    unsigned long long *__restrict__ step_ctr,
+                                                    int nch)
 {
-    const int b = blockIdx.x;
+    const int b = blockIdx.x / nch, c = blockIdx.x - b * nch;
     const long o = (long)b * 3 * a.NAP;
+    const int t0 = c * SC_THREADS + threadIdx.x, dt = nch * SC_THREADS;
     if (b == a.P) {
         if (threadIdx.x == 0) {
             double tot = 0.0, vop = 0.0;
@@ -138,7 +141,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__re
         const int k = b >> 1;
         const double fscale = scal[b * 4 + 3];
         const long op = (long)(2 * k) * 3 * a.NAP, om = (long)(2 * k + 1) * 3 * a.NAP;
-        for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) {
+        for (int t = t0; t < 3 * a.NAP; t += dt) {
             double F = 4.0 * fph[o + t] / 3.0;
             const double fp = f2 ? fpm[op + t] + f2[op + t] : fpm[op + t];
             const double fm = f2 ? fpm[om + t] + f2[om + t] : fpm[om + t];
@@ -147,7 +150,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__re
             if (kick) p[o + t] += a.halfdt * F;
         }
     } else if (kick) {
-        for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) p[o + t] += a.halfdt * Fsc[o + t];
+        for (int t = t0; t < 3 * a.NAP; t += dt) p[o + t] += a.halfdt * Fsc[o + t];
     }
 }
 
