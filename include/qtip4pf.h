@@ -69,6 +69,8 @@ typedef struct {
 
 int qtp_create(const qtp_params *prm, qtp_ctx **out);
 void qtp_destroy(qtp_ctx *ctx);
+/* ctx == NULL: the message of the last failed context-free call
+ * (qtp_get_kcv_noalloc) on the calling thread. */
 const char *qtp_last_error(const qtp_ctx *ctx);
 int qtp_get_derived(const qtp_ctx *ctx, qtp_derived *out);
 /* The HIP stream (hipStream_t) every call of this context runs on. */
@@ -131,6 +133,21 @@ int qtp_update_trajectory_class_nospline(qtp_ctx *ctx, double *q, double *p, con
 
 /* path_integrals.cpp:29-48 get_kcv(q, fphys) */
 int qtp_get_kcv(qtp_ctx *ctx, const double *q, const double *fphys, double *kcv);
+/* get_kcv without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same estimator
+ * as qtp_get_kcv (path_integrals.cpp:29-48) on DEVICE arrays in the reference
+ * layout, q[b*n_dof + j] and fphys[b*n_dof + j] (n_dof = 3*n_atoms); the
+ * result is written to the device double *kcv.  Asynchronous on `stream`
+ * (a hipStream_t; NULL = the default stream).  No context, no
+ * hipMalloc/hipFree: the block partials go to a fixed device
+ * array of 8192 doubles (64 KiB per GPU, held until the process ends), so
+ * n_atoms <= 699,050 (n_dof <= 2,097,152); larger inputs are rejected with
+ * QTP_ERR_ARG.  Every call on one GPU shares that GPU's array: calls that
+ * overlap on one GPU (different streams, concurrent threads) race.  Errors:
+ * qtp_last_error(NULL). */
+int qtp_get_kcv_noalloc(int n_beads, int n_atoms, double beta, const double *q,
+                        const double *fphys, double *kcv, void *stream);
 /* path_integrals.cpp:52-81 get_conserved(q, p, mass, energy) (q and p are
  * not modified here; the reference round-trips them through ftransform). */
 int qtp_get_conserved(qtp_ctx *ctx, const double *q, const double *p, const double *mass,

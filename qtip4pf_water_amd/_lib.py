@@ -70,6 +70,7 @@ SIGNATURES = [
     ("qtp_update_trajectory_class_nospline", C.c_int, [_vp, dp, dp, dp, dp, dp, ip, ip, dp, dp]),
     ("qtp_get_kcv", C.c_int, [_vp, dp, dp, dp]),
     ("qtp_get_conserved", C.c_int, [_vp, dp, dp, dp, C.c_double, dp]),
+    ("qtp_get_kcv_noalloc", C.c_int, [C.c_int, C.c_int, C.c_double, _vp, _vp, _vp, _vp]),
     ("qtp_set_splines", C.c_int, [_vp, dp, C.c_int, dp, C.c_int, dp, C.c_int, C.c_double,
                                   C.c_double]),
     ("qtp_clear_splines", C.c_int, [_vp]),

@@ -13,6 +13,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(PKG)
 SRC = os.path.join(PKG, "csrc", "qtip4pf.hip")
 SETUP_SRC = os.path.join(PKG, "csrc", "qtp_setup.cpp")  # host-only
+NOALLOC_SRC = os.path.join(PKG, "csrc", "qtp_noalloc.hip")
 OUT = os.path.join(PKG, "libqtip4pf.so")
 ARCH = os.environ.get("QTP_OFFLOAD_ARCH", "gfx950")
 
@@ -40,7 +41,7 @@ def hipcc() -> str:
 
 def build(verbose: bool = False) -> str:
     tmp = OUT + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC, SETUP_SRC]
+    cmd = [hipcc(), *HIPCC_FLAGS, "-o", tmp, SRC, NOALLOC_SRC, SETUP_SRC]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)

@@ -23,6 +23,9 @@
 
 using namespace qtp;
 
+// message of the last failed context-free call on this thread (qtp_noalloc.hip)
+const char *qtp_noalloc_last_error();
+
 #include "qtp_host_ctx.hpp"
 #include "qtp_host_setup.hpp"
 #include "qtp_host_run.hpp"
@@ -228,7 +231,10 @@ void qtp_destroy(qtp_ctx *ctx)
     delete ctx;
 }
 
-const char *qtp_last_error(const qtp_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *qtp_last_error(const qtp_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : qtp_noalloc_last_error();
+}
 
 int qtp_get_derived(const qtp_ctx *ctx, qtp_derived *out)
 {
