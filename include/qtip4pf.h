@@ -52,7 +52,9 @@ typedef struct {
     double epsilon_sc;    /* config.txt epsilon_sc */
     int list_size;        /* 0: round(4 r_skin^3 n_mols rho_ox) (water.cpp:135) */
     /* Ewald override; ewald_kappa <= 0 derives (kappa, r_c, k_cut, k_max)
-     * inside each force call as forces.cpp:574-584 does. */
+     * inside each force call as forces.cpp:574-584 does.  Either way
+     * 1 <= k_max <= 45 (the k-space factor tables live in LDS), r_c <= L/2
+     * and kappa * r_c <= 6.2; qtp_create rejects other values. */
     double ewald_kappa, ewald_rcut, ewald_kcut;
     int ewald_kmax;
 } qtp_params;
@@ -246,7 +248,10 @@ int qtp_get_virial(qtp_ctx *ctx, double *virial, double *energy);
  * thermostat seed and step counter; qtp_load_checkpoint (in place of
  * qtp_set_state; charges/masses as there) restores them into a context of
  * the same system and recomputes the forces, so the run continues exactly
- * (bit for bit) as it would have without the stop. */
+ * (bit for bit) as it would have without the stop.  The file also records
+ * T_K, dtfs, eps_ewald, epsilon_sc and the effective Ewald kappa, r_c, k_cut
+ * and k_max; a load into a context where any of them (or the molecule count,
+ * bead count, list size or box) differs fails with QTP_ERR_ARG. */
 int qtp_save_checkpoint(qtp_ctx *ctx, const char *path);
 int qtp_load_checkpoint(qtp_ctx *ctx, const char *path, const double *charge,
                         const double *mass);
@@ -298,6 +303,99 @@ int qtp_get_observables(qtp_ctx *ctx, double out[4]);
  * driver's per-sample download, simulation.cpp:951-960, goes away). */
 int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *trace,
                      int *n_samples);
+
+/* ---- centroid dynamics, thermostats, centroid mean forces, g(r) --------------
+ * None of these has a counterpart in the reference, whose PILE-G is the tau -> 0
+ * limit (thermostat.cpp:64-133) and which has no CMD propagator or g(r)
+ * (its README leaves them to other programs). */
+
+#define QTP_THERMO_PILE_G 0  /* centroid: global Bussi rescale, time constant tau */
+#define QTP_THERMO_PILE_L 1  /* centroid: Langevin, friction 1/tau */
+/* Centroid thermostat of NVT resident steps (the non-centroid modes always
+ * take PILE's critically damped Langevin, thermostat.cpp:114-129).
+ * PILE_G with tau_fs = 0 is pile_g exactly (the default); tau_fs > 0 the
+ * canonical-sampling rescale of Bussi, Donadio and Parrinello (J. Chem.
+ * Phys. 126, 014101 (2007)).  PILE_L (tau_fs > 0): Langevin on every centroid
+ * degree of freedom (Ceriotti et al., J. Chem. Phys. 133, 124104 (2010)).
+ * Classical contexts (n_beads == 1) take PILE_L only: a Langevin step on
+ * every atom at the start of each NVT step.  The drop-in calls
+ * (qtp_pile_g, qtp_update_sc_nospline) always use pile_g. */
+int qtp_set_thermostat(qtp_ctx *ctx, int kind, double tau_fs);
+
+#define QTP_MODE_PIMD 0            /* plain PIMD (default) */
+#define QTP_MODE_PACMD 1           /* partially adiabatic centroid MD */
+#define QTP_MODE_CENTROID_FIXED 2  /* centroid held fixed (mean-force sampling) */
+/* Centroid dynamics of the resident SC steps.  PACMD gives every
+ * non-centroid normal mode the frequency omega_ad (a.u.; <= 0:
+ * P^(P/(P-1)) / (beta hbar), Hone, Rossky and Voth, J. Chem. Phys. 124,
+ * 154103 (2006)) through the mass m (w_k / omega_ad)^2, leaves the physical
+ * spring and forces as they are, and damps those modes critically; the
+ * centroid keeps the physical mass.  CENTROID_FIXED zeroes the centroid
+ * momentum after each kick and skips the centroid thermostat, so the beads
+ * sample the ring-polymer distribution at a fixed centroid. */
+int qtp_set_pimd_mode(qtp_ctx *ctx, int mode, double omega_ad);
+/* on != 0: zero the accumulator and, from the next resident step on, add
+ * (1/P) sum_b F_sc,b (the force on the centroid) after every step; on == 0
+ * stops.  qtp_get_centroid_force returns the mean over the steps since
+ * (reference layout, n_dof), optionally the current centroid positions
+ * (n_dof), and the number of steps. */
+int qtp_centroid_force(qtp_ctx *ctx, int on);
+int qtp_get_centroid_force(qtp_ctx *ctx, double *mean_force, double *centroid, int *n_samples);
+/* Pair-distance histograms of the resident (centroid, for n_beads > 1)
+ * positions: O-O over molecule pairs, O-H over every (O, H) pair including
+ * the molecule's own, bins of r_max / n_bins (bohr), minimum image.
+ * qtp_rdf_sample queues one sample (asynchronous); qtp_rdf_get returns the
+ * accumulated counts and the number of samples. */
+int qtp_rdf_reset(qtp_ctx *ctx, double r_max, int n_bins);
+int qtp_rdf_sample(qtp_ctx *ctx);
+int qtp_rdf_get(qtp_ctx *ctx, double *oo, double *oh, int *n_samples);
+
+/* ---- learned centroid-force correction ------------------------------------------
+ * No counterpart in the reference.  It takes the place of the reference's
+ * centroid hook, the additive spline pair correction of its fast-CMD path
+ * (forces.cpp:452-466, 634-691): V = V_qTIP4P/F + sum_i E_i, E_i the output
+ * of an element's network (O, H: 40 -> 128 -> 128 -> 1, tanh) on atom i's
+ * descriptor of 24 radial Gaussians (0.8-6 A) and 16 angular terms
+ * (within 4 A), each neighbour weighted by an element-pair weight.  The
+ * networks run on BF16 matrix cores with FP32 accumulation; forces come back
+ * in FP64 arrays.  Parameters: theta[2 * 21889] FP32, per element W1[128][40],
+ * b1[128], W2[128][128], b2[128], w3[128], b3 (O first); mean[2][40] and
+ * std[2][40] normalise the descriptors; E_i carries the factor e_scale. */
+typedef struct qtp_nn qtp_nn;
+/* The descriptor constants (r_rad, r_ang, mu0, dmu, eta_r, zeta[16],
+ * lambda[16], eta_a[16], w[2][2]; Angstrom): writes min(n, count) values,
+ * returns the count (57).  No GPU needed. */
+int qtp_nn_descriptor_params(double *out, int n);
+/* A model for systems of n_mols waters in the box boxl (bohr; every edge
+ * above 12 A), with work space for batches of up to max_frames frames. */
+int qtp_nn_create(int n_mols, const double boxl[3], int max_frames, qtp_nn **out);
+void qtp_nn_destroy(qtp_nn *nn);
+const char *qtp_nn_last_error(const qtp_nn *nn);
+int qtp_nn_set_model(qtp_nn *nn, const float *theta, const float *mean, const float *stdv,
+                     double e_scale);
+int qtp_nn_get_model(qtp_nn *nn, float *theta);
+/* Energies (n_frames), forces (n_frames * n_dof, Hartree/bohr) and raw
+ * descriptors (n_frames * n_atoms * 40) of frames in the reference layout;
+ * any output may be NULL. */
+int qtp_nn_eval(qtp_nn *nn, int n_frames, const double *pos, double *energy, double *forces,
+                float *descriptors);
+/* Training data for force matching: positions and target forces (both
+ * n_frames * n_dof, reference layout), kept on the device.  The loss is
+ * the mean over atoms and components of (F_model - target)^2. */
+int qtp_nn_set_data(qtp_nn *nn, int n_frames, const double *pos, const double *target);
+/* Loss and its gradient in theta (2 * 21889) on frames [frame0, frame0 +
+ * n_frames) of the data (n_frames <= max_frames), no update. */
+int qtp_nn_loss_grad(qtp_nn *nn, int frame0, int n_frames, double *loss, float *grad);
+/* n_steps Adam steps (beta 0.9, 0.999) with learning rate lr on batches of
+ * batch_frames consecutive data frames, batches in a seeded random order per
+ * pass; loss_trace (may be NULL) receives each batch's loss before its step. */
+int qtp_nn_train(qtp_nn *nn, int n_steps, int batch_frames, double lr, uint64_t seed,
+                 double *loss_trace);
+/* Attach (nn) or detach (NULL) the correction to a classical context's
+ * resident steps: its forces and energy are added to qTIP4P/F's (the model
+ * must be made for the same molecule count and box).  qtp_set_state must
+ * follow. */
+int qtp_set_nn(qtp_ctx *ctx, qtp_nn *nn);
 
 /* Per-kernel timing of the last qtp_run (HIP events around each launch of the
  * named kernel class on the context's stream); for measurement only. */

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(PKG, "libqtip4pf.so")
 
 dp = C.POINTER(C.c_double)
 ip = C.POINTER(C.c_int)
+fp = C.POINTER(C.c_float)
 u32p = C.POINTER(C.c_uint32)
 
 QTP_OK = 0
@@ -105,6 +106,24 @@ SIGNATURES = [
     ("qtp_batch_last_error", C.c_char_p, [_vp]),
     ("qtp_batch_destroy", None, [_vp]),
     ("qtp_get_scalars", C.c_int, [_vp, dp]),
+    ("qtp_set_thermostat", C.c_int, [_vp, C.c_int, C.c_double]),
+    ("qtp_set_pimd_mode", C.c_int, [_vp, C.c_int, C.c_double]),
+    ("qtp_centroid_force", C.c_int, [_vp, C.c_int]),
+    ("qtp_get_centroid_force", C.c_int, [_vp, dp, dp, ip]),
+    ("qtp_rdf_reset", C.c_int, [_vp, C.c_double, C.c_int]),
+    ("qtp_rdf_sample", C.c_int, [_vp]),
+    ("qtp_rdf_get", C.c_int, [_vp, dp, dp, ip]),
+    ("qtp_nn_descriptor_params", C.c_int, [dp, C.c_int]),
+    ("qtp_nn_create", C.c_int, [C.c_int, dp, C.c_int, C.POINTER(_vp)]),
+    ("qtp_nn_destroy", None, [_vp]),
+    ("qtp_nn_last_error", C.c_char_p, [_vp]),
+    ("qtp_nn_set_model", C.c_int, [_vp, fp, fp, fp, C.c_double]),
+    ("qtp_nn_get_model", C.c_int, [_vp, fp]),
+    ("qtp_nn_eval", C.c_int, [_vp, C.c_int, dp, dp, dp, fp]),
+    ("qtp_nn_set_data", C.c_int, [_vp, C.c_int, dp, dp]),
+    ("qtp_nn_loss_grad", C.c_int, [_vp, C.c_int, C.c_int, dp, fp]),
+    ("qtp_nn_train", C.c_int, [_vp, C.c_int, C.c_int, C.c_double, C.c_uint64, dp]),
+    ("qtp_set_nn", C.c_int, [_vp, _vp]),
     ("qtp_enable_timing", C.c_int, [_vp, C.c_int]),
     ("qtp_get_timing", C.c_int, [_vp, C.c_char_p, dp, ip]),
     ("qtp_philox4x32_10", C.c_int, [u32p, u32p, u32p]),

@@ -28,15 +28,20 @@ from . import _lib
 from ._lib import QtpDerived, QtpError, QtpParams, ListOverflow, dp, ip, ptr
 
 
-def _f64(a):
+def _f64(a, n=0, name="array"):
+    """`a` as passed to the library, which reads or writes n elements of it."""
     if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
-        raise TypeError("expected a C-contiguous float64 numpy array")
+        raise TypeError(f"{name}: expected a C-contiguous float64 numpy array")
+    if a.size < n:
+        raise ValueError(f"{name}: {a.size} elements, the call uses {n}")
     return a
 
 
-def _i32(a):
+def _i32(a, n=0, name="array"):
     if not (isinstance(a, np.ndarray) and a.dtype == np.int32 and a.flags.c_contiguous):
-        raise TypeError("expected a C-contiguous int32 numpy array")
+        raise TypeError(f"{name}: expected a C-contiguous int32 numpy array")
+    if a.size < n:
+        raise ValueError(f"{name}: {a.size} elements, the call uses {n}")
     return a
 
 
@@ -135,7 +140,7 @@ def read_positions(path, n_mols: int, n_beads: int):
 def write_positions(path, pos, n_mols: int, n_beads: int):
     """Every bead, one atom per line, exact on reading back."""
     rc = _lib.load().qtp_write_positions(os.fsencode(path), int(n_mols), int(n_beads),
-                                         ptr(_f64(pos)))
+                                         ptr(_f64(pos, n_beads * 9 * n_mols, "pos")))
     if rc != 0:
         raise QtpError(rc, f"write_positions: {path}")
 
@@ -169,6 +174,34 @@ class Water:
                         for f, _ in QtpDerived._fields_}
         self.n_mols, self.n_atoms, self.n_dof = d.n_mols, d.n_atoms, d.n_dof
         self.n_beads, self.list_size = d.n_beads, d.list_size
+
+    # ------------------------------------------- argument arrays and sizes
+    # Every array the library reads or writes is checked for its length
+    # first: the C calls take bare pointers.
+    def _q(self, a, name="q"):
+        """A whole P * n_dof array (positions, momenta, forces)."""
+        return ptr(_f64(a, self.n_beads * self.n_dof, name))
+
+    def _opt_q(self, a, name):
+        return self._q(a, name) if a is not None else None
+
+    def _charge(self, a):
+        return ptr(_f64(a, self.n_atoms, "charge")) if a is not None else None
+
+    def _mass(self, a):
+        return ptr(_f64(a, self.n_dof, "mass")) if a is not None else None
+
+    def _point(self, a):
+        return ptr(_i32(a, self.n_beads * self.n_mols, "point"), ip)
+
+    def _lst(self, a):
+        return ptr(_i32(a, self.n_beads * self.list_size, "list"), ip)
+
+    def _disp(self, a):
+        return ptr(_f64(a, self.n_beads * 3 * self.n_mols, "disp"))
+
+    def _dev(self, a):
+        return ptr(_f64(a, 3 + (self.n_beads - 1) * self.n_dof, "dev")) if a is not None else None
 
     # ---------------------------------------------------------------- misc
     def _chk(self, rc):
@@ -208,9 +241,9 @@ class Water:
     # ------------------------------------------------ reference-shaped calls
     def get_forces_nospline(self, q, F, charge, point, lst, bead=0) -> float:
         e = C.c_double()
-        self._chk(self._L.qtp_get_forces_nospline(self._h, ptr(_f64(q)), ptr(_f64(F)),
-                                                  ptr(_f64(charge)) if charge is not None else None,
-                                                  ptr(_i32(point), ip), ptr(_i32(lst), ip),
+        self._chk(self._L.qtp_get_forces_nospline(self._h, self._q(q), self._q(F, "F"),
+                                                  self._charge(charge),
+                                                  self._point(point), self._lst(lst),
                                                   int(bead), C.byref(e)))
         return e.value
 
@@ -218,9 +251,9 @@ class Water:
         """get_forces_nospline plus the bead's virial tensor W = -dV/d(strain)
         at fixed Ewald parameters.  Returns (energy, W as a 3x3 array)."""
         e, v = C.c_double(), np.zeros(6)
-        self._chk(self._L.qtp_get_forces_virial(self._h, ptr(_f64(q)), ptr(_f64(F)),
-                                                ptr(_f64(charge)) if charge is not None else None,
-                                                ptr(_i32(point), ip), ptr(_i32(lst), ip),
+        self._chk(self._L.qtp_get_forces_virial(self._h, self._q(q), self._q(F, "F"),
+                                                self._charge(charge),
+                                                self._point(point), self._lst(lst),
                                                 int(bead), C.byref(e), ptr(v)))
         return e.value, voigt_to_matrix(v)
 
@@ -229,11 +262,11 @@ class Water:
         sc_pot reference argument."""
         e, sc = C.c_double(), C.c_double()
         self._chk(self._L.qtp_force_eval_sc_nospline(
-            self._h, ptr(_f64(q)), ptr(_f64(F_sc)),
-            ptr(_f64(charge)) if charge is not None else None,
-            ptr(_f64(mass)) if mass is not None else None,
-            ptr(_i32(point), ip), ptr(_i32(lst), ip), C.byref(sc),
-            ptr(_f64(fphys)) if fphys is not None else None, C.byref(e)))
+            self._h, self._q(q), self._q(F_sc, "F_sc"),
+            self._charge(charge),
+            self._mass(mass),
+            self._point(point), self._lst(lst), C.byref(sc),
+            self._opt_q(fphys, "fphys"), C.byref(e)))
         return e.value, sc.value
 
     # ---------------------------------------------- splined pair potential
@@ -249,57 +282,58 @@ class Water:
 
     def get_forces_wspline(self, q, F, charge, point, lst, bead=0) -> float:
         e = C.c_double()
-        self._chk(self._L.qtp_get_forces_wspline(self._h, ptr(_f64(q)), ptr(_f64(F)),
-                                                 ptr(_f64(charge)) if charge is not None else None,
-                                                 ptr(_i32(point), ip), ptr(_i32(lst), ip),
+        self._chk(self._L.qtp_get_forces_wspline(self._h, self._q(q), self._q(F, "F"),
+                                                 self._charge(charge),
+                                                 self._point(point), self._lst(lst),
                                                  int(bead), C.byref(e)))
         return e.value
 
     def force_eval_sc_wspline(self, q, F_sc, charge, mass, point, lst, fphys=None):
         e, sc = C.c_double(), C.c_double()
         self._chk(self._L.qtp_force_eval_sc_wspline(
-            self._h, ptr(_f64(q)), ptr(_f64(F_sc)),
-            ptr(_f64(charge)) if charge is not None else None,
-            ptr(_f64(mass)) if mass is not None else None,
-            ptr(_i32(point), ip), ptr(_i32(lst), ip), C.byref(sc),
-            ptr(_f64(fphys)) if fphys is not None else None, C.byref(e)))
+            self._h, self._q(q), self._q(F_sc, "F_sc"),
+            self._charge(charge),
+            self._mass(mass),
+            self._point(point), self._lst(lst), C.byref(sc),
+            self._opt_q(fphys, "fphys"), C.byref(e)))
         return e.value, sc.value
 
     def update_sc_wspline(self, q, p, mass, F, fphys, charge, point, lst, disp, dev, is_nvt, heat):
         h, sc, e = C.c_double(heat), C.c_double(), C.c_double()
         self._chk(self._L.qtp_update_sc_wspline(
-            self._h, ptr(_f64(q)), ptr(_f64(p)),
-            ptr(_f64(mass)) if mass is not None else None, ptr(_f64(F)),
-            ptr(_f64(fphys)) if fphys is not None else None,
-            ptr(_f64(charge)) if charge is not None else None,
-            ptr(_i32(point), ip), ptr(_i32(lst), ip), ptr(_f64(disp)),
-            ptr(_f64(dev)) if dev is not None else None, 1 if is_nvt else 0,
+            self._h, self._q(q), self._q(p, "p"),
+            self._mass(mass), self._q(F, "F"),
+            self._opt_q(fphys, "fphys"),
+            self._charge(charge),
+            self._point(point), self._lst(lst), self._disp(disp),
+            self._dev(dev), 1 if is_nvt else 0,
             C.byref(h), C.byref(sc), C.byref(e)))
         return e.value, h.value, sc.value
 
     def update_trajectory_class_wspline(self, q, p, mass, F, charge, point, lst, disp) -> float:
         e = C.c_double()
         self._chk(self._L.qtp_update_trajectory_class_wspline(
-            self._h, ptr(_f64(q)), ptr(_f64(p)),
-            ptr(_f64(mass)) if mass is not None else None, ptr(_f64(F)),
-            ptr(_f64(charge)) if charge is not None else None,
-            ptr(_i32(point), ip), ptr(_i32(lst), ip), ptr(_f64(disp)), C.byref(e)))
+            self._h, self._q(q), self._q(p, "p"),
+            self._mass(mass), self._q(F, "F"),
+            self._charge(charge),
+            self._point(point), self._lst(lst), self._disp(disp), C.byref(e)))
         return e.value
 
     def neighbour_list(self, q, point, lst, disp, bead=0):
-        self._chk(self._L.qtp_neighbour_list(self._h, ptr(_f64(q)), ptr(_i32(point), ip),
-                                             ptr(_i32(lst), ip),
-                                             ptr(_f64(disp)) if disp is not None else None,
+        self._chk(self._L.qtp_neighbour_list(self._h, self._q(q), self._point(point),
+                                             self._lst(lst),
+                                             self._disp(disp) if disp is not None else None,
                                              int(bead)))
 
     def ftransform(self, a, length, is_forwards):
-        self._chk(self._L.qtp_ftransform(self._h, ptr(_f64(a)), int(length), 1 if is_forwards else 0))
+        self._chk(self._L.qtp_ftransform(self._h, ptr(_f64(a, self.n_beads * int(length), "a")),
+                                         int(length), 1 if is_forwards else 0))
 
     def pile_g(self, p_nm, mass, dev) -> float:
         h = C.c_double()
-        self._chk(self._L.qtp_pile_g(self._h, ptr(_f64(p_nm)),
-                                     ptr(_f64(mass)) if mass is not None else None,
-                                     ptr(_f64(dev)), C.byref(h)))
+        self._chk(self._L.qtp_pile_g(self._h, self._q(p_nm, "p_nm"),
+                                     self._mass(mass),
+                                     self._dev(dev), C.byref(h)))
         return h.value
 
     def update_sc_nospline(self, q, p, mass, F, fphys, charge, point, lst, disp, dev, is_nvt,
@@ -307,41 +341,41 @@ class Water:
         """One SC step.  Returns (energy, heat, sc_pot)."""
         h, sc, e = C.c_double(heat), C.c_double(), C.c_double()
         self._chk(self._L.qtp_update_sc_nospline(
-            self._h, ptr(_f64(q)), ptr(_f64(p)),
-            ptr(_f64(mass)) if mass is not None else None, ptr(_f64(F)),
-            ptr(_f64(fphys)) if fphys is not None else None,
-            ptr(_f64(charge)) if charge is not None else None,
-            ptr(_i32(point), ip), ptr(_i32(lst), ip), ptr(_f64(disp)),
-            ptr(_f64(dev)) if dev is not None else None, 1 if is_nvt else 0,
+            self._h, self._q(q), self._q(p, "p"),
+            self._mass(mass), self._q(F, "F"),
+            self._opt_q(fphys, "fphys"),
+            self._charge(charge),
+            self._point(point), self._lst(lst), self._disp(disp),
+            self._dev(dev), 1 if is_nvt else 0,
             C.byref(h), C.byref(sc), C.byref(e)))
         return e.value, h.value, sc.value
 
     def update_trajectory_class_nospline(self, q, p, mass, F, charge, point, lst, disp) -> float:
         e = C.c_double()
         self._chk(self._L.qtp_update_trajectory_class_nospline(
-            self._h, ptr(_f64(q)), ptr(_f64(p)),
-            ptr(_f64(mass)) if mass is not None else None, ptr(_f64(F)),
-            ptr(_f64(charge)) if charge is not None else None,
-            ptr(_i32(point), ip), ptr(_i32(lst), ip), ptr(_f64(disp)), C.byref(e)))
+            self._h, self._q(q), self._q(p, "p"),
+            self._mass(mass), self._q(F, "F"),
+            self._charge(charge),
+            self._point(point), self._lst(lst), self._disp(disp), C.byref(e)))
         return e.value
 
     def get_kcv(self, q, fphys) -> float:
         v = C.c_double()
-        self._chk(self._L.qtp_get_kcv(self._h, ptr(_f64(q)), ptr(_f64(fphys)), C.byref(v)))
+        self._chk(self._L.qtp_get_kcv(self._h, self._q(q), self._q(fphys, "fphys"), C.byref(v)))
         return v.value
 
     def get_conserved(self, q, p, mass, energy) -> float:
         v = C.c_double()
-        self._chk(self._L.qtp_get_conserved(self._h, ptr(_f64(q)), ptr(_f64(p)),
-                                            ptr(_f64(mass)) if mass is not None else None,
+        self._chk(self._L.qtp_get_conserved(self._h, self._q(q), self._q(p, "p"),
+                                            self._mass(mass),
                                             float(energy), C.byref(v)))
         return v.value
 
     # ------------------------------------------------------- resident mode
     def set_state(self, q, p, charge=None, mass=None):
-        self._chk(self._L.qtp_set_state(self._h, ptr(_f64(q)), ptr(_f64(p)),
-                                        ptr(_f64(charge)) if charge is not None else None,
-                                        ptr(_f64(mass)) if mass is not None else None))
+        self._chk(self._L.qtp_set_state(self._h, self._q(q), self._q(p, "p"),
+                                        self._charge(charge),
+                                        self._mass(mass)))
 
     def get_state(self):
         n = self.n_beads * self.n_dof
@@ -367,8 +401,8 @@ class Water:
 
     def load_checkpoint(self, path, charge=None, mass=None):
         self._chk(self._L.qtp_load_checkpoint(self._h, os.fsencode(path),
-                                              ptr(_f64(charge)) if charge is not None else None,
-                                              ptr(_f64(mass)) if mass is not None else None))
+                                              self._charge(charge),
+                                              self._mass(mass)))
 
     def set_seed(self, seed: int):
         self._chk(self._L.qtp_set_seed(self._h, C.c_uint64(seed)))
@@ -400,6 +434,51 @@ class Water:
         self._chk(self._L.qtp_get_scalars(self._h, ptr(o)))
         return {"energy": o[0], "sc_pot": o[1], "heat": o[2], "rebuilds": int(o[3]),
                 "steps": int(o[4]), "status": int(o[5])}
+
+    # ------------------------------ centroid dynamics, thermostats, g(r)
+    def set_thermostat(self, kind: str = "pile_g", tau_fs: float = 0.0):
+        """Centroid thermostat of NVT resident steps: "pile_g" (Bussi rescale,
+        time constant tau_fs; 0 = the reference's pile_g) or "pile_l"
+        (Langevin, friction 1/tau_fs).  Classical contexts: "pile_l" only."""
+        k = {"pile_g": 0, "pile_l": 1}[kind]
+        self._chk(self._L.qtp_set_thermostat(self._h, k, float(tau_fs)))
+
+    def set_pimd_mode(self, mode: str = "pimd", omega_ad: float = 0.0):
+        """"pimd", "pacmd" (partially adiabatic CMD at omega_ad, a.u.; 0 =
+        P^(P/(P-1))/(beta hbar)) or "centroid_fixed"."""
+        m = {"pimd": 0, "pacmd": 1, "centroid_fixed": 2}[mode]
+        self._chk(self._L.qtp_set_pimd_mode(self._h, m, float(omega_ad)))
+
+    def centroid_force(self, on: bool = True):
+        """Start (zeroing the sum) or stop accumulating the force on the
+        centroid, (1/P) sum_b F_sc,b, after every resident step."""
+        self._chk(self._L.qtp_centroid_force(self._h, 1 if on else 0))
+
+    def get_centroid_force(self):
+        """(mean force since centroid_force(True), centroid positions,
+        number of steps); arrays in reference layout (n_dof)."""
+        f, c, n = np.zeros(self.n_dof), np.zeros(self.n_dof), C.c_int()
+        self._chk(self._L.qtp_get_centroid_force(self._h, ptr(f), ptr(c), C.byref(n)))
+        return f, c, n.value
+
+    def rdf_reset(self, r_max_bohr: float, n_bins: int):
+        self._chk(self._L.qtp_rdf_reset(self._h, float(r_max_bohr), int(n_bins)))
+        self._rdf_bins = int(n_bins)
+
+    def rdf_sample(self):
+        self._chk(self._L.qtp_rdf_sample(self._h))
+
+    def rdf_get(self):
+        """(O-O counts, O-H counts, number of samples)."""
+        n = self._rdf_bins
+        oo, oh, k = np.zeros(n), np.zeros(n), C.c_int()
+        self._chk(self._L.qtp_rdf_get(self._h, ptr(oo), ptr(oh), C.byref(k)))
+        return oo, oh, k.value
+
+    def set_nn(self, model):
+        """Attach a Surrogate (or None) to the classical resident steps."""
+        self._chk(self._L.qtp_set_nn(self._h, model._h if model is not None else None))
+        self._nn = model
 
     def enable_timing(self, on=True):
         self._chk(self._L.qtp_enable_timing(self._h, 1 if on else 0))
@@ -459,3 +538,102 @@ class Batch:
 
     def __exit__(self, *a):
         self.close()
+
+
+NN_PARAMS_PER_ELEMENT = 128 * 40 + 128 + 128 * 128 + 128 + 128 + 1
+
+
+def nn_descriptor_params() -> dict:
+    """The descriptor constants compiled into the library (Angstrom)."""
+    L = _lib.load()
+    n = L.qtp_nn_descriptor_params(None, 0)
+    v = np.zeros(n)
+    L.qtp_nn_descriptor_params(ptr(v), n)
+    return dict(r_rad=v[0], r_ang=v[1], mu0=v[2], dmu=v[3], eta_r=v[4], zeta=v[5:21].copy(),
+                lam=v[21:37].copy(), eta_a=v[37:53].copy(), w=v[53:57].reshape(2, 2).copy())
+
+
+class Surrogate:
+    """The learned centroid-force correction on the GPU (qtp_nn_*): an
+    atom-centred BF16 network whose energies add to qTIP4P/F's, trained by
+    force matching on centroid mean forces."""
+
+    def __init__(self, n_mols, boxl, max_frames=16):
+        self._L = _lib.load()
+        h = C.c_void_p()
+        b = np.ascontiguousarray(boxl, dtype=np.float64)
+        rc = self._L.qtp_nn_create(int(n_mols), ptr(b), int(max_frames), C.byref(h))
+        self._h = h
+        if rc != 0:
+            msg = self._L.qtp_nn_last_error(h).decode() if h else "qtp_nn_create"
+            self.close()
+            raise QtpError(rc, msg)
+        self.n_mols, self.n_atoms, self.n_dof = int(n_mols), 3 * int(n_mols), 9 * int(n_mols)
+        self.max_frames = int(max_frames)
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise QtpError(rc, self._L.qtp_nn_last_error(self._h).decode())
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.qtp_nn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_model(self, theta, mean, std, e_scale):
+        t = np.ascontiguousarray(theta, dtype=np.float32)
+        m = np.ascontiguousarray(mean, dtype=np.float32).ravel()
+        s = np.ascontiguousarray(std, dtype=np.float32).ravel()
+        if t.size != 2 * NN_PARAMS_PER_ELEMENT or m.size != 80 or s.size != 80:
+            raise ValueError("theta: 2 x 21889 values; mean, std: 2 x 40")
+        self._chk(self._L.qtp_nn_set_model(self._h, ptr(t, _lib.fp), ptr(m, _lib.fp),
+                                           ptr(s, _lib.fp), float(e_scale)))
+
+    def get_model(self):
+        t = np.zeros(2 * NN_PARAMS_PER_ELEMENT, np.float32)
+        self._chk(self._L.qtp_nn_get_model(self._h, ptr(t, _lib.fp)))
+        return t
+
+    def eval(self, pos, descriptors=False):
+        """(energies (n_frames), forces (n_frames, n_dof)[, raw descriptors
+        (n_frames, n_atoms, 40)]) of frames in reference layout."""
+        x = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, self.n_dof)
+        nf = x.shape[0]
+        e, f = np.zeros(nf), np.zeros((nf, self.n_dof))
+        g = np.zeros((nf, self.n_atoms, 40), np.float32) if descriptors else None
+        self._chk(self._L.qtp_nn_eval(self._h, nf, ptr(x), ptr(e), ptr(f),
+                                      ptr(g, _lib.fp) if g is not None else None))
+        return (e, f, g) if descriptors else (e, f)
+
+    def set_data(self, pos, target):
+        x = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1, self.n_dof)
+        t = np.ascontiguousarray(target, dtype=np.float64).reshape(-1, self.n_dof)
+        if x.shape != t.shape:
+            raise ValueError("positions and targets differ in shape")
+        self._chk(self._L.qtp_nn_set_data(self._h, x.shape[0], ptr(x), ptr(t)))
+        self.n_data = x.shape[0]
+
+    def loss_grad(self, frame0, n_frames):
+        loss = C.c_double()
+        g = np.zeros(2 * NN_PARAMS_PER_ELEMENT, np.float32)
+        self._chk(self._L.qtp_nn_loss_grad(self._h, int(frame0), int(n_frames), C.byref(loss),
+                                           ptr(g, _lib.fp)))
+        return loss.value, g
+
+    def train(self, n_steps, batch_frames, lr, seed=0):
+        tr = np.zeros(max(1, int(n_steps)))
+        self._chk(self._L.qtp_nn_train(self._h, int(n_steps), int(batch_frames), float(lr),
+                                       C.c_uint64(seed), ptr(tr)))
+        return tr[:int(n_steps)]

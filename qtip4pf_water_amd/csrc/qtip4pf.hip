@@ -28,6 +28,7 @@ const char *qtp_noalloc_last_error();
 
 #include "qtp_host_ctx.hpp"
 #include "qtp_host_setup.hpp"
+#include "qtp_host_nn.hpp"
 #include "qtp_host_run.hpp"
 
 // ===========================================================================
@@ -57,10 +58,16 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->fcap = 2L * ctx->list_size;
     // the real-space erfc approximation covers kappa * r <= QTP_ERFCX_XMAX
     // (derived parameters: kappa * r_c = sqrt(-ln eps_ewald) <= 6.2 for eps >= 2e-17)
+    // the largest k_max whose k-space factor tables fit the 160 KiB of LDS
+    int kmax_ceiling = 1;
+    while (kforce_lds_bytes(kmax_ceiling + 2, KF_WAVES,
+                            kforce_chunk(kmax_ceiling + 2, KF_WAVES, 2)) <= 160 * 1024)
+        ++kmax_ceiling;
     if (2.0 * ctx->drv.rcut > ctx->drv.min_box * (1.0 + 1e-12) || ctx->drv.k_max < 1 ||
-        ctx->drv.k_max > 250 || !(ctx->drv.kappa * ctx->drv.rcut <= QTP_ERFCX_XMAX)) {
-        ctx->err = "Ewald parameters out of range (r_c must be <= L/2, 1 <= k_max <= 250, "
-                   "kappa * r_c <= 6.2)";
+        ctx->drv.k_max > kmax_ceiling || !(ctx->drv.kappa * ctx->drv.rcut <= QTP_ERFCX_XMAX)) {
+        ctx->err = "Ewald parameters out of range (r_c must be <= L/2, 1 <= k_max <= " +
+                   std::to_string(kmax_ceiling) + ", kappa * r_c <= 6.2); k_max = " +
+                   std::to_string(ctx->drv.k_max);
         *out = ctx;
         return QTP_ERR_ARG;
     }
@@ -174,11 +181,18 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
             return QTP_ERR_ARG;
         }
     }
+    // the generic bead transform keeps P x 64 doubles in LDS (128 KiB at P = 256)
+    if (nm_generic_lds(ctx->P) > 64 * 1024 &&
+        hipFuncSetAttribute((const void *)k_nm_generic, hipFuncAttributeMaxDynamicSharedMemorySize,
+                            (int)nm_generic_lds(ctx->P)) != hipSuccess) {
+        ctx->err = "LDS request of the bead transform above the device limit";
+        return QTP_ERR_ARG;
+    }
     int st = QTP_OK;
     if ((st = dalloc(ctx, &ctx->groups, ctx->G)) || (st = dalloc(ctx, &ctx->gcf, ctx->G)) ||
         (st = dalloc(ctx, &ctx->grk, ctx->G)) || (st = dalloc(ctx, &ctx->tiles, ctx->ntile)) ||
         (st = dalloc(ctx, &ctx->Cm, (size_t)ctx->P * ctx->P)) ||
-        (st = dalloc(ctx, &ctx->mode, ctx->P)) ||
+        (st = dalloc(ctx, &ctx->mode, ctx->P)) || (st = dalloc(ctx, &ctx->ms, ctx->P)) ||
         (ctx->kf_mf && ((st = dalloc(ctx, &ctx->gcol, ctx->G)) ||
                         (st = dalloc(ctx, &ctx->mf_rt, ctx->mf_nrt)) ||
                         (st = dalloc(ctx, &ctx->mf_rt_small, ctx->mf_nrt_small)) ||
@@ -220,7 +234,9 @@ void qtp_destroy(qtp_ctx *ctx)
                     ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->wm, ctx->gcol, ctx->mf_rt, ctx->mf_rt_small, ctx->ks_col, ctx->ks_unit, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
-                    ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace};
+                    ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace,
+                    ctx->nn_e, ctx->ms, ctx->cmf_acc, ctx->cmf_n, ctx->qc, ctx->rdf_oo,
+                    ctx->rdf_oh};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->s2) hipStreamSynchronize(ctx->s2);
@@ -365,7 +381,7 @@ int qtp_ftransform(qtp_ctx *ctx, double *a, int length, int is_forwards)
         own = true;
     }
     HIPCHK(hipMemcpyAsync(d, a, n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-    const size_t lds = (size_t)P * P * sizeof(double) + (size_t)P * 64 * sizeof(double);
+    const size_t lds = nm_generic_lds(P);
     {
         TimeScope ts(ctx, "nm");
         k_nm_generic<<<(length + 63) / 64, 256, lds, ctx->stream>>>(P, length, ctx->Cm, d,
@@ -520,7 +536,7 @@ int qtp_get_conserved(qtp_ctx *ctx, const double *q, const double *p, const doub
     CHK(upload_planar(ctx, ctx->nq, q, P));
     CHK(upload_planar(ctx, ctx->np, p, P));
     const int len = 3 * ctx->NAP;
-    const size_t lds = (size_t)P * P * sizeof(double) + (size_t)P * 64 * sizeof(double);
+    const size_t lds = nm_generic_lds(P);
     k_nm_generic<<<(len + 63) / 64, 256, lds, ctx->stream>>>(P, len, ctx->Cm, ctx->nq, 1);
     k_nm_generic<<<(len + 63) / 64, 256, lds, ctx->stream>>>(P, len, ctx->Cm, ctx->np, 1);
     const int nb = (3 * ctx->NAP + 255) / 256;
@@ -756,10 +772,7 @@ int qtp_set_state(qtp_ctx *ctx, const double *q, const double *p, const double *
     if (P >= 2) {
         CHK(run_sc_forces(ctx, 0, nullptr));
     } else {
-        CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, ctx->Ve));
-        HIPCHK(hipMemcpyAsync(ctx->F, ctx->fph, 3 * ctx->NAP * sizeof(double),
-                              hipMemcpyDeviceToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->out, ctx->Ve, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        CHK(class_initial_forces(ctx));
     }
     HIPCHK(hipMemsetAsync(ctx->nreb, 0, P * sizeof(double), ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -784,12 +797,31 @@ int qtp_get_state(qtp_ctx *ctx, double *q, double *p, double *F, double *fphys, 
 
 // ---- checkpoint / restart ------------------------------------------------------
 
-// File layout (little endian): "QTPCKPT1", int32 version (1), int32 n_mols,
-// n_beads, list_size; double boxl[3], T_K, dtfs; uint64 seed, step counter;
-// double heat; then in the reference layout: q, p (n_beads * n_dof),
-// disp (n_beads * 3 * n_mols), point (n_beads * n_mols int32),
-// list (n_beads * list_size int32), rebuild counts (n_beads doubles).
+// File layout (little endian): "QTPCKPT1", int32 version (2), int32 n_mols,
+// n_beads, list_size, k_max; double boxl[3], T_K, dtfs, eps_ewald,
+// epsilon_sc, kappa, r_c, k_cut (the effective Ewald parameters); uint64
+// seed, step counter; double heat; then in the reference layout: q, p
+// (n_beads * n_dof), disp (n_beads * 3 * n_mols), point (n_beads * n_mols
+// int32), list (n_beads * list_size int32), rebuild counts (n_beads doubles).
+// A load into a context whose system, temperature, time step or force
+// parameters differ is rejected: the run would not continue as saved.
 static const char QTP_CKPT_MAGIC[8] = {'Q', 'T', 'P', 'C', 'K', 'P', 'T', '1'};
+static const int32_t QTP_CKPT_VERSION = 2;
+static const int QTP_CKPT_NPAR = 10;
+
+static void ckpt_header(const qtp_ctx *ctx, int32_t hdr[5], double par[QTP_CKPT_NPAR])
+{
+    hdr[0] = QTP_CKPT_VERSION;
+    hdr[1] = ctx->n_mols;
+    hdr[2] = ctx->P;
+    hdr[3] = ctx->list_size;
+    hdr[4] = ctx->drv.k_max;
+    const double v[QTP_CKPT_NPAR] = {ctx->drv.boxl[0], ctx->drv.boxl[1], ctx->drv.boxl[2],
+                                     ctx->prm.T_K,     ctx->prm.dtfs,    ctx->prm.eps_ewald,
+                                     ctx->prm.epsilon_sc, ctx->drv.kappa, ctx->drv.rcut,
+                                     ctx->drv.k_cut};
+    for (int i = 0; i < QTP_CKPT_NPAR; ++i) par[i] = v[i];
+}
 
 int qtp_save_checkpoint(qtp_ctx *ctx, const char *path)
 {
@@ -815,12 +847,13 @@ int qtp_save_checkpoint(qtp_ctx *ctx, const char *path)
         ctx->err = std::string("cannot open ") + path;
         return QTP_ERR_ARG;
     }
-    const int32_t hdr[4] = {1, ctx->n_mols, P, ctx->list_size};
-    const double par[5] = {ctx->drv.boxl[0], ctx->drv.boxl[1], ctx->drv.boxl[2], ctx->prm.T_K,
-                           ctx->prm.dtfs};
+    int32_t hdr[5];
+    double par[QTP_CKPT_NPAR];
+    ckpt_header(ctx, hdr, par);
     const uint64_t u[2] = {ctx->seed, ctr};
-    bool ok = std::fwrite(QTP_CKPT_MAGIC, 1, 8, f) == 8 && std::fwrite(hdr, 4, 4, f) == 4 &&
-              std::fwrite(par, 8, 5, f) == 5 && std::fwrite(u, 8, 2, f) == 2 &&
+    bool ok = std::fwrite(QTP_CKPT_MAGIC, 1, 8, f) == 8 && std::fwrite(hdr, 4, 5, f) == 5 &&
+              std::fwrite(par, 8, QTP_CKPT_NPAR, f) == (size_t)QTP_CKPT_NPAR &&
+              std::fwrite(u, 8, 2, f) == 2 &&
               std::fwrite(&heat, 8, 1, f) == 1 && std::fwrite(q.data(), 8, nd, f) == nd &&
               std::fwrite(p.data(), 8, nd, f) == nd && std::fwrite(disp.data(), 8, nm, f) == nm &&
               std::fwrite(point.data(), 4, point.size(), f) == point.size() &&
@@ -845,14 +878,16 @@ int qtp_load_checkpoint(qtp_ctx *ctx, const char *path, const double *charge, co
     const int P = ctx->P;
     const size_t nd = (size_t)P * ctx->n_dof, nm = (size_t)P * 3 * ctx->n_mols;
     char magic[8];
-    int32_t hdr[4];
-    double par[5], heat = 0.0;
+    int32_t hdr[5], want_hdr[5];
+    double par[QTP_CKPT_NPAR], want_par[QTP_CKPT_NPAR], heat = 0.0;
+    ckpt_header(ctx, want_hdr, want_par);
     uint64_t u[2];
     std::vector<double> q(nd), p(nd), disp(nm), nreb(P);
     std::vector<int> point((size_t)P * ctx->n_mols), list((size_t)P * ctx->list_size);
     bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, QTP_CKPT_MAGIC, 8) == 0 &&
-              std::fread(hdr, 4, 4, f) == 4 && hdr[0] == 1 && hdr[1] == ctx->n_mols &&
-              hdr[2] == P && hdr[3] == ctx->list_size && std::fread(par, 8, 5, f) == 5 &&
+              std::fread(hdr, 4, 5, f) == 5 && hdr[0] == QTP_CKPT_VERSION &&
+              hdr[1] == ctx->n_mols && hdr[2] == P && hdr[3] == ctx->list_size &&
+              std::fread(par, 8, QTP_CKPT_NPAR, f) == (size_t)QTP_CKPT_NPAR &&
               std::fread(u, 8, 2, f) == 2 && std::fread(&heat, 8, 1, f) == 1 &&
               std::fread(q.data(), 8, nd, f) == nd && std::fread(p.data(), 8, nd, f) == nd &&
               std::fread(disp.data(), 8, nm, f) == nm &&
@@ -864,11 +899,18 @@ int qtp_load_checkpoint(qtp_ctx *ctx, const char *path, const double *charge, co
         ctx->err = std::string("not a checkpoint of this system: ") + path;
         return QTP_ERR_ARG;
     }
-    for (int k = 0; k < 3; ++k)
-        if (par[k] != ctx->drv.boxl[k]) {
-            ctx->err = "checkpoint box differs from the context's";
+    static const char *const what[QTP_CKPT_NPAR] = {
+        "box", "box", "box", "T_K", "dtfs", "eps_ewald", "epsilon_sc", "Ewald kappa",
+        "Ewald r_c", "Ewald k_cut"};
+    for (int k = 0; k < QTP_CKPT_NPAR; ++k)
+        if (par[k] != want_par[k]) {
+            ctx->err = std::string("checkpoint ") + what[k] + " differs from the context's";
             return QTP_ERR_ARG;
         }
+    if (hdr[4] != want_hdr[4]) {
+        ctx->err = "checkpoint Ewald k_max differs from the context's";
+        return QTP_ERR_ARG;
+    }
     SplineScope sp(ctx, ctx->spl_resident);
     CHK(check_finite(ctx, q.data(), (long)nd, "positions"));
     CHK(check_finite(ctx, p.data(), (long)nd, "momenta"));
@@ -885,10 +927,7 @@ int qtp_load_checkpoint(qtp_ctx *ctx, const char *path, const double *charge, co
     if (P >= 2) {
         CHK(run_sc_forces(ctx, 0, nullptr));
     } else {
-        CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, ctx->Ve));
-        HIPCHK(hipMemcpyAsync(ctx->F, ctx->fph, 3 * ctx->NAP * sizeof(double),
-                              hipMemcpyDeviceToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(ctx->out, ctx->Ve, sizeof(double), hipMemcpyDeviceToDevice, ctx->stream));
+        CHK(class_initial_forces(ctx));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     CHK(flush_timing(ctx));
@@ -944,7 +983,20 @@ static int step_once(qtp_ctx *ctx, int nvt)
 {
     SplineScope sp(ctx, ctx->spl_resident);
     if (ctx->P >= 2) return run_sc_step(ctx, nvt, 0, ctx->step_ctr);
-    return run_class_step(ctx, ctx->step_ctr);
+    return run_class_step(ctx, ctx->step_ctr, true, nvt);
+}
+
+// NVT flag of a resident step: SC steps always take it; classical steps only
+// with the Langevin thermostat (PILE-L kind), the one they implement.
+static int resident_nvt(qtp_ctx *ctx, int is_nvt, int *nvt)
+{
+    *nvt = is_nvt ? 1 : 0;
+    if (*nvt && ctx->P == 1 && !(ctx->cthermo == QTP_THERMO_PILE_L && ctx->tau_fs > 0.0)) {
+        ctx->err = "classical NVT steps take the Langevin thermostat: "
+                   "qtp_set_thermostat(ctx, QTP_THERMO_PILE_L, tau_fs > 0)";
+        return QTP_ERR_STATE;
+    }
+    return QTP_OK;
 }
 
 // One step of the resident state captured as a graph (not instantiated).
@@ -973,7 +1025,8 @@ int qtp_run(qtp_ctx *ctx, int n_steps, int is_nvt)
         ctx->err = "qtp_run: no resident state (call qtp_set_state)";
         return QTP_ERR_STATE;
     }
-    const int nvt = (is_nvt && ctx->P >= 2) ? 1 : 0;
+    int nvt = 0;
+    CHK(resident_nvt(ctx, is_nvt, &nvt));
     if (ctx->timing) {
         for (int i = 0; i < n_steps; ++i) CHK(step_once(ctx, nvt));
         CHK(flush_timing(ctx));
@@ -1027,7 +1080,11 @@ int qtp_batch_create(qtp_ctx *const *ctxs, const int *is_nvt, int n, qtp_batch *
             b->err = "batch member without resident state (or timing on)";
             return QTP_ERR_STATE;
         }
-        const int nvt = (is_nvt && is_nvt[i] && c->P >= 2) ? 1 : 0;
+        int nvt = 0;
+        if (resident_nvt(c, is_nvt && is_nvt[i], &nvt) != QTP_OK) {
+            b->err = "replica " + std::to_string(i) + ": " + c->err;
+            return QTP_ERR_STATE;
+        }
         // capture and instantiate the replica's step graph now (zero steps run)
         const int st = qtp_run(c, 0, nvt);
         if (st != QTP_OK) {
@@ -1115,7 +1172,8 @@ int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *t
         ctx->err = "qtp_run_observed: no resident state (call qtp_set_state)";
         return QTP_ERR_STATE;
     }
-    const int nvt = (is_nvt && ctx->P >= 2) ? 1 : 0;
+    int nvt = 0;
+    CHK(resident_nvt(ctx, is_nvt, &nvt));
     const int need = n_steps / every + 1;
     if (need > ctx->trace_cap) {
         HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1222,6 +1280,440 @@ int qtp_get_timing(qtp_ctx *ctx, const char *kernel, double *ms_total, int *laun
     auto it = ctx->timers.find(kernel);
     if (ms_total) *ms_total = it == ctx->timers.end() ? 0.0 : it->second.ms;
     if (launches) *launches = it == ctx->timers.end() ? 0 : it->second.n;
+    return QTP_OK;
+}
+
+// ---- learned centroid-force correction --------------------------------------------
+
+int qtp_nn_descriptor_params(double *out, int n)
+{
+    // r_rad, r_ang, mu0, dmu, eta_r (Angstrom), zeta[16], lambda[16], eta_a[16],
+    // w[2][2] (element-pair weights, O = 0, H = 1)
+    std::vector<double> v = {RR, RA, MU0, DMU, ETAR};
+    for (int t = 0; t < NANG; ++t) v.push_back((double)(1 << ((t >> 1) & 3)));
+    for (int t = 0; t < NANG; ++t) v.push_back((t & 1) ? -1.0 : 1.0);
+    for (int t = 0; t < NANG; ++t) v.push_back((t >> 3) ? (double)ETA_A1 : (double)ETA_A0);
+    v.push_back(1.0);
+    v.push_back(0.5);
+    v.push_back(1.0);
+    v.push_back(0.5);
+    if (out)
+        for (int t = 0; t < std::min(n, (int)v.size()); ++t) out[t] = v[t];
+    return (int)v.size();
+}
+
+int qtp_nn_create(int n_mols, const double boxl[3], int max_frames, qtp_nn **out)
+{
+    if (!out || n_mols < 2 || !boxl || max_frames < 1) return QTP_ERR_ARG;
+    qtp_nn *nn = new qtp_nn();
+    *out = nn;
+    nn->n_mols = n_mols;
+    nn->n_atoms = 3 * n_mols;
+    nn->NAP = ((nn->n_atoms + 63) / 64) * 64;
+    nn->max_frames = max_frames;
+    for (int k = 0; k < 3; ++k) {
+        if (!(boxl[k] > 0.0) || 2.0 * RR / ANGD > boxl[k]) {
+            nn->err = "box edges must exceed twice the 6 A descriptor cutoff";
+            return QTP_ERR_ARG;
+        }
+        nn->box.L[k] = boxl[k];
+        nn->box.inv[k] = 1.0 / boxl[k];
+    }
+    NNCHK(hipStreamCreate(&nn->stream));
+    const long NA = (long)max_frames * nn->n_atoms, N3 = 3L * nn->NAP * max_frames;
+    const NNBatch b = nn_batch(nn, max_frames);
+    int st = QTP_OK;
+    if ((st = nalloc(nn, &nn->theta, 2 * NPE)) || (st = nalloc(nn, &nn->am, 2 * NPE)) ||
+        (st = nalloc(nn, &nn->av, 2 * NPE)) || (st = nalloc(nn, &nn->grad, 2 * NPE)) ||
+        (st = nalloc(nn, &nn->w1, 2 * W1E)) || (st = nalloc(nn, &nn->w1t, 2 * W1E)) ||
+        (st = nalloc(nn, &nn->w2, 2 * W2E)) || (st = nalloc(nn, &nn->w2t, 2 * W2E)) ||
+        (st = nalloc(nn, &nn->b1, 2 * NH)) || (st = nalloc(nn, &nn->b2, 2 * NH)) ||
+        (st = nalloc(nn, &nn->w3, 2 * NH)) || (st = nalloc(nn, &nn->b3, 2)) ||
+        (st = nalloc(nn, &nn->mean, 2 * NIN)) || (st = nalloc(nn, &nn->stdv, 2 * NIN)) ||
+        (st = nalloc(nn, &nn->pos, N3)) || (st = nalloc(nn, &nn->tgt, N3)) ||
+        (st = nalloc(nn, &nn->fwork, N3)) || (st = nalloc(nn, &nn->dsum, max_frames)) ||
+        (st = nalloc(nn, &nn->gt, NA * NIN)) || (st = nalloc(nn, &nn->ut, NA * NIN)) ||
+        (st = nalloc(nn, &nn->dEdG, NA * NIN)) || (st = nalloc(nn, &nn->graw, NA * NIN)) ||
+        (st = nalloc(nn, &nn->Eatom, NA)) || (st = nalloc(nn, &nn->res, 3 * NA)) ||
+        (st = nalloc(nn, &nn->sq, NA)) || (st = nalloc(nn, &nn->nbr, NA * NMAX)) ||
+        (st = nalloc(nn, &nn->nn, NA)) || (st = nalloc(nn, &nn->nba, NA * NAMAX)) ||
+        (st = nalloc(nn, &nn->nna, NA)) || (st = nalloc(nn, &nn->status, 1)) ||
+        (st = nalloc(nn, &nn->tp, (size_t)(b.tiles_o + b.tiles_h) * 3 * NH)) ||
+        (st = nalloc(nn, &nn->part, (size_t)(b.nco + b.nch) * WG_PART)) ||
+        (st = nalloc(nn, &nn->inter, nn_inter_elems(b))))
+        return st;
+    std::vector<float> one(2 * NIN, 1.0f);
+    NNCHK(hipMemcpy(nn->stdv, one.data(), one.size() * sizeof(float), hipMemcpyHostToDevice));
+    const void *big[] = {(const void *)k_nn_mlp, (const void *)k_nn_train_fwd};
+    const int lds[] = {MLP_LDS, TRAIN_LDS};
+    for (int t = 0; t < 2; ++t)
+        NNCHK(hipFuncSetAttribute(big[t], hipFuncAttributeMaxDynamicSharedMemorySize, lds[t]));
+    return QTP_OK;
+}
+
+void qtp_nn_destroy(qtp_nn *nn)
+{
+    if (!nn) return;
+    if (nn->stream) hipStreamSynchronize(nn->stream);
+    void *ptrs[] = {nn->theta, nn->am, nn->av, nn->grad, nn->w1, nn->w1t, nn->w2, nn->w2t,
+                    nn->b1, nn->b2, nn->w3, nn->b3, nn->mean, nn->stdv, nn->pos, nn->tgt,
+                    nn->fwork, nn->dsum, nn->gt, nn->ut, nn->dEdG, nn->graw, nn->Eatom,
+                    nn->res, nn->sq, nn->nbr, nn->nn, nn->nba, nn->nna, nn->status, nn->tp,
+                    nn->part, nn->inter, nn->dpos, nn->dtgt};
+    for (void *p : ptrs)
+        if (p) hipFree(p);
+    if (nn->stream) hipStreamDestroy(nn->stream);
+    delete nn;
+}
+
+const char *qtp_nn_last_error(const qtp_nn *nn) { return nn ? nn->err.c_str() : "null model"; }
+
+int qtp_nn_set_model(qtp_nn *nn, const float *theta, const float *mean, const float *stdv,
+                     double e_scale)
+{
+    if (!nn || !theta || !mean || !stdv || !(e_scale > 0.0)) return QTP_ERR_ARG;
+    for (int t = 0; t < 2 * NPE; ++t)
+        if (!std::isfinite(theta[t])) {
+            nn->err = "parameters contain a non-finite value";
+            return QTP_ERR_NONFINITE;
+        }
+    for (int t = 0; t < 2 * NIN; ++t)
+        if (!std::isfinite(mean[t]) || !(stdv[t] > 0.0f)) {
+            nn->err = "normalisation: finite means and positive deviations required";
+            return QTP_ERR_ARG;
+        }
+    NNCHK(hipStreamSynchronize(nn->stream));
+    NNCHK(hipMemcpy(nn->theta, theta, 2 * NPE * sizeof(float), hipMemcpyHostToDevice));
+    NNCHK(hipMemcpy(nn->mean, mean, 2 * NIN * sizeof(float), hipMemcpyHostToDevice));
+    NNCHK(hipMemcpy(nn->stdv, stdv, 2 * NIN * sizeof(float), hipMemcpyHostToDevice));
+    NNCHK(hipMemset(nn->am, 0, 2 * NPE * sizeof(float)));
+    NNCHK(hipMemset(nn->av, 0, 2 * NPE * sizeof(float)));
+    nn->adam_t = 0;
+    nn->e_scale = e_scale;
+    k_nn_update<<<(2 * NPE + 255) / 256, 256, 0, nn->stream>>>(2, nullptr, 0, 0, nullptr, 0, 0, 0.0,
+                                                               nn->theta, nn->am, nn->av,
+                                                               AdamArgs{}, nullptr, nn_dev(nn));
+    NNCHK(hipGetLastError());
+    NNCHK(hipStreamSynchronize(nn->stream));
+    nn->model_set = true;
+    return QTP_OK;
+}
+
+int qtp_nn_get_model(qtp_nn *nn, float *theta)
+{
+    if (!nn || !theta) return QTP_ERR_ARG;
+    NNCHK(hipStreamSynchronize(nn->stream));
+    NNCHK(hipMemcpy(theta, nn->theta, 2 * NPE * sizeof(float), hipMemcpyDeviceToHost));
+    return QTP_OK;
+}
+
+static int nn_need_model(qtp_nn *nn)
+{
+    if (!nn->model_set) {
+        nn->err = "no model (qtp_nn_set_model)";
+        return QTP_ERR_STATE;
+    }
+    return QTP_OK;
+}
+
+int qtp_nn_eval(qtp_nn *nn, int n_frames, const double *pos, double *energy, double *forces,
+                float *descriptors)
+{
+    if (!nn || n_frames < 1 || !pos) return QTP_ERR_ARG;
+    CHK(nn_need_model(nn));
+    const long nd = 3L * nn->n_atoms, N3 = 3L * nn->NAP;
+    for (int f0 = 0; f0 < n_frames; f0 += nn->max_frames) {
+        const int F = std::min(nn->max_frames, n_frames - f0);
+        for (long t = 0; t < F * nd; ++t)
+            if (!std::isfinite(pos[f0 * nd + t])) {
+                nn->err = "positions contain a non-finite value";
+                return QTP_ERR_NONFINITE;
+            }
+        CHK(nn_upload_frames(nn, nn->pos, pos + f0 * nd, F));
+        const NNBatch b = nn_batch(nn, F);
+        const NNGeom g = nn_geom(nn, F);
+        hipStream_t s = nn->stream;
+        NNCHK(hipMemsetAsync(nn->status, 0, sizeof(int), s));
+        NNCHK(hipMemsetAsync(nn->fwork, 0, (size_t)F * N3 * sizeof(double), s));
+        k_nn_desc<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, nn->pos, nn->mean, nn->stdv, nn->gt,
+                                                            nn->nbr, nn->nn, nn->nba, nn->nna,
+                                                            descriptors ? nn->graw : nullptr,
+                                                            nn->status);
+        k_nn_mlp<<<b.nbo + b.nbh, 64 * MLP_WAVES, MLP_LDS, s>>>(g, nn_weights(nn), b.nbo, nn->gt,
+                                                               nn->stdv, nn->dEdG, nn->Eatom);
+        k_nn_force<0><<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, nn->pos, nn->nbr, nn->nn, nn->nba,
+                                                             nn->nna, nn->dEdG, nn->fwork, nullptr,
+                                                             nullptr, nullptr);
+        for (int f = 0; f < F; ++f)
+            k_nn_sum<<<1, 1024, 0, s>>>(nn->n_atoms, nn->Eatom + (long)f * nn->n_atoms, 1.0, 0,
+                                        nn->dsum + f);
+        NNCHK(hipGetLastError());
+        NNCHK(hipStreamSynchronize(s));
+        CHK(nn_read_status(nn));
+        if (energy) NNCHK(hipMemcpy(energy + f0, nn->dsum, F * sizeof(double), hipMemcpyDeviceToHost));
+        if (forces) {
+            std::vector<double> h((size_t)F * N3);
+            NNCHK(hipMemcpy(h.data(), nn->fwork, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+            for (int f = 0; f < F; ++f)
+                for (int a = 0; a < nn->n_atoms; ++a)
+                    for (int d = 0; d < 3; ++d)
+                        forces[((long)(f0 + f) * nn->n_atoms + a) * 3 + d] =
+                            h[f * N3 + (long)d * nn->NAP + a];
+        }
+        if (descriptors)
+            NNCHK(hipMemcpy(descriptors + (long)f0 * nn->n_atoms * NIN, nn->graw,
+                            (size_t)F * nn->n_atoms * NIN * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    return QTP_OK;
+}
+
+int qtp_nn_set_data(qtp_nn *nn, int n_frames, const double *pos, const double *target)
+{
+    if (!nn || n_frames < 1 || !pos || !target) return QTP_ERR_ARG;
+    const long nd = 3L * nn->n_atoms, N3 = 3L * nn->NAP;
+    for (long t = 0; t < n_frames * nd; ++t)
+        if (!std::isfinite(pos[t]) || !std::isfinite(target[t])) {
+            nn->err = "training data contain a non-finite value";
+            return QTP_ERR_NONFINITE;
+        }
+    NNCHK(hipStreamSynchronize(nn->stream));
+    if (nn->dpos) hipFree(nn->dpos);
+    if (nn->dtgt) hipFree(nn->dtgt);
+    nn->dpos = nn->dtgt = nullptr;
+    nn->n_data = 0;
+    CHK(nalloc(nn, &nn->dpos, (size_t)n_frames * N3));
+    CHK(nalloc(nn, &nn->dtgt, (size_t)n_frames * N3));
+    CHK(nn_upload_frames(nn, nn->dpos, pos, n_frames));
+    CHK(nn_upload_frames(nn, nn->dtgt, target, n_frames));
+    nn->n_data = n_frames;
+    return QTP_OK;
+}
+
+int qtp_nn_loss_grad(qtp_nn *nn, int frame0, int n_frames, double *loss, float *grad)
+{
+    if (!nn || frame0 < 0 || n_frames < 1 || n_frames > nn->max_frames) return QTP_ERR_ARG;
+    CHK(nn_need_model(nn));
+    if (frame0 + n_frames > nn->n_data) {
+        nn->err = "frames beyond the training data (qtp_nn_set_data)";
+        return QTP_ERR_ARG;
+    }
+    hipStream_t s = nn->stream;
+    NNCHK(hipMemsetAsync(nn->status, 0, sizeof(int), s));
+    CHK(nn_train_batch(nn, s, frame0, n_frames, 0, 0.0, nn->dsum));
+    NNCHK(hipStreamSynchronize(s));
+    CHK(nn_read_status(nn));
+    if (loss) NNCHK(hipMemcpy(loss, nn->dsum, sizeof(double), hipMemcpyDeviceToHost));
+    if (grad) NNCHK(hipMemcpy(grad, nn->grad, 2 * NPE * sizeof(float), hipMemcpyDeviceToHost));
+    return QTP_OK;
+}
+
+int qtp_nn_train(qtp_nn *nn, int n_steps, int batch_frames, double lr, uint64_t seed,
+                 double *loss_trace)
+{
+    if (!nn || n_steps < 0 || batch_frames < 1 || batch_frames > nn->max_frames || !(lr > 0.0))
+        return QTP_ERR_ARG;
+    CHK(nn_need_model(nn));
+    if (nn->n_data < batch_frames) {
+        nn->err = "fewer training frames than one batch (qtp_nn_set_data)";
+        return QTP_ERR_ARG;
+    }
+    const int nb = nn->n_data / batch_frames;
+    double *trace = nullptr;
+    CHK(nalloc(nn, &trace, std::max(1, n_steps)));
+    std::mt19937_64 rng(seed);
+    std::vector<int> order(nb);
+    hipStream_t s = nn->stream;
+    NNCHK(hipMemsetAsync(nn->status, 0, sizeof(int), s));
+    int st = QTP_OK;
+    for (int t = 0; t < n_steps && st == QTP_OK; ++t) {
+        if (t % nb == 0) {
+            for (int q = 0; q < nb; ++q) order[q] = q;
+            std::shuffle(order.begin(), order.end(), rng);
+        }
+        st = nn_train_batch(nn, s, order[t % nb] * batch_frames, batch_frames, 1, lr, trace + t);
+    }
+    if (st == QTP_OK) {
+        if (hipStreamSynchronize(s) != hipSuccess) st = QTP_ERR_HIP;
+        else st = nn_read_status(nn);
+    }
+    if (st == QTP_OK && loss_trace && n_steps > 0 &&
+        hipMemcpy(loss_trace, trace, n_steps * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+        st = QTP_ERR_HIP;
+    hipFree(trace);
+    return st;
+}
+
+int qtp_set_nn(qtp_ctx *ctx, qtp_nn *nn)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    if (nn) {
+        if (ctx->P != 1) {
+            ctx->err = "the learned correction drives classical (centroid) dynamics: n_beads = 1";
+            return QTP_ERR_ARG;
+        }
+        if (nn->n_mols != ctx->n_mols || nn->box.L[0] != ctx->box.L[0] ||
+            nn->box.L[1] != ctx->box.L[1] || nn->box.L[2] != ctx->box.L[2]) {
+            ctx->err = "model made for another system (molecule count or box)";
+            return QTP_ERR_ARG;
+        }
+        if (!nn->model_set) {
+            ctx->err = "model without parameters (qtp_nn_set_model)";
+            return QTP_ERR_STATE;
+        }
+        if (!ctx->nn_e) CHK(dalloc(ctx, &ctx->nn_e, 1));
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    invalidate_graphs(ctx);
+    ctx->nn = nn;
+    ctx->state_set = false;
+    return QTP_OK;
+}
+
+// ---- centroid dynamics, thermostats, mean forces, g(r) ------------------------------
+
+int qtp_set_thermostat(qtp_ctx *ctx, int kind, double tau_fs)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    if ((kind != QTP_THERMO_PILE_G && kind != QTP_THERMO_PILE_L) || !(tau_fs >= 0.0) ||
+        !std::isfinite(tau_fs) || (kind == QTP_THERMO_PILE_L && tau_fs == 0.0)) {
+        ctx->err = "thermostat: PILE_G with tau >= 0 or PILE_L with tau > 0";
+        return QTP_ERR_ARG;
+    }
+    if (ctx->P == 1 && kind != QTP_THERMO_PILE_L) {
+        ctx->err = "classical resident steps take the Langevin (PILE_L) thermostat only";
+        return QTP_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->cthermo = kind;
+    ctx->tau_fs = tau_fs;
+    invalidate_graphs(ctx);
+    return QTP_OK;
+}
+
+int qtp_set_pimd_mode(qtp_ctx *ctx, int mode, double omega_ad)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    if (mode != QTP_MODE_PIMD && mode != QTP_MODE_PACMD && mode != QTP_MODE_CENTROID_FIXED) {
+        ctx->err = "pimd mode: QTP_MODE_PIMD, QTP_MODE_PACMD or QTP_MODE_CENTROID_FIXED";
+        return QTP_ERR_ARG;
+    }
+    if (ctx->P < 2 && mode != QTP_MODE_PIMD) {
+        ctx->err = "centroid dynamics modes need n_beads >= 2";
+        return QTP_ERR_ARG;
+    }
+    double W = 0.0;
+    if (mode == QTP_MODE_PACMD) {
+        // Hone, Rossky, Voth, J. Chem. Phys. 124, 154103 (2006): W = P^(P/(P-1)) / (beta hbar)
+        const double P = ctx->P;
+        W = omega_ad > 0.0 ? omega_ad : std::pow(P, P / (P - 1.0)) / ctx->drv.beta;
+        if (!std::isfinite(W)) {
+            ctx->err = "adiabatic frequency not finite";
+            return QTP_ERR_ARG;
+        }
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->pimd_mode = mode;
+    ctx->omega_ad = W;
+    CHK(upload_modes(ctx));
+    invalidate_graphs(ctx);
+    return QTP_OK;
+}
+
+int qtp_centroid_force(qtp_ctx *ctx, int on)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    if (ctx->P < 2) {
+        ctx->err = "centroid mean force needs n_beads >= 2";
+        return QTP_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (on) {
+        HIPCHK(hipMemset(ctx->cmf_acc, 0, 3 * ctx->NAP * sizeof(double)));
+        HIPCHK(hipMemset(ctx->cmf_n, 0, sizeof(unsigned long long)));
+    }
+    if ((on != 0) != ctx->cmf_on) invalidate_graphs(ctx);
+    ctx->cmf_on = on != 0;
+    return QTP_OK;
+}
+
+int qtp_get_centroid_force(qtp_ctx *ctx, double *mean_force, double *centroid, int *n_samples)
+{
+    if (!ctx || !mean_force) return QTP_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    unsigned long long n = 0;
+    HIPCHK(hipMemcpy(&n, ctx->cmf_n, sizeof(n), hipMemcpyDeviceToHost));
+    std::vector<double> h(3 * (size_t)ctx->NAP);
+    HIPCHK(hipMemcpy(h.data(), ctx->cmf_acc, h.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (int a = 0; a < ctx->n_atoms; ++a)
+        for (int d = 0; d < 3; ++d)
+            mean_force[3 * a + d] = n ? h[(size_t)d * ctx->NAP + a] / (double)n : 0.0;
+    if (centroid) {
+        k_centroid<<<(3 * ctx->NAP + 255) / 256, 256, 0, ctx->stream>>>(ctx->P, ctx->NAP, ctx->q,
+                                                                       ctx->qc);
+        HIPCHK(hipGetLastError());
+        CHK(download_planar(ctx, ctx->qc, 1, centroid));
+    }
+    if (n_samples) *n_samples = (int)n;
+    return QTP_OK;
+}
+
+int qtp_rdf_reset(qtp_ctx *ctx, double r_max, int n_bins)
+{
+    if (!ctx || !(r_max > 0.0) || n_bins < 1 || n_bins > (1 << 20)) return QTP_ERR_ARG;
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (n_bins != ctx->rdf_bins) {
+        if (ctx->rdf_oo) hipFree(ctx->rdf_oo);
+        if (ctx->rdf_oh) hipFree(ctx->rdf_oh);
+        ctx->rdf_oo = ctx->rdf_oh = nullptr;
+        ctx->rdf_bins = 0;
+        CHK(dalloc(ctx, &ctx->rdf_oo, n_bins));
+        CHK(dalloc(ctx, &ctx->rdf_oh, n_bins));
+        ctx->rdf_bins = n_bins;
+    }
+    HIPCHK(hipMemset(ctx->rdf_oo, 0, n_bins * sizeof(unsigned int)));
+    HIPCHK(hipMemset(ctx->rdf_oh, 0, n_bins * sizeof(unsigned int)));
+    ctx->rdf_dr = r_max / n_bins;
+    ctx->rdf_n = 0;
+    return QTP_OK;
+}
+
+int qtp_rdf_sample(qtp_ctx *ctx)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    if (!ctx->rdf_bins || !ctx->state_set) {
+        ctx->err = "qtp_rdf_sample: no histogram (qtp_rdf_reset) or no resident state";
+        return QTP_ERR_STATE;
+    }
+    const double *x = ctx->q;
+    if (ctx->P > 1) {
+        k_centroid<<<(3 * ctx->NAP + 255) / 256, 256, 0, ctx->stream>>>(ctx->P, ctx->NAP, ctx->q,
+                                                                       ctx->qc);
+        x = ctx->qc;
+    }
+    k_rdf<<<(ctx->n_mols + 3) / 4, 256, 0, ctx->stream>>>(ctx->n_mols, ctx->NAP, x, ctx->box,
+                                                          1.0 / ctx->rdf_dr, ctx->rdf_bins,
+                                                          ctx->rdf_oo, ctx->rdf_oh);
+    HIPCHK(hipGetLastError());
+    ctx->rdf_n += 1;
+    return QTP_OK;
+}
+
+int qtp_rdf_get(qtp_ctx *ctx, double *oo, double *oh, int *n_samples)
+{
+    if (!ctx || !oo || !oh) return QTP_ERR_ARG;
+    if (!ctx->rdf_bins) {
+        ctx->err = "qtp_rdf_get: no histogram (qtp_rdf_reset)";
+        return QTP_ERR_STATE;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    std::vector<unsigned int> a(ctx->rdf_bins), b(ctx->rdf_bins);
+    HIPCHK(hipMemcpy(a.data(), ctx->rdf_oo, a.size() * sizeof(unsigned int), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(b.data(), ctx->rdf_oh, b.size() * sizeof(unsigned int), hipMemcpyDeviceToHost));
+    for (int t = 0; t < ctx->rdf_bins; ++t) {
+        oo[t] = a[t];
+        oh[t] = b[t];
+    }
+    if (n_samples) *n_samples = ctx->rdf_n;
     return QTP_OK;
 }
 

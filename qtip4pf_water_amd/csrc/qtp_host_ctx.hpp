@@ -14,6 +14,8 @@ struct Timer {
 
 }  // namespace
 
+struct qtp_nn;
+
 struct qtp_ctx {
     qtp_params prm{};
     qtp_derived drv{};
@@ -82,6 +84,22 @@ struct qtp_ctx {
     int2 *tiles = nullptr;
     double *Cm = nullptr;
     double4 *mode = nullptr;
+    double2 *ms = nullptr;   // per-mode propagation-mass factor {f, sqrt f}
+    // centroid dynamics and thermostat of the resident steps: pimd_mode 0
+    // PIMD, 1 partially adiabatic CMD at frequency omega_ad, 2 centroid held
+    // fixed; cthermo 0 PILE-G (Bussi, time constant tau_fs; 0 = the
+    // reference's pile_g), 1 PILE-L (Langevin, friction 1/tau_fs)
+    int pimd_mode = 0, cthermo = 0;
+    double omega_ad = 0.0, tau_fs = 0.0;
+    // centroid mean force accumulated over resident steps (k_cmf_acc)
+    bool cmf_on = false;
+    double *cmf_acc = nullptr;
+    unsigned long long *cmf_n = nullptr;
+    // radial distribution functions of the (centroid) positions
+    double *qc = nullptr;
+    unsigned int *rdf_oo = nullptr, *rdf_oh = nullptr;
+    int rdf_bins = 0, rdf_n = 0;
+    double rdf_dr = 0.0;
     // propagation scratch
     double *nq = nullptr, *np = nullptr, *ppart = nullptr, *hpart = nullptr, *dev = nullptr;
     double *obs = nullptr;
@@ -97,6 +115,9 @@ struct qtp_ctx {
     int *ntrace = nullptr;
     int trace_cap = 0, obs_every = 0;
     hipGraphExec_t graph_obs[2] = {nullptr, nullptr};
+    // learned centroid-force correction added to classical resident steps
+    qtp_nn *nn = nullptr;
+    double *nn_e = nullptr;   // its energy of the last evaluation
     bool timing = false;
     std::map<std::string, Timer> timers;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;

@@ -261,6 +261,12 @@ static PropArgs prop_args(qtp_ctx *ctx, int nvt, int host_dev)
     A.host_dev = host_dev;
     A.n_dof = ctx->n_dof;
     A.seed = ctx->seed;
+    A.cmode = ctx->pimd_mode == 2 ? 2 : 0;
+    A.cthermo = ctx->cthermo;
+    const double c = ctx->tau_fs > 0.0 ? std::exp(-ctx->prm.dtfs / ctx->tau_fs) : 0.0;
+    A.cbussi = ctx->cthermo == 0 ? c : 0.0;
+    A.c1_0 = c;
+    A.c2_0 = std::sqrt(1.0 - c * c);
     return A;
 }
 
@@ -272,7 +278,7 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
     const int grid = ctx->ptile * ctx->P;
     {
         TimeScope ts(ctx, "prop_a");
-        k_prop_fwd<<<grid, 64, 0, s>>>(A, ctx->Cm, ctx->mode, ctx->q, ctx->p, ctx->F, ctx->m_atom,
+        k_prop_fwd<<<grid, 64, 0, s>>>(A, ctx->Cm, ctx->mode, ctx->ms, ctx->q, ctx->p, ctx->F, ctx->m_atom,
                                        ctx->sqm_atom, ctx->dev, ctx->nq, ctx->np, ctx->ppart,
                                        ctx->hpart, ctr);
     }
@@ -285,26 +291,60 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
     HIPCHK(hipGetLastError());
     CHK(run_lists(ctx, 0, nvt != 0));
     CHK(run_sc_forces(ctx, 1, ctr));
+    if (ctx->cmf_on)
+        k_cmf_acc<<<(3 * ctx->NAP + 255) / 256, 256, 0, s>>>(ctx->P, ctx->NAP, ctx->F, ctx->cmf_acc,
+                                                            ctx->cmf_n);
+    HIPCHK(hipGetLastError());
     return QTP_OK;
 }
 
-// One classical velocity-Verlet step (update.cpp:78-137).
-static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr)
+// One classical velocity-Verlet step (update.cpp:78-137).  Resident steps
+// (resident) add the learned correction when a model is attached and, when
+// nvt, a Langevin step on every atom first.
+static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr, bool resident = false,
+                          int nvt = 0)
 {
     hipStream_t s = ctx->stream;
     const int nb = (3 * ctx->NAP + 255) / 256;
+    VVLang lang{};
+    if (resident && nvt) {
+        const double c = std::exp(-ctx->prm.dtfs / ctx->tau_fs);
+        lang.on = 1;
+        lang.c1 = c;
+        lang.c2 = std::sqrt(1.0 - c * c);
+        lang.seed = ctx->seed;
+    }
     {
         TimeScope ts(ctx, "vv_a");
         k_vv_a<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->NMP, ctx->drv.halfdt, ctx->drv.dt,
-                                  ctx->m_atom, ctx->F, ctx->q, ctx->p, ctx->disp);
+                                  ctx->m_atom, ctx->F, ctx->q, ctx->p, ctx->disp, lang,
+                                  ctx->sqm_atom, ctx->step_ctr);
     }
     CHK(run_lists(ctx, 0));
     CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, nullptr));
+    const bool with_nn = resident && ctx->nn;
+    if (with_nn) CHK(run_nn_forces(ctx));
     {
         TimeScope ts(ctx, "vv_b");
         k_vv_b<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->drv.halfdt, ctx->fph, ctx->epart,
-                                  ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p, ctx->out, ctr);
+                                  ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p, ctx->out,
+                                  ctr, with_nn ? ctx->nn_e : nullptr);
     }
+    HIPCHK(hipGetLastError());
+    return QTP_OK;
+}
+
+// Forces (with the learned correction when attached) and energy of the
+// resident classical state into F and out[0], no step: the set-up of
+// qtp_set_state / qtp_load_checkpoint.
+static int class_initial_forces(qtp_ctx *ctx)
+{
+    CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, nullptr));
+    if (ctx->nn) CHK(run_nn_forces(ctx));
+    const int nb = (3 * ctx->NAP + 255) / 256;
+    k_vv_b<<<nb, 256, 0, ctx->stream>>>(ctx->n_atoms, ctx->NAP, 0.0, ctx->fph, ctx->epart,
+                                        ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p,
+                                        ctx->out, nullptr, ctx->nn ? ctx->nn_e : nullptr);
     HIPCHK(hipGetLastError());
     return QTP_OK;
 }
@@ -447,6 +487,11 @@ static int read_status(qtp_ctx *ctx)
     if (st & 1) {
         ctx->err = "list size exceeded! (neighbour.cpp:56-58): increase list_size";
         return QTP_ERR_LIST_OVERFLOW;
+    }
+    if (st & 4) {
+        ctx->err = "learned correction: an atom has more neighbours than its descriptor lists "
+                   "hold (192 within 6 A, 64 within 4 A)";
+        return QTP_ERR_STATE;
     }
     return QTP_OK;
 }

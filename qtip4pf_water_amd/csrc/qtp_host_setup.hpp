@@ -259,6 +259,9 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->heat, 1));
     CHK(dalloc(ctx, &ctx->heat_step, 1));
     CHK(dalloc(ctx, &ctx->step_ctr, 1));
+    CHK(dalloc(ctx, &ctx->cmf_acc, (size_t)N3));
+    CHK(dalloc(ctx, &ctx->cmf_n, 1));
+    CHK(dalloc(ctx, &ctx->qc, (size_t)N3));
     CHK(dalloc(ctx, &ctx->nq, (size_t)P * N3));
     CHK(dalloc(ctx, &ctx->np, (size_t)P * N3));
     CHK(dalloc(ctx, &ctx->ppart, 2 * (size_t)ctx->ptile + 2));
@@ -268,6 +271,42 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->ntrace, 1));
     ctx->dev_cap = 3 + (long)(P > 1 ? P - 1 : 0) * ctx->n_dof;
     CHK(dalloc(ctx, &ctx->dev, (size_t)ctx->dev_cap));
+    return QTP_OK;
+}
+
+// Per-mode propagator and thermostat constants {scale, w^2, c1, c2 * sqrt(mass
+// factor)} and propagation-mass factors {f, sqrt f} (update.cpp:302-304,
+// thermostat.cpp:119-121).  Partially adiabatic CMD (pimd_mode 1) moves every
+// non-centroid mode to the frequency W with mass m (w_k / W)^2 (the spring
+// m w_k^2 is unchanged) and damps it critically, c1 = exp(-2 W dt).
+static int upload_modes(qtp_ctx *ctx)
+{
+    const int P = ctx->P;
+    std::vector<double4> md(P);
+    std::vector<double2> ms(P, make_double2(1.0, 1.0));
+    const double pre = 2.0 / (1.0 * ctx->drv.beta_n), pre_sq = pre * pre;
+    const double tpre = 4.0 / (ctx->drv.beta_n * 1.0);
+    const double dt = ctx->drv.dt;
+    const double W = ctx->omega_ad;
+    for (int k = 0; k < P; ++k) {
+        const double s = std::sin(k * ctx->drv.piovern);              // update.cpp:302
+        const double wsq = pre_sq * s * s;
+        if (ctx->pimd_mode == 1 && k > 0) {
+            const double f = wsq / (W * W);
+            const double c1 = std::exp(-2.0 * W * dt);
+            md[k] = make_double4(1.0 / std::sqrt(1.0 + 0.25 * W * W * dt * dt), W * W, c1,
+                                 std::sqrt(1.0 - c1 * c1) * std::sqrt(f));
+            ms[k] = make_double2(f, std::sqrt(f));
+            continue;
+        }
+        const double scale = 1.0 / std::sqrt(1.0 + 0.25 * wsq * dt * dt);
+        const double gk = tpre * std::sin(k * RC.pi / P);              // thermostat.cpp:119
+        const double c1 = std::exp(-dt * gk);
+        const double c2 = std::sqrt(1.0 - c1 * c1);
+        md[k] = make_double4(scale, wsq, c1, c2);
+    }
+    HIPCHK(hipMemcpy(ctx->mode, md.data(), P * sizeof(double4), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(ctx->ms, ms.data(), P * sizeof(double2), hipMemcpyHostToDevice));
     return QTP_OK;
 }
 
@@ -299,20 +338,7 @@ static int upload_constants(qtp_ctx *ctx)
     std::vector<double> C;
     build_nm(P, C);
     HIPCHK(hipMemcpy(ctx->Cm, C.data(), C.size() * sizeof(double), hipMemcpyHostToDevice));
-    std::vector<double4> md(P);
-    const double pre = 2.0 / (1.0 * ctx->drv.beta_n), pre_sq = pre * pre;
-    const double tpre = 4.0 / (ctx->drv.beta_n * 1.0);
-    const double dt = ctx->drv.dt;
-    for (int k = 0; k < P; ++k) {
-        const double s = std::sin(k * ctx->drv.piovern);              // update.cpp:302
-        const double wsq = pre_sq * s * s;
-        const double scale = 1.0 / std::sqrt(1.0 + 0.25 * wsq * dt * dt);
-        const double gk = tpre * std::sin(k * RC.pi / P);              // thermostat.cpp:119
-        const double c1 = std::exp(-dt * gk);
-        const double c2 = std::sqrt(1.0 - c1 * c1);
-        md[k] = make_double4(scale, wsq, c1, c2);
-    }
-    HIPCHK(hipMemcpy(ctx->mode, md.data(), P * sizeof(double4), hipMemcpyHostToDevice));
-    return QTP_OK;
+    return upload_modes(ctx);
 }
+
 

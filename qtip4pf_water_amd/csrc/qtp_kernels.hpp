@@ -6,3 +6,4 @@
 #include "qtp_kspace.hpp"
 #include "qtp_pair.hpp"
 #include "qtp_step.hpp"
+#include "qtp_nn.hpp"

@@ -1,0 +1,1021 @@
+// qtp_nn.hpp -- gfx950 device code of the learned centroid-force correction:
+// an atom-centred network whose atomic energies sum to an additive
+// correction V_corr of the qTIP4P/F potential, the place the reference's
+// centroid hook takes (the splined pair corrections of forces.cpp:452-466,
+// 634-691).  No counterpart in the reference otherwise.
+//
+// Descriptor of atom i (element e = 0 O, 1 H), positions in Angstrom:
+//   G_m    = sum_j w[e_i][e_j] exp(-ETAR (r_ij - mu_m)^2) fc(r_ij; RR),
+//            mu_m = MU0 + m DMU, m < 24                  (radial, r < 6 A)
+//   G_24+n = 2^(1-z) sum_{j<k} w_ij w_ik (1 + l cos t_jik)^z
+//            exp(-h (r_ij^2 + r_ik^2)) fc(r_ij; RA) fc(r_ik; RA)
+//            (angular, r < 4 A; n = 8 hi + 2 zi + li: h = {0.02, 0.2},
+//             z = {1, 2, 4, 8}, l = {+1, -1})
+//   fc(r; R) = (cos(pi r / R) + 1) / 2
+// (Behler-Parrinello symmetry functions with the element-pair weights of
+// weighted ACSFs), normalised per element, g = (G - mean) / std.  Network per
+// element: 40 -> 128 -> 128 -> 1, tanh, on BF16 matrix cores with FP32
+// accumulation (v_mfma_f32_16x16x32_bf16).  Descriptors, forces and tangents
+// are FP32; positions and the forces handed back are FP64.
+#pragma once
+#include "qtp_common.hpp"
+
+namespace qtp {
+namespace nn {
+
+constexpr int NR = 24, NANG = 16, NIN = 40, NH = 128, KIN = 64;
+constexpr int NMAX = 192;    // neighbours within RR per atom
+constexpr int NAMAX = 64;    // of which within RA
+constexpr int NPE = NH * NIN + NH + NH * NH + NH + NH + 1;  // parameters per element
+constexpr float ANGF = 0.529177210903f;                       // Angstrom per bohr
+constexpr double ANGD = 0.529177210903;
+constexpr float RR = 6.0f, RA = 4.0f, MU0 = 0.8f, DMU = 5.2f / 23.0f;
+constexpr float ETAR = 0.5f / (DMU * DMU);
+constexpr float ETA_A0 = 0.02f, ETA_A1 = 0.2f;
+constexpr float PIF = 3.14159265358979f;
+constexpr int DESC_WAVES = 4;    // atoms per block of the per-atom kernels
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float welem(int ei, int ej)
+{
+    // w[e_i][e_j]: O neighbours 1, H neighbours 1/2 (either centre)
+    (void)ei;
+    return ej == 0 ? 1.0f : 0.5f;
+}
+__device__ __forceinline__ int elem_of(int a) { return (a % 3) == 0 ? 0 : 1; }
+
+__device__ __forceinline__ float wave_sumf(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Frame f, atom a of the element-sorted row r (O rows first per element:
+// O rows (f, mol), H rows (f, mol, h)).
+__device__ __forceinline__ int row_atom(int e, int r, int n_mols, int n_frames, int n_atoms)
+{
+    if (e == 0) {
+        if (r >= n_frames * n_mols) return -1;
+        const int f = r / n_mols, m = r - f * n_mols;
+        return f * n_atoms + 3 * m;
+    }
+    if (r >= 2 * n_frames * n_mols) return -1;
+    const int f = r / (2 * n_mols), q = r - f * 2 * n_mols;
+    return f * n_atoms + 3 * (q >> 1) + 1 + (q & 1);
+}
+
+struct NNGeom {
+    int n_atoms, n_mols, n_frames, NAP;
+    Box box;              // bohr
+};
+
+// Minimum-image separation x_j - x_i (Angstrom, float) of atoms i, j of one
+// frame (planar FP64 positions, bohr).
+__device__ __forceinline__ float3 sep(const double *__restrict__ X, int NAP, const Box &box, int i,
+                                      int j)
+{
+    double dx = X[j] - X[i], dy = X[NAP + j] - X[NAP + i], dz = X[2 * NAP + j] - X[2 * NAP + i];
+    dx = mimg_fast(dx, box.L[0], box.inv[0]);
+    dy = mimg_fast(dy, box.L[1], box.inv[1]);
+    dz = mimg_fast(dz, box.L[2], box.inv[2]);
+    return make_float3((float)(dx * ANGD), (float)(dy * ANGD), (float)(dz * ANGD));
+}
+
+__device__ __forceinline__ float fcut(float r, float R)
+{
+    return r < R ? 0.5f * (__cosf(PIF * r / R) + 1.0f) : 0.0f;
+}
+__device__ __forceinline__ float dfcut(float r, float R)
+{
+    return r < R ? -0.5f * PIF / R * __sinf(PIF * r / R) : 0.0f;
+}
+
+// The 16 angular terms of one (j, k) pair and, when D, their partial
+// derivatives in cos t, r_ij (a) and r_ik (b).
+template <bool D>
+__device__ __forceinline__ void ang_terms(float c, float ra, float rb, float wt, float *T, float *Tc,
+                                          float *Ta, float *Tb)
+{
+    const float fa = fcut(ra, RA), fb = fcut(rb, RA);
+    const float s2 = ra * ra + rb * rb;
+    const float ex[2] = {__expf(-ETA_A0 * s2), __expf(-ETA_A1 * s2)};
+    const float eta[2] = {ETA_A0, ETA_A1};
+    float dfa = 0.f, dfb = 0.f;
+    if (D) {
+        dfa = dfcut(ra, RA);
+        dfb = dfcut(rb, RA);
+    }
+#pragma unroll
+    for (int l = 0; l < 2; ++l) {
+        const float base = l == 0 ? 1.0f + c : 1.0f - c;
+        const float sg = l == 0 ? 1.0f : -1.0f;
+        const float p1 = base, p2 = p1 * p1, p4 = p2 * p2, p8 = p4 * p4;
+        // 2^(1-z) base^z and 2^(1-z) z base^(z-1)
+        const float pw[4] = {p1, 0.5f * p2, 0.125f * p4, 0.0078125f * p8};
+        const float dp[4] = {1.0f, p1, 0.5f * p2 * p1, 0.0625f * p4 * p2 * p1};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+#pragma unroll
+            for (int z = 0; z < 4; ++z) {
+                const int n = 8 * h + 2 * z + l;
+                const float e = ex[h] * wt;
+                T[n] = pw[z] * e * fa * fb;
+                if (D) {
+                    Tc[n] = sg * dp[z] * e * fa * fb;
+                    Ta[n] = pw[z] * e * fb * (dfa - 2.0f * eta[h] * ra * fa);
+                    Tb[n] = pw[z] * e * fa * (dfb - 2.0f * eta[h] * rb * fb);
+                }
+            }
+        }
+    }
+}
+
+// Pair (j, k), j < k, of the upper triangle enumerated as p = k (k-1)/2 + j.
+__device__ __forceinline__ void tri_decode(int p, int &j, int &k)
+{
+    int kk = (int)((1.0f + sqrtf(1.0f + 8.0f * (float)p)) * 0.5f);
+    while (kk * (kk - 1) / 2 > p) --kk;
+    while ((kk + 1) * kk / 2 <= p) ++kk;
+    k = kk;
+    j = p - kk * (kk - 1) / 2;
+}
+
+// ---------------------------------------------------------------------------
+// Neighbour lists and normalised descriptors.  One wavefront per atom,
+// DESC_WAVES atoms per block.  Lists hold, in ascending index order, every
+// atom within RR (nbr, count nn) and the subset within RA (nba, count nna).
+// status bit 2: a list was full (the atom's lists are then truncated).
+// grid: ceil(n_frames * n_atoms / DESC_WAVES), 64 * DESC_WAVES threads.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_desc(
+    NNGeom g, const double *__restrict__ pos, const float *__restrict__ mean,
+    const float *__restrict__ stdv, float *__restrict__ gt, int *__restrict__ nbr,
+    int *__restrict__ nn, int *__restrict__ nba, int *__restrict__ nna, float *__restrict__ graw,
+    int *__restrict__ status)
+{
+    __shared__ float4 sv[DESC_WAVES][NMAX];   // x_j - x_i (A), r
+    __shared__ int sj[DESC_WAVES][NMAX];      // neighbour atom
+    __shared__ int sa[DESC_WAVES][NAMAX];     // angular neighbours: slot in sv
+    __shared__ float sg[DESC_WAVES][NIN];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gi = blockIdx.x * DESC_WAVES + w;
+    const int N = g.n_frames * g.n_atoms;
+    if (gi >= N) return;   // whole wavefronts leave together; no block barrier below
+    const int f = gi / g.n_atoms, i = gi - f * g.n_atoms;
+    const double *X = pos + (long)f * 3 * g.NAP;
+    const int ei = elem_of(i);
+    const float rr2 = RR * RR, ra2 = RA * RA;
+    int cnt = 0, cnta = 0, over = 0;
+    for (int j0 = 0; j0 < g.n_atoms; j0 += 64) {
+        const int j = j0 + lane;
+        float3 d = make_float3(0.f, 0.f, 0.f);
+        float r2 = 1e30f;
+        if (j < g.n_atoms && j != i) {
+            d = sep(X, g.NAP, g.box, i, j);
+            r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+        }
+        const bool in = r2 < rr2;
+        const unsigned long long b = __ballot(in);
+        const int s = cnt + __popcll(b & ((1ull << lane) - 1ull));
+        const bool keep = in && s < NMAX;
+        const bool ina = keep && r2 < ra2;
+        const unsigned long long ba = __ballot(ina);
+        const int sa_slot = cnta + __popcll(ba & ((1ull << lane) - 1ull));
+        if (keep) {
+            sv[w][s] = make_float4(d.x, d.y, d.z, sqrtf(r2));
+            sj[w][s] = j;
+            nbr[(long)gi * NMAX + s] = j;
+            if (ina && sa_slot < NAMAX) {
+                sa[w][sa_slot] = s;
+                nba[(long)gi * NAMAX + sa_slot] = j;
+            }
+        }
+        over |= (cnt + __popcll(b) > NMAX) || (cnta + __popcll(ba) > NAMAX);
+        cnt += __popcll(b);
+        cnta += __popcll(ba);
+    }
+    cnt = min(cnt, NMAX);
+    cnta = min(cnta, NAMAX);
+    if (lane == 0) {
+        nn[gi] = cnt;
+        nna[gi] = cnta;
+        if (over) atomicOr(status, 4);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // radial
+    float acc[NR];
+#pragma unroll
+    for (int m = 0; m < NR; ++m) acc[m] = 0.f;
+    for (int s = lane; s < cnt; s += 64) {
+        const float r = sv[w][s].w;
+        const float wf = welem(ei, elem_of(sj[w][s])) * fcut(r, RR);
+#pragma unroll
+        for (int m = 0; m < NR; ++m) {
+            const float x = r - (MU0 + m * DMU);
+            acc[m] += wf * __expf(-ETAR * x * x);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < NR; ++m) {
+        const float v = wave_sumf(acc[m]);
+        if (lane == m) sg[w][m] = v;
+    }
+    // angular
+    float T[NANG];
+    float ang[NANG];
+#pragma unroll
+    for (int n = 0; n < NANG; ++n) ang[n] = 0.f;
+    const int npair = cnta * (cnta - 1) / 2;
+    for (int p = lane; p < npair; p += 64) {
+        int j, k;
+        tri_decode(p, j, k);
+        const float4 A = sv[w][sa[w][j]], B = sv[w][sa[w][k]];
+        const int aj = sj[w][sa[w][j]], ak = sj[w][sa[w][k]];
+        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
+        const float wt = welem(ei, elem_of(aj)) * welem(ei, elem_of(ak));
+        ang_terms<false>(c, A.w, B.w, wt, T, nullptr, nullptr, nullptr);
+#pragma unroll
+        for (int n = 0; n < NANG; ++n) ang[n] += T[n];
+    }
+#pragma unroll
+    for (int n = 0; n < NANG; ++n) {
+        const float v = wave_sumf(ang[n]);
+        if (lane == n) sg[w][NR + n] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < NIN) {
+        const float G = sg[w][lane];
+        gt[(long)gi * NIN + lane] = (G - mean[ei * NIN + lane]) / stdv[ei * NIN + lane];
+        if (graw) graw[(long)gi * NIN + lane] = G;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// BF16 matrix-core pieces.  v_mfma_f32_16x16x32_bf16: lane l holds
+//   A: row l%16, k = 8 (l/16) + 0..7;  B: column l%16, k = 8 (l/16) + 0..7;
+//   C: column l%16, rows 4 (l/16) + 0..3.
+// Weights of one element, as BF16 rows of 16-byte groups:
+//   w1 [128][64] (W1, inputs padded to 64), w1t [64][128] (its transpose),
+//   w2 [128][128], w2t [128][128].
+// ---------------------------------------------------------------------------
+struct NNW {
+    const __bf16 *w1, *w1t, *w2, *w2t;   // [2][...] per element
+    const float *b1, *b2, *w3, *b3;       // [2][128], [2][128], [2][128], [2]
+    float e_scale;
+};
+constexpr int W1E = NH * KIN, W2E = NH * NH;   // bf16 elements per element matrix
+
+__device__ __forceinline__ f32x4 mfma16(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ bf16x8 lds8(const __bf16 *p)
+{
+    return *reinterpret_cast<const bf16x8 *>(p);
+}
+
+// Load an A fragment of 16 rows x 32 k (k-step s) from a float matrix
+// src[row][ld] (rows of the tile: atom index or -1), zero beyond kmax.
+__device__ __forceinline__ bf16x8 afrag_f32(const float *__restrict__ src, long row, int ld,
+                                            int k0, int kmax)
+{
+    bf16x8 a;
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+        const int k = k0 + t;
+        a[t] = (__bf16)((row >= 0 && k < kmax) ? src[row * ld + k] : 0.0f);
+    }
+    return a;
+}
+
+// Stage one element's weights in LDS (block-cooperative, 16-byte copies).
+__device__ __forceinline__ void stage_weights(const NNW &W, int e, __bf16 *s_w1, __bf16 *s_w1t,
+                                              __bf16 *s_w2, __bf16 *s_w2t)
+{
+    const uint4 *g1 = reinterpret_cast<const uint4 *>(W.w1 + (long)e * W1E);
+    const uint4 *g1t = reinterpret_cast<const uint4 *>(W.w1t + (long)e * W1E);
+    const uint4 *g2 = reinterpret_cast<const uint4 *>(W.w2 + (long)e * W2E);
+    const uint4 *g2t = reinterpret_cast<const uint4 *>(W.w2t + (long)e * W2E);
+    uint4 *d1 = reinterpret_cast<uint4 *>(s_w1), *d1t = reinterpret_cast<uint4 *>(s_w1t);
+    uint4 *d2 = reinterpret_cast<uint4 *>(s_w2), *d2t = reinterpret_cast<uint4 *>(s_w2t);
+    for (int t = threadIdx.x; t < W1E / 8; t += blockDim.x) {
+        d1[t] = g1[t];
+        d1t[t] = g1t[t];
+    }
+    for (int t = threadIdx.x; t < W2E / 8; t += blockDim.x) {
+        d2[t] = g2[t];
+        d2t[t] = g2t[t];
+    }
+}
+
+// C-layout values (4 rows x 8 column tiles per lane) -> a 16 x 128 BF16 tile
+// in LDS (row-major), then A fragments of it for k-step s.
+__device__ __forceinline__ void c_to_lds(const float (*v)[4], __bf16 *tile, int lane)
+{
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) tile[(4 * (lane >> 4) + r) * NH + 16 * t + (lane & 15)] = (__bf16)v[t][r];
+}
+
+constexpr int MLP_WAVES = 4;   // 16-row tiles per block
+constexpr int MLP_LDS = 2 * (2 * W1E + 2 * W2E) + MLP_WAVES * 16 * NH * 2;  // bytes
+
+// Per-atom energies and dE/dG (G the raw descriptor) for 16-row tiles of one
+// element.  Blocks [0, nbo) take O tiles, the rest H tiles (each element's
+// tile count padded to a multiple of MLP_WAVES).
+// grid: nbo + nbh blocks of 64 * MLP_WAVES threads, MLP_LDS bytes of LDS.
+__global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
+    NNGeom g, NNW W, int nbo, const float *__restrict__ gt, const float *__restrict__ stdv,
+    float *__restrict__ dEdG, float *__restrict__ Eatom)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __bf16 *s_w1 = reinterpret_cast<__bf16 *>(smem);
+    __bf16 *s_w1t = s_w1 + W1E;
+    __bf16 *s_w2 = s_w1t + W1E;
+    __bf16 *s_w2t = s_w2 + W2E;
+    __bf16 *s_tile = s_w2t + W2E;
+    const int e = blockIdx.x < nbo ? 0 : 1;
+    const int tile = (e == 0 ? blockIdx.x : blockIdx.x - nbo) * MLP_WAVES + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    __bf16 *tl = s_tile + (threadIdx.x >> 6) * 16 * NH;
+    stage_weights(W, e, s_w1, s_w1t, s_w2, s_w2t);
+    __syncthreads();
+    const int ra = row_atom(e, tile * 16 + (lane & 15), g.n_mols, g.n_frames, g.n_atoms);
+    const int col = lane & 15, kq = 8 * (lane >> 4);
+    // layer 1: z1 = g W1^T
+    float a1[8][4];
+    {
+        f32x4 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const bf16x8 a = afrag_f32(gt, ra, NIN, 32 * s + kq, NIN);
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                acc[t] = mfma16(a, lds8(s_w1 + (16 * t + col) * KIN + 32 * s + kq), acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float b = W.b1[e * NH + 16 * t + col];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a1[t][r] = tanhf(acc[t][r] + b);
+        }
+    }
+    c_to_lds(a1, tl, lane);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // layer 2: z2 = a1 W2^T; energy; dE/dz2
+    float dz2[8][4];
+    {
+        f32x4 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const bf16x8 a = lds8(tl + col * NH + 32 * s + kq);
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                acc[t] = mfma16(a, lds8(s_w2 + (16 * t + col) * NH + 32 * s + kq), acc[t]);
+        }
+        float er[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float b = W.b2[e * NH + 16 * t + col], w3 = W.w3[e * NH + 16 * t + col];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float a2 = tanhf(acc[t][r] + b);
+                er[r] += w3 * a2;
+                dz2[t][r] = W.e_scale * w3 * (1.0f - a2 * a2);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = er[r];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            v += __shfl_xor(v, 8, 64);
+            const int rr = row_atom(e, tile * 16 + 4 * (lane >> 4) + r, g.n_mols, g.n_frames,
+                                    g.n_atoms);
+            if (col == 0 && rr >= 0 && Eatom) Eatom[rr] = W.e_scale * (v + W.b3[e]);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    c_to_lds(dz2, tl, lane);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // dE/da1 = dz2 W2, dE/dz1 = (1 - a1^2) dE/da1
+    {
+        f32x4 acc[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const bf16x8 a = lds8(tl + col * NH + 32 * s + kq);
+#pragma unroll
+            for (int t = 0; t < 8; ++t)
+                acc[t] = mfma16(a, lds8(s_w2t + (16 * t + col) * NH + 32 * s + kq), acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) a1[t][r] = (1.0f - a1[t][r] * a1[t][r]) * acc[t][r];
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    c_to_lds(a1, tl, lane);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // dE/dg = dz1 W1 (columns 0..47), dE/dG = dE/dg / std
+    {
+        f32x4 acc[3];
+#pragma unroll
+        for (int t = 0; t < 3; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const bf16x8 a = lds8(tl + col * NH + 32 * s + kq);
+#pragma unroll
+            for (int t = 0; t < 3; ++t)
+                acc[t] = mfma16(a, lds8(s_w1t + (16 * t + col) * NH + 32 * s + kq), acc[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int d = 16 * t + col;
+            if (d >= NIN) continue;
+            const float is = 1.0f / stdv[e * NIN + d];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int rr = row_atom(e, tile * 16 + 4 * (lane >> 4) + r, g.n_mols, g.n_frames,
+                                        g.n_atoms);
+                if (rr >= 0) dEdG[(long)rr * NIN + d] = acc[t][r] * is;
+            }
+        }
+    }
+}
+
+
+// d(cos t)/dA and d(cos t)/dB for A = x_j - x_i, B = x_k - x_i.
+__device__ __forceinline__ void dcos(float4 A, float4 B, float c, float3 &dA, float3 &dB)
+{
+    const float iab = 1.0f / (A.w * B.w), ia2 = c / (A.w * A.w), ib2 = c / (B.w * B.w);
+    dA = make_float3(B.x * iab - ia2 * A.x, B.y * iab - ia2 * A.y, B.z * iab - ia2 * A.z);
+    dB = make_float3(A.x * iab - ib2 * B.x, A.y * iab - ib2 * B.y, A.z * iab - ib2 * B.z);
+}
+
+__device__ __forceinline__ float4 sepr(const double *X, int NAP, const Box &box, int i, int j)
+{
+    const float3 d = sep(X, NAP, box, i, j);
+    return make_float4(d.x, d.y, d.z, sqrtf(d.x * d.x + d.y * d.y + d.z * d.z));
+}
+
+// Radial derivative d phi_m / dr of exp(-ETAR (r - mu_m)^2) fc(r; RR).
+__device__ __forceinline__ float dphi(int m, float r, float fc, float dfc)
+{
+    const float x = r - (MU0 + m * DMU);
+    return __expf(-ETAR * x * x) * (dfc - 2.0f * ETAR * x * fc);
+}
+
+// ---------------------------------------------------------------------------
+// Forces of the correction, F_a = -dE/dx_a, gathered per atom a (one
+// wavefront per atom, fixed summation order, no atomics):
+//   radial:  dE/dx_a = -sum_j [sum_m (e_am w_aj + e_jm w_ja) phi'_m(r_aj)] u_aj
+//   angular: own pairs (j, k):     -(dE/dA + dE/dB) with e_a
+//            as a neighbour of i:  dE/dA of T(i; a, k) with e_i, every k != a
+// with e = dE/dG and u_aj the unit vector from a to j.  MODE 0 adds F to an
+// FP64 planar force array; MODE 1 writes the residual F - target (training)
+// and the atom's squared residual.
+// grid: ceil(n_frames * n_atoms / DESC_WAVES), 64 * DESC_WAVES threads.
+// ---------------------------------------------------------------------------
+template <int MODE>
+__global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_force(
+    NNGeom g, const double *__restrict__ pos, const int *__restrict__ nbr,
+    const int *__restrict__ nn, const int *__restrict__ nba, const int *__restrict__ nna,
+    const float *__restrict__ dEdG, double *__restrict__ fout, const double *__restrict__ target,
+    float *__restrict__ res, float *__restrict__ sq)
+{
+    __shared__ float4 sv[DESC_WAVES][NAMAX];
+    __shared__ int sj[DESC_WAVES][NAMAX];
+    __shared__ int soff[DESC_WAVES][NAMAX + 1];
+    __shared__ float se[DESC_WAVES][NIN];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ga = blockIdx.x * DESC_WAVES + w;
+    if (ga >= g.n_frames * g.n_atoms) return;
+    const int f = ga / g.n_atoms, a = ga - f * g.n_atoms, base = f * g.n_atoms;
+    const double *X = pos + (long)f * 3 * g.NAP;
+    const int ea = elem_of(a);
+    if (lane < NIN) se[w][lane] = dEdG[(long)ga * NIN + lane];
+    const int cnt = nn[ga], cnta = nna[ga];
+    if (lane < cnta) {
+        const int j = nba[(long)ga * NAMAX + lane];
+        sj[w][lane] = j;
+        sv[w][lane] = sepr(X, g.NAP, g.box, a, j);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane == 0) {
+        int o = 0;
+        for (int t = 0; t < cnta; ++t) {
+            soff[w][t] = o;
+            o += nna[base + sj[w][t]];
+        }
+        soff[w][cnta] = o;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    float gx = 0.f, gy = 0.f, gz = 0.f;   // dE/dx_a (Angstrom)
+    // radial
+    for (int s = lane; s < cnt; s += 64) {
+        const int j = nbr[(long)ga * NMAX + s];
+        const float4 d = sepr(X, g.NAP, g.box, a, j);
+        const float fc = fcut(d.w, RR), dfc = dfcut(d.w, RR);
+        const int ej = elem_of(j);
+        const float waj = welem(ea, ej), wja = welem(ej, ea);
+        const float *ejv = dEdG + (long)(base + j) * NIN;
+        float c = 0.f;
+#pragma unroll
+        for (int m = 0; m < NR; ++m) c += (se[w][m] * waj + ejv[m] * wja) * dphi(m, d.w, fc, dfc);
+        const float ir = c / d.w;
+        gx -= ir * d.x;
+        gy -= ir * d.y;
+        gz -= ir * d.z;
+    }
+    // own angular pairs
+    float T[NANG], Tc[NANG], Ta[NANG], Tb[NANG];
+    const int npair = cnta * (cnta - 1) / 2;
+    for (int p = lane; p < npair; p += 64) {
+        int j, k;
+        tri_decode(p, j, k);
+        const float4 A = sv[w][j], B = sv[w][k];
+        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
+        const float wt = welem(ea, elem_of(sj[w][j])) * welem(ea, elem_of(sj[w][k]));
+        ang_terms<true>(c, A.w, B.w, wt, T, Tc, Ta, Tb);
+        float Sc = 0.f, Sa = 0.f, Sb = 0.f;
+#pragma unroll
+        for (int n = 0; n < NANG; ++n) {
+            const float e = se[w][NR + n];
+            Sc += e * Tc[n];
+            Sa += e * Ta[n];
+            Sb += e * Tb[n];
+        }
+        float3 dA, dB;
+        dcos(A, B, c, dA, dB);
+        const float sa = Sa / A.w, sb = Sb / B.w;
+        gx -= Sc * (dA.x + dB.x) + sa * A.x + sb * B.x;
+        gy -= Sc * (dA.y + dB.y) + sa * A.y + sb * B.y;
+        gz -= Sc * (dA.z + dB.z) + sa * A.z + sb * B.z;
+    }
+    // a as a neighbour of each angular neighbour i: T(i; a, k), k != a
+    const int ntot = soff[w][cnta];
+    for (int q = lane; q < ntot; q += 64) {
+        int t = 0;
+        while (soff[w][t + 1] <= q) ++t;
+        const int i = sj[w][t], gi = base + i;
+        const int k = nba[(long)gi * NAMAX + (q - soff[w][t])];
+        if (k == a) continue;
+        const float4 A = sepr(X, g.NAP, g.box, i, a), B = sepr(X, g.NAP, g.box, i, k);
+        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
+        const int ei = elem_of(i);
+        const float wt = welem(ei, ea) * welem(ei, elem_of(k));
+        ang_terms<true>(c, A.w, B.w, wt, T, Tc, Ta, Tb);
+        const float *eiv = dEdG + (long)gi * NIN + NR;
+        float Sc = 0.f, Sa = 0.f;
+#pragma unroll
+        for (int n = 0; n < NANG; ++n) {
+            const float e = eiv[n];
+            Sc += e * Tc[n];
+            Sa += e * Ta[n];
+        }
+        float3 dA, dB;
+        dcos(A, B, c, dA, dB);
+        const float sa = Sa / A.w;
+        gx += Sc * dA.x + sa * A.x;
+        gy += Sc * dA.y + sa * A.y;
+        gz += Sc * dA.z + sa * A.z;
+    }
+    gx = wave_sumf(gx);
+    gy = wave_sumf(gy);
+    gz = wave_sumf(gz);
+    if (lane == 0) {
+        const double F[3] = {-ANGD * gx, -ANGD * gy, -ANGD * gz};
+        double *fo = fout ? fout + (long)f * 3 * g.NAP : nullptr;
+        if (MODE == 0) {
+            for (int d = 0; d < 3; ++d) fo[d * g.NAP + a] += F[d];
+        } else {
+            const double *tg = target + (long)f * 3 * g.NAP;
+            float s2 = 0.f;
+            for (int d = 0; d < 3; ++d) {
+                const float r = (float)(F[d] - tg[d * g.NAP + a]);
+                res[(long)ga * 3 + d] = r;
+                s2 += r * r;
+                if (fo) fo[d * g.NAP + a] = F[d];
+            }
+            sq[ga] = s2;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Tangent of the normalised descriptor along the residual field R (training):
+//   u_i = ANG * (dG_i/dx . R) / std     (R per atom, Hartree/bohr)
+// radial:  sum_j w phi'_m(r_ij) u_ij . (R_j - R_i)
+// angular: sum_{j<k} Tc dc + Ta da + Tb db, dc = dcos/dA . dA + dcos/dB . dB,
+//          da = A . (R_j - R_i) / r_ij, db = B . (R_k - R_i) / r_ik
+// grid: ceil(n_frames * n_atoms / DESC_WAVES), 64 * DESC_WAVES threads.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_tan(
+    NNGeom g, const double *__restrict__ pos, const int *__restrict__ nbr,
+    const int *__restrict__ nn, const int *__restrict__ nba, const int *__restrict__ nna,
+    const float *__restrict__ res, const float *__restrict__ stdv, float *__restrict__ ut)
+{
+    __shared__ float4 sv[DESC_WAVES][NAMAX];
+    __shared__ float4 sr[DESC_WAVES][NAMAX];   // R_j - R_i of the angular neighbours
+    __shared__ int sj[DESC_WAVES][NAMAX];
+    __shared__ float su[DESC_WAVES][NIN];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gi = blockIdx.x * DESC_WAVES + w;
+    if (gi >= g.n_frames * g.n_atoms) return;
+    const int f = gi / g.n_atoms, i = gi - f * g.n_atoms, base = f * g.n_atoms;
+    const double *X = pos + (long)f * 3 * g.NAP;
+    const int ei = elem_of(i);
+    const float Rix = res[(long)gi * 3], Riy = res[(long)gi * 3 + 1], Riz = res[(long)gi * 3 + 2];
+    const int cnt = nn[gi], cnta = nna[gi];
+    if (lane < cnta) {
+        const int j = nba[(long)gi * NAMAX + lane];
+        sj[w][lane] = j;
+        sv[w][lane] = sepr(X, g.NAP, g.box, i, j);
+        const long o = (long)(base + j) * 3;
+        sr[w][lane] = make_float4(res[o] - Rix, res[o + 1] - Riy, res[o + 2] - Riz, 0.f);
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    float acc[NIN];
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) acc[m] = 0.f;
+    for (int s = lane; s < cnt; s += 64) {
+        const int j = nbr[(long)gi * NMAX + s];
+        const float4 d = sepr(X, g.NAP, g.box, i, j);
+        const long o = (long)(base + j) * 3;
+        const float dr = (d.x * (res[o] - Rix) + d.y * (res[o + 1] - Riy) + d.z * (res[o + 2] - Riz)) / d.w;
+        const float wf = welem(ei, elem_of(j)) * dr;
+        const float fc = fcut(d.w, RR), dfc = dfcut(d.w, RR);
+#pragma unroll
+        for (int m = 0; m < NR; ++m) acc[m] += wf * dphi(m, d.w, fc, dfc);
+    }
+    float T[NANG], Tc[NANG], Ta[NANG], Tb[NANG];
+    const int npair = cnta * (cnta - 1) / 2;
+    for (int p = lane; p < npair; p += 64) {
+        int j, k;
+        tri_decode(p, j, k);
+        const float4 A = sv[w][j], B = sv[w][k], dA4 = sr[w][j], dB4 = sr[w][k];
+        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
+        const float wt = welem(ei, elem_of(sj[w][j])) * welem(ei, elem_of(sj[w][k]));
+        ang_terms<true>(c, A.w, B.w, wt, T, Tc, Ta, Tb);
+        float3 cA, cB;
+        dcos(A, B, c, cA, cB);
+        const float dc = cA.x * dA4.x + cA.y * dA4.y + cA.z * dA4.z + cB.x * dB4.x + cB.y * dB4.y +
+                         cB.z * dB4.z;
+        const float da = (A.x * dA4.x + A.y * dA4.y + A.z * dA4.z) / A.w;
+        const float db = (B.x * dB4.x + B.y * dB4.y + B.z * dB4.z) / B.w;
+#pragma unroll
+        for (int n = 0; n < NANG; ++n) acc[NR + n] += Tc[n] * dc + Ta[n] * da + Tb[n] * db;
+    }
+#pragma unroll
+    for (int m = 0; m < NIN; ++m) {
+        const float v = wave_sumf(acc[m]);
+        if (lane == m) su[w][m] = v;
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (lane < NIN) ut[(long)gi * NIN + lane] = ANGF * su[w][lane] / stdv[ei * NIN + lane];
+}
+
+// Sum of per-atom values in a fixed order (one block): out[0] = scale * sum
+// (+ out[0] when accumulate).  grid: 1, 1024 threads.
+__global__ __launch_bounds__(1024) void k_nn_sum(int n, const float *__restrict__ v, double scale,
+                                                 int accumulate, double *__restrict__ out)
+{
+    __shared__ double sh[16];
+    double s = 0.0;
+    for (int t = threadIdx.x; t < n; t += 1024) s += (double)v[t];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[0] = (accumulate ? out[0] : 0.0) + scale * s;
+}
+
+// ---------------------------------------------------------------------------
+// Training, forward-over-reverse on 16-row tiles (one wavefront per tile):
+// with z1 = W1 g + b1, a1 = tanh z1, s1 = 1 - a1^2, the tangent zd1 = W1 u,
+// ad1 = s1 zd1, likewise z2, a2, s2, zd2, ad2 = s2 zd2, and
+// J = e_scale sum_rows w3 . ad2, the adjoints
+//   d_ad2 = e_scale w3, d_zd2 = s2 d_ad2, d_z2 = -2 a2 s2 zd2 d_ad2,
+//   d_ad1 = d_zd2 W2, d_a1 = d_z2 W2 - 2 a1 zd1 d_ad1, d_zd1 = s1 d_ad1,
+//   d_z1 = s1 d_a1
+// give dJ/dW2 = sum d_zd2 (x) ad1 + d_z2 (x) a1, dJ/dW1 = sum d_zd1 (x) u +
+// d_z1 (x) g, dJ/db = sum d_z, dJ/dw3 = e_scale sum ad2.  The weight
+// gradients are left for k_nn_wgrad as transposed BF16 matrices [unit][row]
+// per element (rows padded to WG_ROWS); bias and w3 parts as per-tile sums.
+// grid: nbo + nbh blocks (as k_nn_mlp), 64 * MLP_WAVES threads, TRAIN_LDS.
+// ---------------------------------------------------------------------------
+constexpr int WG_ROWS = 256;
+constexpr int TRAIN_LDS = 2 * (W1E + 2 * W2E) + MLP_WAVES * 2 * 16 * NH * 2;
+
+struct NNTrainBuf {
+    __bf16 *d2t[2], *d2p[2], *a1t[2], *a1p[2], *d1t[2], *d1p[2], *a0t[2], *a0p[2];
+    int rows[2];          // padded row count per element (leading dimension)
+    float *tp;            // [tile][3][128]: d_z1, d_z2 sums, e_scale ad2 sums
+};
+
+__device__ __forceinline__ void store_t(__bf16 *dst, int ld, int unit, int row, const float *v4)
+{
+    // four consecutive rows of one unit: one 8-byte store
+    __bf16 b[4] = {(__bf16)v4[0], (__bf16)v4[1], (__bf16)v4[2], (__bf16)v4[3]};
+    *reinterpret_cast<uint2 *>(dst + (long)unit * ld + row) = *reinterpret_cast<uint2 *>(b);
+}
+
+__global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_train_fwd(
+    NNGeom g, NNW W, int nbo, const float *__restrict__ gt, const float *__restrict__ ut,
+    NNTrainBuf B)
+{
+    extern __shared__ __align__(16) unsigned char smem[];
+    __bf16 *s_w1 = reinterpret_cast<__bf16 *>(smem);
+    __bf16 *s_w2 = s_w1 + W1E;
+    __bf16 *s_w2t = s_w2 + W2E;
+    __bf16 *s_tile = s_w2t + W2E;
+    const int e = blockIdx.x < nbo ? 0 : 1;
+    const int tile = (e == 0 ? blockIdx.x : blockIdx.x - nbo) * MLP_WAVES + (threadIdx.x >> 6);
+    const int gtile = blockIdx.x * MLP_WAVES + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63, col = lane & 15, kq = 8 * (lane >> 4);
+    __bf16 *tA = s_tile + (threadIdx.x >> 6) * 2 * 16 * NH, *tB = tA + 16 * NH;
+    {
+        const uint4 *g1 = reinterpret_cast<const uint4 *>(W.w1 + (long)e * W1E);
+        const uint4 *g2 = reinterpret_cast<const uint4 *>(W.w2 + (long)e * W2E);
+        const uint4 *g2t = reinterpret_cast<const uint4 *>(W.w2t + (long)e * W2E);
+        uint4 *d1 = reinterpret_cast<uint4 *>(s_w1), *d2 = reinterpret_cast<uint4 *>(s_w2);
+        uint4 *d2t = reinterpret_cast<uint4 *>(s_w2t);
+        for (int t = threadIdx.x; t < W1E / 8; t += blockDim.x) d1[t] = g1[t];
+        for (int t = threadIdx.x; t < W2E / 8; t += blockDim.x) {
+            d2[t] = g2[t];
+            d2t[t] = g2t[t];
+        }
+    }
+    __syncthreads();
+    const int ld = B.rows[e];
+    const int row0 = tile * 16;
+    const int ra = row_atom(e, row0 + col, g.n_mols, g.n_frames, g.n_atoms);
+    const int rq = row0 + 4 * (lane >> 4);   // first of this lane's four C rows
+    // layer 1, primal and tangent; the inputs go to a0p / a0t
+    float a1[8][4], zd1[8][4];
+    {
+        f32x4 az[8], ad[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) az[t] = ad[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+            const bf16x8 ag = afrag_f32(gt, ra, NIN, 32 * s + kq, NIN);
+            const bf16x8 au = afrag_f32(ut, ra, NIN, 32 * s + kq, NIN);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const bf16x8 b = lds8(s_w1 + (16 * t + col) * KIN + 32 * s + kq);
+                az[t] = mfma16(ag, b, az[t]);
+                ad[t] = mfma16(au, b, ad[t]);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                B.a0p[e][(long)(32 * s + kq + q) * ld + row0 + col] = ag[q];
+                B.a0t[e][(long)(32 * s + kq + q) * ld + row0 + col] = au[q];
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const float b = W.b1[e * NH + 16 * t + col];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                a1[t][r] = tanhf(az[t][r] + b);
+                zd1[t][r] = ad[t][r];
+            }
+        }
+    }
+    {
+        float ad1[8][4];
+#pragma unroll
+        for (int t = 0; t < 8; ++t)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) ad1[t][r] = (1.0f - a1[t][r] * a1[t][r]) * zd1[t][r];
+        c_to_lds(a1, tA, lane);
+        c_to_lds(ad1, tB, lane);
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            store_t(B.a1p[e], ld, 16 * t + col, rq, a1[t]);
+            store_t(B.a1t[e], ld, 16 * t + col, rq, ad1[t]);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    // layer 2, primal and tangent; adjoints of the output
+    float dzd2[8][4], dz2[8][4];
+    {
+        f32x4 az[8], ad[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) az[t] = ad[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const bf16x8 xa = lds8(tA + col * NH + 32 * s + kq);
+            const bf16x8 xd = lds8(tB + col * NH + 32 * s + kq);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const bf16x8 b = lds8(s_w2 + (16 * t + col) * NH + 32 * s + kq);
+                az[t] = mfma16(xa, b, az[t]);
+                ad[t] = mfma16(xd, b, ad[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int n = 16 * t + col;
+            const float b = W.b2[e * NH + n], dad2 = W.e_scale * W.w3[e * NH + n];
+            float sz = 0.f, sw = 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float a2 = tanhf(az[t][r] + b), s2 = 1.0f - a2 * a2;
+                const float zd2 = ad[t][r];
+                dzd2[t][r] = s2 * dad2;
+                dz2[t][r] = -2.0f * a2 * s2 * zd2 * dad2;
+                sz += dz2[t][r];
+                sw += W.e_scale * s2 * zd2;
+            }
+            sz += __shfl_xor(sz, 16, 64);
+            sz += __shfl_xor(sz, 32, 64);
+            sw += __shfl_xor(sw, 16, 64);
+            sw += __shfl_xor(sw, 32, 64);
+            if (lane < 16) {
+                B.tp[((long)gtile * 3 + 1) * NH + n] = sz;
+                B.tp[((long)gtile * 3 + 2) * NH + n] = sw;
+            }
+            store_t(B.d2t[e], ld, n, rq, dzd2[t]);
+            store_t(B.d2p[e], ld, n, rq, dz2[t]);
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    c_to_lds(dzd2, tA, lane);
+    c_to_lds(dz2, tB, lane);
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    {
+        f32x4 gd[8], gp[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) gd[t] = gp[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            const bf16x8 xd = lds8(tA + col * NH + 32 * s + kq);
+            const bf16x8 xp = lds8(tB + col * NH + 32 * s + kq);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const bf16x8 b = lds8(s_w2t + (16 * t + col) * NH + 32 * s + kq);
+                gd[t] = mfma16(xd, b, gd[t]);
+                gp[t] = mfma16(xp, b, gp[t]);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const int n = 16 * t + col;
+            float dzd1[4], dz1[4], sz = 0.f;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float s1 = 1.0f - a1[t][r] * a1[t][r];
+                const float dad1 = gd[t][r];
+                const float da1 = gp[t][r] - 2.0f * a1[t][r] * zd1[t][r] * dad1;
+                dzd1[r] = s1 * dad1;
+                dz1[r] = s1 * da1;
+                sz += dz1[r];
+            }
+            sz += __shfl_xor(sz, 16, 64);
+            sz += __shfl_xor(sz, 32, 64);
+            if (lane < 16) B.tp[((long)gtile * 3 + 0) * NH + n] = sz;
+            store_t(B.d1t[e], ld, n, rq, dzd1);
+            store_t(B.d1p[e], ld, n, rq, dz1);
+        }
+    }
+}
+
+// Weight gradients as split-K products over WG_ROWS-row chunks of one
+// element: part[e][chunk] = [dJ/dW1 (128 x 64) | dJ/dW2 (128 x 128)],
+//   dJ/dW2[m][n] = sum_rows d2t[m] a1t[n] + d2p[m] a1p[n]
+//   dJ/dW1[m][k] = sum_rows d1t[m] a0t[k] + d1p[m] a0p[k]
+// 88 output tiles of 16 x 16 (64 of W2, 24 of W1 for k < 48) over 8 waves.
+// grid: nco + nch blocks (chunks of O, then of H), 512 threads.
+constexpr int WG_WAVES = 8;
+constexpr int WG_PART = NH * KIN + NH * NH;
+__global__ __launch_bounds__(64 * WG_WAVES) void k_nn_wgrad(NNTrainBuf B, int nco,
+                                                            float *__restrict__ part)
+{
+    const int e = blockIdx.x < nco ? 0 : 1;
+    const int chunk = e == 0 ? blockIdx.x : blockIdx.x - nco;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, kq = 8 * (lane >> 4);
+    const int ld = B.rows[e];
+    const long r0 = (long)chunk * WG_ROWS;
+    float *P = part + (long)blockIdx.x * WG_PART;
+    for (int tt = wv; tt < 88; tt += WG_WAVES) {
+        const bool w2 = tt < 64;
+        const int mt = w2 ? tt >> 3 : (tt - 64) / 3, nt = w2 ? tt & 7 : (tt - 64) % 3;
+        const __bf16 *At = w2 ? B.d2t[e] : B.d1t[e], *Ap = w2 ? B.d2p[e] : B.d1p[e];
+        const __bf16 *Bt = w2 ? B.a1t[e] : B.a0t[e], *Bp = w2 ? B.a1p[e] : B.a0p[e];
+        const long ra = (long)(16 * mt + col) * ld + r0 + kq, rb = (long)(16 * nt + col) * ld + r0 + kq;
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+        for (int s = 0; s < WG_ROWS / 32; ++s) {
+            acc = mfma16(*reinterpret_cast<const bf16x8 *>(At + ra + 32 * s),
+                         *reinterpret_cast<const bf16x8 *>(Bt + rb + 32 * s), acc);
+            acc = mfma16(*reinterpret_cast<const bf16x8 *>(Ap + ra + 32 * s),
+                         *reinterpret_cast<const bf16x8 *>(Bp + rb + 32 * s), acc);
+        }
+        const int n = 16 * nt + col;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int m = 16 * mt + 4 * (lane >> 4) + r;
+            if (w2) P[NH * KIN + m * NH + n] = acc[r];
+            else P[m * KIN + n] = acc[r];
+        }
+    }
+}
+
+// Parameters: reduce the gradient parts in a fixed order, scale by
+// -(2/n_comp) (the force-matching loss), then (mode 1) an Adam step of the
+// FP32 master copy, or (mode 0) only the gradient into grad_out; mode 2
+// only re-derives the device copies from theta.  Every mode rewrites the
+// BF16 matrices (and transposes) and the FP32 vectors the kernels read.
+// grid: ceil(2 NPE / 256), 256 threads.
+struct NNDev {
+    __bf16 *w1, *w1t, *w2, *w2t;
+    float *b1, *b2, *w3, *b3;
+};
+struct AdamArgs {
+    float lr, beta1, beta2, eps, c1, c2;   // c1 = 1 - beta1^t, c2 = 1 - beta2^t
+};
+__global__ __launch_bounds__(256) void k_nn_update(int mode, const float *__restrict__ part,
+                                                   int nco, int nch, const float *__restrict__ tp,
+                                                   int nto, int nth, double gscale,
+                                                   float *__restrict__ theta,
+                                                   float *__restrict__ am, float *__restrict__ av,
+                                                   AdamArgs ad, float *__restrict__ grad_out,
+                                                   NNDev D)
+{
+    const int pidx = blockIdx.x * 256 + threadIdx.x;
+    if (pidx >= 2 * NPE) return;
+    const int e = pidx / NPE, o = pidx - e * NPE;
+    // which block of the packed element vector
+    int kind, n = 0, k = 0;
+    if (o < NH * NIN) { kind = 0; n = o / NIN; k = o - n * NIN; }
+    else if (o < NH * NIN + NH) { kind = 1; n = o - NH * NIN; }
+    else if (o < NH * NIN + NH + NH * NH) { kind = 2; const int q = o - NH * NIN - NH; n = q / NH; k = q - n * NH; }
+    else if (o < NH * NIN + 2 * NH + NH * NH) { kind = 3; n = o - NH * NIN - NH - NH * NH; }
+    else if (o < NPE - 1) { kind = 4; n = o - NH * NIN - 2 * NH - NH * NH; }
+    else { kind = 5; }
+    float th = theta[pidx];
+    if (mode != 2) {
+        double s = 0.0;
+        if (kind == 0 || kind == 2) {
+            const int c0 = e == 0 ? 0 : nco, c1 = e == 0 ? nco : nco + nch;
+            const long off = kind == 0 ? (long)n * KIN + k : (long)NH * KIN + (long)n * NH + k;
+            for (int c = c0; c < c1; ++c) s += (double)part[(long)c * WG_PART + off];
+        } else if (kind != 5) {
+            const int t0 = e == 0 ? 0 : nto, t1 = e == 0 ? nto : nto + nth;
+            const int slot = kind == 1 ? 0 : (kind == 3 ? 1 : 2);
+            for (int t = t0; t < t1; ++t) s += (double)tp[((long)t * 3 + slot) * NH + n];
+        }
+        const float gr = (float)(gscale * s);
+        if (mode == 0) {
+            grad_out[pidx] = gr;
+        } else {
+            const float m = ad.beta1 * am[pidx] + (1.0f - ad.beta1) * gr;
+            const float v = ad.beta2 * av[pidx] + (1.0f - ad.beta2) * gr * gr;
+            am[pidx] = m;
+            av[pidx] = v;
+            th -= ad.lr * (m / ad.c1) / (sqrtf(v / ad.c2) + ad.eps);
+            theta[pidx] = th;
+        }
+    }
+    switch (kind) {
+    case 0:
+        D.w1[(long)e * W1E + n * KIN + k] = (__bf16)th;
+        D.w1t[(long)e * W1E + k * NH + n] = (__bf16)th;
+        break;
+    case 1: D.b1[e * NH + n] = th; break;
+    case 2:
+        D.w2[(long)e * W2E + n * NH + k] = (__bf16)th;
+        D.w2t[(long)e * W2E + k * NH + n] = (__bf16)th;
+        break;
+    case 3: D.b2[e * NH + n] = th; break;
+    case 4: D.w3[e * NH + n] = th; break;
+    default: D.b3[e] = th; break;
+    }
+}
+
+}  // namespace nn
+}  // namespace qtp

@@ -162,6 +162,12 @@ struct PropArgs {
     double halfdt, dt, beta_n;
     int nvt, host_dev, n_dof;  // host_dev: deviates from the host buffer
     uint64_t seed;
+    // centroid: cmode 0 free (PIMD / PA-CMD), 2 held fixed (its momentum is
+    // zeroed after the kick, no thermostat); cthermo 0 = PILE-G, the Bussi
+    // rescale with c = exp(-dt/tau) (c = 0: thermostat.cpp:88-111), 1 =
+    // PILE-L, Langevin c1_0 = exp(-dt/tau), c2_0 = sqrt(1 - c1_0^2)
+    int cmode, cthermo;
+    double cbussi, c1_0, c2_0;
 };
 
 // Per-mode constants: mode[k] = {scale_k, wsq_k, c1_k, c2_k}
@@ -188,6 +194,7 @@ __device__ __forceinline__ void cayley_half(double &uq, double &up, double m, do
 // hpart[k * ntile + tile] = KE before - KE after of mode k's Langevin step.
 __global__ __launch_bounds__(64) void k_prop_fwd(PropArgs A, const double *__restrict__ Cm,
                                                  const double4 *__restrict__ mode,
+                                                 const double2 *__restrict__ mscale,
                                                  const double *__restrict__ q,
                                                  const double *__restrict__ p,
                                                  const double *__restrict__ F,
@@ -205,7 +212,9 @@ __global__ __launch_bounds__(64) void k_prop_fwd(PropArgs A, const double *__res
     const int f = tile * 64 + lane;
     const int at = f % A.NAP, d = f / A.NAP;
     const bool va = at < A.n_atoms;
-    const double m = va ? m_atom[at] : 1.0;
+    // propagation mass of mode k: the physical mass, times (w_k / W)^2 for
+    // the non-centroid modes of partially adiabatic CMD
+    const double m = va ? m_atom[at] * mscale[k].x : 1.0;
     const double *cr = Cm + (long)k * P;
     double uq = 0.0, up = 0.0;
 #pragma unroll 8
@@ -217,27 +226,40 @@ __global__ __launch_bounds__(64) void k_prop_fwd(PropArgs A, const double *__res
         up = fma(c, pk, up);
     }
     const double4 md = mode[k];
-    cayley_half(uq, up, m, A.halfdt, md);
     if (k == 0) {
-        nq[f] = uq;
-        np[f] = up;
-        double ke0 = va ? 0.5 * up * up / m : 0.0;
-        double chi = 0.0;
-        if (A.nvt && !A.host_dev && va) {
+        if (A.cmode == 2) up = 0.0;   // centroid held fixed
+        cayley_half(uq, up, m, A.halfdt, md);
+        const bool pile_g = A.nvt && A.cthermo == 0 && A.cmode != 2;
+        double ke0 = (va && pile_g) ? 0.5 * up * up / m : 0.0;
+        double chi = 0.0, h = 0.0;
+        if (pile_g && !A.host_dev && va) {
             const int ridx = 3 * at + d;
             if (ridx >= 1) {
                 const double z = philox_normal(A.seed, *step_ctr, (uint32_t)(P + 1), (uint32_t)ridx);
                 chi = z * z;
             }
         }
+        if (A.nvt && A.cthermo == 1 && A.cmode != 2 && va) {
+            // PILE-L: Langevin on the centroid (random stream 0)
+            h = 0.5 * up * up / m;
+            const double xi = philox_normal(A.seed, *step_ctr, 0u, (uint32_t)(3 * at + d));
+            up = A.c1_0 * up + sqm_atom[at] * A.c2_0 * xi;
+            h -= 0.5 * up * up / m;
+        }
+        nq[f] = uq;
+        np[f] = up;
         ke0 = wave_sum(ke0);
         chi = wave_sum(chi);
+        h = wave_sum(h);
         if (lane == 0) {
             part[tile] = ke0;
             part[A.ntile + tile] = chi;
+            // the centroid's heat slot: PILE-L's here, PILE-G's in k_prop_inv
+            if (A.nvt && !pile_g) hpart[tile] = h;
         }
         return;
     }
+    cayley_half(uq, up, m, A.halfdt, md);
     if (A.nvt) {
         double h = 0.0;
         if (va) {
@@ -280,7 +302,7 @@ __global__ __launch_bounds__(64) void k_prop_inv(PropArgs A, const double *__res
     const bool va = at < A.n_atoms;
     const double m = va ? m_atom[at] : 1.0;
     double u0q = nq[f], u0p = np[f];
-    if (A.nvt) {
+    if (A.nvt && A.cthermo == 0 && A.cmode != 2) {
         const double ke = wave_sum_array(part, A.ntile);
         double R, g2;
         if (A.host_dev) {
@@ -290,7 +312,16 @@ __global__ __launch_bounds__(64) void k_prop_inv(PropArgs A, const double *__res
             g2 = wave_sum_array(part + A.ntile, A.ntile);
             R = philox_normal(A.seed, *step_ctr, (uint32_t)P, 0u);
         }
-        const double alpha = sqrt((R * R + g2) / (2.0 * A.beta_n * ke));
+        double alpha;
+        if (A.cbussi == 0.0) {
+            alpha = sqrt((R * R + g2) / (2.0 * A.beta_n * ke));   // thermostat.cpp:99-104
+        } else {
+            // Bussi, Donadio, Parrinello, J. Chem. Phys. 126, 014101 (2007),
+            // eq. (A7): relaxation towards the canonical KE with time constant tau
+            const double c = A.cbussi, x = 1.0 / (2.0 * A.beta_n * ke);
+            alpha = sqrt(c + (1.0 - c) * (R * R + g2) * x + 2.0 * R * sqrt(c * (1.0 - c) * x));
+            if (R + sqrt(c / ((1.0 - c) * x)) < 0.0) alpha = -alpha;
+        }
         u0p *= alpha;
         if (b == 0) {
             double h = va ? -0.5 * u0p * u0p / m : 0.0;
@@ -334,22 +365,22 @@ __global__ __launch_bounds__(1024) void k_heat(int ntile, int nh, const double *
 }
 
 // Generic orthonormal transform along the bead axis of a[P][len] (fourier.cpp:12-69).
-// grid: ceil(len/64), 256 threads.  fwd: u_k = sum_b C[k][b] x_b, else the transpose.
+// grid: ceil(len/64), 256 threads, nm_generic_lds(P) bytes of LDS (the
+// block's 64 columns of all P beads; the matrix entries are wave-uniform
+// loads from global memory).  fwd: u_k = sum_b C[k][b] x_b, else the transpose.
+inline size_t nm_generic_lds(int P) { return (size_t)P * 64 * sizeof(double); }
 __global__ __launch_bounds__(256) void k_nm_generic(int P, int len, const double *__restrict__ Cm,
                                                     double *__restrict__ a, int fwd)
 {
-    extern __shared__ double sm[];
-    double *Cs = sm;
-    double *X = Cs + P * P;
+    extern __shared__ double X[];
     const int lane = threadIdx.x & 63, kg = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + lane;
-    for (int t = threadIdx.x; t < P * P; t += 256) Cs[t] = Cm[t];
     for (int b = kg; b < P; b += 4) X[b * 64 + lane] = c < len ? a[(long)b * len + c] : 0.0;
     __syncthreads();
     for (int k = kg; k < P; k += 4) {
         double u = 0.0;
         for (int b = 0; b < P; ++b) {
-            const double cf = fwd ? Cs[k * P + b] : Cs[b * P + k];
+            const double cf = fwd ? Cm[k * P + b] : Cm[b * P + k];
             u = fma(cf, X[b * 64 + lane], u);
         }
         if (c < len) a[(long)k * len + c] = u;
@@ -795,16 +826,31 @@ __global__ __launch_bounds__(1024) void k_nlist_bead(int n_mols, int NAP, int NM
 // Classical velocity Verlet (update.cpp:78-137)
 // ---------------------------------------------------------------------------
 // p += dt/2 F; q += p dt / m; O displacements.  grid: ceil(3 NAP / 256).
+// With a thermostat (lang != 0) a Langevin step p = c1 p + sqrt(m/beta) c2 xi
+// comes first (random stream 0 of the step counter; the counter advances in
+// k_vv_b, so every block reads the same value here).
+struct VVLang {
+    int on;
+    double c1, c2;
+    uint64_t seed;
+};
 __global__ __launch_bounds__(256) void k_vv_a(int n_atoms, int NAP, int NMP, double halfdt,
                                               double dt, const double *__restrict__ m_atom,
                                               const double *__restrict__ F, double *__restrict__ q,
-                                              double *__restrict__ p, double *__restrict__ disp)
+                                              double *__restrict__ p, double *__restrict__ disp,
+                                              VVLang lang, const double *__restrict__ sqm_atom,
+                                              const unsigned long long *__restrict__ step_ctr)
 {
     const int f = blockIdx.x * 256 + threadIdx.x;
     if (f >= 3 * NAP) return;
     const int at = f % NAP, d = f / NAP;
     if (at >= n_atoms) return;
-    double pp = p[f] + halfdt * F[f];
+    double p0 = p[f];
+    if (lang.on) {
+        const double xi = philox_normal(lang.seed, *step_ctr, 0u, (uint32_t)(3 * at + d));
+        p0 = lang.c1 * p0 + sqm_atom[at] * lang.c2 * xi;
+    }
+    double pp = p0 + halfdt * F[f];
     p[f] = pp;
     const double m = m_atom[at];
     q[f] += pp * dt / m;
@@ -812,13 +858,16 @@ __global__ __launch_bounds__(256) void k_vv_a(int n_atoms, int NAP, int NMP, dou
 }
 
 // F = f_phys of the single evaluation, p += dt/2 F, energy out.
+// extra_e (may be null): a further energy term added to V (the learned
+// correction's).
 __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfdt,
                                               const double *__restrict__ fe,
                                               const double *__restrict__ epart, int ep_stride,
                                               int ep_n, double eself,
                                               double *__restrict__ F, double *__restrict__ p,
                                               double *__restrict__ out,
-                                              unsigned long long *__restrict__ step_ctr)
+                                              unsigned long long *__restrict__ step_ctr,
+                                              const double *__restrict__ extra_e)
 {
     const int f = blockIdx.x * 256 + threadIdx.x;
     if (f < 3 * NAP) {
@@ -830,7 +879,7 @@ __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfd
     if (blockIdx.x == 0 && threadIdx.x < 64) {
         const double v = eval_energy(epart, 0, ep_stride, ep_n, eself);
         if (threadIdx.x == 0) {
-            out[0] = v;
+            out[0] = extra_e ? v + extra_e[0] : v;
             out[1] = 0.0;
             if (step_ctr) step_ctr[0] += 1ull;
         }
@@ -964,6 +1013,66 @@ __global__ __launch_bounds__(256) void k_cons(int P, int n_atoms, int NAP,
     }
     acc = block_sum(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Centroid mean force, radial distribution functions (no counterpart in the
+// reference; the CMD and g(r) tools its README leaves to other programs)
+// ---------------------------------------------------------------------------
+// acc[t] += (1/P) sum_b F[b][t]: the force on the centroid of dof t, summed
+// over the steps of a run (the mean force of centroid-constrained runs).
+// grid: ceil(3 NAP / 256), 256 threads.
+__global__ __launch_bounds__(256) void k_cmf_acc(int P, int NAP, const double *__restrict__ F,
+                                                 double *__restrict__ acc,
+                                                 unsigned long long *__restrict__ n)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (blockIdx.x == 0 && threadIdx.x == 0) n[0] += 1ull;
+    if (t >= 3 * NAP) return;
+    const long N3 = 3L * NAP;
+    double s = 0.0;
+    for (int b = 0; b < P; ++b) s += F[b * N3 + t];
+    acc[t] += s / P;
+}
+
+// Centroid positions (bead average) of the planar state.  grid: ceil(3 NAP/256).
+__global__ __launch_bounds__(256) void k_centroid(int P, int NAP, const double *__restrict__ q,
+                                                  double *__restrict__ qc)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= 3 * NAP) return;
+    const long N3 = 3L * NAP;
+    double s = 0.0;
+    for (int b = 0; b < P; ++b) s += q[b * N3 + t];
+    qc[t] = s / P;
+}
+
+// Pair-distance histograms of the centroid positions: O-O over molecule
+// pairs i < j, O-H over every (O of i, H of j) including j = i.  Bin
+// floor(r / dr) of the minimum-image distance (bohr) below n_bins * dr.
+// One wavefront per molecule i (lanes over j); integer counts, so the
+// result does not depend on the order of the atomic adds.
+// grid: ceil(n_mols / 4), 256 threads.
+__global__ __launch_bounds__(256) void k_rdf(int n_mols, int NAP, const double *__restrict__ qc,
+                                             Box box, double inv_dr, int n_bins,
+                                             unsigned int *__restrict__ hoo,
+                                             unsigned int *__restrict__ hoh)
+{
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= n_mols) return;
+    const double xo = qc[3 * i], yo = qc[NAP + 3 * i], zo = qc[2 * NAP + 3 * i];
+    for (int j = lane; j < n_mols; j += 64) {
+        for (int s = 0; s < 3; ++s) {
+            if (s == 0 && j <= i) continue;
+            const int a = 3 * j + s;
+            const double dx = mimg_exact(qc[a] - xo, box.L[0]);
+            const double dy = mimg_exact(qc[NAP + a] - yo, box.L[1]);
+            const double dz = mimg_exact(qc[2 * NAP + a] - zo, box.L[2]);
+            const double r = sqrt(dx * dx + dy * dy + dz * dz);
+            const double bf = r * inv_dr;
+            if (bf < (double)n_bins) atomicAdd(s == 0 ? &hoo[(int)bf] : &hoh[(int)bf], 1u);
+        }
+    }
 }
 
 }  // namespace qtp

@@ -87,6 +87,12 @@ def test_checkpoint_rejects_another_system(tmp_path):
     with Water(n, 4, [b * 1.01 for b in box]) as w:
         with pytest.raises(QtpError):
             w.load_checkpoint(path)
+    # same system, another temperature, time step or Ewald accuracy
+    for kw, word in ((dict(T_K=300.0), "T_K"), (dict(dtfs=0.5), "dtfs"),
+                     (dict(eps_ewald=1e-7), "eps_ewald")):
+        with Water(n, 4, box, **kw) as w:
+            with pytest.raises(QtpError, match=word):
+                w.load_checkpoint(path)
     with Water(n, 4, box) as w:
         with pytest.raises(QtpError):
             w.save_checkpoint(tmp_path / "none.qtp")  # no state yet

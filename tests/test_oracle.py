@@ -387,24 +387,60 @@ def test_ftransform_is_orthonormal_and_exact_inverse(oracle_mod):
         assert np.abs(y - x).max() < 1e-14
 
 
-def test_free_ring_polymer_step_is_exact_cayley(oracle_mod):
-    """Force-free SC step (no atoms interacting is impossible with the full
-    force field, so the propagation half is checked on the NM matrix level):
-    each normal mode follows the 2x2 sqrt-Cayley map, det = 1."""
-    P = 8
-    o = oracle_mod.Oracle(1, P, (30.0, 30.0, 30.0), dtfs=0.5)
-    s = o.sys
-    for k in range(P):
-        w2 = (2.0 / s.beta_n) ** 2 * math.sin(k * s.piovern) ** 2
-        sc = 1.0 / math.sqrt(1.0 + 0.25 * w2 * s.dt ** 2)
-        m = 1.0
-        A = sc * np.array([[1.0, s.halfdt / m], [-s.halfdt * w2 * m, 1.0]])
-        assert np.linalg.det(A) == pytest.approx(1.0, abs=1e-14)
-        # two half steps equal the Cayley map of the full step for the harmonic mode
-        full = A @ A
-        cay = np.linalg.solve(np.eye(2) - 0.5 * s.dt * np.array([[0, 1 / m], [-w2 * m, 0]]),
-                              np.eye(2) + 0.5 * s.dt * np.array([[0, 1 / m], [-w2 * m, 0]]))
-        assert np.allclose(full, cay, atol=1e-12)
+def test_update_sc_propagates_by_the_exact_sqrt_cayley_map(oracle_mod):
+    """The oracle's update_sc (update.cpp:271-401) in NVE against its closed
+    form, built here independently of the oracle's code: the half kick
+    p += dt/2 F, the explicit orthonormal normal-mode matrix (cos/sin rows,
+    fourier.cpp:12-69 with FFTW's halfcomplex convention), per mode k two
+    sqrt-Cayley half steps scale * [[1, dt/(2m)], [-dt/2 w_k^2 m, 1]] with
+    w_k = (2/beta_n) sin(k pi_ref / P) (update.cpp:301-348), and the
+    transpose back.  The new positions depend on nothing else; the momenta
+    are compared after removing the final kick dt/2 F_new (F_new is the force
+    array the step leaves behind).  A wrong frequency, a dropped half step, a
+    transposed transform or a missed kick fails by many orders of magnitude."""
+    for P in (4, 8):
+        q0, box = cfg.lattice_water_box(2, 12.43, seed=3)
+        o = oracle_mod.Oracle(8, P, box, dtfs=0.5)
+        s = o.sys
+        rng = np.random.default_rng(10 + P)
+        q = cfg.replicate_beads(q0, P) + rng.normal(0, 0.05, P * o.n_dof)
+        p = cfg.sample_momenta(8, P, s.beta_n, seed=20 + P)
+        F0 = rng.normal(0, 0.01, P * o.n_dof)
+        point, lst, disp = o.all_lists(q)
+        fph = np.zeros(P * o.n_dof)
+        qo, po, Fo = q.copy(), p.copy(), F0.copy()
+        o.update_sc(qo, po, Fo, fph, point, lst, disp, False, None, 0.0)
+
+        nd = o.n_dof
+        j = np.arange(P)
+        M = np.zeros((P, P))
+        M[0] = 1.0 / np.sqrt(P)
+        for k in range(1, (P + 1) // 2):
+            ang = 2 * np.pi * ((j * k) % P) / P
+            M[k] = np.sqrt(2.0 / P) * np.cos(ang)
+            M[P - k] = -np.sqrt(2.0 / P) * np.sin(ang)
+        M[P // 2] = (-1.0) ** j / np.sqrt(P)
+        Q = q.reshape(P, nd)
+        Pm = (p + s.halfdt * F0).reshape(P, nd)
+        uq, up = M @ Q, M @ Pm
+        m = o.mass
+        for k in range(P):
+            w2 = (2.0 / s.beta_n) ** 2 * math.sin(k * s.piovern) ** 2
+            sc = 1.0 / math.sqrt(1.0 + 0.25 * w2 * s.dt ** 2)
+            for _ in range(2):
+                nq = sc * (uq[k] + s.halfdt * up[k] / m)
+                npk = sc * (up[k] - s.halfdt * w2 * m * uq[k])
+                uq[k], up[k] = nq, npk
+        q_want = (M.T @ uq).ravel()
+        p_want = (M.T @ up).ravel()
+        p_prop = po - s.halfdt * Fo
+        assert np.max(np.abs(qo - q_want)) <= 1e-13 * np.max(np.abs(q_want)), P
+        assert np.max(np.abs(p_prop - p_want)) <= 1e-12 * np.max(np.abs(p_want)), P
+        # the final kick used the forces of the new positions
+        Fchk, fchk = np.zeros_like(Fo), np.zeros_like(Fo)
+        pt2, ls2, _ = o.all_lists(qo)
+        o.force_eval_sc(qo, Fchk, fchk, pt2, ls2)
+        assert np.max(np.abs(Fchk - Fo)) <= 1e-12 * np.max(np.abs(Fo))
 
 
 @needs_ref
