@@ -114,34 +114,15 @@ def cpu_baseline(wl, q, p, box, n_mols, seconds):
                       f"{el:.1f} s"}
 
 
-# 64 independent replicas (qtp_batch_*): molecule counts from {64, ..., 512}
-# at 0.997 g/cm^3 (cubic boxes; lattice sites, 128 of a 6^3 lattice for 128
-# molecules), bead counts from {8, 16, 32, 64},
-# temperatures uniform in 250-350 K, NVT, one master seed.
-BATCH_SIZES = {64: 12.43, 128: 15.66, 216: 18.64, 343: 21.75, 512: 24.86}
-
-
-def batch_replicas(n_rep, seed):
-    rng = np.random.default_rng(seed)
-    reps = []
-    for r in range(n_rep):
-        n = int(rng.choice(sorted(BATCH_SIZES)))
-        reps.append(dict(n_mols=n, L_angs=BATCH_SIZES[n], P=int(rng.choice([8, 16, 32, 64])),
-                         T_K=float(rng.uniform(250.0, 350.0)), seed=int(seed * 1000 + r)))
-    return reps
-
-
 def run_batch(a):
     """Replica-steps/s of a batch; value = total replica steps per second."""
     from qtip4pf_water_amd import Batch, Water
-    reps = batch_replicas(a.replicas, 2024)
+    reps = cfg.batch_replicas(a.replicas, 2024)
     ws = []
     for r in reps:
-        q0, box = cfg.water_box(r["n_mols"], r["L_angs"], seed=r["seed"])
-        beta_n = 1.0 / (cfg.K2AU * r["T_K"]) / r["P"]
-        q = cfg.free_ring_polymer_beads(q0, r["P"], beta_n, seed=r["seed"] + 1)
-        p = cfg.sample_momenta(r["n_mols"], r["P"], beta_n, seed=r["seed"] + 2)
-        w = Water(r["n_mols"], r["P"], box, T_K=r["T_K"], dtfs=0.25, eps_ewald=1e-6)
+        q, p, box = cfg.replica_state(r)
+        w = Water(r["n_mols"], r["P"], box, T_K=r["T_K"], dtfs=0.25, eps_ewald=1e-6,
+                  lj_rcut=r["lj_rcut"])
         w.set_seed(r["seed"])
         try:
             w.set_state(q, p)

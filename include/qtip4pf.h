@@ -57,6 +57,12 @@ typedef struct {
      * and kappa * r_c <= 6.2; qtp_create rejects other values. */
     double ewald_kappa, ewald_rcut, ewald_kcut;
     int ewald_kmax;
+    /* O-O Lennard-Jones cutoff (bohr); <= 0: the reference's compile-time
+     * 9 A (qtip4pfparams.h:41).  A shorter one keeps boxes narrower than 18 A
+     * physical (the reference only warns, water.cpp:176-178); the Verlet
+     * radius is 1.3 of it and e_cut and the list size follow it as
+     * qtip4pfparams.h:106-110 and water.cpp:135 derive them. */
+    double lj_rcut;
 } qtp_params;
 
 /* Derived globals (water.cpp:74-157) and Ewald parameters (forces.cpp:574-584). */
@@ -67,6 +73,7 @@ typedef struct {
     double acc, rho, kwald_prefactor, sd_ox, sd_h;
     double kappa, rcut, k_cut;
     int k_max, n_kgroups;
+    double lj_rcut, r_skin;   /* LJ cutoff and Verlet radius (bohr) */
 } qtp_derived;
 
 int qtp_create(const qtp_params *prm, qtp_ctx **out);
@@ -249,8 +256,8 @@ int qtp_get_virial(qtp_ctx *ctx, double *virial, double *energy);
  * qtp_set_state; charges/masses as there) restores them into a context of
  * the same system and recomputes the forces, so the run continues exactly
  * (bit for bit) as it would have without the stop.  The file also records
- * T_K, dtfs, eps_ewald, epsilon_sc and the effective Ewald kappa, r_c, k_cut
- * and k_max; a load into a context where any of them (or the molecule count,
+ * T_K, dtfs, eps_ewald, epsilon_sc, the effective Ewald kappa, r_c, k_cut
+ * and k_max, and the LJ cutoff; a load into a context where any of them (or the molecule count,
  * bead count, list size or box) differs fails with QTP_ERR_ARG. */
 int qtp_save_checkpoint(qtp_ctx *ctx, const char *path);
 int qtp_load_checkpoint(qtp_ctx *ctx, const char *path, const double *charge,

@@ -33,6 +33,8 @@ class OqSys(C.Structure):
         ("kwald_prefactor", C.c_double),
         ("sd_ox", C.c_double), ("sd_h", C.c_double),
         ("use_spline", C.c_int),
+        ("r_cut", C.c_double), ("r_cut_sq", C.c_double), ("r_skin", C.c_double),
+        ("r_skin_sq", C.c_double), ("e_cut", C.c_double),
     ]
 
 
@@ -78,6 +80,7 @@ def lib():
         L.oq_sys_init.argtypes = [C.POINTER(OqSys), C.c_int, C.c_int, _dp, C.c_double,
                                   C.c_double, C.c_double, C.c_double]
         L.oq_ewald_params.argtypes = [C.POINTER(OqSys), C.POINTER(OqEwald)]
+        L.oq_sys_set_lj_rcut.argtypes = [C.POINTER(OqSys), C.c_double]
         L.oq_get_consts.argtypes = [C.POINTER(OqConsts)]
         L.oq_charges_masses.argtypes = [C.POINTER(OqSys), _dp, _dp]
         L.oq_lj_oxygen.restype = C.c_double
@@ -153,11 +156,14 @@ def consts() -> dict:
 class Oracle:
     """Reference-shaped CPU restatement for one system (the reference's globals)."""
 
-    def __init__(self, n_mols, n_beads, boxl, T_K=298.0, dtfs=0.25, eps=1e-6, epsilonsc=1e-5):
+    def __init__(self, n_mols, n_beads, boxl, T_K=298.0, dtfs=0.25, eps=1e-6, epsilonsc=1e-5,
+                 lj_rcut=None):
         self.sys = OqSys()
         b = np.ascontiguousarray(np.asarray(boxl, dtype=np.float64).reshape(3))
         lib().oq_sys_init(C.byref(self.sys), int(n_mols), int(n_beads), _ptr(b), float(T_K),
                           float(dtfs), float(eps), float(epsilonsc))
+        if lj_rcut is not None:
+            lib().oq_sys_set_lj_rcut(C.byref(self.sys), float(lj_rcut))
         s = self.sys
         self.n_mols, self.n_atoms, self.n_dof = s.n_mols, s.n_atoms, s.n_dof
         self.n_beads, self.list_size = s.n_beads, s.list_size

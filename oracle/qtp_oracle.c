@@ -110,6 +110,11 @@ void oq_sys_init(oq_sys *s, int n_mols, int n_beads, const double boxl[3],
     s->vol = boxl[0] * boxl[1] * boxl[2];
     double rho_ox = n_mols / s->vol;
     s->list_size = (int)round(4.0 * C.r_skin * C.r_skin * C.r_skin * n_mols * rho_ox);
+    s->r_cut = C.r_cut;
+    s->r_cut_sq = C.r_cut_sq;
+    s->r_skin = C.r_skin;
+    s->r_skin_sq = C.r_skin_sq;
+    s->e_cut = C.e_cut;
     s->T_K = T_K;
     double T_red = C.K2au * T_K;
     s->beta = 1.0 / T_red;
@@ -130,6 +135,19 @@ void oq_sys_init(oq_sys *s, int n_mols, int n_beads, const double boxl[3],
     s->kwald_prefactor = 2.0 * C.pi / s->vol;
     double max_xy = fmax(boxl[0], boxl[1]);
     s->max_box = fmax(max_xy, boxl[2]);
+}
+
+/* qtip4pfparams.h:106-110 and water.cpp:135 with the cutoff given */
+void oq_sys_set_lj_rcut(oq_sys *s, double r_cut)
+{
+    init_consts();
+    s->r_cut = r_cut;
+    s->r_cut_sq = r_cut * r_cut;
+    s->r_skin = 1.30 * r_cut;
+    s->r_skin_sq = s->r_skin * s->r_skin;
+    s->e_cut = 4.0 * C.epslj * (pow((C.sigmalj / r_cut), 12.0) - pow((C.sigmalj / r_cut), 6.0));
+    double rho_ox = s->n_mols / s->vol;
+    s->list_size = (int)round(4.0 * s->r_skin * s->r_skin * s->r_skin * s->n_mols * rho_ox);
 }
 
 /* forces.cpp:574-584 */
@@ -177,9 +195,16 @@ static inline void min_image(const oq_sys *s, double r[3])
  * ------------------------------------------------------------------------- */
 
 /* forces.cpp:25-59 */
+static double lj_oxygen_cut(const double r[3], double rsq, double *F, int i, int j, double e_cut);
+
 double oq_lj_oxygen(const double r[3], double rsq, double *F, int i, int j)
 {
     init_consts();
+    return lj_oxygen_cut(r, rsq, F, i, j, C.e_cut);
+}
+
+static double lj_oxygen_cut(const double r[3], double rsq, double *F, int i, int j, double e_cut)
+{
     double rm2 = 1.0 / rsq;
     double rm6 = rm2 * rm2 * rm2;
     double rm12 = rm6 * rm6;
@@ -188,7 +213,7 @@ double oq_lj_oxygen(const double r[3], double rsq, double *F, int i, int j)
     double energy = rm12 - rm6;
     double pf = energy + rm12;
     energy *= 4.0 * C.epslj;
-    energy -= C.e_cut;
+    energy -= e_cut;
     pf *= 24.0 * C.epslj * rm2;
     for (int d = 0; d < 3; ++d) {
         F[9 * i + d] += pf * r[d];
@@ -467,8 +492,8 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
             for (int d = 0; d < 3; ++d) r[d] = q[9 * i + d] - q[9 * j + d];
             min_image(s, r);
             double rsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-            if (rsq < C.r_cut_sq) {
-                double e = oq_lj_oxygen(r, rsq, F, i, j);
+            if (rsq < s->r_cut_sq) {
+                double e = lj_oxygen_cut(r, rsq, F, i, j, s->e_cut);
                 energy += e;
                 parts[0] += e;
             }
@@ -798,7 +823,7 @@ int oq_neighbour_list(const oq_sys *s, const double *q_all, int *point_all, int 
             for (int d = 0; d < 3; ++d) r[d] = q[9 * i + d] - q[9 * j + d];
             min_image(s, r);
             double rsq = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-            if (rsq < C.r_skin_sq) {
+            if (rsq < s->r_skin_sq) {
                 if (k < ls) list[k] = j;
                 k += 1;
             }
@@ -947,7 +972,7 @@ static int needs_rebuild(const oq_sys *s, const double *disp)
     }
     m1 = sqrt(m1);
     m2 = sqrt(m2);
-    return (m1 + m2 > C.r_skin - C.r_cut);
+    return (m1 + m2 > s->r_skin - s->r_cut);
 }
 
 /* update.cpp:271-401 */

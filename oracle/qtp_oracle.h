@@ -38,6 +38,9 @@ typedef struct {
     double sd_ox, sd_h;
     /* 1: force calls add the splined pair potential (get_forces_wspline) */
     int use_spline;
+    /* O-O LJ cutoff and Verlet radius (qtip4pfparams.h:41, 106-110): the
+     * reference's 9 A and 1.3 of it unless oq_sys_set_lj_rcut changed them */
+    double r_cut, r_cut_sq, r_skin, r_skin_sq, e_cut;
 } oq_sys;
 
 /* Ewald parameters as derived inside every force call (forces.cpp:574-584). */
@@ -61,6 +64,12 @@ void oq_sys_init(oq_sys *s, int n_mols, int n_beads, const double boxl[3],
                  double T_K, double dtfs, double eps, double epsilonsc);
 
 void oq_ewald_params(const oq_sys *s, oq_ewald *e);
+
+/* A shorter O-O LJ cutoff (bohr) for boxes narrower than twice the
+ * reference's 9 A: r_skin = 1.3 r_cut, e_cut and list_size follow as
+ * qtip4pfparams.h:106-110 and water.cpp:135 derive them.  No counterpart in
+ * the reference, where the cutoff is a compile-time constant. */
+void oq_sys_set_lj_rcut(oq_sys *s, double r_cut);
 
 /* charges[n_atoms] and masses[n_dof] as simulation.cpp:811-822 */
 void oq_charges_masses(const oq_sys *s, double *charge, double *mass);

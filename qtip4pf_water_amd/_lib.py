@@ -35,7 +35,8 @@ class QtpParams(C.Structure):
                 ("T_K", C.c_double), ("dtfs", C.c_double), ("eps_ewald", C.c_double),
                 ("epsilon_sc", C.c_double), ("list_size", C.c_int),
                 ("ewald_kappa", C.c_double), ("ewald_rcut", C.c_double),
-                ("ewald_kcut", C.c_double), ("ewald_kmax", C.c_int)]
+                ("ewald_kcut", C.c_double), ("ewald_kmax", C.c_int),
+                ("lj_rcut", C.c_double)]
 
 
 class QtpDerived(C.Structure):
@@ -49,7 +50,8 @@ class QtpDerived(C.Structure):
                 ("acc", C.c_double), ("rho", C.c_double), ("kwald_prefactor", C.c_double),
                 ("sd_ox", C.c_double), ("sd_h", C.c_double),
                 ("kappa", C.c_double), ("rcut", C.c_double), ("k_cut", C.c_double),
-                ("k_max", C.c_int), ("n_kgroups", C.c_int)]
+                ("k_max", C.c_int), ("n_kgroups", C.c_int),
+                ("lj_rcut", C.c_double), ("r_skin", C.c_double)]
 
 
 # (name, restype, argtypes) for every entry point of include/qtip4pf.h

@@ -149,7 +149,7 @@ class Water:
     """One qTIP4P/F system on the GPU (the reference's globals + device state)."""
 
     def __init__(self, n_mols, n_beads, boxl, T_K=298.0, dtfs=0.25, eps_ewald=1e-6,
-                 epsilon_sc=1e-5, list_size=0, ewald=None):
+                 epsilon_sc=1e-5, list_size=0, ewald=None, lj_rcut=None):
         self._L = _lib.load()
         p = QtpParams()
         p.n_mols = int(n_mols)
@@ -161,6 +161,8 @@ class Water:
         if ewald is not None:
             p.ewald_kappa, p.ewald_rcut, p.ewald_kcut, p.ewald_kmax = (
                 float(ewald["kappa"]), float(ewald["rcut"]), float(ewald["k_cut"]), int(ewald["k_max"]))
+        if lj_rcut is not None:
+            p.lj_rcut = float(lj_rcut)
         h = C.c_void_p()
         rc = self._L.qtp_create(C.byref(p), C.byref(h))
         self._h = h

@@ -77,6 +77,7 @@ def centroid_dataset(n_frames, n_mols=216, P=32, T_K=298.0, dtfs=0.25, stride=80
     pos = np.zeros((n_frames, nd))
     fmean = np.zeros((n_frames, nd))
     fcls = np.zeros((n_frames, nd))
+    fhalf = np.zeros((n_frames, nd))   # mean over the first half of each run
     zero = np.zeros(nd)
     t0 = time.perf_counter()
     for f in range(n_frames):
@@ -85,21 +86,23 @@ def centroid_dataset(n_frames, n_mols=216, P=32, T_K=298.0, dtfs=0.25, stride=80
         cons.set_state(qb, pb)
         cons.run(n_eq, True)
         cons.centroid_force(True)
-        cons.run(n_acc, True)
+        cons.run(n_acc // 2, True)
+        mh, _, _ = cons.get_centroid_force()
+        cons.run(n_acc - n_acc // 2, True)
         mf, qc, n = cons.get_centroid_force()
         cons.centroid_force(False)
         if n != n_acc:
             raise RuntimeError(f"mean force over {n} steps, expected {n_acc}")
         cls.set_state(qc, zero)
         _, _, Fc, _, _ = cls.get_state()
-        pos[f], fmean[f], fcls[f] = qc, mf, Fc
+        pos[f], fmean[f], fcls[f], fhalf[f] = qc, mf, Fc, mh
         if (f + 1) % log_every == 0:
             el = time.perf_counter() - t0
             say(f"frame {f + 1}/{n_frames}  {el:.1f} s  "
                 f"rms(F_mean - F_cl) = {np.sqrt(np.mean((fmean[:f+1] - fcls[:f+1]) ** 2)):.3e}")
     for w in (main, cons, cls):
         w.close()
-    return dict(pos=pos, fmean=fmean, fcls=fcls, box=np.array(box), n_mols=n_mols, P=P,
+    return dict(pos=pos, fmean=fmean, fcls=fcls, fhalf=fhalf, box=np.array(box), n_mols=n_mols, P=P,
                 T_K=T_K, dtfs=dtfs, stride=stride, n_eq=n_eq, n_acc=n_acc, tau_fs=tau_fs)
 
 
@@ -166,11 +169,15 @@ def force_errors(model, data, frames):
         g.set_model(model["theta"], model["mean"], model["std"], model["e_scale"])
         _, Fnn = g.eval(data["pos"][frames])
     tgt = data["fmean"][frames] - data["fcls"][frames]
+    # sampling noise of each mean force: the two half-run means differ by
+    # twice the full-run mean's error, rms(first half - full) estimates it
+    noise = data["fhalf"][frames] - data["fmean"][frames]
     el = np.repeat(np.tile(np.array([0, 1, 1]), n_mols), 3)
     out = {}
     for name, sel in (("all", slice(None)), ("O", el == 0), ("H", el == 1)):
         out[name] = {"without": float(np.sqrt(np.mean(tgt[:, sel] ** 2))),
-                     "with": float(np.sqrt(np.mean((tgt[:, sel] - Fnn[:, sel]) ** 2)))}
+                     "with": float(np.sqrt(np.mean((tgt[:, sel] - Fnn[:, sel]) ** 2))),
+                     "target_noise": float(np.sqrt(np.mean(noise[:, sel] ** 2)))}
     return out
 
 
@@ -268,6 +275,7 @@ def main(argv=None):
     b.add_argument("--stride", type=int, default=80)
     b.add_argument("--n-acc", type=int, default=200)
     b.add_argument("--n-eq", type=int, default=20)
+    b.add_argument("--n-acc-val", type=int, default=1000)
     b.add_argument("--epochs", type=int, default=200)
     b.add_argument("--batch", type=int, default=8)
     b.add_argument("--lr", type=float, default=2e-3)
@@ -281,21 +289,30 @@ def main(argv=None):
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     if a.cmd == "build":
-        n = a.frames + a.val_frames
         t0 = time.perf_counter()
-        data = centroid_dataset(n, stride=a.stride, n_eq=a.n_eq, n_acc=a.n_acc)
+        data = centroid_dataset(a.frames, stride=a.stride, n_eq=a.n_eq, n_acc=a.n_acc, seed=11)
         tgen = time.perf_counter() - t0
-        np.savez_compressed(os.path.join(a.out, "centroid_data.npz"),
-                            **{k: v for k, v in data.items()})
-        say(f"data: {n} frames in {tgen:.1f} s")
+        say(f"training data: {a.frames} frames in {tgen:.1f} s")
+        # validation frames from an independent trajectory, longer averages
+        t0 = time.perf_counter()
+        val = centroid_dataset(a.val_frames, stride=a.stride, n_eq=a.n_eq, n_acc=a.n_acc_val,
+                               seed=97)
+        tval = time.perf_counter() - t0
+        say(f"validation data: {a.val_frames} frames in {tval:.1f} s")
+        # the validation frames only (the training set is ~90 MB)
+        np.savez_compressed(os.path.join(a.out, "centroid_val.npz"),
+                            **{k: (v.astype(np.float32) if isinstance(v, np.ndarray) and v.ndim == 2
+                                   else v) for k, v in val.items()})
         t0 = time.perf_counter()
         model, trace = fit(data, a.frames, epochs=a.epochs, batch=a.batch, lr=a.lr)
         ttr = time.perf_counter() - t0
-        errs = force_errors(model, data, np.arange(a.frames, n))
+        errs = force_errors(model, val, np.arange(a.val_frames))
         errs_train = force_errors(model, data, np.arange(0, min(a.frames, 200)))
         meta = dict(frames=a.frames, val_frames=a.val_frames, stride=a.stride, n_acc=a.n_acc,
-                    n_eq=a.n_eq, epochs=a.epochs, batch=a.batch, lr=a.lr, gen_seconds=tgen,
-                    train_seconds=ttr, val_force_rms=errs, train_force_rms=errs_train,
+                    n_acc_val=a.n_acc_val, n_eq=a.n_eq, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                    gen_seconds=tgen, val_gen_seconds=tval, train_seconds=ttr,
+                    val_force_rms=errs, train_force_rms=errs_train,
+                    final_loss=float(trace[-1]),
                     system="216 H2O x 32 beads, 298 K, PILE-L tau 100 fs, dt 0.25 fs")
         save_model(os.path.join(a.out, "cmd_surrogate_216x32.npz"), model, meta)
         np.save(os.path.join(a.out, "loss_trace.npy"), trace)

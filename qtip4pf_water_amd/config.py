@@ -228,3 +228,37 @@ def free_ring_polymer_beads(q0, n_beads, beta_n, seed=0):
         wk = 2.0 * math.sin(k * math.pi / P) / beta_n
         u[k] = rng.standard_normal(q0.size) / np.sqrt(beta_n * mass * wk * wk)
     return np.ascontiguousarray((C.T @ u).reshape(-1))
+
+
+# Batched replicas: molecule counts from {64, 128, 216, 343, 512} at 0.997
+# g/cm^3 (cubic boxes), bead counts from {8, 16, 32, 64}, temperatures
+# uniform in 250-350 K, one master seed.  Boxes narrower than twice the 9 A
+# LJ cutoff (64 and 128 molecules) take the cutoff L/2.
+BATCH_SIZES = {64: 12.43, 128: 15.66, 216: 18.64, 343: 21.75, 512: 24.86}
+
+
+def lj_rcut_for(L_angs):
+    """LJ cutoff (bohr) of a cubic box: 9 A, or L/2 when the box is narrower."""
+    return min(9.0, 0.5 * L_angs) * ANGS2BOHR
+
+
+def batch_replicas(n_rep, seed):
+    rng = np.random.default_rng(seed)
+    reps = []
+    for r in range(n_rep):
+        n = int(rng.choice(sorted(BATCH_SIZES)))
+        L = BATCH_SIZES[n]
+        reps.append(dict(n_mols=n, L_angs=L, P=int(rng.choice([8, 16, 32, 64])),
+                         T_K=float(rng.uniform(250.0, 350.0)), seed=int(seed * 1000 + r),
+                         lj_rcut=lj_rcut_for(L)))
+    return reps
+
+
+def replica_state(r):
+    """Seeded start of one replica: lattice molecules, free-ring-polymer beads,
+    Maxwell-Boltzmann momenta at its bead temperature."""
+    q0, box = water_box(r["n_mols"], r["L_angs"], seed=r["seed"])
+    beta_n = 1.0 / (K2AU * r["T_K"]) / r["P"]
+    q = free_ring_polymer_beads(q0, r["P"], beta_n, seed=r["seed"] + 1)
+    p = sample_momenta(r["n_mols"], r["P"], beta_n, seed=r["seed"] + 2)
+    return q, p, box

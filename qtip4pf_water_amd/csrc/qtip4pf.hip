@@ -81,9 +81,15 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     f.epslj24 = 24.0 * RC.epslj;
     f.sigmalj6 = RC.sigmalj6;
     f.sigmalj12 = RC.sigmalj12;
-    f.e_cut = RC.e_cut;
-    f.r_cut_sq = RC.r_cut_sq;
-    f.r_skin_sq = RC.r_skin_sq;
+    {
+        // qtip4pfparams.h:106-110 at the context's cutoff (the reference's by default)
+        const double rc = ctx->drv.lj_rcut, rs = ctx->drv.r_skin;
+        f.e_cut = rc == RC.r_cut ? RC.e_cut
+                                 : 4.0 * RC.epslj * (std::pow((RC.sigmalj / rc), 12.0) -
+                                                     std::pow((RC.sigmalj / rc), 6.0));
+        f.r_cut_sq = rc * rc;
+        f.r_skin_sq = rs * rs;
+    }
     f.roheq = RC.roheq;
     f.ar2 = RC.ar2;
     f.ar3 = RC.ar3;
@@ -235,7 +241,7 @@ void qtp_destroy(qtp_ctx *ctx)
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace,
-                    ctx->nn_e, ctx->ms, ctx->cmf_acc, ctx->cmf_n, ctx->qc, ctx->rdf_oo,
+                    ctx->ms, ctx->cmf_acc, ctx->cmf_n, ctx->qc, ctx->rdf_oo,
                     ctx->rdf_oh};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -347,12 +353,12 @@ int qtp_neighbour_list(qtp_ctx *ctx, const double *q, int *point, int *list, dou
     {
         TimeScope ts(ctx, "nlist");
         k_nl_count<<<ctx->P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box,
-                                                RC.r_skin_sq, ctx->flag, ctx->cnt);
+                                                ctx->ff.r_skin_sq, ctx->flag, ctx->cnt);
         k_nl_scan<<<ctx->P, 1024, 0, s>>>(ctx->n_mols, ctx->list_size, ctx->fcap, ctx->flag,
                                           ctx->cnt, ctx->point, ctx->fpoint, ctx->list, ctx->disp,
                                           ctx->NMP, ctx->status);
         k_nl_fill<<<ctx->P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box,
-                                               RC.r_skin_sq, ctx->list_size, ctx->fcap, ctx->flag,
+                                               ctx->ff.r_skin_sq, ctx->list_size, ctx->fcap, ctx->flag,
                                                ctx->point, ctx->fpoint, ctx->list, ctx->flist);
     }
     HIPCHK(hipGetLastError());
@@ -799,7 +805,8 @@ int qtp_get_state(qtp_ctx *ctx, double *q, double *p, double *F, double *fphys, 
 
 // File layout (little endian): "QTPCKPT1", int32 version (2), int32 n_mols,
 // n_beads, list_size, k_max; double boxl[3], T_K, dtfs, eps_ewald,
-// epsilon_sc, kappa, r_c, k_cut (the effective Ewald parameters); uint64
+// epsilon_sc, kappa, r_c, k_cut (the effective Ewald parameters), the LJ
+// cutoff; uint64
 // seed, step counter; double heat; then in the reference layout: q, p
 // (n_beads * n_dof), disp (n_beads * 3 * n_mols), point (n_beads * n_mols
 // int32), list (n_beads * list_size int32), rebuild counts (n_beads doubles).
@@ -807,7 +814,7 @@ int qtp_get_state(qtp_ctx *ctx, double *q, double *p, double *F, double *fphys, 
 // parameters differ is rejected: the run would not continue as saved.
 static const char QTP_CKPT_MAGIC[8] = {'Q', 'T', 'P', 'C', 'K', 'P', 'T', '1'};
 static const int32_t QTP_CKPT_VERSION = 2;
-static const int QTP_CKPT_NPAR = 10;
+static const int QTP_CKPT_NPAR = 11;
 
 static void ckpt_header(const qtp_ctx *ctx, int32_t hdr[5], double par[QTP_CKPT_NPAR])
 {
@@ -819,7 +826,7 @@ static void ckpt_header(const qtp_ctx *ctx, int32_t hdr[5], double par[QTP_CKPT_
     const double v[QTP_CKPT_NPAR] = {ctx->drv.boxl[0], ctx->drv.boxl[1], ctx->drv.boxl[2],
                                      ctx->prm.T_K,     ctx->prm.dtfs,    ctx->prm.eps_ewald,
                                      ctx->prm.epsilon_sc, ctx->drv.kappa, ctx->drv.rcut,
-                                     ctx->drv.k_cut};
+                                     ctx->drv.k_cut, ctx->drv.lj_rcut};
     for (int i = 0; i < QTP_CKPT_NPAR; ++i) par[i] = v[i];
 }
 
@@ -901,7 +908,7 @@ int qtp_load_checkpoint(qtp_ctx *ctx, const char *path, const double *charge, co
     }
     static const char *const what[QTP_CKPT_NPAR] = {
         "box", "box", "box", "T_K", "dtfs", "eps_ewald", "epsilon_sc", "Ewald kappa",
-        "Ewald r_c", "Ewald k_cut"};
+        "Ewald r_c", "Ewald k_cut", "LJ cutoff"};
     for (int k = 0; k < QTP_CKPT_NPAR; ++k)
         if (par[k] != want_par[k]) {
             ctx->err = std::string("checkpoint ") + what[k] + " differs from the context's";
@@ -1338,16 +1345,15 @@ int qtp_nn_create(int n_mols, const double boxl[3], int max_frames, qtp_nn **out
         (st = nalloc(nn, &nn->sq, NA)) || (st = nalloc(nn, &nn->nbr, NA * NMAX)) ||
         (st = nalloc(nn, &nn->nn, NA)) || (st = nalloc(nn, &nn->nba, NA * NAMAX)) ||
         (st = nalloc(nn, &nn->nna, NA)) || (st = nalloc(nn, &nn->status, 1)) ||
+        (st = nalloc(nn, &nn->dang, NA * NAMAX)) || (st = nalloc(nn, &nn->dself, NA)) ||
         (st = nalloc(nn, &nn->tp, (size_t)(b.tiles_o + b.tiles_h) * 3 * NH)) ||
         (st = nalloc(nn, &nn->part, (size_t)(b.nco + b.nch) * WG_PART)) ||
         (st = nalloc(nn, &nn->inter, nn_inter_elems(b))))
         return st;
     std::vector<float> one(2 * NIN, 1.0f);
     NNCHK(hipMemcpy(nn->stdv, one.data(), one.size() * sizeof(float), hipMemcpyHostToDevice));
-    const void *big[] = {(const void *)k_nn_mlp, (const void *)k_nn_train_fwd};
-    const int lds[] = {MLP_LDS, TRAIN_LDS};
-    for (int t = 0; t < 2; ++t)
-        NNCHK(hipFuncSetAttribute(big[t], hipFuncAttributeMaxDynamicSharedMemorySize, lds[t]));
+    NNCHK(hipFuncSetAttribute((const void *)k_nn_train_fwd,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, TRAIN_LDS));
     return QTP_OK;
 }
 
@@ -1359,6 +1365,7 @@ void qtp_nn_destroy(qtp_nn *nn)
                     nn->b1, nn->b2, nn->w3, nn->b3, nn->mean, nn->stdv, nn->pos, nn->tgt,
                     nn->fwork, nn->dsum, nn->gt, nn->ut, nn->dEdG, nn->graw, nn->Eatom,
                     nn->res, nn->sq, nn->nbr, nn->nn, nn->nba, nn->nna, nn->status, nn->tp,
+                    nn->dang, nn->dself,
                     nn->part, nn->inter, nn->dpos, nn->dtgt};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -1435,15 +1442,10 @@ int qtp_nn_eval(qtp_nn *nn, int n_frames, const double *pos, double *energy, dou
         hipStream_t s = nn->stream;
         NNCHK(hipMemsetAsync(nn->status, 0, sizeof(int), s));
         NNCHK(hipMemsetAsync(nn->fwork, 0, (size_t)F * N3 * sizeof(double), s));
-        k_nn_desc<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, nn->pos, nn->mean, nn->stdv, nn->gt,
-                                                            nn->nbr, nn->nn, nn->nba, nn->nna,
-                                                            descriptors ? nn->graw : nullptr,
-                                                            nn->status);
-        k_nn_mlp<<<b.nbo + b.nbh, 64 * MLP_WAVES, MLP_LDS, s>>>(g, nn_weights(nn), b.nbo, nn->gt,
-                                                               nn->stdv, nn->dEdG, nn->Eatom);
+        CHK(nn_forward(nn, s, nn->pos, F, b, nn->status, descriptors ? nn->graw : nullptr));
         k_nn_force<0><<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, nn->pos, nn->nbr, nn->nn, nn->nba,
-                                                             nn->nna, nn->dEdG, nn->fwork, nullptr,
-                                                             nullptr, nullptr);
+                                                             nn->nna, nn->dEdG, nn->dang, nn->dself,
+                                                             nn->fwork, nullptr, nullptr, nullptr);
         for (int f = 0; f < F; ++f)
             k_nn_sum<<<1, 1024, 0, s>>>(nn->n_atoms, nn->Eatom + (long)f * nn->n_atoms, 1.0, 0,
                                         nn->dsum + f);
@@ -1560,7 +1562,6 @@ int qtp_set_nn(qtp_ctx *ctx, qtp_nn *nn)
             ctx->err = "model without parameters (qtp_nn_set_model)";
             return QTP_ERR_STATE;
         }
-        if (!ctx->nn_e) CHK(dalloc(ctx, &ctx->nn_e, 1));
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
     invalidate_graphs(ctx);

@@ -117,7 +117,6 @@ struct qtp_ctx {
     hipGraphExec_t graph_obs[2] = {nullptr, nullptr};
     // learned centroid-force correction added to classical resident steps
     qtp_nn *nn = nullptr;
-    double *nn_e = nullptr;   // its energy of the last evaluation
     bool timing = false;
     std::map<std::string, Timer> timers;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;

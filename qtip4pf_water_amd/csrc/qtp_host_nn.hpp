@@ -22,6 +22,7 @@ struct qtp_nn {
     float *gt = nullptr, *ut = nullptr, *dEdG = nullptr, *Eatom = nullptr, *res = nullptr;
     float *sq = nullptr, *graw = nullptr, *part = nullptr, *tp = nullptr, *grad = nullptr;
     int *nbr = nullptr, *nn = nullptr, *nba = nullptr, *nna = nullptr, *status = nullptr;
+    float4 *dang = nullptr, *dself = nullptr;   // angular force pieces per centre
     __bf16 *inter = nullptr;   // the eight training intermediates, both elements
     size_t inter_elems = 0;
     // training data (device resident)
@@ -50,21 +51,24 @@ static int nalloc(qtp_nn *nn, T **p, size_t n)
     return QTP_OK;
 }
 
-// Launch geometry of a batch of F frames.
+// Launch geometry of a batch of F frames: per-atom kernels (DESC_WAVES atoms
+// of one frame per block), inference tiles of 16 rows per element, training
+// tiles padded to TRAIN_WAVES per block, weight-gradient chunks.
 struct NNBatch {
-    int F, nblk_atom, tiles_o, tiles_h, nbo, nbh, rows_o, rows_h, nco, nch;
+    int F, nblk_atom, to, th, nbo, nbh, tiles_o, tiles_h, rows_o, rows_h, nco, nch;
 };
 
 static NNBatch nn_batch(const qtp_nn *nn, int F)
 {
     NNBatch b;
     b.F = F;
-    b.nblk_atom = (F * nn->n_atoms + DESC_WAVES - 1) / DESC_WAVES;
-    const int to = (F * nn->n_mols + 15) / 16, th = (2 * F * nn->n_mols + 15) / 16;
-    b.nbo = (to + MLP_WAVES - 1) / MLP_WAVES;
-    b.nbh = (th + MLP_WAVES - 1) / MLP_WAVES;
-    b.tiles_o = b.nbo * MLP_WAVES;
-    b.tiles_h = b.nbh * MLP_WAVES;
+    b.nblk_atom = F * ((nn->n_atoms + DESC_WAVES - 1) / DESC_WAVES);
+    b.to = (F * nn->n_mols + 15) / 16;
+    b.th = (2 * F * nn->n_mols + 15) / 16;
+    b.nbo = (b.to + TRAIN_WAVES - 1) / TRAIN_WAVES;
+    b.nbh = (b.th + TRAIN_WAVES - 1) / TRAIN_WAVES;
+    b.tiles_o = b.nbo * TRAIN_WAVES;
+    b.tiles_h = b.nbh * TRAIN_WAVES;
     b.rows_o = ((b.tiles_o * 16 + WG_ROWS - 1) / WG_ROWS) * WG_ROWS;
     b.rows_h = ((b.tiles_h * 16 + WG_ROWS - 1) / WG_ROWS) * WG_ROWS;
     b.nco = b.rows_o / WG_ROWS;
@@ -80,6 +84,10 @@ static NNGeom nn_geom(const qtp_nn *nn, int F)
     g.n_frames = F;
     g.NAP = nn->NAP;
     g.box = nn->box;
+    for (int k = 0; k < 3; ++k) {
+        g.Lf[k] = (float)(nn->box.L[k] * ANGD);
+        g.iLf[k] = 1.0f / g.Lf[k];
+    }
     return g;
 }
 
@@ -137,15 +145,18 @@ static size_t nn_inter_elems(const NNBatch &b)
     return (size_t)(6 * NH + 2 * KIN) * (b.rows_o + b.rows_h);
 }
 
-// Lists, descriptors, energies and dE/dG of F frames at pos (planar, device).
-static int nn_forward(qtp_nn *nn, hipStream_t s, const double *pos, int F, const NNBatch &b)
+// Lists, descriptors, energies, dE/dG and the angular force pieces of F
+// frames at pos (planar, device), the atoms' status bits into status.
+static int nn_forward(qtp_nn *nn, hipStream_t s, const double *pos, int F, const NNBatch &b,
+                      int *status, float *graw = nullptr)
 {
     const NNGeom g = nn_geom(nn, F);
     k_nn_desc<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, pos, nn->mean, nn->stdv, nn->gt, nn->nbr,
-                                                        nn->nn, nn->nba, nn->nna, nullptr,
-                                                        nn->status);
-    k_nn_mlp<<<b.nbo + b.nbh, 64 * MLP_WAVES, MLP_LDS, s>>>(g, nn_weights(nn), b.nbo, nn->gt,
-                                                           nn->stdv, nn->dEdG, nn->Eatom);
+                                                        nn->nn, nn->nba, nn->nna, graw, status);
+    k_nn_mlp<<<b.to + b.th, 64 * MLP_WAVES, 0, s>>>(g, nn_weights(nn), b.to, nn->gt, nn->stdv,
+                                                    nn->dEdG, nn->Eatom);
+    k_nn_dang<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, pos, nn->nba, nn->nna, nn->dEdG,
+                                                        nn->dang, nn->dself);
     NNCHK(hipGetLastError());
     return QTP_OK;
 }
@@ -184,9 +195,10 @@ static int nn_train_batch(qtp_nn *nn, hipStream_t s, int f0, int F, int mode, do
     const NNGeom g = nn_geom(nn, F);
     const long N3 = 3L * nn->NAP;
     const double *pos = nn->dpos + (long)f0 * N3, *tgt = nn->dtgt + (long)f0 * N3;
-    CHK(nn_forward(nn, s, pos, F, b));
+    CHK(nn_forward(nn, s, pos, F, b, nn->status));
     k_nn_force<1><<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, pos, nn->nbr, nn->nn, nn->nba, nn->nna,
-                                                         nn->dEdG, nullptr, tgt, nn->res, nn->sq);
+                                                         nn->dEdG, nn->dang, nn->dself, nullptr,
+                                                         tgt, nn->res, nn->sq);
     const long ncomp = 3L * F * nn->n_atoms;
     k_nn_sum<<<1, 1024, 0, s>>>(F * nn->n_atoms, nn->sq, 1.0 / (double)ncomp, 0, loss_dev);
     k_nn_tan<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, pos, nn->nbr, nn->nn, nn->nba, nn->nna,
@@ -216,31 +228,35 @@ static int nn_train_batch(qtp_nn *nn, hipStream_t s, int f0, int F, int mode, do
 
 }  // namespace
 
-// The correction's force on the resident classical state, added into fph,
-// and its energy into ctx->nn_e (read by k_vv_b).
+// The correction's force on the resident classical state, added into fph;
+// its per-atom energies stay in nn->Eatom (summed by k_vv_b).
 static int run_nn_forces(qtp_ctx *ctx)
 {
     qtp_nn *nn = ctx->nn;
     hipStream_t s = ctx->stream;
     const NNBatch b = nn_batch(nn, 1);
+    const NNGeom g = nn_geom(nn, 1);
     {
         TimeScope ts(ctx, "nn_desc");
-        k_nn_desc<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(nn_geom(nn, 1), ctx->q, nn->mean,
-                                                            nn->stdv, nn->gt, nn->nbr, nn->nn,
-                                                            nn->nba, nn->nna, nullptr, ctx->status);
+        k_nn_desc<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, ctx->q, nn->mean, nn->stdv, nn->gt,
+                                                            nn->nbr, nn->nn, nn->nba, nn->nna,
+                                                            nullptr, ctx->status);
     }
     {
         TimeScope ts(ctx, "nn_mlp");
-        k_nn_mlp<<<b.nbo + b.nbh, 64 * MLP_WAVES, MLP_LDS, s>>>(nn_geom(nn, 1), nn_weights(nn),
-                                                               b.nbo, nn->gt, nn->stdv, nn->dEdG,
-                                                               nn->Eatom);
+        k_nn_mlp<<<b.to + b.th, 64 * MLP_WAVES, 0, s>>>(g, nn_weights(nn), b.to, nn->gt, nn->stdv,
+                                                        nn->dEdG, nn->Eatom);
+    }
+    {
+        TimeScope ts(ctx, "nn_dang");
+        k_nn_dang<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, ctx->q, nn->nba, nn->nna, nn->dEdG,
+                                                            nn->dang, nn->dself);
     }
     {
         TimeScope ts(ctx, "nn_force");
-        k_nn_force<0><<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(
-            nn_geom(nn, 1), ctx->q, nn->nbr, nn->nn, nn->nba, nn->nna, nn->dEdG, ctx->fph,
-            nullptr, nullptr, nullptr);
-        k_nn_sum<<<1, 1024, 0, s>>>(nn->n_atoms, nn->Eatom, 1.0, 0, ctx->nn_e);
+        k_nn_force<0><<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, ctx->q, nn->nbr, nn->nn, nn->nba,
+                                                             nn->nna, nn->dEdG, nn->dang, nn->dself,
+                                                             ctx->fph, nullptr, nullptr, nullptr);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;

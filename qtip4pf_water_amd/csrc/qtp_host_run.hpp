@@ -216,7 +216,7 @@ static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
     if (ctx->n_mols <= NL_FUSED_MAXMOL) {
         TimeScope ts(ctx, "nlist");
         k_nlist_bead<<<P + (with_heat ? 1 : 0), 1024, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box,
-                                        RC.r_skin_sq, RC.r_skin - RC.r_cut, force_all,
+                                        ctx->ff.r_skin_sq, ctx->drv.r_skin - ctx->drv.lj_rcut, force_all,
                                         ctx->list_size, ctx->fcap, ctx->disp, ctx->flag, ctx->nreb,
                                         ctx->point, ctx->fpoint, ctx->list, ctx->flist,
                                         ctx->status, heat);
@@ -228,18 +228,18 @@ static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
                                 heat.heat_step);
     {
         TimeScope ts(ctx, "check");
-        k_check<<<P, 256, 0, s>>>(ctx->n_mols, ctx->NMP, ctx->disp, RC.r_skin - RC.r_cut, force_all,
+        k_check<<<P, 256, 0, s>>>(ctx->n_mols, ctx->NMP, ctx->disp, ctx->drv.r_skin - ctx->drv.lj_rcut, force_all,
                                   ctx->flag, ctx->nreb);
     }
     const int nrb = (ctx->n_mols + 3) / 4;
     {
         TimeScope ts(ctx, "nlist");
-        k_nl_count<<<P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box, RC.r_skin_sq,
+        k_nl_count<<<P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box, ctx->ff.r_skin_sq,
                                            ctx->flag, ctx->cnt);
         k_nl_scan<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->list_size, ctx->fcap, ctx->flag, ctx->cnt,
                                      ctx->point, ctx->fpoint, ctx->list, ctx->disp, ctx->NMP,
                                      ctx->status);
-        k_nl_fill<<<P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box, RC.r_skin_sq,
+        k_nl_fill<<<P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box, ctx->ff.r_skin_sq,
                                           ctx->list_size, ctx->fcap, ctx->flag, ctx->point,
                                           ctx->fpoint, ctx->list, ctx->flist);
     }
@@ -328,7 +328,7 @@ static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr, bool resident =
         TimeScope ts(ctx, "vv_b");
         k_vv_b<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->drv.halfdt, ctx->fph, ctx->epart,
                                   ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p, ctx->out,
-                                  ctr, with_nn ? ctx->nn_e : nullptr);
+                                  ctr, with_nn ? ctx->nn->Eatom : nullptr);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;
@@ -344,7 +344,7 @@ static int class_initial_forces(qtp_ctx *ctx)
     const int nb = (3 * ctx->NAP + 255) / 256;
     k_vv_b<<<nb, 256, 0, ctx->stream>>>(ctx->n_atoms, ctx->NAP, 0.0, ctx->fph, ctx->epart,
                                         ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p,
-                                        ctx->out, nullptr, ctx->nn ? ctx->nn_e : nullptr);
+                                        ctx->out, nullptr, ctx->nn ? ctx->nn->Eatom : nullptr);
     HIPCHK(hipGetLastError());
     return QTP_OK;
 }

@@ -18,9 +18,12 @@ static void derive(qtp_ctx *ctx)
     for (int k = 0; k < 3; ++k) d.boxl[k] = P.boxl[k];
     d.vol = P.boxl[0] * P.boxl[1] * P.boxl[2];
     const double rho_ox = P.n_mols / d.vol;
+    // qtip4pfparams.h:41, 109-110 (the cutoff optionally shorter)
+    d.lj_rcut = P.lj_rcut > 0.0 ? P.lj_rcut : RC.r_cut;
+    d.r_skin = 1.30 * d.lj_rcut;
     d.list_size = P.list_size > 0
                       ? P.list_size
-                      : (int)std::round(4.0 * RC.r_skin * RC.r_skin * RC.r_skin * P.n_mols * rho_ox);
+                      : (int)std::round(4.0 * d.r_skin * d.r_skin * d.r_skin * P.n_mols * rho_ox);
     // water.cpp:76-83, 138-157
     const double T_red = RC.K2au * P.T_K;
     d.beta = 1.0 / T_red;

@@ -33,7 +33,8 @@ constexpr float RR = 6.0f, RA = 4.0f, MU0 = 0.8f, DMU = 5.2f / 23.0f;
 constexpr float ETAR = 0.5f / (DMU * DMU);
 constexpr float ETA_A0 = 0.02f, ETA_A1 = 0.2f;
 constexpr float PIF = 3.14159265358979f;
-constexpr int DESC_WAVES = 4;    // atoms per block of the per-atom kernels
+constexpr int DESC_WAVES = 8;    // atoms (wavefronts) per block of the per-atom kernels
+constexpr int CHUNK = 512;       // candidate atoms staged in LDS per round of the list search
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -68,19 +69,48 @@ __device__ __forceinline__ int row_atom(int e, int r, int n_mols, int n_frames, 
 
 struct NNGeom {
     int n_atoms, n_mols, n_frames, NAP;
-    Box box;              // bohr
+    Box box;               // bohr
+    float Lf[3], iLf[3];   // box edges in Angstrom (float) and their inverses
 };
 
-// Minimum-image separation x_j - x_i (Angstrom, float) of atoms i, j of one
-// frame (planar FP64 positions, bohr).
-__device__ __forceinline__ float3 sep(const double *__restrict__ X, int NAP, const Box &box, int i,
-                                      int j)
+// Coordinate d of atom a in Angstrom (float), wrapped into the box first in
+// FP64 (positions are never wrapped in the state).  Every kernel forms
+// separations from these values the same way (sepf), so lists, descriptors,
+// forces and tangents see identical geometry.
+__device__ __forceinline__ float posf(const double *__restrict__ X, int NAP, const Box &box, int d,
+                                      int a)
 {
-    double dx = X[j] - X[i], dy = X[NAP + j] - X[NAP + i], dz = X[2 * NAP + j] - X[2 * NAP + i];
-    dx = mimg_fast(dx, box.L[0], box.inv[0]);
-    dy = mimg_fast(dy, box.L[1], box.inv[1]);
-    dz = mimg_fast(dz, box.L[2], box.inv[2]);
-    return make_float3((float)(dx * ANGD), (float)(dy * ANGD), (float)(dz * ANGD));
+    const double x = X[d * NAP + a];
+    return (float)((x - box.L[d] * floor(x * box.inv[d])) * ANGD);
+}
+__device__ __forceinline__ float mimf(float d, float L, float iL) { return d - L * rintf(d * iL); }
+
+// Minimum-image separation x_j - x_i (Angstrom) and its length.
+__device__ __forceinline__ float4 sepf(const double *__restrict__ X, const NNGeom &g, int i, int j)
+{
+    const float dx = mimf(posf(X, g.NAP, g.box, 0, j) - posf(X, g.NAP, g.box, 0, i), g.Lf[0], g.iLf[0]);
+    const float dy = mimf(posf(X, g.NAP, g.box, 1, j) - posf(X, g.NAP, g.box, 1, i), g.Lf[1], g.iLf[1]);
+    const float dz = mimf(posf(X, g.NAP, g.box, 2, j) - posf(X, g.NAP, g.box, 2, i), g.Lf[2], g.iLf[2]);
+    return make_float4(dx, dy, dz, sqrtf(dx * dx + dy * dy + dz * dz));
+}
+
+// Blocks of the per-atom kernels hold DESC_WAVES atoms of one frame:
+// wavefront w of block b takes atom i of frame f; false past the frame's end.
+__device__ __forceinline__ int atoms_blocks(int n_atoms) { return (n_atoms + DESC_WAVES - 1) / DESC_WAVES; }
+__device__ __forceinline__ bool block_atom(const NNGeom &g, int &f, int &i)
+{
+    const int bpf = atoms_blocks(g.n_atoms);
+    f = blockIdx.x / bpf;
+    i = (blockIdx.x - f * bpf) * DESC_WAVES + (threadIdx.x >> 6);
+    return i < g.n_atoms;
+}
+
+// tanh(x) = 1 - 2 / (exp(2x) + 1): saturates to +-1 without overflow; its
+// absolute error (a few 1e-7) is far below the BF16 rounding of the
+// activations that follows.
+__device__ __forceinline__ float tanh_fast(float x)
+{
+    return 1.0f - __fdividef(2.0f, __expf(2.0f * x) + 1.0f);
 }
 
 __device__ __forceinline__ float fcut(float r, float R)
@@ -144,10 +174,12 @@ __device__ __forceinline__ void tri_decode(int p, int &j, int &k)
 
 // ---------------------------------------------------------------------------
 // Neighbour lists and normalised descriptors.  One wavefront per atom,
-// DESC_WAVES atoms per block.  Lists hold, in ascending index order, every
-// atom within RR (nbr, count nn) and the subset within RA (nba, count nna).
-// status bit 2: a list was full (the atom's lists are then truncated).
-// grid: ceil(n_frames * n_atoms / DESC_WAVES), 64 * DESC_WAVES threads.
+// DESC_WAVES atoms of one frame per block; the frame's atoms pass through
+// LDS in rounds of CHUNK (float, Angstrom) and every wavefront tests them
+// against its atom.  Lists hold, in ascending index order, every atom within
+// RR (nbr, count nn) and the subset within RA (nba, count nna).  status bit
+// 2: a list was full (the atom's lists are then truncated).
+// grid: n_frames * atoms_blocks(n_atoms), 64 * DESC_WAVES threads.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_desc(
     NNGeom g, const double *__restrict__ pos, const float *__restrict__ mean,
@@ -155,47 +187,68 @@ __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_desc(
     int *__restrict__ nn, int *__restrict__ nba, int *__restrict__ nna, float *__restrict__ graw,
     int *__restrict__ status)
 {
+    __shared__ float4 sp[CHUNK];
     __shared__ float4 sv[DESC_WAVES][NMAX];   // x_j - x_i (A), r
     __shared__ int sj[DESC_WAVES][NMAX];      // neighbour atom
     __shared__ int sa[DESC_WAVES][NAMAX];     // angular neighbours: slot in sv
     __shared__ float sg[DESC_WAVES][NIN];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int gi = blockIdx.x * DESC_WAVES + w;
-    const int N = g.n_frames * g.n_atoms;
-    if (gi >= N) return;   // whole wavefronts leave together; no block barrier below
-    const int f = gi / g.n_atoms, i = gi - f * g.n_atoms;
+    const int bpf = atoms_blocks(g.n_atoms);
+    const int f = blockIdx.x / bpf, i = (blockIdx.x - f * bpf) * DESC_WAVES + w;
+    const bool valid = i < g.n_atoms;
+    const int gi = f * g.n_atoms + i;
     const double *X = pos + (long)f * 3 * g.NAP;
-    const int ei = elem_of(i);
+    float xi = 0.f, yi = 0.f, zi = 0.f;
+    if (valid) {
+        xi = posf(X, g.NAP, g.box, 0, i);
+        yi = posf(X, g.NAP, g.box, 1, i);
+        zi = posf(X, g.NAP, g.box, 2, i);
+    }
     const float rr2 = RR * RR, ra2 = RA * RA;
     int cnt = 0, cnta = 0, over = 0;
-    for (int j0 = 0; j0 < g.n_atoms; j0 += 64) {
-        const int j = j0 + lane;
-        float3 d = make_float3(0.f, 0.f, 0.f);
-        float r2 = 1e30f;
-        if (j < g.n_atoms && j != i) {
-            d = sep(X, g.NAP, g.box, i, j);
-            r2 = d.x * d.x + d.y * d.y + d.z * d.z;
+    for (int c0 = 0; c0 < g.n_atoms; c0 += CHUNK) {
+        __syncthreads();
+        for (int t = threadIdx.x; t < CHUNK; t += 64 * DESC_WAVES) {
+            const int j = c0 + t;
+            sp[t] = j < g.n_atoms ? make_float4(posf(X, g.NAP, g.box, 0, j), posf(X, g.NAP, g.box, 1, j),
+                                                posf(X, g.NAP, g.box, 2, j), 0.f)
+                                  : make_float4(0.f, 0.f, 0.f, 0.f);
         }
-        const bool in = r2 < rr2;
-        const unsigned long long b = __ballot(in);
-        const int s = cnt + __popcll(b & ((1ull << lane) - 1ull));
-        const bool keep = in && s < NMAX;
-        const bool ina = keep && r2 < ra2;
-        const unsigned long long ba = __ballot(ina);
-        const int sa_slot = cnta + __popcll(ba & ((1ull << lane) - 1ull));
-        if (keep) {
-            sv[w][s] = make_float4(d.x, d.y, d.z, sqrtf(r2));
-            sj[w][s] = j;
-            nbr[(long)gi * NMAX + s] = j;
-            if (ina && sa_slot < NAMAX) {
-                sa[w][sa_slot] = s;
-                nba[(long)gi * NAMAX + sa_slot] = j;
+        __syncthreads();
+        if (!valid) continue;
+        const int m = min(CHUNK, g.n_atoms - c0);
+        for (int s0 = 0; s0 < m; s0 += 64) {
+            const int t = s0 + lane, j = c0 + t;
+            float dx = 0.f, dy = 0.f, dz = 0.f, r2 = 1e30f;
+            if (t < m && j != i) {
+                const float4 pj = sp[t];
+                dx = mimf(pj.x - xi, g.Lf[0], g.iLf[0]);
+                dy = mimf(pj.y - yi, g.Lf[1], g.iLf[1]);
+                dz = mimf(pj.z - zi, g.Lf[2], g.iLf[2]);
+                r2 = dx * dx + dy * dy + dz * dz;
             }
+            const bool in = r2 < rr2;
+            const unsigned long long b = __ballot(in);
+            const int s = cnt + __popcll(b & ((1ull << lane) - 1ull));
+            const bool keep = in && s < NMAX;
+            const bool ina = keep && r2 < ra2;
+            const unsigned long long ba = __ballot(ina);
+            const int sa_slot = cnta + __popcll(ba & ((1ull << lane) - 1ull));
+            if (keep) {
+                sv[w][s] = make_float4(dx, dy, dz, sqrtf(r2));
+                sj[w][s] = j;
+                nbr[(long)gi * NMAX + s] = j;
+                if (ina && sa_slot < NAMAX) {
+                    sa[w][sa_slot] = s;
+                    nba[(long)gi * NAMAX + sa_slot] = j;
+                }
+            }
+            over |= (cnt + __popcll(b) > NMAX) || (cnta + __popcll(ba) > NAMAX);
+            cnt += __popcll(b);
+            cnta += __popcll(ba);
         }
-        over |= (cnt + __popcll(b) > NMAX) || (cnta + __popcll(ba) > NAMAX);
-        cnt += __popcll(b);
-        cnta += __popcll(ba);
     }
+    if (!valid) return;   // past the block's last barrier
     cnt = min(cnt, NMAX);
     cnta = min(cnta, NAMAX);
     if (lane == 0) {
@@ -205,6 +258,7 @@ __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_desc(
     }
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const int ei = elem_of(i);
     // radial
     float acc[NR];
 #pragma unroll
@@ -293,26 +347,6 @@ __device__ __forceinline__ bf16x8 afrag_f32(const float *__restrict__ src, long 
     return a;
 }
 
-// Stage one element's weights in LDS (block-cooperative, 16-byte copies).
-__device__ __forceinline__ void stage_weights(const NNW &W, int e, __bf16 *s_w1, __bf16 *s_w1t,
-                                              __bf16 *s_w2, __bf16 *s_w2t)
-{
-    const uint4 *g1 = reinterpret_cast<const uint4 *>(W.w1 + (long)e * W1E);
-    const uint4 *g1t = reinterpret_cast<const uint4 *>(W.w1t + (long)e * W1E);
-    const uint4 *g2 = reinterpret_cast<const uint4 *>(W.w2 + (long)e * W2E);
-    const uint4 *g2t = reinterpret_cast<const uint4 *>(W.w2t + (long)e * W2E);
-    uint4 *d1 = reinterpret_cast<uint4 *>(s_w1), *d1t = reinterpret_cast<uint4 *>(s_w1t);
-    uint4 *d2 = reinterpret_cast<uint4 *>(s_w2), *d2t = reinterpret_cast<uint4 *>(s_w2t);
-    for (int t = threadIdx.x; t < W1E / 8; t += blockDim.x) {
-        d1[t] = g1[t];
-        d1t[t] = g1t[t];
-    }
-    for (int t = threadIdx.x; t < W2E / 8; t += blockDim.x) {
-        d2[t] = g2[t];
-        d2t[t] = g2t[t];
-    }
-}
-
 // C-layout values (4 rows x 8 column tiles per lane) -> a 16 x 128 BF16 tile
 // in LDS (row-major), then A fragments of it for k-step s.
 __device__ __forceinline__ void c_to_lds(const float (*v)[4], __bf16 *tile, int lane)
@@ -323,76 +357,75 @@ __device__ __forceinline__ void c_to_lds(const float (*v)[4], __bf16 *tile, int 
         for (int r = 0; r < 4; ++r) tile[(4 * (lane >> 4) + r) * NH + 16 * t + (lane & 15)] = (__bf16)v[t][r];
 }
 
-constexpr int MLP_WAVES = 4;   // 16-row tiles per block
-constexpr int MLP_LDS = 2 * (2 * W1E + 2 * W2E) + MLP_WAVES * 16 * NH * 2;  // bytes
+__device__ __forceinline__ bf16x8 gload8(const __bf16 *p)
+{
+    return *reinterpret_cast<const bf16x8 *>(p);
+}
 
-// Per-atom energies and dE/dG (G the raw descriptor) for 16-row tiles of one
-// element.  Blocks [0, nbo) take O tiles, the rest H tiles (each element's
-// tile count padded to a multiple of MLP_WAVES).
-// grid: nbo + nbh blocks of 64 * MLP_WAVES threads, MLP_LDS bytes of LDS.
+// Per-atom energies and dE/dG (G the raw descriptor) of one 16-row tile of
+// one element per block; its four wavefronts split every layer's 128 (or
+// 48) output columns, exchanging the activations through LDS.  Weights are
+// read as B fragments straight from global memory (L2-resident).
+// Blocks [0, nto) take O tiles, the rest H tiles.
+// grid: nto + nth blocks, 256 threads.
+constexpr int MLP_WAVES = 4;
 __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
-    NNGeom g, NNW W, int nbo, const float *__restrict__ gt, const float *__restrict__ stdv,
+    NNGeom g, NNW W, int nto, const float *__restrict__ gt, const float *__restrict__ stdv,
     float *__restrict__ dEdG, float *__restrict__ Eatom)
 {
-    extern __shared__ __align__(16) unsigned char smem[];
-    __bf16 *s_w1 = reinterpret_cast<__bf16 *>(smem);
-    __bf16 *s_w1t = s_w1 + W1E;
-    __bf16 *s_w2 = s_w1t + W1E;
-    __bf16 *s_w2t = s_w2 + W2E;
-    __bf16 *s_tile = s_w2t + W2E;
-    const int e = blockIdx.x < nbo ? 0 : 1;
-    const int tile = (e == 0 ? blockIdx.x : blockIdx.x - nbo) * MLP_WAVES + (threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-    __bf16 *tl = s_tile + (threadIdx.x >> 6) * 16 * NH;
-    stage_weights(W, e, s_w1, s_w1t, s_w2, s_w2t);
-    __syncthreads();
-    const int ra = row_atom(e, tile * 16 + (lane & 15), g.n_mols, g.n_frames, g.n_atoms);
-    const int col = lane & 15, kq = 8 * (lane >> 4);
-    // layer 1: z1 = g W1^T
-    float a1[8][4];
+    __shared__ __align__(16) __bf16 sA[16 * NH];   // a1, later dE/dz1
+    __shared__ __align__(16) __bf16 sD[16 * NH];   // dE/dz2
+    __shared__ float sE[MLP_WAVES][16];
+    const int e = blockIdx.x < nto ? 0 : 1;
+    const int tile = e == 0 ? blockIdx.x : blockIdx.x - nto;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63, col = lane & 15, kq = 8 * (lane >> 4);
+    const int row0 = tile * 16, rq = 4 * (lane >> 4);
+    const int ra = row_atom(e, row0 + col, g.n_mols, g.n_frames, g.n_atoms);
+    const __bf16 *w1 = W.w1 + (long)e * W1E, *w1t = W.w1t + (long)e * W1E;
+    const __bf16 *w2 = W.w2 + (long)e * W2E, *w2t = W.w2t + (long)e * W2E;
+    // layer 1, columns 32 w .. 32 w + 31
+    float a1[2][4];
     {
-        f32x4 acc[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
         for (int s = 0; s < 2; ++s) {
             const bf16x8 a = afrag_f32(gt, ra, NIN, 32 * s + kq, NIN);
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
-                acc[t] = mfma16(a, lds8(s_w1 + (16 * t + col) * KIN + 32 * s + kq), acc[t]);
+            for (int tt = 0; tt < 2; ++tt)
+                acc[tt] = mfma16(a, gload8(w1 + (32 * w + 16 * tt + col) * KIN + 32 * s + kq), acc[tt]);
         }
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const float b = W.b1[e * NH + 16 * t + col];
+        for (int tt = 0; tt < 2; ++tt) {
+            const int n = 32 * w + 16 * tt + col;
+            const float b = W.b1[e * NH + n];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a1[t][r] = tanhf(acc[t][r] + b);
+            for (int r = 0; r < 4; ++r) {
+                a1[tt][r] = tanh_fast(acc[tt][r] + b);
+                sA[(rq + r) * NH + n] = (__bf16)a1[tt][r];
+            }
         }
     }
-    c_to_lds(a1, tl, lane);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // layer 2: z2 = a1 W2^T; energy; dE/dz2
-    float dz2[8][4];
+    __syncthreads();
+    // layer 2, energy, dE/dz2
     {
-        f32x4 acc[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const bf16x8 a = lds8(tl + col * NH + 32 * s + kq);
+            const bf16x8 a = lds8(sA + col * NH + 32 * s + kq);
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
-                acc[t] = mfma16(a, lds8(s_w2 + (16 * t + col) * NH + 32 * s + kq), acc[t]);
+            for (int tt = 0; tt < 2; ++tt)
+                acc[tt] = mfma16(a, gload8(w2 + (32 * w + 16 * tt + col) * NH + 32 * s + kq), acc[tt]);
         }
         float er[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-            const float b = W.b2[e * NH + 16 * t + col], w3 = W.w3[e * NH + 16 * t + col];
+        for (int tt = 0; tt < 2; ++tt) {
+            const int n = 32 * w + 16 * tt + col;
+            const float b = W.b2[e * NH + n], w3 = W.w3[e * NH + n];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float a2 = tanhf(acc[t][r] + b);
+                const float a2 = tanh_fast(acc[tt][r] + b);
                 er[r] += w3 * a2;
-                dz2[t][r] = W.e_scale * w3 * (1.0f - a2 * a2);
+                sD[(rq + r) * NH + n] = (__bf16)(W.e_scale * w3 * (1.0f - a2 * a2));
             }
         }
 #pragma unroll
@@ -402,65 +435,53 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
             v += __shfl_xor(v, 2, 64);
             v += __shfl_xor(v, 4, 64);
             v += __shfl_xor(v, 8, 64);
-            const int rr = row_atom(e, tile * 16 + 4 * (lane >> 4) + r, g.n_mols, g.n_frames,
-                                    g.n_atoms);
-            if (col == 0 && rr >= 0 && Eatom) Eatom[rr] = W.e_scale * (v + W.b3[e]);
+            if (col == 0) sE[w][rq + r] = v;
         }
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    c_to_lds(dz2, tl, lane);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // dE/da1 = dz2 W2, dE/dz1 = (1 - a1^2) dE/da1
+    __syncthreads();
+    // dE/da1 for columns 32 w ..; dE/dz1 = (1 - a1^2) dE/da1
     {
-        f32x4 acc[8];
-#pragma unroll
-        for (int t = 0; t < 8; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+        f32x4 acc[2] = {f32x4{0.f, 0.f, 0.f, 0.f}, f32x4{0.f, 0.f, 0.f, 0.f}};
 #pragma unroll
         for (int s = 0; s < 4; ++s) {
-            const bf16x8 a = lds8(tl + col * NH + 32 * s + kq);
+            const bf16x8 a = lds8(sD + col * NH + 32 * s + kq);
 #pragma unroll
-            for (int t = 0; t < 8; ++t)
-                acc[t] = mfma16(a, lds8(s_w2t + (16 * t + col) * NH + 32 * s + kq), acc[t]);
+            for (int tt = 0; tt < 2; ++tt)
+                acc[tt] = mfma16(a, gload8(w2t + (32 * w + 16 * tt + col) * NH + 32 * s + kq), acc[tt]);
         }
+        // every wavefront has read sA (layer 2) before the barrier above
 #pragma unroll
-        for (int t = 0; t < 8; ++t)
+        for (int tt = 0; tt < 2; ++tt) {
+            const int n = 32 * w + 16 * tt + col;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) a1[t][r] = (1.0f - a1[t][r] * a1[t][r]) * acc[t][r];
+            for (int r = 0; r < 4; ++r)
+                sA[(rq + r) * NH + n] = (__bf16)((1.0f - a1[tt][r] * a1[tt][r]) * acc[tt][r]);
+        }
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    c_to_lds(a1, tl, lane);
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    // dE/dg = dz1 W1 (columns 0..47), dE/dG = dE/dg / std
-    {
-        f32x4 acc[3];
+    if (w == 0 && lane < 16 && Eatom) {
+        const int rr = row_atom(e, row0 + lane, g.n_mols, g.n_frames, g.n_atoms);
+        const float v = sE[0][lane] + sE[1][lane] + sE[2][lane] + sE[3][lane];
+        if (rr >= 0) Eatom[rr] = W.e_scale * (v + W.b3[e]);
+    }
+    __syncthreads();
+    // dE/dg = dE/dz1 W1: column tile w (w < 3: inputs 0..47), dE/dG = dE/dg / std
+    if (w < 3) {
+        f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int t = 0; t < 3; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            const bf16x8 a = lds8(tl + col * NH + 32 * s + kq);
-#pragma unroll
-            for (int t = 0; t < 3; ++t)
-                acc[t] = mfma16(a, lds8(s_w1t + (16 * t + col) * NH + 32 * s + kq), acc[t]);
-        }
-#pragma unroll
-        for (int t = 0; t < 3; ++t) {
-            const int d = 16 * t + col;
-            if (d >= NIN) continue;
+        for (int s = 0; s < 4; ++s)
+            acc = mfma16(lds8(sA + col * NH + 32 * s + kq),
+                         gload8(w1t + (16 * w + col) * NH + 32 * s + kq), acc);
+        const int d = 16 * w + col;
+        if (d < NIN) {
             const float is = 1.0f / stdv[e * NIN + d];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int rr = row_atom(e, tile * 16 + 4 * (lane >> 4) + r, g.n_mols, g.n_frames,
-                                        g.n_atoms);
-                if (rr >= 0) dEdG[(long)rr * NIN + d] = acc[t][r] * is;
+                const int rr = row_atom(e, row0 + rq + r, g.n_mols, g.n_frames, g.n_atoms);
+                if (rr >= 0) dEdG[(long)rr * NIN + d] = acc[r] * is;
             }
         }
     }
 }
-
 
 // d(cos t)/dA and d(cos t)/dB for A = x_j - x_i, B = x_k - x_i.
 __device__ __forceinline__ void dcos(float4 A, float4 B, float c, float3 &dA, float3 &dB)
@@ -470,11 +491,6 @@ __device__ __forceinline__ void dcos(float4 A, float4 B, float c, float3 &dA, fl
     dB = make_float3(A.x * iab - ib2 * B.x, A.y * iab - ib2 * B.y, A.z * iab - ib2 * B.z);
 }
 
-__device__ __forceinline__ float4 sepr(const double *X, int NAP, const Box &box, int i, int j)
-{
-    const float3 d = sep(X, NAP, box, i, j);
-    return make_float4(d.x, d.y, d.z, sqrtf(d.x * d.x + d.y * d.y + d.z * d.z));
-}
 
 // Radial derivative d phi_m / dr of exp(-ETAR (r - mu_m)^2) fc(r; RR).
 __device__ __forceinline__ float dphi(int m, float r, float fc, float dfc)
@@ -484,57 +500,99 @@ __device__ __forceinline__ float dphi(int m, float r, float fc, float dfc)
 }
 
 // ---------------------------------------------------------------------------
+// Angular force pieces per centre i (one wavefront per atom): for every
+// angular neighbour j (slot of nba), D_i[j] = sum_{k != j} dE_i/dx_j of the
+// pair terms T(i; j, k) (each pair differentiated once per member, fixed
+// order), and Dself_i = dE_i/dx_i|angular = -sum_j D_i[j].
+// grid: n_frames * atoms_blocks(n_atoms), 64 * DESC_WAVES threads.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_dang(
+    NNGeom g, const double *__restrict__ pos, const int *__restrict__ nba,
+    const int *__restrict__ nna, const float *__restrict__ dEdG, float4 *__restrict__ Dang,
+    float4 *__restrict__ Dself)
+{
+    __shared__ float4 sv[DESC_WAVES][NAMAX];
+    __shared__ int sj[DESC_WAVES][NAMAX];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    int f, i;
+    if (!block_atom(g, f, i)) return;
+    const int gi = f * g.n_atoms + i;
+    const double *X = pos + (long)f * 3 * g.NAP;
+    const int ei = elem_of(i), cnta = nna[gi];
+    if (lane < cnta) {
+        const int j = nba[(long)gi * NAMAX + lane];
+        sj[w][lane] = j;
+        sv[w][lane] = sepf(X, g, i, j);
+    }
+    float e[NANG];
+#pragma unroll
+    for (int n = 0; n < NANG; ++n) e[n] = dEdG[(long)gi * NIN + NR + n];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    float dx = 0.f, dy = 0.f, dz = 0.f;
+    if (lane < cnta) {
+        const float4 A = sv[w][lane];
+        const float wj = welem(ei, elem_of(sj[w][lane]));
+        float T[NANG], Tc[NANG], Ta[NANG], Tb[NANG];
+        for (int k = 0; k < cnta; ++k) {
+            if (k == lane) continue;
+            const float4 B = sv[w][k];
+            const float iab = 1.0f / (A.w * B.w);
+            const float c = (A.x * B.x + A.y * B.y + A.z * B.z) * iab;
+            ang_terms<true>(c, A.w, B.w, wj * welem(ei, elem_of(sj[w][k])), T, Tc, Ta, Tb);
+            float Sc = 0.f, Sa = 0.f;
+#pragma unroll
+            for (int n = 0; n < NANG; ++n) {
+                Sc += e[n] * Tc[n];
+                Sa += e[n] * Ta[n];
+            }
+            // d cos / dA = B / (ra rb) - c A / ra^2
+            const float ca = Sc * iab, cb = Sa / A.w - Sc * c / (A.w * A.w);
+            dx += ca * B.x + cb * A.x;
+            dy += ca * B.y + cb * A.y;
+            dz += ca * B.z + cb * A.z;
+        }
+        Dang[(long)gi * NAMAX + lane] = make_float4(dx, dy, dz, 0.f);
+    }
+    dx = wave_sumf(dx);
+    dy = wave_sumf(dy);
+    dz = wave_sumf(dz);
+    if (lane == 0) Dself[gi] = make_float4(-dx, -dy, -dz, 0.f);
+}
+
+// ---------------------------------------------------------------------------
 // Forces of the correction, F_a = -dE/dx_a, gathered per atom a (one
 // wavefront per atom, fixed summation order, no atomics):
 //   radial:  dE/dx_a = -sum_j [sum_m (e_am w_aj + e_jm w_ja) phi'_m(r_aj)] u_aj
-//   angular: own pairs (j, k):     -(dE/dA + dE/dB) with e_a
-//            as a neighbour of i:  dE/dA of T(i; a, k) with e_i, every k != a
+//   angular: Dself_a + sum over angular neighbours i of D_i[slot of a in i]
 // with e = dE/dG and u_aj the unit vector from a to j.  MODE 0 adds F to an
 // FP64 planar force array; MODE 1 writes the residual F - target (training)
-// and the atom's squared residual.
-// grid: ceil(n_frames * n_atoms / DESC_WAVES), 64 * DESC_WAVES threads.
+// and the atom's squared residual (and F to fout when given).
+// grid: n_frames * atoms_blocks(n_atoms), 64 * DESC_WAVES threads.
 // ---------------------------------------------------------------------------
 template <int MODE>
 __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_force(
     NNGeom g, const double *__restrict__ pos, const int *__restrict__ nbr,
     const int *__restrict__ nn, const int *__restrict__ nba, const int *__restrict__ nna,
-    const float *__restrict__ dEdG, double *__restrict__ fout, const double *__restrict__ target,
-    float *__restrict__ res, float *__restrict__ sq)
+    const float *__restrict__ dEdG, const float4 *__restrict__ Dang,
+    const float4 *__restrict__ Dself, double *__restrict__ fout,
+    const double *__restrict__ target, float *__restrict__ res, float *__restrict__ sq)
 {
-    __shared__ float4 sv[DESC_WAVES][NAMAX];
-    __shared__ int sj[DESC_WAVES][NAMAX];
-    __shared__ int soff[DESC_WAVES][NAMAX + 1];
-    __shared__ float se[DESC_WAVES][NIN];
+    __shared__ float se[DESC_WAVES][NR];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int ga = blockIdx.x * DESC_WAVES + w;
-    if (ga >= g.n_frames * g.n_atoms) return;
-    const int f = ga / g.n_atoms, a = ga - f * g.n_atoms, base = f * g.n_atoms;
+    int f, a;
+    if (!block_atom(g, f, a)) return;
+    const int base = f * g.n_atoms, ga = base + a;
     const double *X = pos + (long)f * 3 * g.NAP;
     const int ea = elem_of(a);
-    if (lane < NIN) se[w][lane] = dEdG[(long)ga * NIN + lane];
+    if (lane < NR) se[w][lane] = dEdG[(long)ga * NIN + lane];
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const int cnt = nn[ga], cnta = nna[ga];
-    if (lane < cnta) {
-        const int j = nba[(long)ga * NAMAX + lane];
-        sj[w][lane] = j;
-        sv[w][lane] = sepr(X, g.NAP, g.box, a, j);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (lane == 0) {
-        int o = 0;
-        for (int t = 0; t < cnta; ++t) {
-            soff[w][t] = o;
-            o += nna[base + sj[w][t]];
-        }
-        soff[w][cnta] = o;
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     float gx = 0.f, gy = 0.f, gz = 0.f;   // dE/dx_a (Angstrom)
-    // radial
     for (int s = lane; s < cnt; s += 64) {
         const int j = nbr[(long)ga * NMAX + s];
-        const float4 d = sepr(X, g.NAP, g.box, a, j);
+        const float4 d = sepf(X, g, a, j);
         const float fc = fcut(d.w, RR), dfc = dfcut(d.w, RR);
         const int ej = elem_of(j);
         const float waj = welem(ea, ej), wja = welem(ej, ea);
@@ -547,58 +605,28 @@ __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_force(
         gy -= ir * d.y;
         gz -= ir * d.z;
     }
-    // own angular pairs
-    float T[NANG], Tc[NANG], Ta[NANG], Tb[NANG];
-    const int npair = cnta * (cnta - 1) / 2;
-    for (int p = lane; p < npair; p += 64) {
-        int j, k;
-        tri_decode(p, j, k);
-        const float4 A = sv[w][j], B = sv[w][k];
-        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
-        const float wt = welem(ea, elem_of(sj[w][j])) * welem(ea, elem_of(sj[w][k]));
-        ang_terms<true>(c, A.w, B.w, wt, T, Tc, Ta, Tb);
-        float Sc = 0.f, Sa = 0.f, Sb = 0.f;
-#pragma unroll
-        for (int n = 0; n < NANG; ++n) {
-            const float e = se[w][NR + n];
-            Sc += e * Tc[n];
-            Sa += e * Ta[n];
-            Sb += e * Tb[n];
+    if (lane < cnta) {
+        // a in the (ascending) angular list of its neighbour i
+        const int gi = base + nba[(long)ga * NAMAX + lane];
+        const int *L = nba + (long)gi * NAMAX;
+        int lo = 0, hi = nna[gi];
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (L[mid] < a) lo = mid + 1;
+            else hi = mid;
         }
-        float3 dA, dB;
-        dcos(A, B, c, dA, dB);
-        const float sa = Sa / A.w, sb = Sb / B.w;
-        gx -= Sc * (dA.x + dB.x) + sa * A.x + sb * B.x;
-        gy -= Sc * (dA.y + dB.y) + sa * A.y + sb * B.y;
-        gz -= Sc * (dA.z + dB.z) + sa * A.z + sb * B.z;
+        if (lo < nna[gi] && L[lo] == a) {
+            const float4 D = Dang[(long)gi * NAMAX + lo];
+            gx += D.x;
+            gy += D.y;
+            gz += D.z;
+        }
     }
-    // a as a neighbour of each angular neighbour i: T(i; a, k), k != a
-    const int ntot = soff[w][cnta];
-    for (int q = lane; q < ntot; q += 64) {
-        int t = 0;
-        while (soff[w][t + 1] <= q) ++t;
-        const int i = sj[w][t], gi = base + i;
-        const int k = nba[(long)gi * NAMAX + (q - soff[w][t])];
-        if (k == a) continue;
-        const float4 A = sepr(X, g.NAP, g.box, i, a), B = sepr(X, g.NAP, g.box, i, k);
-        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
-        const int ei = elem_of(i);
-        const float wt = welem(ei, ea) * welem(ei, elem_of(k));
-        ang_terms<true>(c, A.w, B.w, wt, T, Tc, Ta, Tb);
-        const float *eiv = dEdG + (long)gi * NIN + NR;
-        float Sc = 0.f, Sa = 0.f;
-#pragma unroll
-        for (int n = 0; n < NANG; ++n) {
-            const float e = eiv[n];
-            Sc += e * Tc[n];
-            Sa += e * Ta[n];
-        }
-        float3 dA, dB;
-        dcos(A, B, c, dA, dB);
-        const float sa = Sa / A.w;
-        gx += Sc * dA.x + sa * A.x;
-        gy += Sc * dA.y + sa * A.y;
-        gz += Sc * dA.z + sa * A.z;
+    if (lane == 0) {
+        const float4 D = Dself[ga];
+        gx += D.x;
+        gy += D.y;
+        gz += D.z;
     }
     gx = wave_sumf(gx);
     gy = wave_sumf(gy);
@@ -640,9 +668,9 @@ __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_tan(
     __shared__ int sj[DESC_WAVES][NAMAX];
     __shared__ float su[DESC_WAVES][NIN];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int gi = blockIdx.x * DESC_WAVES + w;
-    if (gi >= g.n_frames * g.n_atoms) return;
-    const int f = gi / g.n_atoms, i = gi - f * g.n_atoms, base = f * g.n_atoms;
+    int f, i;
+    if (!block_atom(g, f, i)) return;
+    const int base = f * g.n_atoms, gi = base + i;
     const double *X = pos + (long)f * 3 * g.NAP;
     const int ei = elem_of(i);
     const float Rix = res[(long)gi * 3], Riy = res[(long)gi * 3 + 1], Riz = res[(long)gi * 3 + 2];
@@ -650,7 +678,7 @@ __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_tan(
     if (lane < cnta) {
         const int j = nba[(long)gi * NAMAX + lane];
         sj[w][lane] = j;
-        sv[w][lane] = sepr(X, g.NAP, g.box, i, j);
+        sv[w][lane] = sepf(X, g, i, j);
         const long o = (long)(base + j) * 3;
         sr[w][lane] = make_float4(res[o] - Rix, res[o + 1] - Riy, res[o + 2] - Riz, 0.f);
     }
@@ -661,7 +689,7 @@ __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_tan(
     for (int m = 0; m < NIN; ++m) acc[m] = 0.f;
     for (int s = lane; s < cnt; s += 64) {
         const int j = nbr[(long)gi * NMAX + s];
-        const float4 d = sepr(X, g.NAP, g.box, i, j);
+        const float4 d = sepf(X, g, i, j);
         const long o = (long)(base + j) * 3;
         const float dr = (d.x * (res[o] - Rix) + d.y * (res[o + 1] - Riy) + d.z * (res[o + 2] - Riz)) / d.w;
         const float wf = welem(ei, elem_of(j)) * dr;
@@ -721,10 +749,12 @@ __global__ __launch_bounds__(1024) void k_nn_sum(int n, const float *__restrict_
 // d_z1 (x) g, dJ/db = sum d_z, dJ/dw3 = e_scale sum ad2.  The weight
 // gradients are left for k_nn_wgrad as transposed BF16 matrices [unit][row]
 // per element (rows padded to WG_ROWS); bias and w3 parts as per-tile sums.
-// grid: nbo + nbh blocks (as k_nn_mlp), 64 * MLP_WAVES threads, TRAIN_LDS.
+// grid: nbo + nbh blocks (each element's tiles padded to TRAIN_WAVES per
+// block, O first), 64 * TRAIN_WAVES threads, TRAIN_LDS bytes of LDS.
 // ---------------------------------------------------------------------------
 constexpr int WG_ROWS = 256;
-constexpr int TRAIN_LDS = 2 * (W1E + 2 * W2E) + MLP_WAVES * 2 * 16 * NH * 2;
+constexpr int TRAIN_WAVES = 4;   // 16-row tiles (one per wavefront) per training block
+constexpr int TRAIN_LDS = 2 * (W1E + 2 * W2E) + TRAIN_WAVES * 2 * 16 * NH * 2;
 
 struct NNTrainBuf {
     __bf16 *d2t[2], *d2p[2], *a1t[2], *a1p[2], *d1t[2], *d1p[2], *a0t[2], *a0p[2];
@@ -739,7 +769,7 @@ __device__ __forceinline__ void store_t(__bf16 *dst, int ld, int unit, int row, 
     *reinterpret_cast<uint2 *>(dst + (long)unit * ld + row) = *reinterpret_cast<uint2 *>(b);
 }
 
-__global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_train_fwd(
+__global__ __launch_bounds__(64 * TRAIN_WAVES) void k_nn_train_fwd(
     NNGeom g, NNW W, int nbo, const float *__restrict__ gt, const float *__restrict__ ut,
     NNTrainBuf B)
 {
@@ -749,8 +779,8 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_train_fwd(
     __bf16 *s_w2t = s_w2 + W2E;
     __bf16 *s_tile = s_w2t + W2E;
     const int e = blockIdx.x < nbo ? 0 : 1;
-    const int tile = (e == 0 ? blockIdx.x : blockIdx.x - nbo) * MLP_WAVES + (threadIdx.x >> 6);
-    const int gtile = blockIdx.x * MLP_WAVES + (threadIdx.x >> 6);
+    const int tile = (e == 0 ? blockIdx.x : blockIdx.x - nbo) * TRAIN_WAVES + (threadIdx.x >> 6);
+    const int gtile = blockIdx.x * TRAIN_WAVES + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63, col = lane & 15, kq = 8 * (lane >> 4);
     __bf16 *tA = s_tile + (threadIdx.x >> 6) * 2 * 16 * NH, *tB = tA + 16 * NH;
     {
@@ -797,7 +827,7 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_train_fwd(
             const float b = W.b1[e * NH + 16 * t + col];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                a1[t][r] = tanhf(az[t][r] + b);
+                a1[t][r] = tanh_fast(az[t][r] + b);
                 zd1[t][r] = ad[t][r];
             }
         }
@@ -842,7 +872,7 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_train_fwd(
             float sz = 0.f, sw = 0.f;
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float a2 = tanhf(az[t][r] + b), s2 = 1.0f - a2 * a2;
+                const float a2 = tanh_fast(az[t][r] + b), s2 = 1.0f - a2 * a2;
                 const float zd2 = ad[t][r];
                 dzd2[t][r] = s2 * dad2;
                 dz2[t][r] = -2.0f * a2 * s2 * zd2 * dad2;

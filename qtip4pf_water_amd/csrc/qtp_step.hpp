@@ -858,8 +858,8 @@ __global__ __launch_bounds__(256) void k_vv_a(int n_atoms, int NAP, int NMP, dou
 }
 
 // F = f_phys of the single evaluation, p += dt/2 F, energy out.
-// extra_e (may be null): a further energy term added to V (the learned
-// correction's).
+// extra_e (may be null): per-atom energies of a further term added to V
+// (the learned correction's), summed in a fixed order.
 __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfdt,
                                               const double *__restrict__ fe,
                                               const double *__restrict__ epart, int ep_stride,
@@ -867,7 +867,7 @@ __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfd
                                               double *__restrict__ F, double *__restrict__ p,
                                               double *__restrict__ out,
                                               unsigned long long *__restrict__ step_ctr,
-                                              const double *__restrict__ extra_e)
+                                              const float *__restrict__ extra_e)
 {
     const int f = blockIdx.x * 256 + threadIdx.x;
     if (f < 3 * NAP) {
@@ -877,9 +877,14 @@ __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfd
         if (at < n_atoms) p[f] += halfdt * v;
     }
     if (blockIdx.x == 0 && threadIdx.x < 64) {
-        const double v = eval_energy(epart, 0, ep_stride, ep_n, eself);
+        double v = eval_energy(epart, 0, ep_stride, ep_n, eself);
+        if (extra_e) {
+            double x = 0.0;
+            for (int a = threadIdx.x; a < n_atoms; a += 64) x += (double)extra_e[a];
+            v += wave_sum(x);
+        }
         if (threadIdx.x == 0) {
-            out[0] = extra_e ? v + extra_e[0] : v;
+            out[0] = v;
             out[1] = 0.0;
             if (step_ctr) step_ctr[0] += 1ull;
         }
