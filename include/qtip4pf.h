@@ -311,6 +311,46 @@ int qtp_get_observables(qtp_ctx *ctx, double out[4]);
 int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *trace,
                      int *n_samples);
 
+/* ---- launch configuration and autotuner ------------------------------------
+ * No counterpart in the reference (one CPU code path).  cfg[6] = {lanes per
+ * molecule of the LJ/intramolecular kernel (16 or 64), k-space on the matrix
+ * cores (1; only for k_max <= 15) or the vector kernels (0), real-space
+ * coefficients in scalar registers (1), as literals (0) or by grid size (-1),
+ * 16-wave k-space force blocks (1), 8-wave (0) or by block count (-1),
+ * structure-factor blocks per 64-site chunk (1-4; 0 by launch size),
+ * propagation as one LDS-staged kernel per step where the thermostat allows
+ * it (-1) or always as two (0; the results are the same bit for bit)}.
+ * Every configuration gives the same forces up to summation order. */
+int qtp_get_tuning(const qtp_ctx *ctx, int cfg[6]);
+/* Sets cfg (QTP_ERR_ARG for values the context cannot run); with a state set,
+ * its forces are recomputed under the new configuration. */
+int qtp_set_tuning(qtp_ctx *ctx, const int cfg[6]);
+/* Times one force evaluation of every bead of the resident state (the force
+ * work of one step, both streams, HIP events, median of 5) under each
+ * candidate configuration, keeps the fastest and recomputes the state's
+ * forces with it.  The choice is cached per system (molecules, beads, box,
+ * Ewald and LJ parameters) for the process, so contexts of one shape are
+ * timed once; flags & 1 times again.  report[8] (may be NULL): best ms,
+ * default ms, the chosen cfg[0..4], candidates timed (0: from the cache). */
+int qtp_autotune(qtp_ctx *ctx, int flags, double report[8]);
+
+/* ---- Verlet-list build ------------------------------------------------------
+ * No counterpart in the reference, whose neighbour.cpp:12-62 tests every pair
+ * and whose update.cpp:356-385 rebuilds on the displacement test only. */
+#define QTP_LIST_AUTO 0      /* cells above 1024 molecules when the box holds them */
+#define QTP_LIST_ALLPAIRS 1  /* every pair, as neighbour.cpp */
+#define QTP_LIST_CELLS 2     /* cell-binned candidates; QTP_ERR_ARG if the box is too narrow */
+/* method: how lists are built (the lists are identical either way: the same
+ * FP64 pair test, rows in ascending order).  Cells are at least r_skin / S
+ * wide, S = 2 when every edge holds 5 of them, else S = 1 with 3; up to
+ * 65,536 molecules and 12,288 cells.  rebuild_every > 0: resident steps also
+ * rebuild every bead's lists on each step whose count is a multiple of it
+ * (besides the displacement test); 0 = displacement test only (default). */
+int qtp_set_lists(qtp_ctx *ctx, int method, int rebuild_every);
+/* info[5] = {cells in use (0/1), cells per edge x, y, z, S}; the grid is
+ * reported when the box holds one, whether or not it is in use. */
+int qtp_get_list_info(const qtp_ctx *ctx, int info[5]);
+
 /* ---- centroid dynamics, thermostats, centroid mean forces, g(r) --------------
  * None of these has a counterpart in the reference, whose PILE-G is the tau -> 0
  * limit (thermostat.cpp:64-133) and which has no CMD propagator or g(r)

@@ -35,6 +35,8 @@ class OqSys(C.Structure):
         ("use_spline", C.c_int),
         ("r_cut", C.c_double), ("r_cut_sq", C.c_double), ("r_skin", C.c_double),
         ("r_skin_sq", C.c_double), ("e_cut", C.c_double),
+        ("ew_kappa", C.c_double), ("ew_rcut", C.c_double), ("ew_kcut", C.c_double),
+        ("ew_kmax", C.c_int),
     ]
 
 
@@ -55,9 +57,8 @@ class OqConsts(C.Structure):
 
 
 def build(force: bool = False) -> None:
-    """Compile the oracle (and the reference harness when the tree exists)."""
-    if force or not os.path.exists(ORACLE_SO):
-        subprocess.check_call(["make", "-s", "-C", HERE, "oracle"])
+    """Compile the oracle (make rebuilds it when its sources are newer)."""
+    subprocess.check_call(["make", "-s", "-C", HERE] + (["-B"] if force else []) + ["oracle"])
 
 
 def build_ref() -> None:
@@ -93,6 +94,8 @@ def lib():
         L.oq_get_forces.argtypes = [C.POINTER(OqSys), _dp, _dp, _dp, _ip, _ip, C.c_int]
         L.oq_get_forces_parts.restype = C.c_double
         L.oq_get_forces_parts.argtypes = [C.POINTER(OqSys), _dp, _dp, _dp, _ip, _ip, C.c_int, _dp]
+        L.oq_set_exact_energy.restype = None
+        L.oq_set_exact_energy.argtypes = [C.c_int]
         L.oq_force_eval_sc.restype = C.c_double
         L.oq_force_eval_sc.argtypes = [C.POINTER(OqSys), _dp, _dp, _dp, _dp, _ip, _ip, _dp, _dp,
                                        C.c_int]
@@ -119,6 +122,20 @@ def lib():
         L.oq_get_conserved.argtypes = [C.POINTER(OqSys), _dp, _dp, _dp, C.c_double]
         _lib = L
     return _lib
+
+
+class exact_energy:
+    """Context manager: inside it every oracle bead energy is summed with an
+    80-bit accumulator (oq_set_exact_energy), the reference's FP64 running
+    sum outside.  Forces are unchanged."""
+
+    def __enter__(self):
+        lib().oq_set_exact_energy(1)
+        return self
+
+    def __exit__(self, *exc):
+        lib().oq_set_exact_energy(0)
+        return False
 
 
 def spline_set(oo, oh, hh, a=0.0, h=0.01) -> None:
@@ -157,13 +174,16 @@ class Oracle:
     """Reference-shaped CPU restatement for one system (the reference's globals)."""
 
     def __init__(self, n_mols, n_beads, boxl, T_K=298.0, dtfs=0.25, eps=1e-6, epsilonsc=1e-5,
-                 lj_rcut=None):
+                 lj_rcut=None, ewald=None):
         self.sys = OqSys()
         b = np.ascontiguousarray(np.asarray(boxl, dtype=np.float64).reshape(3))
         lib().oq_sys_init(C.byref(self.sys), int(n_mols), int(n_beads), _ptr(b), float(T_K),
                           float(dtfs), float(eps), float(epsilonsc))
         if lj_rcut is not None:
             lib().oq_sys_set_lj_rcut(C.byref(self.sys), float(lj_rcut))
+        if ewald is not None:   # {"kappa", "rcut", "k_cut", "k_max"} instead of derived
+            self.sys.ew_kappa, self.sys.ew_rcut = float(ewald["kappa"]), float(ewald["rcut"])
+            self.sys.ew_kcut, self.sys.ew_kmax = float(ewald["k_cut"]), int(ewald["k_max"])
         s = self.sys
         self.n_mols, self.n_atoms, self.n_dof = s.n_mols, s.n_atoms, s.n_dof
         self.n_beads, self.list_size = s.n_beads, s.list_size
@@ -203,7 +223,7 @@ class Oracle:
     # -- forces ------------------------------------------------------------
     def get_forces(self, q, F, point, lst, bead=0, parts=False):
         if parts:
-            out = np.zeros(5)
+            out = np.zeros(6)
             e = lib().oq_get_forces_parts(C.byref(self.sys), _ptr(q), _ptr(F), _ptr(self.charge),
                                           _ptr(point, _ip), _ptr(lst, _ip), int(bead), _ptr(out))
             return e, out

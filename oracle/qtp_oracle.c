@@ -154,6 +154,16 @@ void oq_sys_set_lj_rcut(oq_sys *s, double r_cut)
 void oq_ewald_params(const oq_sys *s, oq_ewald *e)
 {
     init_consts();
+    if (s->ew_kappa > 0.0) {
+        e->kappa = s->ew_kappa;
+        e->rcut = s->ew_rcut;
+        e->rcut_sq = s->ew_rcut * s->ew_rcut;
+        e->k_cut = s->ew_kcut;
+        e->k_cut_sq = s->ew_kcut * s->ew_kcut;
+        e->k_max = s->ew_kmax;
+        e->prefactor = 2.0 * s->ew_kappa / sqrt(C.pi);
+        return;
+    }
     double kappa = 2.6 * pow(s->rho / s->vol, 1.0 / 6.0);
     double rcut = s->acc / kappa;
     if (rcut > 0.5 * s->min_box) {
@@ -467,9 +477,18 @@ static double kwald(const oq_sys *s, const double *pos, double *ef, const double
     return energy;
 }
 
+/* Energies in extended precision (test aid, not in the reference): when set,
+ * oq_get_forces returns the bead energy summed with an 80-bit accumulator in
+ * the same order, so that its rounding error (~1e-19 relative per term) lies
+ * far below the FP64 running sum's (the reference's, ~1e-16 per term, which
+ * over the millions of real-space pair terms of a 4096-molecule box reaches
+ * ~1e-11 Eh). */
+static int oq_exact = 0;
+void oq_set_exact_energy(int on) { oq_exact = on; }
+
 /* forces.cpp:468-630 */
 double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, const double *charge,
-                           const int *point_all, const int *list_all, int bead, double out[5])
+                           const int *point_all, const int *list_all, int bead, double out[6])
 {
     init_consts();
     const int n_mols = s->n_mols, n_atoms = s->n_atoms, n_dof = s->n_dof;
@@ -481,7 +500,8 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
     double *ef = (double *)malloc(sizeof(double) * n_dof);
     double r[3], r1[3], r2[3];
     double energy = 0.0;
-    double parts[5] = {0, 0, 0, 0, 0};
+    long double ex = 0.0L;
+    double parts[6] = {0, 0, 0, 0, 0, 0};
 
     for (int i = 0; i < n_dof; ++i) F[i] = 0.0;
 
@@ -495,6 +515,7 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
             if (rsq < s->r_cut_sq) {
                 double e = lj_oxygen_cut(r, rsq, F, i, j, s->e_cut);
                 energy += e;
+                ex += e;
                 parts[0] += e;
             }
         }
@@ -508,6 +529,7 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
         double roh1 = sqrt(r1sq);
         double e = oq_intra_morse_bond(r1, roh1, F, i, 1);
         energy += e;
+        ex += e;
         parts[1] += e;
         for (int d = 0; d < 3; ++d) r2[d] = q[9 * i + d] - q[9 * i + d + 6];
         min_image(s, r2);
@@ -515,9 +537,11 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
         double roh2 = sqrt(r2sq);
         e = oq_intra_morse_bond(r2, roh2, F, i, 2);
         energy += e;
+        ex += e;
         parts[1] += e;
         e = oq_intra_harm_angle(r1, r2, roh1, roh2, F, i);
         energy += e;
+        ex += e;
         parts[1] += e;
     }
 
@@ -544,6 +568,7 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
             if (rsq < ew.rcut_sq) {
                 double e = rwald(r, rsq, ef, charge, i, j, ew.kappa);
                 energy += e;
+                ex += e;
                 parts[2] += e;
             }
         }
@@ -552,6 +577,7 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
     /* reciprocal space, forces.cpp:611 */
     double ek = kwald(s, cp, ef, charge, ew.k_max, ew.kappa, ew.k_cut_sq);
     energy += ek;
+    ex += ek;
     parts[3] = ek;
 
     /* chain rule, forces.cpp:613-619 */
@@ -567,13 +593,15 @@ double oq_get_forces_parts(const oq_sys *s, const double *q_all, double *F_all, 
     for (int i = 0; i < n_atoms; ++i) {
         double e = 0.5 * ew.prefactor * charge[i] * charge[i];
         energy -= e;
+        ex -= e;
         parts[4] -= e;
     }
 
     free(cp);
     free(ef);
+    parts[5] = (double)ex;
     if (out) memcpy(out, parts, sizeof(parts));
-    return energy;
+    return oq_exact ? (double)ex : energy;
 }
 
 /* ---------------------------------------------------------------------------

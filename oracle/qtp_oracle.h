@@ -41,6 +41,10 @@ typedef struct {
     /* O-O LJ cutoff and Verlet radius (qtip4pfparams.h:41, 106-110): the
      * reference's 9 A and 1.3 of it unless oq_sys_set_lj_rcut changed them */
     double r_cut, r_cut_sq, r_skin, r_skin_sq, e_cut;
+    /* Ewald parameters given instead of derived (ew_kappa > 0; not in the
+     * reference, whose forces.cpp:574-584 always derives them) */
+    double ew_kappa, ew_rcut, ew_kcut;
+    int ew_kmax;
 } oq_sys;
 
 /* Ewald parameters as derived inside every force call (forces.cpp:574-584). */
@@ -100,9 +104,13 @@ double oq_get_forces(const oq_sys *s, const double *q, double *F, const double *
                      const int *point, const int *list, int bead);
 
 /* Components of the single-bead energy, for tests: out[0]=LJ, [1]=intra,
- * [2]=real-space Ewald, [3]=reciprocal, [4]=self.  Forces as oq_get_forces. */
+ * [2]=real-space Ewald, [3]=reciprocal, [4]=self, [5]=the total summed with
+ * an 80-bit accumulator.  Forces as oq_get_forces. */
 double oq_get_forces_parts(const oq_sys *s, const double *q, double *F, const double *charge,
-                           const int *point, const int *list, int bead, double out[5]);
+                           const int *point, const int *list, int bead, double out[6]);
+/* on != 0: every bead energy (oq_get_forces and the drivers above it) is the
+ * 80-bit accumulated total instead of the reference's FP64 running sum. */
+void oq_set_exact_energy(int on);
 
 /* Suzuki-Chin force driver (forces.cpp:698-789).  Per-bead reduction is
  * taken in bead order (the reference with OMP_NUM_THREADS=1). */

@@ -108,6 +108,11 @@ SIGNATURES = [
     ("qtp_batch_last_error", C.c_char_p, [_vp]),
     ("qtp_batch_destroy", None, [_vp]),
     ("qtp_get_scalars", C.c_int, [_vp, dp]),
+    ("qtp_get_tuning", C.c_int, [_vp, ip]),
+    ("qtp_set_tuning", C.c_int, [_vp, ip]),
+    ("qtp_autotune", C.c_int, [_vp, C.c_int, dp]),
+    ("qtp_set_lists", C.c_int, [_vp, C.c_int, C.c_int]),
+    ("qtp_get_list_info", C.c_int, [_vp, ip]),
     ("qtp_set_thermostat", C.c_int, [_vp, C.c_int, C.c_double]),
     ("qtp_set_pimd_mode", C.c_int, [_vp, C.c_int, C.c_double]),
     ("qtp_centroid_force", C.c_int, [_vp, C.c_int]),

@@ -437,6 +437,47 @@ class Water:
         return {"energy": o[0], "sc_pot": o[1], "heat": o[2], "rebuilds": int(o[3]),
                 "steps": int(o[4]), "status": int(o[5])}
 
+    # ------------------------------ launch configuration and autotuner
+    TUNING_KEYS = ("lanes", "matrix_kspace", "rs_registers", "kforce_16_waves",
+                   "ksum_blocks_per_chunk", "fused_propagation")
+
+    def tuning(self) -> dict:
+        out = np.zeros(6, np.int32)
+        self._chk(self._L.qtp_get_tuning(self._h, ptr(out, ip)))
+        return dict(zip(self.TUNING_KEYS, (int(v) for v in out)))
+
+    def set_tuning(self, **kw):
+        """Set launch choices (keys of tuning()); forces of a set state are
+        recomputed."""
+        cur = self.tuning()
+        cur.update(kw)
+        arr = np.array([cur[k] for k in self.TUNING_KEYS], np.int32)
+        self._chk(self._L.qtp_set_tuning(self._h, ptr(arr, ip)))
+
+    def autotune(self, retime: bool = False) -> dict:
+        """Time every candidate launch configuration on the resident state's
+        force evaluation and keep the fastest (cached per system shape)."""
+        r = np.zeros(8)
+        self._chk(self._L.qtp_autotune(self._h, 1 if retime else 0, ptr(r)))
+        return {"best_ms": float(r[0]), "default_ms": float(r[1]),
+                "chosen": dict(zip(self.TUNING_KEYS[:5], (int(v) for v in r[2:7]))),
+                "timed": int(r[7])}
+
+    # ------------------------------ Verlet-list build
+    def set_lists(self, method: str = "auto", rebuild_every: int = 0):
+        """How lists are built: "auto" (cell-binned above 1024 molecules when
+        the box holds the grid), "allpairs" (neighbour.cpp's loop) or "cells";
+        rebuild_every > 0 also rebuilds every bead's lists on each such step
+        of the resident runs.  The lists are identical whichever builds them."""
+        m = {"auto": 0, "allpairs": 1, "cells": 2}[method]
+        self._chk(self._L.qtp_set_lists(self._h, m, int(rebuild_every)))
+
+    def list_info(self) -> dict:
+        """{"cells": in use, "grid": cells per edge, "S": stencil half-width}."""
+        out = np.zeros(5, np.int32)
+        self._chk(self._L.qtp_get_list_info(self._h, ptr(out, ip)))
+        return {"cells": bool(out[0]), "grid": tuple(int(v) for v in out[1:4]), "S": int(out[4])}
+
     # ------------------------------ centroid dynamics, thermostats, g(r)
     def set_thermostat(self, kind: str = "pile_g", tau_fs: float = 0.0):
         """Centroid thermostat of NVT resident steps: "pile_g" (Bussi rescale,

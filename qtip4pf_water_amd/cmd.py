@@ -286,6 +286,8 @@ def main(argv=None):
     v.add_argument("--equil-ps", type=float, default=2.0)
     v.add_argument("--kinds", default="pacmd,surrogate,classical")
     v.add_argument("--n-mols", type=int, default=512)
+    v.add_argument("--ref", default=None,
+                   help="g(r) file of an earlier plain-CMD run (rdf_pacmd.npz) instead of a new one")
     a = ap.parse_args(argv)
     os.makedirs(a.out, exist_ok=True)
     if a.cmd == "build":
@@ -322,7 +324,12 @@ def main(argv=None):
     else:
         model = load_model(a.model)
         res = {}
+        if a.ref:
+            with np.load(a.ref) as z:
+                res["pacmd"] = {k: (z[k] if z[k].ndim else z[k].item()) for k in z.files}
         for kind in a.kinds.split(","):
+            if kind in res:
+                continue
             r = cmd_run(kind, n_mols=a.n_mols, model=model, ps=a.ps, equil_ps=a.equil_ps)
             res[kind] = r
             np.savez(os.path.join(a.out, f"rdf_{kind}.npz"),

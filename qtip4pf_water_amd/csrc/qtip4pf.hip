@@ -30,6 +30,7 @@ const char *qtp_noalloc_last_error();
 #include "qtp_host_setup.hpp"
 #include "qtp_host_nn.hpp"
 #include "qtp_host_run.hpp"
+#include "qtp_host_tune.hpp"
 
 // ===========================================================================
 // C ABI
@@ -125,7 +126,8 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->kf_chunk = kforce_chunk(K, KF_WAVES, 2);
     ctx->kf_chunk_split = kforce_chunk(K, KF_SPLIT_WAVES, 3);
     ctx->kforce_lds = kforce_lds_bytes(K, KF_WAVES, ctx->kf_chunk);
-    ctx->kf_split = (long)ctx->P * ((ctx->n_atoms + KF_SITES - 1) / KF_SITES) <= 512;
+    ctx->kf_split_ok = (long)ctx->P * ((ctx->n_atoms + KF_SITES - 1) / KF_SITES) <= 512;
+    ctx->kf_split = ctx->kf_split_ok;
     ctx->kforce_lds_split = kforce_lds_bytes(K, KF_SPLIT_WAVES, ctx->kf_chunk_split);
     std::vector<int> gcol;
     std::vector<int4> mf_tiles, mf_rt, mf_rt_small, ks_col;
@@ -133,9 +135,10 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     {
         const char *kv = std::getenv("QTP_KSPACE");
         const bool want_mf = !(kv && std::strcmp(kv, "vector") == 0);
-        ctx->kf_mf = want_mf && 2 * K <= 4 * MF_MAXKS && ctx->G > 0 && ctx->AC % 4 == 0 &&
-                     build_mf_layout(g, gcol, mf_tiles, ctx->mf_Mp, ctx->mf_nks, ks_col, ks_unit);
-        if (ctx->kf_mf) {
+        ctx->kf_mf_ok = 2 * K <= 4 * MF_MAXKS && ctx->G > 0 && ctx->AC % 4 == 0 &&
+                        build_mf_layout(g, gcol, mf_tiles, ctx->mf_Mp, ctx->mf_nks, ks_col, ks_unit);
+        ctx->kf_mf = ctx->kf_mf_ok && want_mf;
+        if (ctx->kf_mf_ok) {
             mf_rt = mf_tile_order(mf_tiles, MF_WAVES);
             mf_rt_small = mf_tile_order(mf_tiles, MF_WAVES_SMALL);
             ctx->mf_nrt = (int)mf_rt.size();
@@ -145,8 +148,8 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
             ctx->kforce_mf_lds_small = kforce_mf_lds_bytes(K, ctx->mf_nks, MF_WAVES_SMALL);
             ctx->ks_ncol4 = (int)ks_col.size();
             ctx->ksum_mf_lds = ksum_mf_lds_bytes(K, ctx->AC, ctx->ks_ncol4, ctx->ks_nunit);
-            ctx->kf_split = false;
         }
+        if (ctx->kf_mf) ctx->kf_split = false;
     }
     std::vector<int2> tl;
     for (int I = 0; I < ctx->nblk; ++I)
@@ -154,23 +157,24 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->ntile = (int)tl.size();
     ctx->rs_cull = std::min(ctx->box.L[0], std::min(ctx->box.L[1], ctx->box.L[2])) >=
                    RS_CULL_MIN_BOX * std::sqrt(ctx->ew.rcut_sq);
+    // energy partials sized for the widest layout (a wave per molecule)
+    set_lj_lanes(ctx, 64);
+    ctx->ep_cap = ctx->ep_stride;
     // a whole wave per molecule when one launch covers few molecules
-    ctx->lj_lpm = (long)std::max(ctx->P, 1) * ctx->n_mols <= 2048 ? 64 : 16;
-    ctx->nmb = (ctx->n_mols + (256 / ctx->lj_lpm) - 1) / (256 / ctx->lj_lpm);
-    ctx->ep_lj = 0;
-    ctx->ep_real = ctx->nmb;
-    ctx->ep_k = ctx->nmb + ctx->ntile;
-    ctx->ep_spl = ctx->nmb + ctx->ntile + ctx->ngr;
-    ctx->ep_stride = ctx->ep_spl + ctx->nblk;
+    set_lj_lanes(ctx, (long)std::max(ctx->P, 1) * ctx->n_mols <= 2048 ? 64 : 16);
     ctx->ptile = (3 * ctx->NAP + 63) / 64;
     ctx->prop_lds = 0;
+    ctx->cells_ok = cell_grid(ctx->drv.r_skin, ctx->box.L, ctx->n_mols, ctx->cg);
     *out = ctx;
 
     // a blocking stream: host-synchronous hipMemcpy/hipMemset on the null
     // stream are ordered with this stream's kernels
     if (hipStreamCreate(&ctx->stream) != hipSuccess || hipStreamCreate(&ctx->s2) != hipSuccess ||
+        hipStreamCreate(&ctx->s3) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&ctx->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_nn_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_nn_join, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "hipStreamCreate failed";
         return QTP_ERR_HIP;
     }
@@ -199,14 +203,14 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
         (st = dalloc(ctx, &ctx->grk, ctx->G)) || (st = dalloc(ctx, &ctx->tiles, ctx->ntile)) ||
         (st = dalloc(ctx, &ctx->Cm, (size_t)ctx->P * ctx->P)) ||
         (st = dalloc(ctx, &ctx->mode, ctx->P)) || (st = dalloc(ctx, &ctx->ms, ctx->P)) ||
-        (ctx->kf_mf && ((st = dalloc(ctx, &ctx->gcol, ctx->G)) ||
+        (ctx->kf_mf_ok && ((st = dalloc(ctx, &ctx->gcol, ctx->G)) ||
                         (st = dalloc(ctx, &ctx->mf_rt, ctx->mf_nrt)) ||
                         (st = dalloc(ctx, &ctx->mf_rt_small, ctx->mf_nrt_small)) ||
                         (st = dalloc(ctx, &ctx->ks_col, ks_col.size())) ||
                         (st = dalloc(ctx, &ctx->ks_unit, ks_unit.size())))) ||
         (st = alloc_all(ctx)))
         return st;
-    if (ctx->kf_mf &&
+    if (ctx->kf_mf_ok &&
         (hipMemcpy(ctx->gcol, gcol.data(), gcol.size() * sizeof(int), hipMemcpyHostToDevice) ||
          hipMemcpy(ctx->mf_rt, mf_rt.data(), mf_rt.size() * sizeof(int4), hipMemcpyHostToDevice) ||
          hipMemcpy(ctx->mf_rt_small, mf_rt_small.data(), mf_rt_small.size() * sizeof(int4),
@@ -241,14 +245,18 @@ void qtp_destroy(qtp_ctx *ctx)
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,
                     ctx->ppart, ctx->hpart, ctx->dev, ctx->obs, ctx->obs_out, ctx->ntrace, ctx->trace,
-                    ctx->ms, ctx->cmf_acc, ctx->cmf_n, ctx->qc, ctx->rdf_oo,
+                    ctx->fnn, ctx->ms, ctx->cmf_acc, ctx->cmf_n, ctx->qc, ctx->rdf_oo,
                     ctx->rdf_oh};
     for (void *p : ptrs)
         if (p) hipFree(p);
     if (ctx->s2) hipStreamSynchronize(ctx->s2);
+    if (ctx->s3) hipStreamSynchronize(ctx->s3);
     if (ctx->ev_fork) hipEventDestroy(ctx->ev_fork);
     if (ctx->ev_join) hipEventDestroy(ctx->ev_join);
+    if (ctx->ev_nn_fork) hipEventDestroy(ctx->ev_nn_fork);
+    if (ctx->ev_nn_join) hipEventDestroy(ctx->ev_nn_join);
     if (ctx->s2) hipStreamDestroy(ctx->s2);
+    if (ctx->s3) hipStreamDestroy(ctx->s3);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -350,7 +358,9 @@ int qtp_neighbour_list(qtp_ctx *ctx, const double *q, int *point, int *list, dou
     HIPCHK(hipMemset(ctx->status, 0, sizeof(int)));
     const int nrb = (ctx->n_mols + 3) / 4;
     hipStream_t s = ctx->stream;
-    {
+    if (use_cells(ctx)) {
+        CHK(run_cell_lists(ctx, 0, 0, nullptr));
+    } else {
         TimeScope ts(ctx, "nlist");
         k_nl_count<<<ctx->P * nrb, 256, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->q, ctx->box,
                                                 ctx->ff.r_skin_sq, ctx->flag, ctx->cnt);
@@ -438,6 +448,10 @@ static int update_sc_impl(qtp_ctx *ctx, double *q, double *p, const double *mass
 {
     if (!ctx || !q || !p || !F || !point || !list || !disp || ctx->P < 2 || (is_nvt && !dev))
         return QTP_ERR_ARG;
+    if (ctx->pimd_mode != QTP_MODE_PIMD) {
+        ctx->err = "update_sc_nospline steps plain PIMD (qtp_set_pimd_mode QTP_MODE_PIMD)";
+        return QTP_ERR_STATE;
+    }
     const int P = ctx->P;
     CHK(check_finite(ctx, q, (long)P * ctx->n_dof, "positions"));
     CHK(upload_charges_masses(ctx, charge, mass));
@@ -1346,6 +1360,7 @@ int qtp_nn_create(int n_mols, const double boxl[3], int max_frames, qtp_nn **out
         (st = nalloc(nn, &nn->nn, NA)) || (st = nalloc(nn, &nn->nba, NA * NAMAX)) ||
         (st = nalloc(nn, &nn->nna, NA)) || (st = nalloc(nn, &nn->status, 1)) ||
         (st = nalloc(nn, &nn->dang, NA * NAMAX)) || (st = nalloc(nn, &nn->dself, NA)) ||
+        (st = nalloc(nn, &nn->P4, NA)) || (st = nalloc(nn, &nn->Etile, b.to + b.th)) ||
         (st = nalloc(nn, &nn->tp, (size_t)(b.tiles_o + b.tiles_h) * 3 * NH)) ||
         (st = nalloc(nn, &nn->part, (size_t)(b.nco + b.nch) * WG_PART)) ||
         (st = nalloc(nn, &nn->inter, nn_inter_elems(b))))
@@ -1365,7 +1380,7 @@ void qtp_nn_destroy(qtp_nn *nn)
                     nn->b1, nn->b2, nn->w3, nn->b3, nn->mean, nn->stdv, nn->pos, nn->tgt,
                     nn->fwork, nn->dsum, nn->gt, nn->ut, nn->dEdG, nn->graw, nn->Eatom,
                     nn->res, nn->sq, nn->nbr, nn->nn, nn->nba, nn->nna, nn->status, nn->tp,
-                    nn->dang, nn->dself,
+                    nn->dang, nn->dself, nn->P4, nn->Etile,
                     nn->part, nn->inter, nn->dpos, nn->dtgt};
     for (void *p : ptrs)
         if (p) hipFree(p);
@@ -1443,9 +1458,9 @@ int qtp_nn_eval(qtp_nn *nn, int n_frames, const double *pos, double *energy, dou
         NNCHK(hipMemsetAsync(nn->status, 0, sizeof(int), s));
         NNCHK(hipMemsetAsync(nn->fwork, 0, (size_t)F * N3 * sizeof(double), s));
         CHK(nn_forward(nn, s, nn->pos, F, b, nn->status, descriptors ? nn->graw : nullptr));
-        k_nn_force<0><<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, nn->pos, nn->nbr, nn->nn, nn->nba,
-                                                             nn->nna, nn->dEdG, nn->dang, nn->dself,
-                                                             nn->fwork, nullptr, nullptr, nullptr);
+        k_nn_force<0><<<b.nblk_atom, AT, 0, s>>>(g, nn->P4, nn->nbr, nn->nn, nn->nba, nn->nna,
+                                                 nn->dEdG, nn->dang, nn->dself, nn->fwork, nullptr,
+                                                 nullptr, nullptr);
         for (int f = 0; f < F; ++f)
             k_nn_sum<<<1, 1024, 0, s>>>(nn->n_atoms, nn->Eatom + (long)f * nn->n_atoms, 1.0, 0,
                                         nn->dsum + f);
@@ -1564,9 +1579,142 @@ int qtp_set_nn(qtp_ctx *ctx, qtp_nn *nn)
         }
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (nn && !ctx->fnn) CHK(dalloc(ctx, &ctx->fnn, 3 * (size_t)ctx->NAP));
     invalidate_graphs(ctx);
     ctx->nn = nn;
     ctx->state_set = false;
+    return QTP_OK;
+}
+
+// ---- launch configuration and autotuner -----------------------------------------------
+
+int qtp_get_tuning(const qtp_ctx *ctx, int cfg[6])
+{
+    if (!ctx || !cfg) return QTP_ERR_ARG;
+    const TuneCfg c = current_tuning(ctx);
+    cfg[0] = c.lpm;
+    cfg[1] = c.kmf;
+    cfg[2] = c.carg;
+    cfg[3] = c.small;
+    cfg[4] = c.usplit;
+    cfg[5] = c.fused;
+    return QTP_OK;
+}
+
+int qtp_set_tuning(qtp_ctx *ctx, const int cfg[6])
+{
+    if (!ctx || !cfg) return QTP_ERR_ARG;
+    TuneCfg c;
+    c.lpm = cfg[0];
+    c.kmf = cfg[1];
+    c.carg = cfg[2];
+    c.small = cfg[3];
+    c.usplit = cfg[4];
+    c.fused = cfg[5];
+    if (!tuning_valid(ctx, c)) {
+        ctx->err = "tuning: lanes 16 or 64, matrix k-space 0/1 (1 only for k_max <= 15), "
+                   "registers and 16-wave blocks -1/0/1, blocks per chunk 0-4, fused "
+                   "propagation -1/0";
+        return QTP_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    apply_tuning(ctx, c);
+    SplineScope sp(ctx, ctx->spl_resident);
+    return tune_refresh(ctx);
+}
+
+int qtp_autotune(qtp_ctx *ctx, int flags, double report[8])
+{
+    if (!ctx) return QTP_ERR_ARG;
+    if (!ctx->state_set) {
+        ctx->err = "autotune: set the state first (qtp_set_state)";
+        return QTP_ERR_STATE;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    SplineScope sp(ctx, ctx->spl_resident);
+    const bool timing = ctx->timing;
+    ctx->timing = false;
+    const std::string key = tune_key(ctx);
+    TuneCfg best = current_tuning(ctx);
+    float best_ms = 0.f, dflt_ms = 0.f;
+    int n_timed = 0, st = QTP_OK;
+    bool cached = false;
+    if (!(flags & 1)) {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            best = it->second;
+            cached = true;
+        }
+    }
+    if (!cached) {
+        const TuneCfg dflt = current_tuning(ctx);
+        st = tune_time(ctx, 5, &dflt_ms);
+        best_ms = dflt_ms;
+        for (const TuneCfg &c : tune_candidates(ctx)) {
+            if (st != QTP_OK) break;
+            apply_tuning(ctx, c);
+            float ms = 0.f;
+            st = tune_time(ctx, 5, &ms);
+            ++n_timed;
+            if (st == QTP_OK && ms < best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+        if (st == QTP_OK) {
+            std::lock_guard<std::mutex> g(g_tune_mu);
+            g_tune_cache[key] = best;
+        } else {
+            best = dflt;
+        }
+    }
+    best.fused = ctx->prop_fused;   // not timed here: the caller's choice stays
+    apply_tuning(ctx, best);
+    ctx->timing = timing;
+    CHK(st);
+    CHK(tune_refresh(ctx));
+    CHK(read_status(ctx));
+    if (report) {
+        report[0] = best_ms;
+        report[1] = dflt_ms;
+        report[2] = best.lpm;
+        report[3] = best.kmf;
+        report[4] = best.carg;
+        report[5] = best.small;
+        report[6] = best.usplit;
+        report[7] = n_timed;
+    }
+    return QTP_OK;
+}
+
+// ---- Verlet-list build -------------------------------------------------------------
+
+int qtp_set_lists(qtp_ctx *ctx, int method, int rebuild_every)
+{
+    if (!ctx) return QTP_ERR_ARG;
+    if (method < QTP_LIST_AUTO || method > QTP_LIST_CELLS || rebuild_every < 0) {
+        ctx->err = "lists: method QTP_LIST_AUTO / ALLPAIRS / CELLS and rebuild_every >= 0";
+        return QTP_ERR_ARG;
+    }
+    if (method == QTP_LIST_CELLS && !ctx->cells_ok) {
+        ctx->err = "lists: the box holds fewer than 3 cells of r_skin a side (or the system "
+                   "exceeds 65536 molecules / 12288 cells)";
+        return QTP_ERR_ARG;
+    }
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx->list_method = method;
+    ctx->rebuild_every = rebuild_every;
+    invalidate_graphs(ctx);
+    return QTP_OK;
+}
+
+int qtp_get_list_info(const qtp_ctx *ctx, int info[5])
+{
+    if (!ctx || !info) return QTP_ERR_ARG;
+    info[0] = use_cells(ctx) ? 1 : 0;
+    for (int d = 0; d < 3; ++d) info[1 + d] = ctx->cells_ok ? ctx->cg.nc[d] : 0;
+    info[4] = ctx->cells_ok ? ctx->cg.S : 0;
     return QTP_OK;
 }
 

@@ -24,6 +24,9 @@ struct qtp_ctx {
     // second stream for the k-space half of a force phase (fork/join by events)
     hipStream_t s2 = nullptr;
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // third stream for the learned correction of classical steps
+    hipStream_t s3 = nullptr;
+    hipEvent_t ev_nn_fork = nullptr, ev_nn_join = nullptr;
 
     int P = 0, n_mols = 0, n_atoms = 0, n_dof = 0, NAP = 0, NMP = 0, nblk = 0, list_size = 0;
     long fcap = 0;
@@ -51,11 +54,20 @@ struct qtp_ctx {
     int kf_chunk = KF_CHUNK, kf_chunk_split = KF_CHUNK;  // k_kforce record staging
     // split k_kforce in the SC phases when its P * ceil(n_atoms/63) blocks
     // would not fill two per CU (small systems); large ones have blocks enough
-    bool kf_split = false;
+    bool kf_split = false, kf_split_ok = false;
     // k-space sums and forces on the matrix cores (k_ksum_mf, k_kforce_mf)
     // for k_max <= 15; the vector kernels above that, or when
     // QTP_KSPACE=vector is set at create
-    bool kf_mf = false;
+    bool kf_mf = false, kf_mf_ok = false;
+    // launch choices the autotuner may set (qtp_set_tuning): k_realspace
+    // coefficients in SGPRs (-1: for grids of RS_CARG_BLOCKS blocks or more),
+    // k_kforce_mf 16-wave blocks (-1: below MF_SMALL_BLOCKS blocks), k_ksum_mf
+    // blocks per chunk (0: several when a launch has under 128 chunks)
+    int rs_carg = -1, mf_small = -1, ks_usplit = 0;
+    // -1: one LDS-staged propagation kernel per step where the thermostat
+    // allows it (k_prop_tile); 0: always k_prop_fwd + k_prop_inv
+    int prop_fused = -1;
+    int ep_cap = 0;   // allocated stride of the energy partials
     int mf_Mp = 0, mf_nks = 0, mf_nrt = 0, mf_nrt_small = 0, ks_nunit = 0, ks_ncol4 = 0;
     size_t kforce_mf_lds = 0, kforce_mf_lds_small = 0, ksum_mf_lds = 0;
 
@@ -64,6 +76,13 @@ struct qtp_ctx {
     double *m_atom = nullptr, *sqm_atom = nullptr, *charge = nullptr;
     int *point = nullptr, *list = nullptr, *fpoint = nullptr, *flist = nullptr, *cnt = nullptr;
     int *flag = nullptr, *status = nullptr;
+    // cell-binned list build (k_cell_build, k_nl_cell): the grid when the box
+    // holds one (cells_ok), the method (QTP_LIST_AUTO / ALLPAIRS / CELLS) and
+    // the forced rebuild period of resident steps (0: displacement test only)
+    CellGrid cg{};
+    bool cells_ok = false;
+    int list_method = 0, rebuild_every = 0;
+    int *cell_of = nullptr, *crank = nullptr, *cstart = nullptr, *member = nullptr;
     double *nreb = nullptr;
     // force-phase scratch (E <= max(P,1))
     double *qpm = nullptr, *vpart = nullptr, *vout = nullptr, *rpart = nullptr, *spart = nullptr, *sprime = nullptr;
@@ -117,6 +136,7 @@ struct qtp_ctx {
     hipGraphExec_t graph_obs[2] = {nullptr, nullptr};
     // learned centroid-force correction added to classical resident steps
     qtp_nn *nn = nullptr;
+    double *fnn = nullptr;   // its forces (planar)
     bool timing = false;
     std::map<std::string, Timer> timers;
     std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> pending;

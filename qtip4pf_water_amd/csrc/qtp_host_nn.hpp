@@ -23,6 +23,8 @@ struct qtp_nn {
     float *sq = nullptr, *graw = nullptr, *part = nullptr, *tp = nullptr, *grad = nullptr;
     int *nbr = nullptr, *nn = nullptr, *nba = nullptr, *nna = nullptr, *status = nullptr;
     float4 *dang = nullptr, *dself = nullptr;   // angular force pieces per centre
+    float4 *P4 = nullptr;                         // positions (Angstrom, wrapped)
+    float *Etile = nullptr;                       // energy per inference tile
     __bf16 *inter = nullptr;   // the eight training intermediates, both elements
     size_t inter_elems = 0;
     // training data (device resident)
@@ -51,9 +53,9 @@ static int nalloc(qtp_nn *nn, T **p, size_t n)
     return QTP_OK;
 }
 
-// Launch geometry of a batch of F frames: per-atom kernels (DESC_WAVES atoms
-// of one frame per block), inference tiles of 16 rows per element, training
-// tiles padded to TRAIN_WAVES per block, weight-gradient chunks.
+// Launch geometry of a batch of F frames: per-atom kernels (one block per
+// atom), inference tiles of 16 rows per element, training tiles padded to
+// TRAIN_WAVES per block, weight-gradient chunks.
 struct NNBatch {
     int F, nblk_atom, to, th, nbo, nbh, tiles_o, tiles_h, rows_o, rows_h, nco, nch;
 };
@@ -62,7 +64,7 @@ static NNBatch nn_batch(const qtp_nn *nn, int F)
 {
     NNBatch b;
     b.F = F;
-    b.nblk_atom = F * ((nn->n_atoms + DESC_WAVES - 1) / DESC_WAVES);
+    b.nblk_atom = F * nn->n_atoms;   // one block of AT threads per atom
     b.to = (F * nn->n_mols + 15) / 16;
     b.th = (2 * F * nn->n_mols + 15) / 16;
     b.nbo = (b.to + TRAIN_WAVES - 1) / TRAIN_WAVES;
@@ -145,18 +147,20 @@ static size_t nn_inter_elems(const NNBatch &b)
     return (size_t)(6 * NH + 2 * KIN) * (b.rows_o + b.rows_h);
 }
 
-// Lists, descriptors, energies, dE/dG and the angular force pieces of F
-// frames at pos (planar, device), the atoms' status bits into status.
+// Positions, lists, descriptors, energies, dE/dG and the angular force
+// pieces of F frames at pos (planar FP64, device), the atoms' status bits
+// into status; Etile (may be null) receives the tile energy sums.
 static int nn_forward(qtp_nn *nn, hipStream_t s, const double *pos, int F, const NNBatch &b,
-                      int *status, float *graw = nullptr)
+                      int *status, float *graw = nullptr, float *Etile = nullptr)
 {
     const NNGeom g = nn_geom(nn, F);
-    k_nn_desc<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, pos, nn->mean, nn->stdv, nn->gt, nn->nbr,
-                                                        nn->nn, nn->nba, nn->nna, graw, status);
+    k_nn_pos<<<(F * nn->n_atoms + 255) / 256, 256, 0, s>>>(g, pos, nn->P4);
+    k_nn_desc<<<b.nblk_atom, AT, 0, s>>>(g, nn->P4, nn->mean, nn->stdv, nn->gt, nn->nbr, nn->nn,
+                                         nn->nba, nn->nna, graw, status);
     k_nn_mlp<<<b.to + b.th, 64 * MLP_WAVES, 0, s>>>(g, nn_weights(nn), b.to, nn->gt, nn->stdv,
-                                                    nn->dEdG, nn->Eatom);
-    k_nn_dang<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, pos, nn->nba, nn->nna, nn->dEdG,
-                                                        nn->dang, nn->dself);
+                                                    nn->dEdG, nn->Eatom, Etile);
+    k_nn_dang<<<b.nblk_atom, AT, 0, s>>>(g, nn->P4, nn->nba, nn->nna, nn->dEdG, nn->dang,
+                                         nn->dself);
     NNCHK(hipGetLastError());
     return QTP_OK;
 }
@@ -196,13 +200,13 @@ static int nn_train_batch(qtp_nn *nn, hipStream_t s, int f0, int F, int mode, do
     const long N3 = 3L * nn->NAP;
     const double *pos = nn->dpos + (long)f0 * N3, *tgt = nn->dtgt + (long)f0 * N3;
     CHK(nn_forward(nn, s, pos, F, b, nn->status));
-    k_nn_force<1><<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, pos, nn->nbr, nn->nn, nn->nba, nn->nna,
-                                                         nn->dEdG, nn->dang, nn->dself, nullptr,
-                                                         tgt, nn->res, nn->sq);
+    k_nn_force<1><<<b.nblk_atom, AT, 0, s>>>(g, nn->P4, nn->nbr, nn->nn, nn->nba, nn->nna,
+                                             nn->dEdG, nn->dang, nn->dself, nullptr, tgt, nn->res,
+                                             nn->sq);
     const long ncomp = 3L * F * nn->n_atoms;
     k_nn_sum<<<1, 1024, 0, s>>>(F * nn->n_atoms, nn->sq, 1.0 / (double)ncomp, 0, loss_dev);
-    k_nn_tan<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, pos, nn->nbr, nn->nn, nn->nba, nn->nna,
-                                                    nn->res, nn->stdv, nn->ut);
+    k_nn_tan<<<b.nblk_atom, AT, 0, s>>>(g, nn->P4, nn->nbr, nn->nn, nn->nba, nn->nna, nn->res,
+                                        nn->stdv, nn->ut);
     NNTrainBuf T = nn_trainbuf(nn, b);
     NNCHK(hipMemsetAsync(nn->inter, 0, nn_inter_elems(b) * sizeof(__bf16), s));
     k_nn_train_fwd<<<b.nbo + b.nbh, 64 * MLP_WAVES, TRAIN_LDS, s>>>(g, nn_weights(nn), b.nbo, nn->gt,
@@ -228,36 +232,41 @@ static int nn_train_batch(qtp_nn *nn, hipStream_t s, int f0, int F, int mode, do
 
 }  // namespace
 
-// The correction's force on the resident classical state, added into fph;
-// its per-atom energies stay in nn->Eatom (summed by k_vv_b).
-static int run_nn_forces(qtp_ctx *ctx)
+// The correction's forces on the resident classical state into ctx->fnn and
+// its energy as tile sums in nn->Etile (both read by k_vv_b), on stream s.
+static int run_nn_forces(qtp_ctx *ctx, hipStream_t s)
 {
     qtp_nn *nn = ctx->nn;
-    hipStream_t s = ctx->stream;
     const NNBatch b = nn_batch(nn, 1);
     const NNGeom g = nn_geom(nn, 1);
     {
         TimeScope ts(ctx, "nn_desc");
-        k_nn_desc<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, ctx->q, nn->mean, nn->stdv, nn->gt,
-                                                            nn->nbr, nn->nn, nn->nba, nn->nna,
-                                                            nullptr, ctx->status);
+        k_nn_pos<<<(nn->n_atoms + 255) / 256, 256, 0, s>>>(g, ctx->q, nn->P4);
+        k_nn_desc<<<b.nblk_atom, AT, 0, s>>>(g, nn->P4, nn->mean, nn->stdv, nn->gt, nn->nbr, nn->nn,
+                                             nn->nba, nn->nna, nullptr, ctx->status);
     }
     {
         TimeScope ts(ctx, "nn_mlp");
         k_nn_mlp<<<b.to + b.th, 64 * MLP_WAVES, 0, s>>>(g, nn_weights(nn), b.to, nn->gt, nn->stdv,
-                                                        nn->dEdG, nn->Eatom);
+                                                        nn->dEdG, nullptr, nn->Etile);
     }
     {
         TimeScope ts(ctx, "nn_dang");
-        k_nn_dang<<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, ctx->q, nn->nba, nn->nna, nn->dEdG,
-                                                            nn->dang, nn->dself);
+        k_nn_dang<<<b.nblk_atom, AT, 0, s>>>(g, nn->P4, nn->nba, nn->nna, nn->dEdG, nn->dang,
+                                             nn->dself);
     }
     {
         TimeScope ts(ctx, "nn_force");
-        k_nn_force<0><<<b.nblk_atom, 64 * DESC_WAVES, 0, s>>>(g, ctx->q, nn->nbr, nn->nn, nn->nba,
-                                                             nn->nna, nn->dEdG, nn->dang, nn->dself,
-                                                             ctx->fph, nullptr, nullptr, nullptr);
+        k_nn_force<0><<<b.nblk_atom, AT, 0, s>>>(g, nn->P4, nn->nbr, nn->nn, nn->nba, nn->nna,
+                                                 nn->dEdG, nn->dang, nn->dself, ctx->fnn, nullptr,
+                                                 nullptr, nullptr);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;
+}
+
+static int nn_tiles(const qtp_nn *nn)
+{
+    const NNBatch b = nn_batch(nn, 1);
+    return b.to + b.th;
 }

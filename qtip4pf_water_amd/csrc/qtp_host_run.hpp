@@ -46,7 +46,9 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         TimeScope ts(ctx, "ksum");
         if (ctx->kf_mf) {
             // few chunks in the launch: several blocks per chunk
-            const int usplit = E * ctx->nac < 128 ? std::min(4, std::max(1, ctx->ks_nunit / KS_WAVES)) : 1;
+            const int umax = std::min(4, std::max(1, ctx->ks_nunit / KS_WAVES));
+            const int usplit = ctx->ks_usplit > 0 ? std::min(ctx->ks_usplit, umax)
+                                                  : (E * ctx->nac < 128 ? umax : 1);
             k_ksum_mf<<<E * ctx->nac * usplit, 64 * KS_WAVES, ctx->ksum_mf_lds, sk>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->ks_col,
                 ctx->ks_ncol4, ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, usplit,
@@ -60,7 +62,8 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         TimeScope ts(ctx, "sreduce");
         k_sreduce<<<E * ctx->ngr, 256, 0, sk>>>(ctx->spart, ctx->nac, ctx->G, ctx->groups, ctx->gcf,
                                                  ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
-                                                 ctx->grk, 0.25 / ctx->ew.kappa2, vp, ctx->wm,
+                                                 ctx->grk, 0.25 / ctx->ew.kappa2, vp,
+                                                 ctx->kf_mf ? ctx->wm : nullptr,
                                                  ctx->mf_Mp, 4 * ctx->mf_nks, ctx->gcol,
                                                  ctx->ew.bk[2], ctx->kf_mf ? 1 : 0);
     }
@@ -103,7 +106,8 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         // that measured faster (4096 x 32: 3.42 against 3.60 ms); literals
         // for small ones (216 x 32: no gain, and the second stream's kernels
         // lost more than the launch gained)
-        const bool carg = (long)E * ctx->ntile >= RS_CARG_BLOCKS;
+        const bool carg = ctx->rs_carg >= 0 ? ctx->rs_carg == 1
+                                            : (long)E * ctx->ntile >= RS_CARG_BLOCKS;
         const RsCoef cf = rs_coef();
 #define QTP_RS_LAUNCH(VIR, CARG, CULL)                                                             \
     k_realspace<VIR, CARG, CULL><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(                         \
@@ -130,7 +134,8 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         const int nab = (ctx->n_atoms + KF_SITES - 1) / KF_SITES;
         const int nbm = E * ((ctx->n_atoms + MF_SITES - 1) / MF_SITES);
         // few blocks: twice the waves per block (when its LDS fits the default 64 KB)
-        if (ctx->kf_mf && nbm < MF_SMALL_BLOCKS && ctx->kforce_mf_lds_small <= 64 * 1024)
+        const bool small = ctx->mf_small >= 0 ? ctx->mf_small == 1 : nbm < MF_SMALL_BLOCKS;
+        if (ctx->kf_mf && small && ctx->kforce_mf_lds_small <= 64 * 1024)
             k_kforce_mf<MF_WAVES_SMALL><<<nbm, 64 * MF_WAVES_SMALL, ctx->kforce_mf_lds_small, s>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
                 ctx->mf_nks, ctx->mf_rt_small, ctx->mf_nrt_small, ctx->rpart, ctx->nblk, ctx->flj,
@@ -200,8 +205,47 @@ static int run_sc_forces(qtp_ctx *ctx, int kick, unsigned long long *ctr)
 
 // with_heat: also reduce a thermostatted step's heat (folded into the fused
 // list kernel; a kernel of its own on the multi-kernel path)
-static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
+// Cell-binned build of the beads flagged in ctx->flag (check = 0) or of
+// those the displacement test flags (check = 1): bins, row counts, offsets,
+// rows.
+static int run_cell_lists(qtp_ctx *ctx, int check, int force_all,
+                          const unsigned long long *ctr)
 {
+    hipStream_t s = ctx->stream;
+    const int P = ctx->P, nrb = (ctx->n_mols + 3) / 4;
+    const size_t bm_lds = 4 * (size_t)((ctx->n_mols + 63) / 64) * sizeof(unsigned long long);
+    {
+        TimeScope ts(ctx, "check");
+        k_cell_build<<<P, 1024, ctx->cg.ncell * sizeof(int), s>>>(
+            ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box, ctx->cg,
+            ctx->drv.r_skin - ctx->drv.lj_rcut, check, force_all, ctr ? ctx->rebuild_every : 0, ctr,
+            ctx->disp, ctx->flag, check ? ctx->nreb : nullptr, ctx->cell_of, ctx->crank,
+            ctx->cstart, ctx->member);
+    }
+    {
+        TimeScope ts(ctx, "nlist");
+        k_nl_cell<false><<<P * nrb, 256, bm_lds, s>>>(
+            ctx->n_mols, ctx->NAP, ctx->q, ctx->box, ctx->cg, ctx->ff.r_skin_sq, ctx->list_size,
+            ctx->fcap, ctx->flag, ctx->cell_of, ctx->cstart, ctx->member, ctx->cnt, nullptr, nullptr,
+            nullptr, nullptr);
+        k_nl_scan<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->list_size, ctx->fcap, ctx->flag, ctx->cnt,
+                                     ctx->point, ctx->fpoint, ctx->list, ctx->disp, ctx->NMP,
+                                     ctx->status);
+        k_nl_cell<true><<<P * nrb, 256, bm_lds, s>>>(
+            ctx->n_mols, ctx->NAP, ctx->q, ctx->box, ctx->cg, ctx->ff.r_skin_sq, ctx->list_size,
+            ctx->fcap, ctx->flag, ctx->cell_of, ctx->cstart, ctx->member, nullptr, ctx->point,
+            ctx->fpoint, ctx->list, ctx->flist);
+    }
+    HIPCHK(hipGetLastError());
+    return QTP_OK;
+}
+
+// ctr: the device step counter of resident steps (forced rebuilds every
+// ctx->rebuild_every steps), or null
+static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false,
+                     const unsigned long long *ctr = nullptr)
+{
+    const int every = ctr ? ctx->rebuild_every : 0;
     hipStream_t s = ctx->stream;
     const int P = ctx->P;
     HeatArgs heat{};
@@ -213,11 +257,12 @@ static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
         heat.ntile = ctx->ptile;
         heat.nh = ctx->P * ctx->ptile;
     }
-    if (ctx->n_mols <= NL_FUSED_MAXMOL) {
+    const bool cells = use_cells(ctx);
+    if (ctx->n_mols <= NL_FUSED_MAXMOL && !cells) {
         TimeScope ts(ctx, "nlist");
         k_nlist_bead<<<P + (with_heat ? 1 : 0), 1024, 0, s>>>(ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box,
                                         ctx->ff.r_skin_sq, ctx->drv.r_skin - ctx->drv.lj_rcut, force_all,
-                                        ctx->list_size, ctx->fcap, ctx->disp, ctx->flag, ctx->nreb,
+                                        every, ctr, ctx->list_size, ctx->fcap, ctx->disp, ctx->flag, ctx->nreb,
                                         ctx->point, ctx->fpoint, ctx->list, ctx->flist,
                                         ctx->status, heat);
         HIPCHK(hipGetLastError());
@@ -226,10 +271,11 @@ static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false)
     if (with_heat)
         k_heat<<<1, 1024, 0, s>>>(heat.ntile, heat.nh, heat.part, heat.hpart, heat.heat_acc,
                                 heat.heat_step);
+    if (cells) return run_cell_lists(ctx, 1, force_all, ctr);
     {
         TimeScope ts(ctx, "check");
         k_check<<<P, 256, 0, s>>>(ctx->n_mols, ctx->NMP, ctx->disp, ctx->drv.r_skin - ctx->drv.lj_rcut, force_all,
-                                  ctx->flag, ctx->nreb);
+                                  every, ctr, ctx->flag, ctx->nreb);
     }
     const int nrb = (ctx->n_mols + 3) / 4;
     {
@@ -263,8 +309,12 @@ static PropArgs prop_args(qtp_ctx *ctx, int nvt, int host_dev)
     A.seed = ctx->seed;
     A.cmode = ctx->pimd_mode == 2 ? 2 : 0;
     A.cthermo = ctx->cthermo;
-    const double c = ctx->tau_fs > 0.0 ? std::exp(-ctx->prm.dtfs / ctx->tau_fs) : 0.0;
-    A.cbussi = ctx->cthermo == 0 ? c : 0.0;
+    double c = ctx->tau_fs > 0.0 ? std::exp(-ctx->prm.dtfs / ctx->tau_fs) : 0.0;
+    if (host_dev) {   // drop-in steps: the reference's pile_g (tau -> 0)
+        A.cthermo = 0;
+        c = 0.0;
+    }
+    A.cbussi = A.cthermo == 0 ? c : 0.0;
     A.c1_0 = c;
     A.c2_0 = std::sqrt(1.0 - c * c);
     return A;
@@ -276,6 +326,16 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
     hipStream_t s = ctx->stream;
     PropArgs A = prop_args(ctx, nvt, host_dev);
     const int grid = ctx->ptile * ctx->P;
+    // one kernel per tile, the beads staged in LDS, when the centroid needs
+    // no rescale over the whole system between the two transforms
+    const bool bussi = nvt && A.cthermo == 0 && A.cmode != 2;
+    if (ctx->prop_fused != 0 && !bussi && !(nvt && host_dev) &&
+        ctx->P <= PT_WAVES * PT_PER_WAVE) {
+        TimeScope ts(ctx, "prop");
+        k_prop_tile<<<ctx->ptile, 64 * PT_WAVES, prop_tile_lds(ctx->P), s>>>(
+            A, ctx->Cm, ctx->mode, ctx->ms, ctx->q, ctx->p, ctx->F, ctx->m_atom, ctx->sqm_atom,
+            ctx->ppart, ctx->hpart, ctx->disp, ctx->NMP, ctr);
+    } else {
     {
         TimeScope ts(ctx, "prop_a");
         k_prop_fwd<<<grid, 64, 0, s>>>(A, ctx->Cm, ctx->mode, ctx->ms, ctx->q, ctx->p, ctx->F, ctx->m_atom,
@@ -288,8 +348,9 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
                                        ctx->ppart, ctx->dev, ctx->q, ctx->p, ctx->disp, ctx->NMP,
                                        ctx->hpart, ctr);
     }
+    }
     HIPCHK(hipGetLastError());
-    CHK(run_lists(ctx, 0, nvt != 0));
+    CHK(run_lists(ctx, 0, nvt != 0, ctr));
     CHK(run_sc_forces(ctx, 1, ctr));
     if (ctx->cmf_on)
         k_cmf_acc<<<(3 * ctx->NAP + 255) / 256, 256, 0, s>>>(ctx->P, ctx->NAP, ctx->F, ctx->cmf_acc,
@@ -320,15 +381,27 @@ static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr, bool resident =
                                   ctx->m_atom, ctx->F, ctx->q, ctx->p, ctx->disp, lang,
                                   ctx->sqm_atom, ctx->step_ctr);
     }
-    CHK(run_lists(ctx, 0));
-    CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, nullptr));
+    // the learned correction depends on the new positions only: it runs on a
+    // third stream beside the lists and the qTIP4P/F forces (on the main
+    // stream when kernels are timed one by one)
     const bool with_nn = resident && ctx->nn;
-    if (with_nn) CHK(run_nn_forces(ctx));
+    const bool nn_fork = with_nn && !ctx->timing;
+    if (nn_fork) {
+        HIPCHK(hipEventRecord(ctx->ev_nn_fork, s));
+        HIPCHK(hipStreamWaitEvent(ctx->s3, ctx->ev_nn_fork, 0));
+        CHK(run_nn_forces(ctx, ctx->s3));
+        HIPCHK(hipEventRecord(ctx->ev_nn_join, ctx->s3));
+    }
+    CHK(run_lists(ctx, 0, false, resident ? ctr : nullptr));
+    CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, nullptr));
+    if (with_nn && !nn_fork) CHK(run_nn_forces(ctx, s));
+    if (nn_fork) HIPCHK(hipStreamWaitEvent(s, ctx->ev_nn_join, 0));
     {
         TimeScope ts(ctx, "vv_b");
         k_vv_b<<<nb, 256, 0, s>>>(ctx->n_atoms, ctx->NAP, ctx->drv.halfdt, ctx->fph, ctx->epart,
                                   ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p, ctx->out,
-                                  ctr, with_nn ? ctx->nn->Eatom : nullptr);
+                                  ctr, with_nn ? ctx->fnn : nullptr,
+                                  with_nn ? ctx->nn->Etile : nullptr, with_nn ? nn_tiles(ctx->nn) : 0);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;
@@ -340,11 +413,13 @@ static int run_class_step(qtp_ctx *ctx, unsigned long long *ctr, bool resident =
 static int class_initial_forces(qtp_ctx *ctx)
 {
     CHK(run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, nullptr));
-    if (ctx->nn) CHK(run_nn_forces(ctx));
+    if (ctx->nn) CHK(run_nn_forces(ctx, ctx->stream));
     const int nb = (3 * ctx->NAP + 255) / 256;
     k_vv_b<<<nb, 256, 0, ctx->stream>>>(ctx->n_atoms, ctx->NAP, 0.0, ctx->fph, ctx->epart,
                                         ctx->ep_stride, ep_live(ctx), ctx->eself, ctx->F, ctx->p,
-                                        ctx->out, nullptr, ctx->nn ? ctx->nn->Eatom : nullptr);
+                                        ctx->out, nullptr, ctx->nn ? ctx->fnn : nullptr,
+                                        ctx->nn ? ctx->nn->Etile : nullptr,
+                                        ctx->nn ? nn_tiles(ctx->nn) : 0);
     HIPCHK(hipGetLastError());
     return QTP_OK;
 }
@@ -504,10 +579,7 @@ static int upload_charges_masses(qtp_ctx *ctx, const double *charge, const doubl
         if (c != ctx->h_charge) {
             ctx->h_charge = c;
             HIPCHK(hipMemcpy(ctx->charge, c.data(), c.size() * sizeof(double), hipMemcpyHostToDevice));
-            const double pref = 2.0 * ctx->drv.kappa / std::sqrt(RC.pi);
-            double es = 0.0;
-            for (int a = 0; a < ctx->n_atoms; ++a) es += 0.5 * pref * c[a] * c[a];
-            ctx->eself = es;
+            ctx->eself = self_energy(ctx, c.data());
         }
     }
     if (mass) {

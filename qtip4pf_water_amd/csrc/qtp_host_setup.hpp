@@ -218,6 +218,66 @@ static void build_nm(int P, std::vector<double> &C)
     }
 }
 
+// Cell grid of the cell-binned list build: cells at least r_skin / S wide
+// (relative margin 1e-9, far above the rounding of a cell index), S = 2 when
+// every side holds 5 such cells, else S = 1 with 3; false when neither fits,
+// or the bins or row bitmaps would not fit in LDS.
+static bool cell_grid(double r_skin, const double L[3], int n_mols, CellGrid &cg)
+{
+    if (n_mols > CELL_MAXMOL) return false;
+    for (int S = 2; S >= 1; --S) {
+        int nc[3];
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) {
+            nc[d] = (int)std::floor(S * L[d] / (r_skin * (1.0 + 1e-9)));
+            ok = ok && nc[d] >= 2 * S + 1;
+        }
+        if (!ok || (long)nc[0] * nc[1] * nc[2] > CELL_MAXCELL) continue;
+        cg.S = S;
+        cg.ncell = nc[0] * nc[1] * nc[2];
+        for (int d = 0; d < 3; ++d) {
+            cg.nc[d] = nc[d];
+            cg.sc[d] = nc[d] / L[d];
+        }
+        return true;
+    }
+    return false;
+}
+
+// Lanes per molecule of k_ljintra (16 or 64) and the energy-partial layout
+// that follows from it: LJ/intra blocks, real-space tiles, k-space blocks,
+// spline blocks.
+static void set_lj_lanes(qtp_ctx *ctx, int lpm)
+{
+    ctx->lj_lpm = lpm;
+    ctx->nmb = (ctx->n_mols + (256 / lpm) - 1) / (256 / lpm);
+    ctx->ep_lj = 0;
+    ctx->ep_real = ctx->nmb;
+    ctx->ep_k = ctx->nmb + ctx->ntile;
+    ctx->ep_spl = ctx->nmb + ctx->ntile + ctx->ngr;
+    ctx->ep_stride = ctx->ep_spl + ctx->nblk;
+}
+
+// whether the next list build takes the cell-binned kernels
+static bool use_cells(const qtp_ctx *ctx)
+{
+    if (!ctx->cells_ok || ctx->list_method == QTP_LIST_ALLPAIRS) return false;
+    return ctx->list_method == QTP_LIST_CELLS || ctx->n_mols > NL_FUSED_MAXMOL;
+}
+
+// Ewald self energy sum_i (kappa / sqrt(pi)) q_i^2 (forces.cpp:622-624), in
+// the reference's per-site order but with an 80-bit accumulator: the FP64
+// running sum of thousands of equal terms drifts by up to ~5e-14 relative
+// (3e-11 Eh of the 571 Eh at 4096 waters), which V, a difference of such
+// terms, would carry at ~1e-11 relative.
+static double self_energy(const qtp_ctx *ctx, const double *charge)
+{
+    const double pref = 2.0 * ctx->drv.kappa / std::sqrt(RC.pi);
+    long double es = 0.0L;
+    for (int a = 0; a < ctx->n_atoms; ++a) es += (long double)(0.5 * pref * charge[a] * charge[a]);
+    return (double)es;
+}
+
 static int alloc_all(qtp_ctx *ctx)
 {
     const int P = ctx->P, E = std::max(P, 1);
@@ -235,11 +295,17 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->fpoint, (size_t)P * (ctx->n_mols + 1)));
     CHK(dalloc(ctx, &ctx->flist, (size_t)P * ctx->fcap));
     CHK(dalloc(ctx, &ctx->cnt, (size_t)P * 2 * ctx->n_mols));
+    if (ctx->cells_ok) {
+        CHK(dalloc(ctx, &ctx->cell_of, (size_t)P * ctx->n_mols));
+        CHK(dalloc(ctx, &ctx->crank, (size_t)P * ctx->n_mols));
+        CHK(dalloc(ctx, &ctx->member, (size_t)P * ctx->n_mols));
+        CHK(dalloc(ctx, &ctx->cstart, (size_t)P * (ctx->cg.ncell + 1)));
+    }
     CHK(dalloc(ctx, &ctx->flag, P));
     CHK(dalloc(ctx, &ctx->status, 1));
     CHK(dalloc(ctx, &ctx->nreb, P));
     CHK(dalloc(ctx, &ctx->qpm, (size_t)E * N3));
-    CHK(dalloc(ctx, &ctx->vpart, (size_t)E * 6 * ctx->ep_stride));
+    CHK(dalloc(ctx, &ctx->vpart, (size_t)E * 6 * ctx->ep_cap));
     CHK(dalloc(ctx, &ctx->vout, (size_t)E * 6));
     CHK(dalloc(ctx, &ctx->rpart, (size_t)E * ctx->nblk * N3));
     if (ctx->rs_cull) {
@@ -248,15 +314,15 @@ static int alloc_all(qtp_ctx *ctx)
         CHK(dalloc(ctx, &ctx->rs_far, (size_t)E * ctx->ntile));
     }
     CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));
-    if (!ctx->kf_mf) CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));
+    CHK(dalloc(ctx, &ctx->sprime, (size_t)E * ctx->G * KF_REC));
 
     CHK(dalloc(ctx, &ctx->flj, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fspl, (size_t)E * N3));
     CHK(dalloc(ctx, &ctx->fpm, (size_t)E * N3));
-    if (ctx->kf_split) CHK(dalloc(ctx, &ctx->fk2, (size_t)E * N3));
-    if (ctx->kf_mf) CHK(dalloc(ctx, &ctx->wm, (size_t)E * 4 * ctx->mf_nks * ctx->mf_Mp));
+    if (ctx->kf_split_ok) CHK(dalloc(ctx, &ctx->fk2, (size_t)E * N3));
+    if (ctx->kf_mf_ok) CHK(dalloc(ctx, &ctx->wm, (size_t)E * 4 * ctx->mf_nks * ctx->mf_Mp));
     CHK(dalloc(ctx, &ctx->Ve, E));
-    CHK(dalloc(ctx, &ctx->epart, (size_t)E * ctx->ep_stride));
+    CHK(dalloc(ctx, &ctx->epart, (size_t)E * ctx->ep_cap));
     CHK(dalloc(ctx, &ctx->scal, (size_t)E * 4));
     CHK(dalloc(ctx, &ctx->out, 8));
     CHK(dalloc(ctx, &ctx->heat, 1));
@@ -331,11 +397,7 @@ static int upload_constants(qtp_ctx *ctx)
     HIPCHK(hipMemcpy(ctx->m_atom, ctx->h_mass_atom.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->sqm_atom, sqm.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(ctx->charge, ctx->h_charge.data(), ctx->NAP * sizeof(double), hipMemcpyHostToDevice));
-    // self energy constant in the reference's per-site order
-    const double pref = 2.0 * ctx->drv.kappa / std::sqrt(RC.pi);
-    double es = 0.0;
-    for (int a = 0; a < ctx->n_atoms; ++a) es += 0.5 * pref * ctx->h_charge[a] * ctx->h_charge[a];
-    ctx->eself = es;
+    ctx->eself = self_energy(ctx, ctx->h_charge.data());
     // normal modes and per-mode propagator / thermostat constants
     const int P = ctx->P;
     std::vector<double> C;

@@ -6,4 +6,5 @@
 #include "qtp_kspace.hpp"
 #include "qtp_pair.hpp"
 #include "qtp_step.hpp"
+#include "qtp_cells.hpp"
 #include "qtp_nn.hpp"

@@ -33,8 +33,6 @@ constexpr float RR = 6.0f, RA = 4.0f, MU0 = 0.8f, DMU = 5.2f / 23.0f;
 constexpr float ETAR = 0.5f / (DMU * DMU);
 constexpr float ETA_A0 = 0.02f, ETA_A1 = 0.2f;
 constexpr float PIF = 3.14159265358979f;
-constexpr int DESC_WAVES = 8;    // atoms (wavefronts) per block of the per-atom kernels
-constexpr int CHUNK = 512;       // candidate atoms staged in LDS per round of the list search
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
@@ -74,9 +72,7 @@ struct NNGeom {
 };
 
 // Coordinate d of atom a in Angstrom (float), wrapped into the box first in
-// FP64 (positions are never wrapped in the state).  Every kernel forms
-// separations from these values the same way (sepf), so lists, descriptors,
-// forces and tangents see identical geometry.
+// FP64 (positions are never wrapped in the state).
 __device__ __forceinline__ float posf(const double *__restrict__ X, int NAP, const Box &box, int d,
                                       int a)
 {
@@ -85,25 +81,6 @@ __device__ __forceinline__ float posf(const double *__restrict__ X, int NAP, con
 }
 __device__ __forceinline__ float mimf(float d, float L, float iL) { return d - L * rintf(d * iL); }
 
-// Minimum-image separation x_j - x_i (Angstrom) and its length.
-__device__ __forceinline__ float4 sepf(const double *__restrict__ X, const NNGeom &g, int i, int j)
-{
-    const float dx = mimf(posf(X, g.NAP, g.box, 0, j) - posf(X, g.NAP, g.box, 0, i), g.Lf[0], g.iLf[0]);
-    const float dy = mimf(posf(X, g.NAP, g.box, 1, j) - posf(X, g.NAP, g.box, 1, i), g.Lf[1], g.iLf[1]);
-    const float dz = mimf(posf(X, g.NAP, g.box, 2, j) - posf(X, g.NAP, g.box, 2, i), g.Lf[2], g.iLf[2]);
-    return make_float4(dx, dy, dz, sqrtf(dx * dx + dy * dy + dz * dz));
-}
-
-// Blocks of the per-atom kernels hold DESC_WAVES atoms of one frame:
-// wavefront w of block b takes atom i of frame f; false past the frame's end.
-__device__ __forceinline__ int atoms_blocks(int n_atoms) { return (n_atoms + DESC_WAVES - 1) / DESC_WAVES; }
-__device__ __forceinline__ bool block_atom(const NNGeom &g, int &f, int &i)
-{
-    const int bpf = atoms_blocks(g.n_atoms);
-    f = blockIdx.x / bpf;
-    i = (blockIdx.x - f * bpf) * DESC_WAVES + (threadIdx.x >> 6);
-    return i < g.n_atoms;
-}
 
 // tanh(x) = 1 - 2 / (exp(2x) + 1): saturates to +-1 without overflow; its
 // absolute error (a few 1e-7) is far below the BF16 rounding of the
@@ -122,21 +99,24 @@ __device__ __forceinline__ float dfcut(float r, float R)
     return r < R ? -0.5f * PIF / R * __sinf(PIF * r / R) : 0.0f;
 }
 
-// The 16 angular terms of one (j, k) pair and, when D, their partial
-// derivatives in cos t, r_ij (a) and r_ik (b).
-template <bool D>
-__device__ __forceinline__ void ang_terms(float c, float ra, float rb, float wt, float *T, float *Tc,
-                                          float *Ta, float *Tb)
+// Per angular neighbour (r < RA): fc(r; RA), its derivative, and the two
+// radial factors exp(-h r^2) of the angular terms (the pair factor
+// exp(-h (r_ij^2 + r_ik^2)) is their product).
+__device__ __forceinline__ float4 ang_pre(float r)
 {
-    const float fa = fcut(ra, RA), fb = fcut(rb, RA);
-    const float s2 = ra * ra + rb * rb;
-    const float ex[2] = {__expf(-ETA_A0 * s2), __expf(-ETA_A1 * s2)};
+    return make_float4(fcut(r, RA), dfcut(r, RA), __expf(-ETA_A0 * r * r), __expf(-ETA_A1 * r * r));
+}
+
+// The 16 angular terms of one (j, k) pair (cos t = c, lengths ra, rb, their
+// ang_pre values pa, pb, pair weight wt) and, when D, their partial
+// derivatives in cos t, r_ij (Ta) and r_ik (Tb).
+template <bool D>
+__device__ __forceinline__ void ang_pair(float c, float ra, float rb, float4 pa, float4 pb, float wt,
+                                         float *T, float *Tc, float *Ta, float *Tb)
+{
+    const float ff = pa.x * pb.x * wt;
+    const float ex[2] = {pa.z * pb.z, pa.w * pb.w};
     const float eta[2] = {ETA_A0, ETA_A1};
-    float dfa = 0.f, dfb = 0.f;
-    if (D) {
-        dfa = dfcut(ra, RA);
-        dfb = dfcut(rb, RA);
-    }
 #pragma unroll
     for (int l = 0; l < 2; ++l) {
         const float base = l == 0 ? 1.0f + c : 1.0f - c;
@@ -150,12 +130,12 @@ __device__ __forceinline__ void ang_terms(float c, float ra, float rb, float wt,
 #pragma unroll
             for (int z = 0; z < 4; ++z) {
                 const int n = 8 * h + 2 * z + l;
-                const float e = ex[h] * wt;
-                T[n] = pw[z] * e * fa * fb;
+                T[n] = pw[z] * ex[h] * ff;
                 if (D) {
-                    Tc[n] = sg * dp[z] * e * fa * fb;
-                    Ta[n] = pw[z] * e * fb * (dfa - 2.0f * eta[h] * ra * fa);
-                    Tb[n] = pw[z] * e * fa * (dfb - 2.0f * eta[h] * rb * fb);
+                    const float e = pw[z] * ex[h] * wt;
+                    Tc[n] = sg * dp[z] * ex[h] * ff;
+                    Ta[n] = e * pb.x * (pa.y - 2.0f * eta[h] * ra * pa.x);
+                    Tb[n] = e * pa.x * (pb.y - 2.0f * eta[h] * rb * pb.x);
                 }
             }
         }
@@ -173,138 +153,215 @@ __device__ __forceinline__ void tri_decode(int p, int &j, int &k)
 }
 
 // ---------------------------------------------------------------------------
-// Neighbour lists and normalised descriptors.  One wavefront per atom,
-// DESC_WAVES atoms of one frame per block; the frame's atoms pass through
-// LDS in rounds of CHUNK (float, Angstrom) and every wavefront tests them
-// against its atom.  Lists hold, in ascending index order, every atom within
-// RR (nbr, count nn) and the subset within RA (nba, count nna).  status bit
-// 2: a list was full (the atom's lists are then truncated).
-// grid: n_frames * atoms_blocks(n_atoms), 64 * DESC_WAVES threads.
+// Per-atom kernels: one block of AT threads (four wavefronts) per atom, which
+// share its neighbours, pairs and sums.  Positions come from k_nn_pos as
+// float4 (Angstrom, wrapped into the box); separations are formed the same
+// way everywhere (sep4), so lists, descriptors, forces and tangents see one
+// geometry.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_desc(
-    NNGeom g, const double *__restrict__ pos, const float *__restrict__ mean,
+constexpr int AT = 256;
+constexpr int RGRP = AT / NR;   // neighbour groups of the radial sums (10 x 24 threads)
+
+// P4[f * n_atoms + a] = (x, y, z) of atom a of frame f in Angstrom, wrapped.
+// grid: ceil(n_frames * n_atoms / 256), 256 threads.
+__global__ __launch_bounds__(256) void k_nn_pos(NNGeom g, const double *__restrict__ pos,
+                                                float4 *__restrict__ P4)
+{
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= g.n_frames * g.n_atoms) return;
+    const int f = t / g.n_atoms, a = t - f * g.n_atoms;
+    const double *X = pos + (long)f * 3 * g.NAP;
+    P4[t] = make_float4(posf(X, g.NAP, g.box, 0, a), posf(X, g.NAP, g.box, 1, a),
+                        posf(X, g.NAP, g.box, 2, a), 0.f);
+}
+
+__device__ __forceinline__ float4 sep4(const float4 *__restrict__ P, const NNGeom &g, int i, int j)
+{
+    const float4 a = P[i], b = P[j];
+    const float dx = mimf(b.x - a.x, g.Lf[0], g.iLf[0]);
+    const float dy = mimf(b.y - a.y, g.Lf[1], g.iLf[1]);
+    const float dz = mimf(b.z - a.z, g.Lf[2], g.iLf[2]);
+    return make_float4(dx, dy, dz, sqrtf(dx * dx + dy * dy + dz * dz));
+}
+
+// Block sum of n per-thread values (n <= NIN) into out[] (thread 0's view:
+// every thread of the block may read out after the call).
+template <int N>
+__device__ __forceinline__ void block_sums(float *v, float (*red)[NIN], float *out)
+{
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+    for (int m = 0; m < N; ++m) {
+        const float x = wave_sumf(v[m]);
+        if (lane == m) red[w][m] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) out[threadIdx.x] = red[0][threadIdx.x] + red[1][threadIdx.x] +
+                                            red[2][threadIdx.x] + red[3][threadIdx.x];
+    __syncthreads();
+}
+
+// Wave sums of 16 per-lane values by recursive halving: each round a lane
+// keeps half of its values and adds its partner's matching half (8, 4, 2, 1
+// values over the lane bits 5..2), then lanes 4c..4c+3 hold the total of
+// component c = lane >> 2 after two plain xor rounds.  63 shuffles instead
+// of 96, in a fixed order.
+__device__ __forceinline__ float halving_sum16(float *v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int o = 32 >> r, h = 8 >> r;
+        const bool up = (lane & o) != 0;
+#pragma unroll
+        for (int i = 0; i < h; ++i) {
+            const float keep = up ? v[i + h] : v[i], give = up ? v[i] : v[i + h];
+            v[i] = keep + __shfl_xor(give, o, 64);
+        }
+    }
+    float x = v[0];
+    x += __shfl_xor(x, 2, 64);
+    x += __shfl_xor(x, 1, 64);
+    return x;
+}
+
+// Neighbour lists and normalised descriptors.  The four wavefronts test
+// consecutive quarters of the frame's atoms; their hits, merged in order,
+// give lists in ascending index order: every atom within RR (nbr, count nn)
+// and the subset within RA (nba, count nna).  status bit 2: a list was full
+// (the atom's lists are then truncated).
+// grid: n_frames * n_atoms blocks of AT threads.
+__global__ __launch_bounds__(AT) void k_nn_desc(
+    NNGeom g, const float4 *__restrict__ P4, const float *__restrict__ mean,
     const float *__restrict__ stdv, float *__restrict__ gt, int *__restrict__ nbr,
     int *__restrict__ nn, int *__restrict__ nba, int *__restrict__ nna, float *__restrict__ graw,
     int *__restrict__ status)
 {
-    __shared__ float4 sp[CHUNK];
-    __shared__ float4 sv[DESC_WAVES][NMAX];   // x_j - x_i (A), r
-    __shared__ int sj[DESC_WAVES][NMAX];      // neighbour atom
-    __shared__ int sa[DESC_WAVES][NAMAX];     // angular neighbours: slot in sv
-    __shared__ float sg[DESC_WAVES][NIN];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int bpf = atoms_blocks(g.n_atoms);
-    const int f = blockIdx.x / bpf, i = (blockIdx.x - f * bpf) * DESC_WAVES + w;
-    const bool valid = i < g.n_atoms;
-    const int gi = f * g.n_atoms + i;
-    const double *X = pos + (long)f * 3 * g.NAP;
-    float xi = 0.f, yi = 0.f, zi = 0.f;
-    if (valid) {
-        xi = posf(X, g.NAP, g.box, 0, i);
-        yi = posf(X, g.NAP, g.box, 1, i);
-        zi = posf(X, g.NAP, g.box, 2, i);
-    }
-    const float rr2 = RR * RR, ra2 = RA * RA;
-    int cnt = 0, cnta = 0, over = 0;
-    for (int c0 = 0; c0 < g.n_atoms; c0 += CHUNK) {
-        __syncthreads();
-        for (int t = threadIdx.x; t < CHUNK; t += 64 * DESC_WAVES) {
-            const int j = c0 + t;
-            sp[t] = j < g.n_atoms ? make_float4(posf(X, g.NAP, g.box, 0, j), posf(X, g.NAP, g.box, 1, j),
-                                                posf(X, g.NAP, g.box, 2, j), 0.f)
-                                  : make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        __syncthreads();
-        if (!valid) continue;
-        const int m = min(CHUNK, g.n_atoms - c0);
-        for (int s0 = 0; s0 < m; s0 += 64) {
-            const int t = s0 + lane, j = c0 + t;
+    __shared__ float4 tv[4][NMAX];
+    __shared__ int tj[4][NMAX];
+    __shared__ int wc[4];
+    __shared__ float4 sv[NMAX];
+    __shared__ int sj[NMAX];
+    __shared__ float4 av[NAMAX], ap[NAMAX];
+    __shared__ float aw[NAMAX];
+    __shared__ int s_cnt, s_cnta;
+    const int tid = threadIdx.x, w = tid >> 6, lane = tid & 63;
+    const int gi = blockIdx.x, f = gi / g.n_atoms, i = gi - f * g.n_atoms;
+    const float4 *P = P4 + (long)f * g.n_atoms;
+    const int ei = elem_of(i);
+    const float rr2 = RR * RR;
+    {
+        const float4 pi = P[i];
+        const int q = ((g.n_atoms + 255) / 256) * 64;
+        const int lo = w * q, hi = min(g.n_atoms, lo + q);
+        int c = 0;
+        for (int j0 = lo; j0 < hi; j0 += 64) {
+            const int j = j0 + lane;
             float dx = 0.f, dy = 0.f, dz = 0.f, r2 = 1e30f;
-            if (t < m && j != i) {
-                const float4 pj = sp[t];
-                dx = mimf(pj.x - xi, g.Lf[0], g.iLf[0]);
-                dy = mimf(pj.y - yi, g.Lf[1], g.iLf[1]);
-                dz = mimf(pj.z - zi, g.Lf[2], g.iLf[2]);
+            if (j < hi && j != i) {
+                const float4 pj = P[j];
+                dx = mimf(pj.x - pi.x, g.Lf[0], g.iLf[0]);
+                dy = mimf(pj.y - pi.y, g.Lf[1], g.iLf[1]);
+                dz = mimf(pj.z - pi.z, g.Lf[2], g.iLf[2]);
                 r2 = dx * dx + dy * dy + dz * dz;
             }
             const bool in = r2 < rr2;
             const unsigned long long b = __ballot(in);
-            const int s = cnt + __popcll(b & ((1ull << lane) - 1ull));
-            const bool keep = in && s < NMAX;
-            const bool ina = keep && r2 < ra2;
-            const unsigned long long ba = __ballot(ina);
-            const int sa_slot = cnta + __popcll(ba & ((1ull << lane) - 1ull));
-            if (keep) {
-                sv[w][s] = make_float4(dx, dy, dz, sqrtf(r2));
-                sj[w][s] = j;
-                nbr[(long)gi * NMAX + s] = j;
-                if (ina && sa_slot < NAMAX) {
-                    sa[w][sa_slot] = s;
-                    nba[(long)gi * NAMAX + sa_slot] = j;
-                }
+            const int s = c + __popcll(b & ((1ull << lane) - 1ull));
+            if (in && s < NMAX) {
+                tv[w][s] = make_float4(dx, dy, dz, sqrtf(r2));
+                tj[w][s] = j;
             }
-            over |= (cnt + __popcll(b) > NMAX) || (cnta + __popcll(ba) > NAMAX);
-            cnt += __popcll(b);
-            cnta += __popcll(ba);
+            c += __popcll(b);
+        }
+        if (lane == 0) wc[w] = c;
+    }
+    __syncthreads();
+    const int total = wc[0] + wc[1] + wc[2] + wc[3];
+    const int cnt = min(total, NMAX);
+    {
+        int off = 0;
+        for (int v = 0; v < w; ++v) off += wc[v];
+        for (int s = lane; s < min(wc[w], NMAX); s += 64) {
+            const int d = off + s;
+            if (d < NMAX) {
+                sv[d] = tv[w][s];
+                sj[d] = tj[w][s];
+                nbr[(long)gi * NMAX + d] = tj[w][s];
+            }
         }
     }
-    if (!valid) return;   // past the block's last barrier
-    cnt = min(cnt, NMAX);
-    cnta = min(cnta, NAMAX);
-    if (lane == 0) {
-        nn[gi] = cnt;
-        nna[gi] = cnta;
-        if (over) atomicOr(status, 4);
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const int ei = elem_of(i);
-    // radial
-    float acc[NR];
-#pragma unroll
-    for (int m = 0; m < NR; ++m) acc[m] = 0.f;
-    for (int s = lane; s < cnt; s += 64) {
-        const float r = sv[w][s].w;
-        const float wf = welem(ei, elem_of(sj[w][s])) * fcut(r, RR);
-#pragma unroll
-        for (int m = 0; m < NR; ++m) {
-            const float x = r - (MU0 + m * DMU);
-            acc[m] += wf * __expf(-ETAR * x * x);
+    __syncthreads();
+    if (w == 0) {
+        int ca = 0;
+        for (int s0 = 0; s0 < cnt; s0 += 64) {
+            const int s = s0 + lane;
+            const bool ina = s < cnt && sv[s].w < RA;
+            const unsigned long long b = __ballot(ina);
+            const int slot = ca + __popcll(b & ((1ull << lane) - 1ull));
+            if (ina && slot < NAMAX) {
+                const int j = sj[s];
+                av[slot] = sv[s];
+                ap[slot] = ang_pre(sv[s].w);
+                aw[slot] = welem(ei, elem_of(j));
+                nba[(long)gi * NAMAX + slot] = j;
+            }
+            ca += __popcll(b);
+        }
+        if (lane == 0) {
+            s_cnt = cnt;
+            s_cnta = min(ca, NAMAX);
+            nn[gi] = cnt;
+            nna[gi] = min(ca, NAMAX);
+            if (total > NMAX || ca > NAMAX) atomicOr(status, 4);
         }
     }
-#pragma unroll
-    for (int m = 0; m < NR; ++m) {
-        const float v = wave_sumf(acc[m]);
-        if (lane == m) sg[w][m] = v;
+    __syncthreads();
+    // radial: the weight times the cutoff function per neighbour, then
+    // thread (m, grp) sums component m over every RGRP-th neighbour
+    __shared__ float rw[NMAX];
+    __shared__ float rsum[RGRP][NR], asum[4][NANG];
+    for (int s = tid; s < s_cnt; s += AT) rw[s] = welem(ei, elem_of(sj[s])) * fcut(sv[s].w, RR);
+    __syncthreads();
+    if (tid < RGRP * NR) {
+        const int m = tid % NR, grp = tid / NR;
+        const float mu = MU0 + m * DMU;
+        float a = 0.f;
+        for (int s = grp; s < s_cnt; s += RGRP) {
+            const float x = sv[s].w - mu;
+            a += rw[s] * __expf(-ETAR * x * x);
+        }
+        rsum[grp][m] = a;
     }
-    // angular
-    float T[NANG];
-    float ang[NANG];
+    // angular: a pair per thread, the 16 sums reduced per wave by halving
+    {
+        float T[NANG], acc[NANG];
 #pragma unroll
-    for (int n = 0; n < NANG; ++n) ang[n] = 0.f;
-    const int npair = cnta * (cnta - 1) / 2;
-    for (int p = lane; p < npair; p += 64) {
-        int j, k;
-        tri_decode(p, j, k);
-        const float4 A = sv[w][sa[w][j]], B = sv[w][sa[w][k]];
-        const int aj = sj[w][sa[w][j]], ak = sj[w][sa[w][k]];
-        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
-        const float wt = welem(ei, elem_of(aj)) * welem(ei, elem_of(ak));
-        ang_terms<false>(c, A.w, B.w, wt, T, nullptr, nullptr, nullptr);
+        for (int n = 0; n < NANG; ++n) acc[n] = 0.f;
+        const int na = s_cnta, npair = na * (na - 1) / 2;
+        for (int p = tid; p < npair; p += AT) {
+            int j, k;
+            tri_decode(p, j, k);
+            const float4 A = av[j], B = av[k];
+            const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
+            ang_pair<false>(c, A.w, B.w, ap[j], ap[k], aw[j] * aw[k], T, nullptr, nullptr, nullptr);
 #pragma unroll
-        for (int n = 0; n < NANG; ++n) ang[n] += T[n];
+            for (int n = 0; n < NANG; ++n) acc[n] += T[n];
+        }
+        const float x = halving_sum16(acc);
+        if ((lane & 3) == 0) asum[w][lane >> 2] = x;
     }
+    __syncthreads();
+    if (tid < NIN) {
+        float G = 0.f;
+        if (tid < NR) {
 #pragma unroll
-    for (int n = 0; n < NANG; ++n) {
-        const float v = wave_sumf(ang[n]);
-        if (lane == n) sg[w][NR + n] = v;
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (lane < NIN) {
-        const float G = sg[w][lane];
-        gt[(long)gi * NIN + lane] = (G - mean[ei * NIN + lane]) / stdv[ei * NIN + lane];
-        if (graw) graw[(long)gi * NIN + lane] = G;
+            for (int grp = 0; grp < RGRP; ++grp) G += rsum[grp][tid];
+        } else {
+            G = asum[0][tid - NR] + asum[1][tid - NR] + asum[2][tid - NR] + asum[3][tid - NR];
+        }
+        gt[(long)gi * NIN + tid] = (G - mean[ei * NIN + tid]) / stdv[ei * NIN + tid];
+        if (graw) graw[(long)gi * NIN + tid] = G;
     }
 }
 
@@ -364,14 +421,15 @@ __device__ __forceinline__ bf16x8 gload8(const __bf16 *p)
 
 // Per-atom energies and dE/dG (G the raw descriptor) of one 16-row tile of
 // one element per block; its four wavefronts split every layer's 128 (or
-// 48) output columns, exchanging the activations through LDS.  Weights are
-// read as B fragments straight from global memory (L2-resident).
-// Blocks [0, nto) take O tiles, the rest H tiles.
+// 48) output columns, exchanging the activations through LDS.  Every B
+// fragment (weights, L2-resident) is loaded at the start, so the loads
+// overlap each other and the first layer.  Blocks [0, nto) take O tiles, the
+// rest H tiles.  Etile (may be null): the tile's energy sum per block.
 // grid: nto + nth blocks, 256 threads.
 constexpr int MLP_WAVES = 4;
 __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
     NNGeom g, NNW W, int nto, const float *__restrict__ gt, const float *__restrict__ stdv,
-    float *__restrict__ dEdG, float *__restrict__ Eatom)
+    float *__restrict__ dEdG, float *__restrict__ Eatom, float *__restrict__ Etile)
 {
     __shared__ __align__(16) __bf16 sA[16 * NH];   // a1, later dE/dz1
     __shared__ __align__(16) __bf16 sD[16 * NH];   // dE/dz2
@@ -383,6 +441,18 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
     const int ra = row_atom(e, row0 + col, g.n_mols, g.n_frames, g.n_atoms);
     const __bf16 *w1 = W.w1 + (long)e * W1E, *w1t = W.w1t + (long)e * W1E;
     const __bf16 *w2 = W.w2 + (long)e * W2E, *w2t = W.w2t + (long)e * W2E;
+    bf16x8 B1[2][2], B2[4][2], B3[4][2], B4[4];
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+#pragma unroll
+        for (int tt = 0; tt < 2; ++tt) {
+            const int n = 32 * w + 16 * tt + col;
+            if (s < 2) B1[s][tt] = gload8(w1 + n * KIN + 32 * s + kq);
+            B2[s][tt] = gload8(w2 + n * NH + 32 * s + kq);
+            B3[s][tt] = gload8(w2t + n * NH + 32 * s + kq);
+        }
+        B4[s] = gload8(w1t + (16 * (w < 3 ? w : 0) + col) * NH + 32 * s + kq);
+    }
     // layer 1, columns 32 w .. 32 w + 31
     float a1[2][4];
     {
@@ -391,8 +461,7 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
         for (int s = 0; s < 2; ++s) {
             const bf16x8 a = afrag_f32(gt, ra, NIN, 32 * s + kq, NIN);
 #pragma unroll
-            for (int tt = 0; tt < 2; ++tt)
-                acc[tt] = mfma16(a, gload8(w1 + (32 * w + 16 * tt + col) * KIN + 32 * s + kq), acc[tt]);
+            for (int tt = 0; tt < 2; ++tt) acc[tt] = mfma16(a, B1[s][tt], acc[tt]);
         }
 #pragma unroll
         for (int tt = 0; tt < 2; ++tt) {
@@ -413,8 +482,7 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
         for (int s = 0; s < 4; ++s) {
             const bf16x8 a = lds8(sA + col * NH + 32 * s + kq);
 #pragma unroll
-            for (int tt = 0; tt < 2; ++tt)
-                acc[tt] = mfma16(a, gload8(w2 + (32 * w + 16 * tt + col) * NH + 32 * s + kq), acc[tt]);
+            for (int tt = 0; tt < 2; ++tt) acc[tt] = mfma16(a, B2[s][tt], acc[tt]);
         }
         float er[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -446,8 +514,7 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
         for (int s = 0; s < 4; ++s) {
             const bf16x8 a = lds8(sD + col * NH + 32 * s + kq);
 #pragma unroll
-            for (int tt = 0; tt < 2; ++tt)
-                acc[tt] = mfma16(a, gload8(w2t + (32 * w + 16 * tt + col) * NH + 32 * s + kq), acc[tt]);
+            for (int tt = 0; tt < 2; ++tt) acc[tt] = mfma16(a, B3[s][tt], acc[tt]);
         }
         // every wavefront has read sA (layer 2) before the barrier above
 #pragma unroll
@@ -458,19 +525,23 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
                 sA[(rq + r) * NH + n] = (__bf16)((1.0f - a1[tt][r] * a1[tt][r]) * acc[tt][r]);
         }
     }
-    if (w == 0 && lane < 16 && Eatom) {
+    if (w == 0 && lane < 16) {
         const int rr = row_atom(e, row0 + lane, g.n_mols, g.n_frames, g.n_atoms);
-        const float v = sE[0][lane] + sE[1][lane] + sE[2][lane] + sE[3][lane];
-        if (rr >= 0) Eatom[rr] = W.e_scale * (v + W.b3[e]);
+        float v = rr >= 0 ? W.e_scale * (sE[0][lane] + sE[1][lane] + sE[2][lane] + sE[3][lane] + W.b3[e])
+                          : 0.f;
+        if (rr >= 0 && Eatom) Eatom[rr] = v;
+        v += __shfl_xor(v, 1, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 8, 64);
+        if (lane == 0 && Etile) Etile[blockIdx.x] = v;
     }
     __syncthreads();
     // dE/dg = dE/dz1 W1: column tile w (w < 3: inputs 0..47), dE/dG = dE/dg / std
     if (w < 3) {
         f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int s = 0; s < 4; ++s)
-            acc = mfma16(lds8(sA + col * NH + 32 * s + kq),
-                         gload8(w1t + (16 * w + col) * NH + 32 * s + kq), acc);
+        for (int s = 0; s < 4; ++s) acc = mfma16(lds8(sA + col * NH + 32 * s + kq), B4[s], acc);
         const int d = 16 * w + col;
         if (d < NIN) {
             const float is = 1.0f / stdv[e * NIN + d];
@@ -500,142 +571,162 @@ __device__ __forceinline__ float dphi(int m, float r, float fc, float dfc)
 }
 
 // ---------------------------------------------------------------------------
-// Angular force pieces per centre i (one wavefront per atom): for every
-// angular neighbour j (slot of nba), D_i[j] = sum_{k != j} dE_i/dx_j of the
-// pair terms T(i; j, k) (each pair differentiated once per member, fixed
-// order), and Dself_i = dE_i/dx_i|angular = -sum_j D_i[j].
-// grid: n_frames * atoms_blocks(n_atoms), 64 * DESC_WAVES threads.
+// Angular force pieces per centre i: for every angular neighbour j (slot of
+// nba), D_i[j] = sum_{k != j} dE_i/dx_j of the pair terms T(i; j, k), and
+// Dself_i = dE_i/dx_i|angular = -sum_j D_i[j].  Each unordered pair (j, k)
+// is differentiated once, by one thread, for both members (its two pieces
+// kept in LDS); then D_i[j] sums its pieces over k ascending (fixed order,
+// no atomics).
+// grid: n_frames * n_atoms blocks of AT threads.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_dang(
-    NNGeom g, const double *__restrict__ pos, const int *__restrict__ nba,
+constexpr int NAPAIR = NAMAX * (NAMAX - 1) / 2;
+__global__ __launch_bounds__(AT) void k_nn_dang(
+    NNGeom g, const float4 *__restrict__ P4, const int *__restrict__ nba,
     const int *__restrict__ nna, const float *__restrict__ dEdG, float4 *__restrict__ Dang,
     float4 *__restrict__ Dself)
 {
-    __shared__ float4 sv[DESC_WAVES][NAMAX];
-    __shared__ int sj[DESC_WAVES][NAMAX];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int f, i;
-    if (!block_atom(g, f, i)) return;
-    const int gi = f * g.n_atoms + i;
-    const double *X = pos + (long)f * 3 * g.NAP;
+    __shared__ float4 av[NAMAX], ap[NAMAX];
+    __shared__ float aw[NAMAX], se[NANG];
+    __shared__ float cd[2][3][NAPAIR];   // pieces of pair p for its lower (0) and upper (1) member
+    const int tid = threadIdx.x;
+    const int gi = blockIdx.x, f = gi / g.n_atoms, i = gi - f * g.n_atoms;
+    const float4 *P = P4 + (long)f * g.n_atoms;
     const int ei = elem_of(i), cnta = nna[gi];
-    if (lane < cnta) {
-        const int j = nba[(long)gi * NAMAX + lane];
-        sj[w][lane] = j;
-        sv[w][lane] = sepf(X, g, i, j);
+    if (tid < cnta) {
+        const int j = nba[(long)gi * NAMAX + tid];
+        const float4 A = sep4(P, g, i, j);
+        av[tid] = A;
+        ap[tid] = ang_pre(A.w);
+        aw[tid] = welem(ei, elem_of(j));
     }
-    float e[NANG];
-#pragma unroll
-    for (int n = 0; n < NANG; ++n) e[n] = dEdG[(long)gi * NIN + NR + n];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    float dx = 0.f, dy = 0.f, dz = 0.f;
-    if (lane < cnta) {
-        const float4 A = sv[w][lane];
-        const float wj = welem(ei, elem_of(sj[w][lane]));
+    if (tid < NANG) se[tid] = dEdG[(long)gi * NIN + NR + tid];
+    __syncthreads();
+    const int npair = cnta * (cnta - 1) / 2;
+    for (int p = tid; p < npair; p += AT) {
+        int j, k;
+        tri_decode(p, j, k);
+        const float4 A = av[j], B = av[k];
         float T[NANG], Tc[NANG], Ta[NANG], Tb[NANG];
-        for (int k = 0; k < cnta; ++k) {
-            if (k == lane) continue;
-            const float4 B = sv[w][k];
-            const float iab = 1.0f / (A.w * B.w);
-            const float c = (A.x * B.x + A.y * B.y + A.z * B.z) * iab;
-            ang_terms<true>(c, A.w, B.w, wj * welem(ei, elem_of(sj[w][k])), T, Tc, Ta, Tb);
-            float Sc = 0.f, Sa = 0.f;
+        const float iab = 1.0f / (A.w * B.w);
+        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) * iab;
+        ang_pair<true>(c, A.w, B.w, ap[j], ap[k], aw[j] * aw[k], T, Tc, Ta, Tb);
+        float Sc = 0.f, Sa = 0.f, Sb = 0.f;
 #pragma unroll
-            for (int n = 0; n < NANG; ++n) {
-                Sc += e[n] * Tc[n];
-                Sa += e[n] * Ta[n];
-            }
-            // d cos / dA = B / (ra rb) - c A / ra^2
-            const float ca = Sc * iab, cb = Sa / A.w - Sc * c / (A.w * A.w);
-            dx += ca * B.x + cb * A.x;
-            dy += ca * B.y + cb * A.y;
-            dz += ca * B.z + cb * A.z;
+        for (int n = 0; n < NANG; ++n) {
+            Sc += se[n] * Tc[n];
+            Sa += se[n] * Ta[n];
+            Sb += se[n] * Tb[n];
         }
-        Dang[(long)gi * NAMAX + lane] = make_float4(dx, dy, dz, 0.f);
+        // d cos / dA = B / (ra rb) - c A / ra^2, and likewise for B
+        const float ca = Sc * iab;
+        const float cj = Sa / A.w - Sc * c / (A.w * A.w);
+        const float ck = Sb / B.w - Sc * c / (B.w * B.w);
+        cd[0][0][p] = ca * B.x + cj * A.x;
+        cd[0][1][p] = ca * B.y + cj * A.y;
+        cd[0][2][p] = ca * B.z + cj * A.z;
+        cd[1][0][p] = ca * A.x + ck * B.x;
+        cd[1][1][p] = ca * A.y + ck * B.y;
+        cd[1][2][p] = ca * A.z + ck * B.z;
     }
-    dx = wave_sumf(dx);
-    dy = wave_sumf(dy);
-    dz = wave_sumf(dz);
-    if (lane == 0) Dself[gi] = make_float4(-dx, -dy, -dz, 0.f);
+    __syncthreads();
+    if (tid < 64) {
+        float4 D = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (tid < cnta) {
+            const int j = tid;
+            for (int k = 0; k < cnta; ++k) {
+                if (k == j) continue;
+                const int side = k < j ? 1 : 0;
+                const int p = k < j ? j * (j - 1) / 2 + k : k * (k - 1) / 2 + j;
+                D.x += cd[side][0][p];
+                D.y += cd[side][1][p];
+                D.z += cd[side][2][p];
+            }
+            Dang[(long)gi * NAMAX + tid] = D;
+        }
+        const float sx = wave_sumf(D.x), sy = wave_sumf(D.y), sz = wave_sumf(D.z);
+        if (tid == 0) Dself[gi] = make_float4(-sx, -sy, -sz, 0.f);
+    }
 }
 
 // ---------------------------------------------------------------------------
-// Forces of the correction, F_a = -dE/dx_a, gathered per atom a (one
-// wavefront per atom, fixed summation order, no atomics):
+// Forces of the correction, F_a = -dE/dx_a, gathered per atom a (fixed
+// summation order, no atomics):
 //   radial:  dE/dx_a = -sum_j [sum_m (e_am w_aj + e_jm w_ja) phi'_m(r_aj)] u_aj
 //   angular: Dself_a + sum over angular neighbours i of D_i[slot of a in i]
-// with e = dE/dG and u_aj the unit vector from a to j.  MODE 0 adds F to an
+// with e = dE/dG and u_aj the unit vector from a to j.  MODE 0 writes F to an
 // FP64 planar force array; MODE 1 writes the residual F - target (training)
 // and the atom's squared residual (and F to fout when given).
-// grid: n_frames * atoms_blocks(n_atoms), 64 * DESC_WAVES threads.
+// grid: n_frames * n_atoms blocks of AT threads.
 // ---------------------------------------------------------------------------
 template <int MODE>
-__global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_force(
-    NNGeom g, const double *__restrict__ pos, const int *__restrict__ nbr,
+__global__ __launch_bounds__(AT) void k_nn_force(
+    NNGeom g, const float4 *__restrict__ P4, const int *__restrict__ nbr,
     const int *__restrict__ nn, const int *__restrict__ nba, const int *__restrict__ nna,
     const float *__restrict__ dEdG, const float4 *__restrict__ Dang,
     const float4 *__restrict__ Dself, double *__restrict__ fout,
     const double *__restrict__ target, float *__restrict__ res, float *__restrict__ sq)
 {
-    __shared__ float se[DESC_WAVES][NR];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int f, a;
-    if (!block_atom(g, f, a)) return;
-    const int base = f * g.n_atoms, ga = base + a;
-    const double *X = pos + (long)f * 3 * g.NAP;
+    __shared__ float se[NR];
+    __shared__ float red[4][NIN], tot[NIN];
+    const int tid = threadIdx.x;
+    const int ga = blockIdx.x, f = ga / g.n_atoms, a = ga - f * g.n_atoms, base = f * g.n_atoms;
+    const float4 *P = P4 + base;
     const int ea = elem_of(a);
-    if (lane < NR) se[w][lane] = dEdG[(long)ga * NIN + lane];
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (tid < NR) se[tid] = dEdG[(long)ga * NIN + tid];
+    __syncthreads();
     const int cnt = nn[ga], cnta = nna[ga];
-    float gx = 0.f, gy = 0.f, gz = 0.f;   // dE/dx_a (Angstrom)
-    for (int s = lane; s < cnt; s += 64) {
+    float v[3] = {0.f, 0.f, 0.f};   // dE/dx_a (Angstrom)
+    // radial: thread pair (2s, 2s + 1) takes neighbour s, each half of the
+    // 24 components
+    for (int t = tid; t < 2 * cnt; t += AT) {
+        const int s = t >> 1, m0 = (t & 1) * (NR / 2);
         const int j = nbr[(long)ga * NMAX + s];
-        const float4 d = sepf(X, g, a, j);
+        const float4 d = sep4(P, g, a, j);
         const float fc = fcut(d.w, RR), dfc = dfcut(d.w, RR);
         const int ej = elem_of(j);
         const float waj = welem(ea, ej), wja = welem(ej, ea);
         const float *ejv = dEdG + (long)(base + j) * NIN;
         float c = 0.f;
 #pragma unroll
-        for (int m = 0; m < NR; ++m) c += (se[w][m] * waj + ejv[m] * wja) * dphi(m, d.w, fc, dfc);
+        for (int mm = 0; mm < NR / 2; ++mm) {
+            const int m = m0 + mm;
+            c += (se[m] * waj + ejv[m] * wja) * dphi(m, d.w, fc, dfc);
+        }
         const float ir = c / d.w;
-        gx -= ir * d.x;
-        gy -= ir * d.y;
-        gz -= ir * d.z;
+        v[0] -= ir * d.x;
+        v[1] -= ir * d.y;
+        v[2] -= ir * d.z;
     }
-    if (lane < cnta) {
+    if (tid < cnta) {
         // a in the (ascending) angular list of its neighbour i
-        const int gi = base + nba[(long)ga * NAMAX + lane];
+        const int gi = base + nba[(long)ga * NAMAX + tid];
         const int *L = nba + (long)gi * NAMAX;
-        int lo = 0, hi = nna[gi];
+        const int ni = nna[gi];
+        int lo = 0, hi = ni;
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             if (L[mid] < a) lo = mid + 1;
             else hi = mid;
         }
-        if (lo < nna[gi] && L[lo] == a) {
+        if (lo < ni && L[lo] == a) {
             const float4 D = Dang[(long)gi * NAMAX + lo];
-            gx += D.x;
-            gy += D.y;
-            gz += D.z;
+            v[0] += D.x;
+            v[1] += D.y;
+            v[2] += D.z;
         }
     }
-    if (lane == 0) {
+    if (tid == 0) {
         const float4 D = Dself[ga];
-        gx += D.x;
-        gy += D.y;
-        gz += D.z;
+        v[0] += D.x;
+        v[1] += D.y;
+        v[2] += D.z;
     }
-    gx = wave_sumf(gx);
-    gy = wave_sumf(gy);
-    gz = wave_sumf(gz);
-    if (lane == 0) {
-        const double F[3] = {-ANGD * gx, -ANGD * gy, -ANGD * gz};
+    block_sums<3>(v, red, tot);
+    if (tid == 0) {
+        const double F[3] = {-ANGD * tot[0], -ANGD * tot[1], -ANGD * tot[2]};
         double *fo = fout ? fout + (long)f * 3 * g.NAP : nullptr;
         if (MODE == 0) {
-            for (int d = 0; d < 3; ++d) fo[d * g.NAP + a] += F[d];
+            for (int d = 0; d < 3; ++d) fo[d * g.NAP + a] = F[d];
         } else {
             const double *tg = target + (long)f * 3 * g.NAP;
             float s2 = 0.f;
@@ -656,40 +747,38 @@ __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_force(
 // radial:  sum_j w phi'_m(r_ij) u_ij . (R_j - R_i)
 // angular: sum_{j<k} Tc dc + Ta da + Tb db, dc = dcos/dA . dA + dcos/dB . dB,
 //          da = A . (R_j - R_i) / r_ij, db = B . (R_k - R_i) / r_ik
-// grid: ceil(n_frames * n_atoms / DESC_WAVES), 64 * DESC_WAVES threads.
+// grid: n_frames * n_atoms blocks of AT threads.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_tan(
-    NNGeom g, const double *__restrict__ pos, const int *__restrict__ nbr,
+__global__ __launch_bounds__(AT) void k_nn_tan(
+    NNGeom g, const float4 *__restrict__ P4, const int *__restrict__ nbr,
     const int *__restrict__ nn, const int *__restrict__ nba, const int *__restrict__ nna,
     const float *__restrict__ res, const float *__restrict__ stdv, float *__restrict__ ut)
 {
-    __shared__ float4 sv[DESC_WAVES][NAMAX];
-    __shared__ float4 sr[DESC_WAVES][NAMAX];   // R_j - R_i of the angular neighbours
-    __shared__ int sj[DESC_WAVES][NAMAX];
-    __shared__ float su[DESC_WAVES][NIN];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    int f, i;
-    if (!block_atom(g, f, i)) return;
-    const int base = f * g.n_atoms, gi = base + i;
-    const double *X = pos + (long)f * 3 * g.NAP;
+    __shared__ float4 av[NAMAX], ap[NAMAX], ar[NAMAX];   // vectors, ang_pre, R_j - R_i
+    __shared__ float aw[NAMAX];
+    __shared__ float red[4][NIN], su[NIN];
+    const int tid = threadIdx.x;
+    const int gi = blockIdx.x, f = gi / g.n_atoms, i = gi - f * g.n_atoms, base = f * g.n_atoms;
+    const float4 *P = P4 + base;
     const int ei = elem_of(i);
     const float Rix = res[(long)gi * 3], Riy = res[(long)gi * 3 + 1], Riz = res[(long)gi * 3 + 2];
     const int cnt = nn[gi], cnta = nna[gi];
-    if (lane < cnta) {
-        const int j = nba[(long)gi * NAMAX + lane];
-        sj[w][lane] = j;
-        sv[w][lane] = sepf(X, g, i, j);
+    if (tid < cnta) {
+        const int j = nba[(long)gi * NAMAX + tid];
+        const float4 A = sep4(P, g, i, j);
+        av[tid] = A;
+        ap[tid] = ang_pre(A.w);
+        aw[tid] = welem(ei, elem_of(j));
         const long o = (long)(base + j) * 3;
-        sr[w][lane] = make_float4(res[o] - Rix, res[o + 1] - Riy, res[o + 2] - Riz, 0.f);
+        ar[tid] = make_float4(res[o] - Rix, res[o + 1] - Riy, res[o + 2] - Riz, 0.f);
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __syncthreads();
     float acc[NIN];
 #pragma unroll
     for (int m = 0; m < NIN; ++m) acc[m] = 0.f;
-    for (int s = lane; s < cnt; s += 64) {
+    for (int s = tid; s < cnt; s += AT) {
         const int j = nbr[(long)gi * NMAX + s];
-        const float4 d = sepf(X, g, i, j);
+        const float4 d = sep4(P, g, i, j);
         const long o = (long)(base + j) * 3;
         const float dr = (d.x * (res[o] - Rix) + d.y * (res[o + 1] - Riy) + d.z * (res[o + 2] - Riz)) / d.w;
         const float wf = welem(ei, elem_of(j)) * dr;
@@ -697,32 +786,27 @@ __global__ __launch_bounds__(64 * DESC_WAVES) void k_nn_tan(
 #pragma unroll
         for (int m = 0; m < NR; ++m) acc[m] += wf * dphi(m, d.w, fc, dfc);
     }
-    float T[NANG], Tc[NANG], Ta[NANG], Tb[NANG];
-    const int npair = cnta * (cnta - 1) / 2;
-    for (int p = lane; p < npair; p += 64) {
-        int j, k;
-        tri_decode(p, j, k);
-        const float4 A = sv[w][j], B = sv[w][k], dA4 = sr[w][j], dB4 = sr[w][k];
-        const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
-        const float wt = welem(ei, elem_of(sj[w][j])) * welem(ei, elem_of(sj[w][k]));
-        ang_terms<true>(c, A.w, B.w, wt, T, Tc, Ta, Tb);
-        float3 cA, cB;
-        dcos(A, B, c, cA, cB);
-        const float dc = cA.x * dA4.x + cA.y * dA4.y + cA.z * dA4.z + cB.x * dB4.x + cB.y * dB4.y +
-                         cB.z * dB4.z;
-        const float da = (A.x * dA4.x + A.y * dA4.y + A.z * dA4.z) / A.w;
-        const float db = (B.x * dB4.x + B.y * dB4.y + B.z * dB4.z) / B.w;
+    {
+        float T[NANG], Tc[NANG], Ta[NANG], Tb[NANG];
+        const int npair = cnta * (cnta - 1) / 2;
+        for (int p = tid; p < npair; p += AT) {
+            int j, k;
+            tri_decode(p, j, k);
+            const float4 A = av[j], B = av[k], dA4 = ar[j], dB4 = ar[k];
+            const float c = (A.x * B.x + A.y * B.y + A.z * B.z) / (A.w * B.w);
+            ang_pair<true>(c, A.w, B.w, ap[j], ap[k], aw[j] * aw[k], T, Tc, Ta, Tb);
+            float3 cA, cB;
+            dcos(A, B, c, cA, cB);
+            const float dc = cA.x * dA4.x + cA.y * dA4.y + cA.z * dA4.z + cB.x * dB4.x + cB.y * dB4.y +
+                             cB.z * dB4.z;
+            const float da = (A.x * dA4.x + A.y * dA4.y + A.z * dA4.z) / A.w;
+            const float db = (B.x * dB4.x + B.y * dB4.y + B.z * dB4.z) / B.w;
 #pragma unroll
-        for (int n = 0; n < NANG; ++n) acc[NR + n] += Tc[n] * dc + Ta[n] * da + Tb[n] * db;
+            for (int n = 0; n < NANG; ++n) acc[NR + n] += Tc[n] * dc + Ta[n] * da + Tb[n] * db;
+        }
     }
-#pragma unroll
-    for (int m = 0; m < NIN; ++m) {
-        const float v = wave_sumf(acc[m]);
-        if (lane == m) su[w][m] = v;
-    }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    if (lane < NIN) ut[(long)gi * NIN + lane] = ANGF * su[w][lane] / stdv[ei * NIN + lane];
+    block_sums<NIN>(acc, red, su);
+    if (tid < NIN) ut[(long)gi * NIN + tid] = ANGF * su[tid] / stdv[ei * NIN + tid];
 }
 
 // Sum of per-atom values in a fixed order (one block): out[0] = scale * sum

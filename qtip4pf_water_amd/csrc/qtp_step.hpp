@@ -344,6 +344,124 @@ __global__ __launch_bounds__(64) void k_prop_inv(PropArgs A, const double *__res
     if (va && (at % 3) == 0) disp[((long)b * 3 + d) * NMP + at / 3] += xq - qold;
 }
 
+// The whole propagation of a step for one 64-dof tile when the centroid needs
+// no global rescale (NVE, PILE-L, or the centroid held fixed): kick, forward
+// normal-mode transform, sqrt-Cayley half steps around the Langevin steps of
+// every mode, inverse transform, displacement tracking (update.cpp:281-352,
+// thermostat.cpp:114-129).  The tile's P beads are read from HBM once and
+// staged in LDS ([P][64] positions and kicked momenta, then the modes in the
+// same space); the arithmetic and its order are k_prop_fwd's and
+// k_prop_inv's, so the results are the same bit for bit.
+// grid: ntile (XCD-remapped), 64 * PT_WAVES threads, 2 P 64 doubles of LDS;
+// P <= PT_WAVES * PT_PER_WAVE.
+constexpr int PT_WAVES = 8, PT_PER_WAVE = 8;
+inline size_t prop_tile_lds(int P) { return (size_t)2 * P * 64 * sizeof(double); }
+__global__ __launch_bounds__(64 * PT_WAVES) void k_prop_tile(
+    PropArgs A, const double *__restrict__ Cm, const double4 *__restrict__ mode,
+    const double2 *__restrict__ mscale, double *__restrict__ q, double *__restrict__ p,
+    const double *__restrict__ F, const double *__restrict__ m_atom,
+    const double *__restrict__ sqm_atom, double *__restrict__ part, double *__restrict__ hpart,
+    double *__restrict__ disp, int NMP, const unsigned long long *__restrict__ step_ctr)
+{
+    extern __shared__ double S[];
+    const int P = A.P;
+    double *SQ = S, *SP = S + (long)P * 64;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const long N3 = 3L * A.NAP;
+    const int f = tile * 64 + lane;
+    const int at = f % A.NAP, d = f / A.NAP;
+    const bool va = at < A.n_atoms;
+    double qold[PT_PER_WAVE];
+#pragma unroll
+    for (int j = 0; j < PT_PER_WAVE; ++j) {
+        const int b = w + j * PT_WAVES;
+        if (b < P) {
+            const long idx = b * N3 + f;
+            const double qq = q[idx];
+            qold[j] = qq;
+            SQ[b * 64 + lane] = qq;
+            SP[b * 64 + lane] = p[idx] + A.halfdt * F[idx];   // update.cpp:281-287
+        }
+    }
+    __syncthreads();
+    double rq[PT_PER_WAVE], rp[PT_PER_WAVE];
+#pragma unroll
+    for (int j = 0; j < PT_PER_WAVE; ++j) {
+        const int k = w + j * PT_WAVES;
+        if (k >= P) continue;
+        const double m = va ? m_atom[at] * mscale[k].x : 1.0;
+        const double *cr = Cm + (long)k * P;
+        double uq = 0.0, up = 0.0;
+        for (int b = 0; b < P; ++b) {
+            const double c = cr[b];
+            uq = fma(c, SQ[b * 64 + lane], uq);
+            up = fma(c, SP[b * 64 + lane], up);
+        }
+        const double4 md = mode[k];
+        if (k == 0) {
+            if (A.cmode == 2) up = 0.0;
+            cayley_half(uq, up, m, A.halfdt, md);
+            double h = 0.0;
+            if (A.nvt && A.cthermo == 1 && A.cmode != 2 && va) {
+                h = 0.5 * up * up / m;
+                const double xi = philox_normal(A.seed, *step_ctr, 0u, (uint32_t)(3 * at + d));
+                up = A.c1_0 * up + sqm_atom[at] * A.c2_0 * xi;
+                h -= 0.5 * up * up / m;
+            }
+            h = wave_sum(h);
+            if (lane == 0) {
+                part[tile] = 0.0;
+                part[A.ntile + tile] = 0.0;
+                if (A.nvt) hpart[tile] = h;
+            }
+            cayley_half(uq, up, va ? m_atom[at] : 1.0, A.halfdt, mode[0]);
+        } else {
+            cayley_half(uq, up, m, A.halfdt, md);
+            if (A.nvt) {
+                double h = 0.0;
+                if (va) {
+                    h += 0.5 * up * up / m;
+                    const double xi =
+                        philox_normal(A.seed, *step_ctr, (uint32_t)k, (uint32_t)(3 * at + d));
+                    up = md.z * up + sqm_atom[at] * md.w * xi;
+                    h -= 0.5 * up * up / m;
+                }
+                h = wave_sum(h);
+                if (lane == 0) hpart[(long)k * A.ntile + tile] = h;
+            }
+            cayley_half(uq, up, m, A.halfdt, md);
+        }
+        rq[j] = uq;
+        rp[j] = up;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PT_PER_WAVE; ++j) {
+        const int k = w + j * PT_WAVES;
+        if (k < P) {
+            SQ[k * 64 + lane] = rq[j];
+            SP[k * 64 + lane] = rp[j];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < PT_PER_WAVE; ++j) {
+        const int b = w + j * PT_WAVES;
+        if (b >= P) continue;
+        double xq = Cm[b] * SQ[lane], xp = Cm[b] * SP[lane];
+        for (int k = 1; k < P; ++k) {
+            const double c = Cm[(long)k * P + b];
+            xq = fma(c, SQ[k * 64 + lane], xq);
+            xp = fma(c, SP[k * 64 + lane], xp);
+        }
+        const long idx = b * N3 + f;
+        q[idx] = xq;
+        p[idx] = xp;
+        if (va && (at % 3) == 0) disp[((long)b * 3 + d) * NMP + at / 3] += xq - qold[j];
+    }
+}
+
 // The thermostat's heat for one step: pile_g's return value (KE before -
 // KE after, thermostat.cpp:86-129); heat_acc -= it (update.cpp:325).
 __global__ __launch_bounds__(1024) void k_heat(int ntile, int nh, const double *__restrict__ part,
@@ -455,11 +573,14 @@ __global__ __launch_bounds__(256) void k_pile_b(int P, int n_atoms, int NAP, int
 // Verlet lists (neighbour.cpp:12-62) and the rebuild test (update.cpp:356-385)
 // ---------------------------------------------------------------------------
 
-// Per bead: two largest O displacement norms; flag = d1 + d2 > r_skin - r_cut.
+// Per bead: two largest O displacement norms; flag = d1 + d2 > r_skin - r_cut,
+// or forced (force_all; or every > 0 and the step about to complete, step_ctr
+// + 1, a multiple of every).
 // grid: P, 256 threads.
 __global__ __launch_bounds__(256) void k_check(int n_mols, int NMP, const double *__restrict__ disp,
-                                               double skin, int force_all, int *__restrict__ flag,
-                                               double *__restrict__ nreb)
+                                               double skin, int force_all, int every,
+                                               const unsigned long long *__restrict__ step_ctr,
+                                               int *__restrict__ flag, double *__restrict__ nreb)
 {
     __shared__ double s1[256], s2[256];
     const int b = blockIdx.x;
@@ -497,7 +618,9 @@ __global__ __launch_bounds__(256) void k_check(int n_mols, int NMP, const double
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const int f = force_all || (sqrt(s1[0]) + sqrt(s2[0]) > skin);
+        int f = force_all || (sqrt(s1[0]) + sqrt(s2[0]) > skin);
+        if (every > 0 && step_ctr && (step_ctr[0] + 1ull) % (unsigned long long)every == 0ull)
+            f = 1;
         flag[b] = f;
         if (f && nreb) nreb[b] += 1.0;
     }
@@ -670,6 +793,8 @@ struct HeatArgs {
 __global__ __launch_bounds__(1024) void k_nlist_bead(int n_mols, int NAP, int NMP,
                                                      const double *__restrict__ q, Box box,
                                                      double r_skin_sq, double skin, int force_all,
+                                                     int every,
+                                                     const unsigned long long *__restrict__ step_ctr,
                                                      int list_size, long fcap,
                                                      double *__restrict__ disp,
                                                      int *__restrict__ flag,
@@ -722,7 +847,9 @@ __global__ __launch_bounds__(1024) void k_nlist_bead(int n_mols, int NAP, int NM
     if (tid == 0) {
         double t1 = s1[0], t2 = s2[0];
         for (int ww = 1; ww < 16; ++ww) top2_merge(t1, t2, s1[ww], s2[ww]);
-        const int f = force_all || (sqrt(t1) + sqrt(t2) > skin);
+        int f = force_all || (sqrt(t1) + sqrt(t2) > skin);
+        if (every > 0 && step_ctr && (step_ctr[0] + 1ull) % (unsigned long long)every == 0ull)
+            f = 1;
         s_flag = f;
         flag[b] = f;
         if (f && nreb) nreb[b] += 1.0;
@@ -858,8 +985,8 @@ __global__ __launch_bounds__(256) void k_vv_a(int n_atoms, int NAP, int NMP, dou
 }
 
 // F = f_phys of the single evaluation, p += dt/2 F, energy out.
-// extra_e (may be null): per-atom energies of a further term added to V
-// (the learned correction's), summed in a fixed order.
+// fx, ex (may be null): forces (planar) and n_ex energy partials of a
+// further term added to V (the learned correction's), summed in a fixed order.
 __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfdt,
                                               const double *__restrict__ fe,
                                               const double *__restrict__ epart, int ep_stride,
@@ -867,20 +994,21 @@ __global__ __launch_bounds__(256) void k_vv_b(int n_atoms, int NAP, double halfd
                                               double *__restrict__ F, double *__restrict__ p,
                                               double *__restrict__ out,
                                               unsigned long long *__restrict__ step_ctr,
-                                              const float *__restrict__ extra_e)
+                                              const double *__restrict__ fx,
+                                              const float *__restrict__ ex, int n_ex)
 {
     const int f = blockIdx.x * 256 + threadIdx.x;
     if (f < 3 * NAP) {
-        const double v = fe[f];
+        const double v = fx ? fe[f] + fx[f] : fe[f];
         F[f] = v;
         const int at = f % NAP;
         if (at < n_atoms) p[f] += halfdt * v;
     }
     if (blockIdx.x == 0 && threadIdx.x < 64) {
         double v = eval_energy(epart, 0, ep_stride, ep_n, eself);
-        if (extra_e) {
+        if (ex) {
             double x = 0.0;
-            for (int a = threadIdx.x; a < n_atoms; a += 64) x += (double)extra_e[a];
+            for (int a = threadIdx.x; a < n_ex; a += 64) x += (double)ex[a];
             v += wave_sum(x);
         }
         if (threadIdx.x == 0) {

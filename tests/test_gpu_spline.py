@@ -10,6 +10,8 @@ import pytest
 import spline_tables
 from qtip4pf_water_amd import config as cfg
 
+from energy_tol import trajectory_bound
+
 pytestmark = pytest.mark.gpu
 
 
@@ -133,15 +135,15 @@ def test_update_trajectory_class_wspline_steps(oracle_mod, tabs):
     o.get_forces(q0, F, point, lst, 0)
     ref = [q0.copy(), p0.copy(), F.copy(), point.copy(), lst.copy(), disp.copy()]
     gpu = [a.copy() for a in ref]
-    self_scale = o.ewald()["prefactor"] * 0.5 * float(np.sum(o.charge ** 2))
-    with _water(Water, 216, 1, box, tabs) as w:
+    with _water(Water, 216, 1, box, tabs) as w, oracle_mod.exact_energy():
         for step in range(6):
             er, _ = o.update_class(*ref)
             eg = w.update_trajectory_class_wspline(gpu[0], gpu[1], o.mass, gpu[2], o.charge,
                                                    gpu[3], gpu[4], gpu[5])
             assert np.max(np.abs(gpu[0] - ref[0])) <= 1e-12 * np.max(np.abs(ref[0])), step
             assert rel_atom_err(gpu[2], ref[2]) < 1e-9, step
-            assert abs(eg - er) <= 1e-12 * self_scale, step
+            tol = trajectory_bound(er, ref[2], gpu[0] - ref[0])
+            assert abs(eg - er) <= tol, (step, eg - er, tol)
 
 
 def test_resident_run_with_splines_matches_drop_in(oracle_mod, tabs):

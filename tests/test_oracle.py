@@ -308,6 +308,52 @@ def test_sc_force_is_gradient_of_sc_potential(oracle_mod):
         assert fd == pytest.approx(Fsc[k], rel=1e-5, abs=1e-9)
 
 
+def test_sc_odd_bead_displacement_scale(oracle_mod):
+    """The odd-bead Hessian term uses the displacement dq = eps_sc delta f / m
+    with delta = 1 / sqrt(sum_i (f_i/m_i)^2 / (n_atoms n_beads))
+    (forces.cpp:746-760) and the prefactor 1 / (9 w_P^2 eps_sc delta).
+    Built here from the oracle's single-bead forces at positions displaced
+    by that dq, the odd-bead SC forces must be the oracle's; a normalisation
+    by n_dof, by n_atoms alone or without the square root changes dq and
+    the finite difference by far more than the 1e-12 allowed."""
+    q0, box = small_box(2, 6.3, 11)
+    P = 4
+    eps = 1e-5
+    o = oracle_mod.Oracle(8, P, box, epsilonsc=eps)
+    rng = np.random.default_rng(3)
+    q = cfg.replicate_beads(q0, P) + rng.normal(0, 0.02, P * o.n_dof)
+    point, lst, _ = o.all_lists(q)
+    Fsc, fph = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+    o.force_eval_sc(q, Fsc, fph, point, lst)
+    m = np.tile(o.mass.reshape(-1)[: o.n_dof], 1)
+    wsq = o.sys.omega_psq
+    for b in (1, 3):
+        sl = slice(b * o.n_dof, (b + 1) * o.n_dof)
+        f = fph[sl]
+        delta = 1.0 / np.sqrt(np.sum((f / m) ** 2) / (o.n_atoms * P))
+        scale = eps * delta
+        dq = scale * f / m
+        fp, fm = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+        qp, qm = q.copy(), q.copy()
+        qp[sl] += dq
+        qm[sl] -= dq
+        o.get_forces(qp, fp, point, lst, b)
+        o.get_forces(qm, fm, point, lst, b)
+        expect = 4.0 * f / 3.0 - (fp[sl] - fm[sl]) / (9.0 * wsq * scale)
+        assert np.max(np.abs(Fsc[sl] - expect)) <= 1e-12 * np.max(np.abs(expect))
+        # the plausible slips move the result far outside that
+        for wrong in (o.n_dof * P, o.n_atoms):
+            d2 = 1.0 / np.sqrt(np.sum((f / m) ** 2) / wrong)
+            qp2, qm2 = q.copy(), q.copy()
+            qp2[sl] += eps * d2 * f / m
+            qm2[sl] -= eps * d2 * f / m
+            fp2, fm2 = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+            o.get_forces(qp2, fp2, point, lst, b)
+            o.get_forces(qm2, fm2, point, lst, b)
+            e2 = 4.0 * f / 3.0 - (fp2[sl] - fm2[sl]) / (9.0 * wsq * eps * d2)
+            assert np.max(np.abs(Fsc[sl] - e2)) > 1e-10 * np.max(np.abs(expect))
+
+
 # ------------------------------------------------------- reference build
 needs_ref = pytest.mark.skipif(not __import__("oracle").ref_available(),
                                reason="oracle/_ref not built (reference tree absent)")
