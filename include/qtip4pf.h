@@ -319,7 +319,8 @@ int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *t
  * 16-wave k-space force blocks (1), 8-wave (0) or by block count (-1),
  * structure-factor blocks per 64-site chunk (1-4; 0 by launch size),
  * propagation as one LDS-staged kernel per step where the thermostat allows
- * it (-1) or always as two (0; the results are the same bit for bit)}.
+ * it: for systems of 256 or more 64-dof tiles (-1), always (1), never (0;
+ * the results are the same bit for bit)}.
  * Every configuration gives the same forces up to summation order. */
 int qtp_get_tuning(const qtp_ctx *ctx, int cfg[6]);
 /* Sets cfg (QTP_ERR_ARG for values the context cannot run); with a state set,

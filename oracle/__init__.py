@@ -124,17 +124,24 @@ def lib():
     return _lib
 
 
+_exact = [0]
+
+
 class exact_energy:
     """Context manager: inside it every oracle bead energy is summed with an
-    80-bit accumulator (oq_set_exact_energy), the reference's FP64 running
-    sum outside.  Forces are unchanged."""
+    80-bit accumulator (oq_set_exact_energy) instead of the reference's FP64
+    running sum; the previous setting comes back on exit.  Forces are
+    unchanged."""
 
     def __enter__(self):
+        self._prev = _exact[0]
+        _exact[0] = 1
         lib().oq_set_exact_energy(1)
         return self
 
     def __exit__(self, *exc):
-        lib().oq_set_exact_energy(0)
+        _exact[0] = self._prev
+        lib().oq_set_exact_energy(self._prev)
         return False
 
 

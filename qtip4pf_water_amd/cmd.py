@@ -20,6 +20,7 @@ Pipeline (everything on the GPU through the library):
 
     python -m qtip4pf_water_amd.cmd build --out DIR [--frames N] ...
     python -m qtip4pf_water_amd.cmd validate --model FILE --out DIR ...
+    python -m qtip4pf_water_amd.cmd steps --kind surrogate --steps 200
 """
 from __future__ import annotations
 
@@ -265,9 +266,46 @@ def cmd_run(kind, n_mols=512, T_K=298.0, model=None, ps=10.0, equil_ps=2.0, samp
                 status=st["status"])
 
 
+def steps(kind, n_steps=200, n_mols=512, T_K=298.0, model=None, P=32, tau_fs=100.0, seed=21):
+    """n_steps resident NVT steps of one CMD kind (no g(r) sampling), for
+    kernel traces: steps/s."""
+    from .api import Surrogate, Water
+    dflt = {"surrogate": 0.5, "classical": 0.5, "pacmd": 0.1, "pimd": 0.25}
+    beads = 1 if kind in ("surrogate", "classical") else P
+    q, p, box = start_state(n_mols, beads, T_K, seed)
+    w = Water(n_mols, beads, box, T_K=T_K, dtfs=dflt[kind])
+    g = None
+    if kind == "surrogate":
+        g = Surrogate(n_mols, box, max_frames=1)
+        g.set_model(model["theta"], model["mean"], model["std"], model["e_scale"])
+        w.set_nn(g)
+    if kind == "pacmd":
+        w.set_pimd_mode("pacmd")
+    w.set_thermostat("pile_l", tau_fs)
+    w.set_seed(seed)
+    w.set_state(q, p)
+    w.run(20, True)
+    w.sync()
+    t0 = time.perf_counter()
+    w.run(n_steps, True)
+    w.sync()
+    el = time.perf_counter() - t0
+    st = w.scalars()["status"]
+    w.close()
+    if g is not None:
+        g.close()
+    return n_steps / el, st
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     sub = ap.add_subparsers(dest="cmd", required=True)
+    r = sub.add_parser("steps", help="time resident steps of one kind (for kernel traces)")
+    r.add_argument("--kind", default="surrogate",
+                   choices=["surrogate", "classical", "pacmd", "pimd"])
+    r.add_argument("--steps", type=int, default=200)
+    r.add_argument("--n-mols", type=int, default=512)
+    r.add_argument("--model", default=MODEL_FILE)
     b = sub.add_parser("build", help="generate mean forces, train, save the model")
     b.add_argument("--out", required=True)
     b.add_argument("--frames", type=int, default=1500)
@@ -289,6 +327,11 @@ def main(argv=None):
     v.add_argument("--ref", default=None,
                    help="g(r) file of an earlier plain-CMD run (rdf_pacmd.npz) instead of a new one")
     a = ap.parse_args(argv)
+    if a.cmd == "steps":
+        model = load_model(a.model) if a.kind == "surrogate" else None
+        sps, st = steps(a.kind, a.steps, a.n_mols, model=model)
+        say(f"{a.kind}: {sps:.1f} steps/s, status {st}")
+        return 0 if st == 0 else 1
     os.makedirs(a.out, exist_ok=True)
     if a.cmd == "build":
         t0 = time.perf_counter()

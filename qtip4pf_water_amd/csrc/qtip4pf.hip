@@ -1614,7 +1614,7 @@ int qtp_set_tuning(qtp_ctx *ctx, const int cfg[6])
     if (!tuning_valid(ctx, c)) {
         ctx->err = "tuning: lanes 16 or 64, matrix k-space 0/1 (1 only for k_max <= 15), "
                    "registers and 16-wave blocks -1/0/1, blocks per chunk 0-4, fused "
-                   "propagation -1/0";
+                   "propagation -1/0/1";
         return QTP_ERR_ARG;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1657,7 +1657,9 @@ int qtp_autotune(qtp_ctx *ctx, int flags, double report[8])
             float ms = 0.f;
             st = tune_time(ctx, 5, &ms);
             ++n_timed;
-            if (st == QTP_OK && ms < best_ms) {
+            // a candidate has to beat the configuration in hand by TUNE_MARGIN,
+            // so that timing noise between near-equal ones does not decide
+            if (st == QTP_OK && ms < best_ms * (1.0f - TUNE_MARGIN)) {
                 best_ms = ms;
                 best = c;
             }

@@ -64,8 +64,9 @@ struct qtp_ctx {
     // k_kforce_mf 16-wave blocks (-1: below MF_SMALL_BLOCKS blocks), k_ksum_mf
     // blocks per chunk (0: several when a launch has under 128 chunks)
     int rs_carg = -1, mf_small = -1, ks_usplit = 0;
-    // -1: one LDS-staged propagation kernel per step where the thermostat
-    // allows it (k_prop_tile); 0: always k_prop_fwd + k_prop_inv
+    // propagation as one LDS-staged kernel per step (k_prop_tile) where the
+    // thermostat allows it: -1 for systems of PT_MIN_TILES tiles or more,
+    // 1 always, 0 never (k_prop_fwd + k_prop_inv)
     int prop_fused = -1;
     int ep_cap = 0;   // allocated stride of the energy partials
     int mf_Mp = 0, mf_nks = 0, mf_nrt = 0, mf_nrt_small = 0, ks_nunit = 0, ks_ncol4 = 0;

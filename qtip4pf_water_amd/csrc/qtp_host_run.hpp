@@ -329,8 +329,9 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
     // one kernel per tile, the beads staged in LDS, when the centroid needs
     // no rescale over the whole system between the two transforms
     const bool bussi = nvt && A.cthermo == 0 && A.cmode != 2;
-    if (ctx->prop_fused != 0 && !bussi && !(nvt && host_dev) &&
-        ctx->P <= PT_WAVES * PT_PER_WAVE) {
+    const bool big = ctx->ptile >= PT_MIN_TILES;
+    if ((ctx->prop_fused == 1 || (ctx->prop_fused == -1 && big)) && !bussi &&
+        !(nvt && host_dev) && ctx->P <= PT_WAVES * PT_PER_WAVE) {
         TimeScope ts(ctx, "prop");
         k_prop_tile<<<ctx->ptile, 64 * PT_WAVES, prop_tile_lds(ctx->P), s>>>(
             A, ctx->Cm, ctx->mode, ctx->ms, ctx->q, ctx->p, ctx->F, ctx->m_atom, ctx->sqm_atom,

@@ -16,6 +16,8 @@
 
 namespace {
 
+constexpr float TUNE_MARGIN = 0.02f;
+
 struct TuneCfg {
     int lpm = 16, kmf = 0, carg = -1, small = -1, usplit = 0, fused = -1;
 };
@@ -39,7 +41,7 @@ bool tuning_valid(const qtp_ctx *ctx, const TuneCfg &c)
     if (c.kmf && !ctx->kf_mf_ok) return false;
     if (c.carg < -1 || c.carg > 1 || c.small < -1 || c.small > 1) return false;
     if (c.small == 1 && ctx->kforce_mf_lds_small > 64 * 1024) return false;
-    if (c.fused != -1 && c.fused != 0) return false;
+    if (c.fused < -1 || c.fused > 1) return false;
     return c.usplit >= 0 && c.usplit <= 4;
 }
 

@@ -161,6 +161,7 @@ __device__ __forceinline__ void tri_decode(int p, int &j, int &k)
 // ---------------------------------------------------------------------------
 constexpr int AT = 256;
 constexpr int RGRP = AT / NR;   // neighbour groups of the radial sums (10 x 24 threads)
+constexpr int SEARCH_BATCH = 8; // candidate loads in flight per lane in k_nn_desc
 
 // P4[f * n_atoms + a] = (x, y, z) of atom a of frame f in Angstrom, wrapped.
 // grid: ceil(n_frames * n_atoms / 256), 256 threads.
@@ -255,24 +256,35 @@ __global__ __launch_bounds__(AT) void k_nn_desc(
         const int q = ((g.n_atoms + 255) / 256) * 64;
         const int lo = w * q, hi = min(g.n_atoms, lo + q);
         int c = 0;
-        for (int j0 = lo; j0 < hi; j0 += 64) {
-            const int j = j0 + lane;
-            float dx = 0.f, dy = 0.f, dz = 0.f, r2 = 1e30f;
-            if (j < hi && j != i) {
-                const float4 pj = P[j];
-                dx = mimf(pj.x - pi.x, g.Lf[0], g.iLf[0]);
-                dy = mimf(pj.y - pi.y, g.Lf[1], g.iLf[1]);
-                dz = mimf(pj.z - pi.z, g.Lf[2], g.iLf[2]);
-                r2 = dx * dx + dy * dy + dz * dz;
+        // SEARCH_BATCH candidates per lane loaded before any is tested, so
+        // their latencies overlap
+        for (int jb = lo; jb < hi; jb += 64 * SEARCH_BATCH) {
+            float4 pb[SEARCH_BATCH];
+#pragma unroll
+            for (int u = 0; u < SEARCH_BATCH; ++u) {
+                const int j = jb + 64 * u + lane;
+                pb[u] = j < hi ? P[j] : pi;
             }
-            const bool in = r2 < rr2;
-            const unsigned long long b = __ballot(in);
-            const int s = c + __popcll(b & ((1ull << lane) - 1ull));
-            if (in && s < NMAX) {
-                tv[w][s] = make_float4(dx, dy, dz, sqrtf(r2));
-                tj[w][s] = j;
+#pragma unroll
+            for (int u = 0; u < SEARCH_BATCH; ++u) {
+                if (jb + 64 * u >= hi) break;   // wave-uniform
+                const int j = jb + 64 * u + lane;
+                float dx = 0.f, dy = 0.f, dz = 0.f, r2 = 1e30f;
+                if (j < hi && j != i) {
+                    dx = mimf(pb[u].x - pi.x, g.Lf[0], g.iLf[0]);
+                    dy = mimf(pb[u].y - pi.y, g.Lf[1], g.iLf[1]);
+                    dz = mimf(pb[u].z - pi.z, g.Lf[2], g.iLf[2]);
+                    r2 = dx * dx + dy * dy + dz * dz;
+                }
+                const bool in = r2 < rr2;
+                const unsigned long long b = __ballot(in);
+                const int s = c + __popcll(b & ((1ull << lane) - 1ull));
+                if (in && s < NMAX) {
+                    tv[w][s] = make_float4(dx, dy, dz, sqrtf(r2));
+                    tj[w][s] = j;
+                }
+                c += __popcll(b);
             }
-            c += __popcll(b);
         }
         if (lane == 0) wc[w] = c;
     }
@@ -574,12 +586,17 @@ __device__ __forceinline__ float dphi(int m, float r, float fc, float dfc)
 // Angular force pieces per centre i: for every angular neighbour j (slot of
 // nba), D_i[j] = sum_{k != j} dE_i/dx_j of the pair terms T(i; j, k), and
 // Dself_i = dE_i/dx_i|angular = -sum_j D_i[j].  Each unordered pair (j, k)
-// is differentiated once, by one thread, for both members (its two pieces
-// kept in LDS); then D_i[j] sums its pieces over k ascending (fixed order,
-// no atomics).
+// is differentiated once, by one thread, for both members; the pieces are
+// summed in LDS as 64-bit fixed-point integers (resolution 2^-40, range
+// +-2^23), whose additions are exact, so the sums do not depend on the order
+// in which the threads add them.
 // grid: n_frames * n_atoms blocks of AT threads.
 // ---------------------------------------------------------------------------
-constexpr int NAPAIR = NAMAX * (NAMAX - 1) / 2;
+constexpr double DANG_FIX = 1099511627776.0;   // 2^40
+__device__ __forceinline__ void fix_add(unsigned long long *a, float v)
+{
+    atomicAdd(a, (unsigned long long)(long long)llrint((double)v * DANG_FIX));
+}
 __global__ __launch_bounds__(AT) void k_nn_dang(
     NNGeom g, const float4 *__restrict__ P4, const int *__restrict__ nba,
     const int *__restrict__ nna, const float *__restrict__ dEdG, float4 *__restrict__ Dang,
@@ -587,7 +604,7 @@ __global__ __launch_bounds__(AT) void k_nn_dang(
 {
     __shared__ float4 av[NAMAX], ap[NAMAX];
     __shared__ float aw[NAMAX], se[NANG];
-    __shared__ float cd[2][3][NAPAIR];   // pieces of pair p for its lower (0) and upper (1) member
+    __shared__ unsigned long long acc[3][NAMAX];
     const int tid = threadIdx.x;
     const int gi = blockIdx.x, f = gi / g.n_atoms, i = gi - f * g.n_atoms;
     const float4 *P = P4 + (long)f * g.n_atoms;
@@ -600,6 +617,7 @@ __global__ __launch_bounds__(AT) void k_nn_dang(
         aw[tid] = welem(ei, elem_of(j));
     }
     if (tid < NANG) se[tid] = dEdG[(long)gi * NIN + NR + tid];
+    if (tid < 3 * NAMAX) (&acc[0][0])[tid] = 0ull;
     __syncthreads();
     const int npair = cnta * (cnta - 1) / 2;
     for (int p = tid; p < npair; p += AT) {
@@ -621,26 +639,20 @@ __global__ __launch_bounds__(AT) void k_nn_dang(
         const float ca = Sc * iab;
         const float cj = Sa / A.w - Sc * c / (A.w * A.w);
         const float ck = Sb / B.w - Sc * c / (B.w * B.w);
-        cd[0][0][p] = ca * B.x + cj * A.x;
-        cd[0][1][p] = ca * B.y + cj * A.y;
-        cd[0][2][p] = ca * B.z + cj * A.z;
-        cd[1][0][p] = ca * A.x + ck * B.x;
-        cd[1][1][p] = ca * A.y + ck * B.y;
-        cd[1][2][p] = ca * A.z + ck * B.z;
+        fix_add(&acc[0][j], ca * B.x + cj * A.x);
+        fix_add(&acc[1][j], ca * B.y + cj * A.y);
+        fix_add(&acc[2][j], ca * B.z + cj * A.z);
+        fix_add(&acc[0][k], ca * A.x + ck * B.x);
+        fix_add(&acc[1][k], ca * A.y + ck * B.y);
+        fix_add(&acc[2][k], ca * A.z + ck * B.z);
     }
     __syncthreads();
     if (tid < 64) {
         float4 D = make_float4(0.f, 0.f, 0.f, 0.f);
         if (tid < cnta) {
-            const int j = tid;
-            for (int k = 0; k < cnta; ++k) {
-                if (k == j) continue;
-                const int side = k < j ? 1 : 0;
-                const int p = k < j ? j * (j - 1) / 2 + k : k * (k - 1) / 2 + j;
-                D.x += cd[side][0][p];
-                D.y += cd[side][1][p];
-                D.z += cd[side][2][p];
-            }
+            D.x = (float)((double)(long long)acc[0][tid] / DANG_FIX);
+            D.y = (float)((double)(long long)acc[1][tid] / DANG_FIX);
+            D.z = (float)((double)(long long)acc[2][tid] / DANG_FIX);
             Dang[(long)gi * NAMAX + tid] = D;
         }
         const float sx = wave_sumf(D.x), sy = wave_sumf(D.y), sz = wave_sumf(D.z);

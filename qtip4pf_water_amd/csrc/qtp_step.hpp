@@ -355,6 +355,11 @@ __global__ __launch_bounds__(64) void k_prop_inv(PropArgs A, const double *__res
 // grid: ntile (XCD-remapped), 64 * PT_WAVES threads, 2 P 64 doubles of LDS;
 // P <= PT_WAVES * PT_PER_WAVE.
 constexpr int PT_WAVES = 8, PT_PER_WAVE = 8;
+// Below this many tiles (one block per CU) the two-kernel form, with a wave
+// per (tile, mode), has more parallelism and is faster (216 x 32: 33 tiles,
+// 25.6 us fused against 16.4 us for the pair; 4096 x 32: 576 tiles, 69
+// against 124 us).
+constexpr int PT_MIN_TILES = 256;
 inline size_t prop_tile_lds(int P) { return (size_t)2 * P * 64 * sizeof(double); }
 __global__ __launch_bounds__(64 * PT_WAVES) void k_prop_tile(
     PropArgs A, const double *__restrict__ Cm, const double4 *__restrict__ mode,

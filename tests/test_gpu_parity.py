@@ -16,7 +16,7 @@ import pytest
 
 from qtip4pf_water_amd import config as cfg
 
-from energy_tol import single, trajectory_bound
+from energy_tol import EPS, single, term_scale, trajectory_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -199,6 +199,7 @@ def test_update_trajectory_class_nospline_steps(W, oracle_mod):
     o.get_forces(q0, F, point, lst, 0)
     ref = [q0.copy(), p0.copy(), F.copy(), point.copy(), lst.copy(), disp.copy()]
     gpu = [a.copy() for a in ref]
+    S = term_scale(o, q0, point, lst)
     with W(216, 1, box) as w, oracle_mod.exact_energy():
         for step in range(10):
             er, _ = o.update_class(*ref)
@@ -208,7 +209,7 @@ def test_update_trajectory_class_nospline_steps(W, oracle_mod):
             assert rel_atom_err(gpu[2], ref[2]) < 1e-9, step
             # 80-bit oracle energy; 1e-12 |V| plus the first-order effect of
             # the measured position difference of the two trajectories
-            tol = trajectory_bound(er, ref[2], gpu[0] - ref[0])
+            tol = trajectory_bound(er, ref[2], gpu[0] - ref[0], scale=S)
             assert abs(eg - er) <= tol, (step, eg - er, tol)
 
 
@@ -324,12 +325,12 @@ def test_reference_adapter_driver(oracle_mod, tmp_path, P, nvt):
     qg, pg, Fg, eg = out[:N], out[N:2 * N], out[2 * N:3 * N], out[3 * N:3 * N + steps]
     # oracle run with the same stream
     point, lst, disp = o.all_lists(q)
+    S = sum(term_scale(o, q, point, lst, b) for b in range(P))
     qr, pr = q.copy(), p.copy()
     F, fph = np.zeros(N), np.zeros(N)
     rng = HostRng(seed)
     heat = 0.0
-    oracle_mod.lib().oq_set_exact_energy(1)
-    try:
+    with oracle_mod.exact_energy():
         if P > 1:
             o.force_eval_sc(qr, F, fph, point, lst)
         else:
@@ -343,10 +344,8 @@ def test_reference_adapter_driver(oracle_mod, tmp_path, P, nvt):
             # the driver returns its positions after the last step only; the
             # trajectories drift apart monotonically, so the final position
             # difference bounds every step's (80-bit oracle energies)
-            tol = P * trajectory_bound(er, F, qg - qr)
+            tol = P * trajectory_bound(er, F, qg - qr) + 16.0 * EPS * S
             assert abs(eg[s] - er) <= tol, (s, eg[s] - er, tol)
-    finally:
-        oracle_mod.lib().oq_set_exact_energy(0)
     assert np.max(np.abs(qg - qr)) <= 1e-12 * np.max(np.abs(qr))
     assert np.max(np.abs(pg - pr)) <= 1e-10 * np.max(np.abs(pr))
 

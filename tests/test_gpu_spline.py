@@ -10,7 +10,7 @@ import pytest
 import spline_tables
 from qtip4pf_water_amd import config as cfg
 
-from energy_tol import trajectory_bound
+from energy_tol import term_scale, trajectory_bound
 
 pytestmark = pytest.mark.gpu
 
@@ -135,6 +135,7 @@ def test_update_trajectory_class_wspline_steps(oracle_mod, tabs):
     o.get_forces(q0, F, point, lst, 0)
     ref = [q0.copy(), p0.copy(), F.copy(), point.copy(), lst.copy(), disp.copy()]
     gpu = [a.copy() for a in ref]
+    S = term_scale(o, q0, point, lst)
     with _water(Water, 216, 1, box, tabs) as w, oracle_mod.exact_energy():
         for step in range(6):
             er, _ = o.update_class(*ref)
@@ -142,7 +143,7 @@ def test_update_trajectory_class_wspline_steps(oracle_mod, tabs):
                                                    gpu[3], gpu[4], gpu[5])
             assert np.max(np.abs(gpu[0] - ref[0])) <= 1e-12 * np.max(np.abs(ref[0])), step
             assert rel_atom_err(gpu[2], ref[2]) < 1e-9, step
-            tol = trajectory_bound(er, ref[2], gpu[0] - ref[0])
+            tol = trajectory_bound(er, ref[2], gpu[0] - ref[0], scale=S)
             assert abs(eg - er) <= tol, (step, eg - er, tol)
 
 
