@@ -314,7 +314,7 @@ int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *t
 /* ---- launch configuration and autotuner ------------------------------------
  * No counterpart in the reference (one CPU code path).  cfg[6] = {lanes per
  * molecule of the LJ/intramolecular kernel (16 or 64), k-space on the matrix
- * cores (1; only for k_max <= 15) or the vector kernels (0), real-space
+ * cores (1; only for k_max <= 23) or the vector kernels (0), real-space
  * coefficients in scalar registers (1), as literals (0) or by grid size (-1),
  * 16-wave k-space force blocks (1), 8-wave (0) or by block count (-1),
  * structure-factor blocks per 64-site chunk (1-4; 0 by launch size),

@@ -119,7 +119,9 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
     ctx->ngc = (ctx->G + 255) / 256;
     ctx->ngr = (ctx->G + SR_GPB - 1) / SR_GPB;
     const int K = ctx->ew.K;
-    ctx->AC = std::max(1, std::min(64, (int)(48 * 1024 / (3 * K * 16))));
+    // sites per structure-factor chunk: 48 KiB of factor tables, a multiple
+    // of 4 (the matrix-core k-steps) below 64
+    ctx->AC = std::max(4, std::min(64, (int)(48 * 1024 / (3 * K * 16))) / 4 * 4);
     ctx->nac = (ctx->n_atoms + ctx->AC - 1) / ctx->AC;
     ctx->ksum_lds = (size_t)ctx->AC * 3 * K * sizeof(double2);
     // two 12-wave blocks or three 6-wave (split) blocks per CU
@@ -177,6 +179,26 @@ int qtp_create(const qtp_params *prm, qtp_ctx **out)
         hipEventCreateWithFlags(&ctx->ev_nn_join, hipEventDisableTiming) != hipSuccess) {
         ctx->err = "hipStreamCreate failed";
         return QTP_ERR_HIP;
+    }
+    // matrix-core k-space kernels with more than the default 64 KiB of LDS
+    // (large k-sets); without the attribute they are not used
+    if (ctx->kf_mf_ok) {
+        const bool hi = ctx->mf_nks > MF_KS_LO;
+        bool ok = ctx->ksum_mf_lds <= 160 * 1024 && ctx->kforce_mf_lds <= 160 * 1024;
+        if (ok && ctx->ksum_mf_lds > 64 * 1024)
+            ok = hipFuncSetAttribute((const void *)k_ksum_mf,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)ctx->ksum_mf_lds) == hipSuccess;
+        if (ok && ctx->kforce_mf_lds > 64 * 1024)
+            ok = hipFuncSetAttribute(hi ? (const void *)k_kforce_mf<MF_WAVES, MF_MAXKS>
+                                        : (const void *)k_kforce_mf<MF_WAVES, MF_KS_LO>,
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     (int)ctx->kforce_mf_lds) == hipSuccess;
+        if (!ok) {
+            ctx->kf_mf_ok = false;
+            ctx->kf_mf = false;
+            ctx->kf_split = ctx->kf_split_ok;
+        }
     }
     if (ctx->kforce_lds > 64 * 1024 || ctx->ksum_lds > 64 * 1024) {
         if (ctx->kforce_lds > 160 * 1024 ||
@@ -1612,7 +1634,7 @@ int qtp_set_tuning(qtp_ctx *ctx, const int cfg[6])
     c.usplit = cfg[4];
     c.fused = cfg[5];
     if (!tuning_valid(ctx, c)) {
-        ctx->err = "tuning: lanes 16 or 64, matrix k-space 0/1 (1 only for k_max <= 15), "
+        ctx->err = "tuning: lanes 16 or 64, matrix k-space 0/1 (1 only for k_max <= 23), "
                    "registers and 16-wave blocks -1/0/1, blocks per chunk 0-4, fused "
                    "propagation -1/0/1";
         return QTP_ERR_ARG;

@@ -135,26 +135,33 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         const int nbm = E * ((ctx->n_atoms + MF_SITES - 1) / MF_SITES);
         // few blocks: twice the waves per block (when its LDS fits the default 64 KB)
         const bool small = ctx->mf_small >= 0 ? ctx->mf_small == 1 : nbm < MF_SMALL_BLOCKS;
-        if (ctx->kf_mf && small && ctx->kforce_mf_lds_small <= 64 * 1024)
-            k_kforce_mf<MF_WAVES_SMALL><<<nbm, 64 * MF_WAVES_SMALL, ctx->kforce_mf_lds_small, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
-                ctx->mf_nks, ctx->mf_rt_small, ctx->mf_nrt_small, ctx->rpart, ctx->nblk, ctx->flj,
-                ctx->spl_on ? ctx->fspl : nullptr, fout, rfar, ctx->ntile);
-        else if (ctx->kf_mf)
-            k_kforce_mf<MF_WAVES><<<nbm, 64 * MF_WAVES, ctx->kforce_mf_lds, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->wm, ctx->mf_Mp,
-                ctx->mf_nks, ctx->mf_rt, ctx->mf_nrt, ctx->rpart, ctx->nblk, ctx->flj,
-                ctx->spl_on ? ctx->fspl : nullptr, fout, rfar, ctx->ntile);
-        else if (fout2)
-            k_kforce<KF_SPLIT_WAVES, true><<<2 * E * nab, 64 * KF_SPLIT_WAVES, ctx->kforce_lds_split, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
-                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, fout2,
-                ctx->kf_chunk_split, rfar, ctx->ntile);
-        else
-            k_kforce<KF_WAVES, false><<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
-                pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
-                ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, nullptr,
-                ctx->kf_chunk, rfar, ctx->ntile);
+        const bool hi = ctx->mf_nks > MF_KS_LO;
+#define QTP_KF_MF(W, KS, LDS, RT, NRT)                                                         \
+    k_kforce_mf<W, KS><<<nbm, 64 * W, LDS, s>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, \
+                                                ctx->ew, ctx->wm, ctx->mf_Mp, ctx->mf_nks, RT, NRT, \
+                                                ctx->rpart, ctx->nblk, ctx->flj,                  \
+                                                ctx->spl_on ? ctx->fspl : nullptr, fout, rfar,    \
+                                                ctx->ntile)
+        if (ctx->kf_mf && small && ctx->kforce_mf_lds_small <= 64 * 1024) {
+            if (hi) QTP_KF_MF(MF_WAVES_SMALL, MF_MAXKS, ctx->kforce_mf_lds_small, ctx->mf_rt_small, ctx->mf_nrt_small);
+            else QTP_KF_MF(MF_WAVES_SMALL, MF_KS_LO, ctx->kforce_mf_lds_small, ctx->mf_rt_small, ctx->mf_nrt_small);
+        } else if (ctx->kf_mf) {
+            if (hi) QTP_KF_MF(MF_WAVES, MF_MAXKS, ctx->kforce_mf_lds, ctx->mf_rt, ctx->mf_nrt);
+            else QTP_KF_MF(MF_WAVES, MF_KS_LO, ctx->kforce_mf_lds, ctx->mf_rt, ctx->mf_nrt);
+        }
+#undef QTP_KF_MF
+        if (!ctx->kf_mf) {
+            if (fout2)
+                k_kforce<KF_SPLIT_WAVES, true><<<2 * E * nab, 64 * KF_SPLIT_WAVES, ctx->kforce_lds_split, s>>>(
+                    pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
+                    ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, fout2,
+                    ctx->kf_chunk_split, rfar, ctx->ntile);
+            else
+                k_kforce<KF_WAVES, false><<<E * nab, 64 * KF_WAVES, ctx->kforce_lds, s>>>(
+                    pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->G, ctx->sprime,
+                    ctx->rpart, ctx->nblk, ctx->flj, ctx->spl_on ? ctx->fspl : nullptr, fout, nullptr,
+                    ctx->kf_chunk, rfar, ctx->ntile);
+        }
     }
     if (eout) {
         TimeScope ts(ctx, "esum");

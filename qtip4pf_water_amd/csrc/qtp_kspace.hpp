@@ -104,9 +104,13 @@ constexpr int KS_WAVES = 8;
 // LDS rows of AC + 2 entries: the B operands of a quarter-wave (16 rows n at
 // one site) and the A factors of a tile's columns then fall in different banks.
 // The column and unit tables are staged in LDS behind the factor tables.
+// z-factor rows of k_ksum_mf: 2 K entries padded to whole 16-row n tiles
+// (at least two)
+__host__ __device__ inline int ksum_mf_zrows(int K) { return 2 * K <= 32 ? 32 : ((2 * K + 15) / 16) * 16; }
 inline size_t ksum_mf_lds_bytes(int K, int AC, int ncol4, int nunit)
 {
-    return ((size_t)4 * K + 32) * (AC + 2) * 8 + (size_t)ncol4 * 16 + (size_t)nunit * 8;
+    return ((size_t)4 * K + ksum_mf_zrows(K)) * (AC + 2) * 8 + (size_t)ncol4 * 16 +
+           (size_t)nunit * 8;
 }
 
 __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
@@ -118,9 +122,10 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     extern __shared__ double smem[];
     const int K = ew.K;
     const int RS = AC + 2;           // LDS row stride
+    const int ZN = ksum_mf_zrows(K);
     double2 *T = (double2 *)smem;    // [2][K][RS] x (charge folded), y factors
-    double *Zt = smem + 4 * K * RS;  // [32][RS]
-    int4 *sc = (int4 *)(Zt + 32 * RS);  // [ncol4] column table
+    double *Zt = smem + 4 * K * RS;  // [ZN][RS]
+    int4 *sc = (int4 *)(Zt + ZN * RS);  // [ncol4] column table
     int2 *su = (int2 *)(sc + ncol4);    // [nunit] unit table
     const int per_e = nac * usplit;
     const int e = blockIdx.x / per_e, r = blockIdx.x - e * per_e;
@@ -156,7 +161,7 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
             }
         }
         if (d == 2)
-            for (int n = 2 * K; n < 32; ++n) Zt[n * RS + j] = 0.0;
+            for (int n = 2 * K; n < ZN; ++n) Zt[n * RS + j] = 0.0;
     }
     __syncthreads();
     const int lane = threadIdx.x & 63;
@@ -556,7 +561,12 @@ constexpr int MF_SITES = 48;
 constexpr int MF_WAVES = 8;       // waves per block of large launches
 constexpr int MF_WAVES_SMALL = 16;  // and of launches of fewer than MF_SMALL_BLOCKS blocks
 constexpr int MF_SMALL_BLOCKS = 64;
-constexpr int MF_MAXKS = 8;  // k-steps per tile (A operands in registers): 2 K <= 32 (k_max <= 15)
+// k-steps of 4 z-factor entries per tile (A operands in registers): a
+// kernel built for MAXKS k-steps takes 2 K <= 4 MAXKS (k_max <= 2 MAXKS - 1);
+// MF_KS_LO (k_max <= 15) keeps the register use of the common sizes low,
+// MF_MAXKS (k_max <= 23) serves the larger k-sets.
+constexpr int MF_KS_LO = 8;
+constexpr int MF_MAXKS = 12;
 
 inline size_t kforce_mf_lds_bytes(int K, int nks, int W)
 {
@@ -589,7 +599,7 @@ __device__ __forceinline__ void mf_tile(const double *__restrict__ Wr, long Mp4,
     }
 }
 
-template <int W>
+template <int W, int MAXKS>
 __global__ __launch_bounds__(64 * W) void k_kforce_mf(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
     int NAP, double gamma, double half1mg, Ewald ew, const double *__restrict__ wm, int Mp,
@@ -690,15 +700,40 @@ __global__ __launch_bounds__(64 * W) void k_kforce_mf(
         mf_v4d d0 = {0.0, 0.0, 0.0, 0.0}, d1 = d0, d2 = d0;
         const double *Wr = We + (long)gq * Mp + r.w + j;
         const long Mp4 = 4L * Mp;
-        switch (r.z) {  // 1 <= r.z <= MF_MAXKS (host table)
-        case 1: mf_tile<1>(Wr, Mp4, Zl, d0, d1, d2); break;
-        case 2: mf_tile<2>(Wr, Mp4, Zl, d0, d1, d2); break;
-        case 3: mf_tile<3>(Wr, Mp4, Zl, d0, d1, d2); break;
-        case 4: mf_tile<4>(Wr, Mp4, Zl, d0, d1, d2); break;
-        case 5: mf_tile<5>(Wr, Mp4, Zl, d0, d1, d2); break;
-        case 6: mf_tile<6>(Wr, Mp4, Zl, d0, d1, d2); break;
-        case 7: mf_tile<7>(Wr, Mp4, Zl, d0, d1, d2); break;
-        default: mf_tile<8>(Wr, Mp4, Zl, d0, d1, d2); break;
+        if constexpr (MAXKS <= MF_KS_LO) {
+            switch (r.z) {  // 1 <= r.z <= MAXKS (host table)
+            case 1: mf_tile<1>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 2: mf_tile<2>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 3: mf_tile<3>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 4: mf_tile<4>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 5: mf_tile<5>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 6: mf_tile<6>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 7: mf_tile<7>(Wr, Mp4, Zl, d0, d1, d2); break;
+            default: mf_tile<8>(Wr, Mp4, Zl, d0, d1, d2); break;
+            }
+        } else {
+            // larger k-sets: tiles of up to 12 k-steps, as two straight-line
+            // halves (registers of one half at a time)
+            const int lo = r.z < 6 ? r.z : 6;
+            switch (lo) {
+            case 1: mf_tile<1>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 2: mf_tile<2>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 3: mf_tile<3>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 4: mf_tile<4>(Wr, Mp4, Zl, d0, d1, d2); break;
+            case 5: mf_tile<5>(Wr, Mp4, Zl, d0, d1, d2); break;
+            default: mf_tile<6>(Wr, Mp4, Zl, d0, d1, d2); break;
+            }
+            const double *Wr2 = Wr + 6 * Mp4;
+            const double *Zl2 = Zl + 4 * 6 * MF_SITES;
+            switch (r.z - lo) {
+            case 0: break;
+            case 1: mf_tile<1>(Wr2, Mp4, Zl2, d0, d1, d2); break;
+            case 2: mf_tile<2>(Wr2, Mp4, Zl2, d0, d1, d2); break;
+            case 3: mf_tile<3>(Wr2, Mp4, Zl2, d0, d1, d2); break;
+            case 4: mf_tile<4>(Wr2, Mp4, Zl2, d0, d1, d2); break;
+            case 5: mf_tile<5>(Wr2, Mp4, Zl2, d0, d1, d2); break;
+            default: mf_tile<6>(Wr2, Mp4, Zl2, d0, d1, d2); break;
+            }
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {

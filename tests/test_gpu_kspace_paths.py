@@ -33,13 +33,14 @@ def system(n_side, L_angs, P, seed):
     return np.ascontiguousarray(q), box
 
 
-def _forces(monkeypatch, path, n, P, box, q, o, point, lst):
+def _forces(monkeypatch, path, n, P, box, q, o, point, lst, ewald=None):
     from qtip4pf_water_amd import Water
     if path == "vector":
         monkeypatch.setenv("QTP_KSPACE", "vector")
     else:
         monkeypatch.delenv("QTP_KSPACE", raising=False)
-    with Water(n, P, box) as w:
+    with Water(n, P, box, ewald=ewald) as w:
+        assert w.tuning()["matrix_kspace"] == (path == "matrix")
         F, f = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
         e, sc = w.force_eval_sc_nospline(q, F, o.charge, o.mass, point, lst, fphys=f)
         F1 = np.zeros(P * o.n_dof)
@@ -47,12 +48,23 @@ def _forces(monkeypatch, path, n, P, box, q, o, point, lst):
     return F, f, e, sc, F1[: o.n_dof], e1
 
 
-@pytest.mark.parametrize("n_side,L,P,seed", [(6, 18.64, 4, 51), (3, 9.4, 2, 52)])
-def test_both_kspace_paths_match_oracle_and_each_other(monkeypatch, oracle_mod, n_side, L, P, seed):
+def _stated(box, n_max):
+    """alpha = 0.3 / A, r_c = 9 A, every k = 2 pi n / L with |n| <= n_max."""
+    return {"kappa": 0.3 / cfg.ANGS2BOHR, "rcut": 9.0 * cfg.ANGS2BOHR,
+            "k_cut": 2.0 * np.pi * n_max / float(box[0]) * (1.0 + 1e-9), "k_max": n_max}
+
+
+@pytest.mark.parametrize("n_side,L,P,seed,n_max", [
+    (6, 18.64, 4, 51, None), (3, 9.4, 2, 52, None),
+    (6, 18.64, 2, 54, 18),    # 2 K = 38 z-factor entries: 10 k-steps, three n tiles
+    (6, 18.64, 2, 55, 23)])   # the largest k-set of the matrix-core kernels
+def test_both_kspace_paths_match_oracle_and_each_other(monkeypatch, oracle_mod, n_side, L, P, seed,
+                                                       n_max):
     n = n_side ** 3
-    say("k-space paths", n, "mols P", P)
+    say("k-space paths", n, "mols P", P, "n_max", n_max)
     q, box = system(n_side, L, P, seed)
-    o = oracle_mod.Oracle(n, P, box)
+    ew = None if n_max is None else _stated(box, n_max)
+    o = oracle_mod.Oracle(n, P, box, ewald=ew)
     point, lst, _ = o.all_lists(q)
     Fr, fr = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
     er, scr = o.force_eval_sc(q, Fr, fr, point, lst)
@@ -60,7 +72,7 @@ def test_both_kspace_paths_match_oracle_and_each_other(monkeypatch, oracle_mod, 
     e1r = o.get_forces(q, F1r, point, lst, 0)
     got = {}
     for path in ("matrix", "vector"):
-        F, f, e, sc, F1, e1 = _forces(monkeypatch, path, n, P, box, q, o, point, lst)
+        F, f, e, sc, F1, e1 = _forces(monkeypatch, path, n, P, box, q, o, point, lst, ew)
         assert rel_atom_err(f, fr) < 1e-10, path
         assert rel_atom_err(F1, F1r[: o.n_dof]) < 1e-10, path
         assert abs(e - er) <= 1e-12 * abs(er), path
@@ -77,8 +89,8 @@ def test_both_kspace_paths_match_oracle_and_each_other(monkeypatch, oracle_mod, 
 
 
 def test_largest_matrix_core_k_max(monkeypatch, oracle_mod):
-    """k_max = 15, the largest the matrix-core kernels take (2 K = 32 z-factor
-    entries, eight k-steps): 5,832 H2O at liquid density, one bead."""
+    """k_max = 15, the largest k-set of the eight-k-step matrix-core kernels
+    (2 K = 32 z-factor entries): 5,832 H2O at liquid density, one bead."""
     from qtip4pf_water_amd import Water
     n_side, L = 18, 55.94
     n = n_side ** 3

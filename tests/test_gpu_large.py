@@ -158,9 +158,8 @@ def _nominal_ewald(box):
 
 def test_large_nominal_ewald_kmax18(oracle_mod):
     """The Ewald point stated for this system (alpha = 0.3 / A, r_c = 9 A,
-    |n| <= 18: the vector k-space kernels, above the matrix-core range
-    k_max <= 15) with cell-binned lists: lists, forces and energy against the
-    oracle given the same parameters."""
+    |n| <= 18: the twelve-k-step matrix-core kernels) with cell-binned lists:
+    lists, forces and energy against the oracle given the same parameters."""
     from qtip4pf_water_amd import Water
     n = N_SIDE ** 3
     say("large nominal Ewald k_max 18", n, "mols")

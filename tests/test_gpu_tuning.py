@@ -90,11 +90,13 @@ def test_autotune_caches_and_runs():
 
 
 def test_vector_only_system_rejects_matrix_kspace():
+    """k_max 24: 2 K = 50 z-factor entries, beyond the matrix-core kernels'
+    twelve k-steps."""
     from qtip4pf_water_amd import Water, QtpError
     q, p, box = _state(2, 51)
     L = float(box[0])
     ew = {"kappa": 0.3 / cfg.ANGS2BOHR, "rcut": 9.0 * cfg.ANGS2BOHR,
-          "k_cut": 2.0 * np.pi * 18.0 / L * (1.0 + 1e-9), "k_max": 18}
+          "k_cut": 2.0 * np.pi * 24.0 / L * (1.0 + 1e-9), "k_max": 24}
     with Water(216, 2, box, ewald=ew) as w:
         assert w.tuning()["matrix_kspace"] == 0
         with pytest.raises(QtpError):
