@@ -56,7 +56,7 @@ struct qtp_ctx {
     // would not fill two per CU (small systems); large ones have blocks enough
     bool kf_split = false, kf_split_ok = false;
     // k-space sums and forces on the matrix cores (k_ksum_mf, k_kforce_mf)
-    // for k_max <= 15; the vector kernels above that, or when
+    // for k_max <= 23; the vector kernels above that, or when
     // QTP_KSPACE=vector is set at create
     bool kf_mf = false, kf_mf_ok = false;
     // launch choices the autotuner may set (qtp_set_tuning): k_realspace

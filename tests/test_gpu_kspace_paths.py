@@ -1,5 +1,5 @@
 """The two implementations of the reciprocal-space sums and forces: the
-matrix-core kernels (k_ksum_mf / k_kforce_mf, used for k_max <= 15) and the
+matrix-core kernels (k_ksum_mf / k_kforce_mf, used for k_max <= 23) and the
 vector kernels (k_ksum / k_kforce, used above that or with QTP_KSPACE=vector
 set when a context is created).  Both against the oracle at the parity
 tolerances, and against each other (both FP64; they differ only in summation
