@@ -40,7 +40,8 @@ MODEL_FILE = os.path.join(DATA_DIR, "cmd_surrogate_216x32.npz")
 
 
 def say(*a):
-    print("[cmd]", *a, flush=True)
+    # progress on stderr: stdout stays free for results (bench.py's JSON line)
+    print("[cmd]", *a, file=sys.stderr, flush=True)
 
 
 def box_for(n_mols, rho_gcc=0.997):

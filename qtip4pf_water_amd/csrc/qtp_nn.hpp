@@ -45,9 +45,16 @@ __device__ __forceinline__ float welem(int ei, int ej)
 }
 __device__ __forceinline__ int elem_of(int a) { return (a % 3) == 0 ? 0 : 1; }
 
+// The value of lane (lane ^ o) of the wavefront (one ds_bpermute).
+__device__ __forceinline__ float xor_lane(float v, int o)
+{
+    const int src = ((int)(threadIdx.x & 63) ^ o) << 2;
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(v)));
+}
+
 __device__ __forceinline__ float wave_sumf(float v)
 {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    for (int o = 32; o > 0; o >>= 1) v += xor_lane(v, o);
     return v;
 }
 
@@ -207,22 +214,28 @@ __device__ __forceinline__ void block_sums(float *v, float (*red)[NIN], float *o
 // values over the lane bits 5..2), then lanes 4c..4c+3 hold the total of
 // component c = lane >> 2 after two plain xor rounds.  63 shuffles instead
 // of 96, in a fixed order.
-__device__ __forceinline__ float halving_sum16(float *v)
+template <int H>
+__device__ __forceinline__ void halving_round(float (&v)[16], int lane)
+{
+    constexpr int o = 4 * H;
+    const bool up = (lane & o) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const float lo = v[i], hi = v[i + H];
+        const float keep = up ? hi : lo, give = up ? lo : hi;
+        v[i] = keep + xor_lane(give, o);
+    }
+}
+__device__ __forceinline__ float halving_sum16(float (&v)[16])
 {
     const int lane = threadIdx.x & 63;
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int o = 32 >> r, h = 8 >> r;
-        const bool up = (lane & o) != 0;
-#pragma unroll
-        for (int i = 0; i < h; ++i) {
-            const float keep = up ? v[i + h] : v[i], give = up ? v[i] : v[i + h];
-            v[i] = keep + __shfl_xor(give, o, 64);
-        }
-    }
+    halving_round<8>(v, lane);
+    halving_round<4>(v, lane);
+    halving_round<2>(v, lane);
+    halving_round<1>(v, lane);
     float x = v[0];
-    x += __shfl_xor(x, 2, 64);
-    x += __shfl_xor(x, 1, 64);
+    x += xor_lane(x, 2);
+    x += xor_lane(x, 1);
     return x;
 }
 
@@ -511,10 +524,10 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             float v = er[r];
-            v += __shfl_xor(v, 1, 64);
-            v += __shfl_xor(v, 2, 64);
-            v += __shfl_xor(v, 4, 64);
-            v += __shfl_xor(v, 8, 64);
+            v += xor_lane(v, 1);
+            v += xor_lane(v, 2);
+            v += xor_lane(v, 4);
+            v += xor_lane(v, 8);
             if (col == 0) sE[w][rq + r] = v;
         }
     }
@@ -542,10 +555,10 @@ __global__ __launch_bounds__(64 * MLP_WAVES) void k_nn_mlp(
         float v = rr >= 0 ? W.e_scale * (sE[0][lane] + sE[1][lane] + sE[2][lane] + sE[3][lane] + W.b3[e])
                           : 0.f;
         if (rr >= 0 && Eatom) Eatom[rr] = v;
-        v += __shfl_xor(v, 1, 64);
-        v += __shfl_xor(v, 2, 64);
-        v += __shfl_xor(v, 4, 64);
-        v += __shfl_xor(v, 8, 64);
+        v += xor_lane(v, 1);
+        v += xor_lane(v, 2);
+        v += xor_lane(v, 4);
+        v += xor_lane(v, 8);
         if (lane == 0 && Etile) Etile[blockIdx.x] = v;
     }
     __syncthreads();
@@ -589,7 +602,7 @@ __device__ __forceinline__ float dphi(int m, float r, float fc, float dfc)
 // is differentiated once, by one thread, for both members; the pieces are
 // summed in LDS as 64-bit fixed-point integers (resolution 2^-40, range
 // +-2^23), whose additions are exact, so the sums do not depend on the order
-// in which the threads add them.
+// in which the threads add them (nor on which member is called j).
 // grid: n_frames * n_atoms blocks of AT threads.
 // ---------------------------------------------------------------------------
 constexpr double DANG_FIX = 1099511627776.0;   // 2^40
@@ -619,10 +632,14 @@ __global__ __launch_bounds__(AT) void k_nn_dang(
     if (tid < NANG) se[tid] = dEdG[(long)gi * NIN + NR + tid];
     if (tid < 3 * NAMAX) (&acc[0][0])[tid] = 0ull;
     __syncthreads();
+    // pairs in circulant order: p = n (d - 1) + j gives (j, (j + d) mod n),
+    // d = 1 .. n/2 (for even n only j < n/2 at d = n/2): every unordered pair
+    // once, and the lanes of a wavefront mostly hold distinct j and distinct
+    // k, so their atomic adds do not pile onto one address
     const int npair = cnta * (cnta - 1) / 2;
     for (int p = tid; p < npair; p += AT) {
-        int j, k;
-        tri_decode(p, j, k);
+        const int d = p / cnta + 1, j = p - (d - 1) * cnta;
+        const int k = j + d < cnta ? j + d : j + d - cnta;
         const float4 A = av[j], B = av[k];
         float T[NANG], Tc[NANG], Ta[NANG], Tb[NANG];
         const float iab = 1.0f / (A.w * B.w);
@@ -975,8 +992,8 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_nn_train_fwd(
                 sz += dz2[t][r];
                 sw += W.e_scale * s2 * zd2;
             }
-            sz += __shfl_xor(sz, 16, 64);
-            sz += __shfl_xor(sz, 32, 64);
+            sz += xor_lane(sz, 16);
+            sz += xor_lane(sz, 32);
             sw += __shfl_xor(sw, 16, 64);
             sw += __shfl_xor(sw, 32, 64);
             if (lane < 16) {
@@ -1021,8 +1038,8 @@ __global__ __launch_bounds__(64 * TRAIN_WAVES) void k_nn_train_fwd(
                 dz1[r] = s1 * da1;
                 sz += dz1[r];
             }
-            sz += __shfl_xor(sz, 16, 64);
-            sz += __shfl_xor(sz, 32, 64);
+            sz += xor_lane(sz, 16);
+            sz += xor_lane(sz, 32);
             if (lane < 16) B.tp[((long)gtile * 3 + 0) * NH + n] = sz;
             store_t(B.d1t[e], ld, n, rq, dzd1);
             store_t(B.d1p[e], ld, n, rq, dz1);

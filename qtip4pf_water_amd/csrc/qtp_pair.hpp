@@ -12,6 +12,7 @@ namespace qtp {
 // group adds the intramolecular terms.  Each LJ pair is met from both sides;
 // its energy is halved.
 // grid: E * ceil(n_mols / (256 / LPM)), 256 threads.
+constexpr int LJ_BATCH = 4;
 template <bool VIR, int LPM>
 __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos, long pstride,
                                                  int n_mols, int NAP,
@@ -41,29 +42,46 @@ __global__ __launch_bounds__(256) void k_ljintra(const double *__restrict__ pos,
         const int *fl = flist + (long)b * fcap;
         // an overflowed list (status flag set by the list kernels) is never read past capacity
         const int k0 = (int)min((long)fp[i], fcap), k1 = (int)min((long)fp[i + 1], fcap);
-        for (int k = k0 + t; k < k1; k += LPM) {
-            const int aj = 3 * fl[k];
-            double r[3];
-            for (int d = 0; d < 3; ++d)
-                r[d] = mimg_fast(o[d] - P[d * NAP + aj], box.L[d], box.inv[d]);
-            const double rsq = fma(r[0], r[0], fma(r[1], r[1], r[2] * r[2]));
-            if (rsq < ff.r_cut_sq) {
-                const double rm2 = 1.0 / rsq;
-                const double rm6 = rm2 * rm2 * rm2;
-                const double a12 = rm6 * rm6 * ff.sigmalj12;
-                const double a6 = rm6 * ff.sigmalj6;
-                const double u = a12 - a6;
-                elj += u * ff.epslj4 - ff.e_cut;
-                const double pf = (u + a12) * ff.epslj24 * rm2;
-                for (int d = 0; d < 3; ++d) fo[d] = fma(pf, r[d], fo[d]);
-                if (VIR) {  // each pair is met twice: half of r (x) f per side
-                    const double hp = 0.5 * pf;
-                    v6[0] += hp * r[0] * r[0];
-                    v6[1] += hp * r[1] * r[1];
-                    v6[2] += hp * r[2] * r[2];
-                    v6[3] += hp * r[0] * r[1];
-                    v6[4] += hp * r[0] * r[2];
-                    v6[5] += hp * r[1] * r[2];
+        // LJ_BATCH entries of the lane's walk per round: their list entries,
+        // then their positions, are loaded before any is used (two round
+        // trips per round instead of two per entry); the entries are taken
+        // in the same order as one at a time
+        for (int k = k0 + t; k < k1; k += LJ_BATCH * LPM) {
+            int aj[LJ_BATCH];
+            double pj[LJ_BATCH][3];
+#pragma unroll
+            for (int u = 0; u < LJ_BATCH; ++u) {
+                const int kk = k + u * LPM;
+                aj[u] = kk < k1 ? 3 * fl[kk] : -1;
+            }
+#pragma unroll
+            for (int u = 0; u < LJ_BATCH; ++u)
+                for (int d = 0; d < 3; ++d) pj[u][d] = aj[u] >= 0 ? P[d * NAP + aj[u]] : o[d];
+#pragma unroll
+            for (int u = 0; u < LJ_BATCH; ++u) {
+                if (aj[u] < 0) break;
+                double r[3];
+                for (int d = 0; d < 3; ++d)
+                    r[d] = mimg_fast(o[d] - pj[u][d], box.L[d], box.inv[d]);
+                const double rsq = fma(r[0], r[0], fma(r[1], r[1], r[2] * r[2]));
+                if (rsq < ff.r_cut_sq) {
+                    const double rm2 = 1.0 / rsq;
+                    const double rm6 = rm2 * rm2 * rm2;
+                    const double a12 = rm6 * rm6 * ff.sigmalj12;
+                    const double a6 = rm6 * ff.sigmalj6;
+                    const double ulj = a12 - a6;
+                    elj += ulj * ff.epslj4 - ff.e_cut;
+                    const double pf = (ulj + a12) * ff.epslj24 * rm2;
+                    for (int d = 0; d < 3; ++d) fo[d] = fma(pf, r[d], fo[d]);
+                    if (VIR) {  // each pair is met twice: half of r (x) f per side
+                        const double hp = 0.5 * pf;
+                        v6[0] += hp * r[0] * r[0];
+                        v6[1] += hp * r[1] * r[1];
+                        v6[2] += hp * r[2] * r[2];
+                        v6[3] += hp * r[0] * r[1];
+                        v6[4] += hp * r[0] * r[2];
+                        v6[5] += hp * r[1] * r[2];
+                    }
                 }
             }
         }
