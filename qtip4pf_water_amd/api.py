@@ -439,10 +439,10 @@ class Water:
 
     # ------------------------------ launch configuration and autotuner
     TUNING_KEYS = ("lanes", "matrix_kspace", "rs_registers", "kforce_16_waves",
-                   "ksum_blocks_per_chunk", "fused_propagation")
+                   "ksum_blocks_per_chunk", "fused_propagation", "rs_waves")
 
     def tuning(self) -> dict:
-        out = np.zeros(6, np.int32)
+        out = np.zeros(len(self.TUNING_KEYS), np.int32)
         self._chk(self._L.qtp_get_tuning(self._h, ptr(out, ip)))
         return dict(zip(self.TUNING_KEYS, (int(v) for v in out)))
 
@@ -457,10 +457,11 @@ class Water:
     def autotune(self, retime: bool = False) -> dict:
         """Time every candidate launch configuration on the resident state's
         force evaluation and keep the fastest (cached per system shape)."""
-        r = np.zeros(8)
+        r = np.zeros(9)
         self._chk(self._L.qtp_autotune(self._h, 1 if retime else 0, ptr(r)))
-        return {"best_ms": float(r[0]), "default_ms": float(r[1]),
-                "chosen": dict(zip(self.TUNING_KEYS[:5], (int(v) for v in r[2:7]))),
+        chosen = dict(zip(self.TUNING_KEYS[:5], (int(v) for v in r[2:7])))
+        chosen["rs_waves"] = int(r[8])
+        return {"best_ms": float(r[0]), "default_ms": float(r[1]), "chosen": chosen,
                 "timed": int(r[7])}
 
     # ------------------------------ Verlet-list build

@@ -1610,7 +1610,7 @@ int qtp_set_nn(qtp_ctx *ctx, qtp_nn *nn)
 
 // ---- launch configuration and autotuner -----------------------------------------------
 
-int qtp_get_tuning(const qtp_ctx *ctx, int cfg[6])
+int qtp_get_tuning(const qtp_ctx *ctx, int cfg[7])
 {
     if (!ctx || !cfg) return QTP_ERR_ARG;
     const TuneCfg c = current_tuning(ctx);
@@ -1620,10 +1620,11 @@ int qtp_get_tuning(const qtp_ctx *ctx, int cfg[6])
     cfg[3] = c.small;
     cfg[4] = c.usplit;
     cfg[5] = c.fused;
+    cfg[6] = c.rsw;
     return QTP_OK;
 }
 
-int qtp_set_tuning(qtp_ctx *ctx, const int cfg[6])
+int qtp_set_tuning(qtp_ctx *ctx, const int cfg[7])
 {
     if (!ctx || !cfg) return QTP_ERR_ARG;
     TuneCfg c;
@@ -1633,10 +1634,11 @@ int qtp_set_tuning(qtp_ctx *ctx, const int cfg[6])
     c.small = cfg[3];
     c.usplit = cfg[4];
     c.fused = cfg[5];
+    c.rsw = cfg[6];
     if (!tuning_valid(ctx, c)) {
         ctx->err = "tuning: lanes 16 or 64, matrix k-space 0/1 (1 only for k_max <= 23), "
                    "registers and 16-wave blocks -1/0/1, blocks per chunk 0-4, fused "
-                   "propagation -1/0/1";
+                   "propagation -1/0/1, real-space waves 2 or 4";
         return QTP_ERR_ARG;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1645,7 +1647,7 @@ int qtp_set_tuning(qtp_ctx *ctx, const int cfg[6])
     return tune_refresh(ctx);
 }
 
-int qtp_autotune(qtp_ctx *ctx, int flags, double report[8])
+int qtp_autotune(qtp_ctx *ctx, int flags, double report[9])
 {
     if (!ctx) return QTP_ERR_ARG;
     if (!ctx->state_set) {
@@ -1708,6 +1710,7 @@ int qtp_autotune(qtp_ctx *ctx, int flags, double report[8])
         report[5] = best.small;
         report[6] = best.usplit;
         report[7] = n_timed;
+        report[8] = best.rsw;
     }
     return QTP_OK;
 }

@@ -109,11 +109,16 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
         const bool carg = ctx->rs_carg >= 0 ? ctx->rs_carg == 1
                                             : (long)E * ctx->ntile >= RS_CARG_BLOCKS;
         const RsCoef cf = rs_coef();
-#define QTP_RS_LAUNCH(VIR, CARG, CULL)                                                             \
-    k_realspace<VIR, CARG, CULL><<<E * ctx->ntile, 64 * RS_WAVES, 0, s>>>(                         \
+#define QTP_RS_LAUNCH_W(VIR, CARG, CULL, W)                                                        \
+    k_realspace<VIR, CARG, CULL, W><<<E * ctx->ntile, 64 * W, 0, s>>>(                             \
         pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles, ctx->ntile,     \
         ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real, vp, cf,         \
         ctx->rs_sites, ctx->rs_box, ctx->rs_far)
+#define QTP_RS_LAUNCH(VIR, CARG, CULL)                                                             \
+    do {                                                                                           \
+        if (ctx->rs_waves == 2) QTP_RS_LAUNCH_W(VIR, CARG, CULL, 2);                              \
+        else QTP_RS_LAUNCH_W(VIR, CARG, CULL, 4);                                                  \
+    } while (0)
         if (ctx->rs_cull) {
             if (vout) QTP_RS_LAUNCH(true, true, true);
             else QTP_RS_LAUNCH(false, true, true);
@@ -125,6 +130,7 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
             else QTP_RS_LAUNCH(false, false, false);
         }
 #undef QTP_RS_LAUNCH
+#undef QTP_RS_LAUNCH_W
     }
     if (fork) HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
     {

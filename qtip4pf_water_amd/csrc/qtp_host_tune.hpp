@@ -3,8 +3,8 @@
 //
 // The tunable launch choices of a context (lanes per molecule of
 // k_ljintra, matrix-core or vector k-space kernels, k_realspace coefficients
-// as literals or in scalar registers, 8- or 16-wave k_kforce_mf blocks,
-// k_ksum_mf blocks per site chunk) are timed on the context's own resident
+// as literals or in scalar registers and two or four waves per tile, 8- or
+// 16-wave k_kforce_mf blocks, k_ksum_mf blocks per site chunk) are timed on the context's own resident
 // state: one force evaluation of every bead (both Suzuki-Chin phases, or the
 // classical phase), with HIP events on the context's stream around the
 // whole fork/join of the two streams, median of several repetitions.  Every
@@ -19,7 +19,7 @@ namespace {
 constexpr float TUNE_MARGIN = 0.02f;
 
 struct TuneCfg {
-    int lpm = 16, kmf = 0, carg = -1, small = -1, usplit = 0, fused = -1;
+    int lpm = 16, kmf = 0, carg = -1, small = -1, usplit = 0, fused = -1, rsw = RS_WAVES;
 };
 
 TuneCfg current_tuning(const qtp_ctx *ctx)
@@ -31,6 +31,7 @@ TuneCfg current_tuning(const qtp_ctx *ctx)
     c.small = ctx->mf_small;
     c.usplit = ctx->ks_usplit;
     c.fused = ctx->prop_fused;
+    c.rsw = ctx->rs_waves;
     return c;
 }
 
@@ -42,6 +43,7 @@ bool tuning_valid(const qtp_ctx *ctx, const TuneCfg &c)
     if (c.carg < -1 || c.carg > 1 || c.small < -1 || c.small > 1) return false;
     if (c.small == 1 && ctx->kforce_mf_lds_small > 64 * 1024) return false;
     if (c.fused < -1 || c.fused > 1) return false;
+    if (c.rsw != 2 && c.rsw != 4) return false;
     return c.usplit >= 0 && c.usplit <= 4;
 }
 
@@ -54,6 +56,7 @@ void apply_tuning(qtp_ctx *ctx, const TuneCfg &c)
     ctx->mf_small = c.small;
     ctx->ks_usplit = c.usplit;
     ctx->prop_fused = c.fused;
+    ctx->rs_waves = c.rsw;
     invalidate_graphs(ctx);
 }
 
@@ -106,26 +109,23 @@ std::vector<TuneCfg> tune_candidates(const qtp_ctx *ctx)
 {
     std::vector<TuneCfg> out;
     const int umax = std::min(4, std::max(1, ctx->ks_nunit / KS_WAVES));
-    for (int lpm : {16, 64}) {
-        for (int kmf = ctx->kf_mf_ok ? 1 : 0; kmf >= 0; --kmf) {
-            for (int carg : ctx->rs_cull ? std::vector<int>{-1} : std::vector<int>{0, 1}) {
-                if (!kmf) {
+    for (int rsw : {4, 2})
+        for (int lpm : {16, 64})
+            for (int kmf = ctx->kf_mf_ok ? 1 : 0; kmf >= 0; --kmf)
+                for (int carg : ctx->rs_cull ? std::vector<int>{-1} : std::vector<int>{0, 1}) {
                     TuneCfg c;
-                    c.lpm = lpm, c.kmf = 0, c.carg = carg, c.fused = ctx->prop_fused;
-                    out.push_back(c);
-                    continue;
-                }
-                const int smax = ctx->kforce_mf_lds_small <= 64 * 1024 ? 1 : 0;
-                for (int small = 0; small <= smax; ++small)
-                    for (int u = 1; u <= umax; u *= 2) {
-                        TuneCfg c;
-                        c.lpm = lpm, c.kmf = 1, c.carg = carg, c.small = small, c.usplit = u;
-                        c.fused = ctx->prop_fused;
+                    c.lpm = lpm, c.kmf = kmf, c.carg = carg, c.fused = ctx->prop_fused, c.rsw = rsw;
+                    if (!kmf) {
                         out.push_back(c);
+                        continue;
                     }
-            }
-        }
-    }
+                    const int smax = ctx->kforce_mf_lds_small <= 64 * 1024 ? 1 : 0;
+                    for (int small = 0; small <= smax; ++small)
+                        for (int u = 1; u <= umax; u *= 2) {
+                            c.small = small, c.usplit = u;
+                            out.push_back(c);
+                        }
+                }
     return out;
 }
 

@@ -4,20 +4,22 @@
 
 namespace qtp {
 // Real-space Ewald over all site pairs i<j within r_c (forces.cpp:586-608,
-// rwald :141-177).  One 256-thread block per 64x64 tile (I<=J) of the site
-// pair matrix; wave w takes the 16 j-columns 16w..16w+15 against all 64 i.
-// Pass 1 (lane <-> i) marks each lane's in-cutoff columns (16-bit mask).
-// The wave then lays its in-cutoff pairs out in an LDS queue, lane-major
-// (offsets by a wave prefix sum of the mask popcounts), and pass 2 gives
-// every lane the same number of consecutive queue entries: lanes stay busy
-// however unevenly the pairs fall on the rows.  Consecutive entries mostly
-// share i, so a lane keeps its i-force in registers and adds it to the
-// wave's LDS row when i changes; j reactions go to the LDS column array with
-// ds_add_f64.  All LDS adds of a row or column come from one wavefront, in
-// program order and fixed lane order, so the result is reproducible.
+// rwald :141-177).  One block of W wavefronts (W = 2 or 4) per 64x64 tile
+// (I<=J) of the site pair matrix; wave w takes the 64/W j-columns
+// c0 = w 64/W ... c0 + 64/W - 1 against all 64 i.
+// Pass 1 (lane <-> i) marks each lane's in-cutoff columns (a bit mask).
+// The wave then lays its in-cutoff pairs out in an LDS queue column by
+// column (a ballot per column; within a column, rows ascending), and pass 2
+// gives every lane the same number of consecutive queue entries: lanes stay
+// busy however unevenly the pairs fall.  Consecutive entries mostly share
+// the column j, so a lane keeps the j reaction in registers and adds it to
+// the LDS column array when j changes; i forces go to the wave's own LDS
+// rows with ds_add_f64.  Columns belong to one wave and every wave has its
+// own rows, and the LDS adds of a wave follow program and lane order, so
+// the result is reproducible.
 // Partial site forces go to slot J (block I's sites) and slot I (block J's
 // sites): every (site, slot) has exactly one writer.
-// grid: E * ntile, 256 threads.
+// grid: E * ntile, 64 W threads.
 // VIR: also accumulate the pair virial r_ij (x) f_ij (6 components) into
 // vpart[(e*6 + c)*ep_stride + ep_off + tile].
 // CARG: the special functions' coefficients come from the argument cf (SGPRs)
@@ -29,7 +31,10 @@ namespace qtp {
 // marked) and returns before loading any site.  The same pairs are evaluated
 // either way.  The host picks it for boxes several r_c wide, where most tiles
 // are far (4096 H2O: 54% of the tiles, sites in molecule order).
-constexpr int RS_WAVES = 4;
+// W: two waves per tile halve the block and double each lane's share of the
+// queue; a launch then holds fewer waves (216 H2O x 32 beads: 4224, which
+// the chip holds at once, against 8448 in two rounds with four).
+constexpr int RS_WAVES = 4;  // the default W
 constexpr long RS_CARG_BLOCKS = 16384;  // grids at least this large use CARG
 // boxes at least this many r_c wide use CULL.  x 32 beads, steps/s against
 // the plain form: 216 H2O (2.06 r_c) -8%, 1000 (2.66) +7%, 1728 (2.91) +7%,
@@ -88,26 +93,29 @@ __global__ __launch_bounds__(64) void k_sites(const double *__restrict__ pos, lo
     else if (l < 6) bb[l] = l == 3 ? hi[0] : (l == 4 ? hi[1] : hi[2]);
 }
 
-template <bool VIR, bool CARG, bool CULL>
-__global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ pos, long pstride,
-                                                   const double *__restrict__ charge, int n_atoms,
-                                                   int NAP, int nblk, double gamma, double half1mg,
-                                                   const int2 *__restrict__ tiles, int ntile,
-                                                   Box box, Ewald ew,
-                                                   double *__restrict__ rpart,
-                                                   double *__restrict__ epart, int ep_stride,
-                                                   int ep_off, double *__restrict__ vpart,
-                                                   const RsCoef cf,
-                                                   const double4 *__restrict__ sites,
-                                                   const double *__restrict__ bbox,
-                                                   unsigned char *__restrict__ rfar)
+template <bool VIR, bool CARG, bool CULL, int W>
+__global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__ pos, long pstride,
+                                                      const double *__restrict__ charge, int n_atoms,
+                                                      int NAP, int nblk, double gamma, double half1mg,
+                                                      const int2 *__restrict__ tiles, int ntile,
+                                                      Box box, Ewald ew,
+                                                      double *__restrict__ rpart,
+                                                      double *__restrict__ epart, int ep_stride,
+                                                      int ep_off, double *__restrict__ vpart,
+                                                      const RsCoef cf,
+                                                      const double4 *__restrict__ sites,
+                                                      const double *__restrict__ bbox,
+                                                      unsigned char *__restrict__ rfar)
 {
+    static_assert(W == 2 || W == 4, "two or four waves per tile");
+    constexpr int CPW = 64 / W;  // columns per wave
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int e = bid / ntile, t = bid - e * ntile;
     const int I = tiles[t].x, J = tiles[t].y;
     const int l = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double *X = pos + (long)e * pstride;
+    const long base_e = (long)e * nblk * 3 * NAP;
     if (CULL) {
         bool far = false;
         if (I != J) {
@@ -124,23 +132,23 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
         }
         if (threadIdx.x == 0) rfar[(long)e * ntile + t] = far ? 1 : 0;
         if (far) {
-            if (VIR && w == 0 && l < 6) vpart[((long)e * 6 + l) * ep_stride + ep_off + t] = 0.0;
-            const long base_e = (long)e * nblk * 3 * NAP;
-            if (w < 3) {
-                rpart[base_e + (long)J * 3 * NAP + w * NAP + I * 64 + l] = 0.0;
-                rpart[base_e + (long)I * 3 * NAP + w * NAP + J * 64 + l] = 0.0;
-            } else if (l == 0) {
-                epart[(long)e * ep_stride + ep_off + t] = 0.0;
+            if (VIR && threadIdx.x < 6)
+                vpart[((long)e * 6 + threadIdx.x) * ep_stride + ep_off + t] = 0.0;
+            for (int k = threadIdx.x; k < 192; k += 64 * W) {
+                const int d = k >> 6, ll = k & 63;
+                rpart[base_e + (long)J * 3 * NAP + d * NAP + I * 64 + ll] = 0.0;
+                rpart[base_e + (long)I * 3 * NAP + d * NAP + J * 64 + ll] = 0.0;
             }
+            if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + t] = 0.0;
             return;
         }
     }
     __shared__ double4 sI[64], sJ[64];  // x, y, z, charge
     __shared__ double fj[3][64];
-    __shared__ double fi[RS_WAVES][3][64];
-    __shared__ unsigned short qe[RS_WAVES][64 * 16];
-    __shared__ double ewv[RS_WAVES];
-    __shared__ double wv[RS_WAVES][6];
+    __shared__ double fi[W][3][64];
+    __shared__ unsigned short qe[W][64 * CPW];
+    __shared__ double ewv[W];
+    __shared__ double wv[W][6];
     if (w < 2) {
         const int s = (w == 0 ? J : I) * 64 + l;
         double4 v = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -161,13 +169,14 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
     fi[w][1][l] = 0.0;
     fi[w][2][l] = 0.0;
     __syncthreads();
-    const int c0 = 16 * w;
+    const int c0 = CPW * w;
     const int i = I * 64 + l;
     // pass 1: in-cutoff columns of row i (j > i; j < n_atoms)
     unsigned int mask = 0u;
     if (i < n_atoms) {
         const double4 a = sI[l];
-        for (int c = 0; c < 16; ++c) {
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
             const int jl = c0 + c;
             const int j = J * 64 + jl;
             const double4 b = sJ[jl];
@@ -178,46 +187,43 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
             if (j > i && j < n_atoms && rsq < ew.rcut_sq) mask |= 1u << c;
         }
     }
-    // the wave's pair queue, lane-major
-    const int pc = __popc(mask);
-    int incl = pc;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int v = __shfl_up(incl, o, 64);
-        if (l >= o) incl += v;
-    }
-    const int total = __shfl(incl, 63, 64);
-    {
-        int pos_q = incl - pc;
-        unsigned int m = mask;
-        while (m) {
-            const int c = __ffs(m) - 1;
-            m &= m - 1u;
-            qe[w][pos_q++] = (unsigned short)(l | (c << 6));
+    // the wave's pair queue, column-major: column c's rows at off_c + (the
+    // lane's rank among the column's set lanes)
+    int total = 0;
+#pragma unroll
+    for (int c = 0; c < CPW; ++c) {
+        const bool on = (mask >> c) & 1u;
+        const unsigned long long bal = __ballot(on);
+        if (on) {
+            const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                    __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+            qe[w][total + r] = (unsigned short)(l | (c << 6));
         }
+        total += __popcll(bal);
     }
     __syncthreads();
     // pass 2: lane l takes queue entries [l*S, l*S + S)
     const bool near = J - I <= 1;
     const int S = (total + 63) >> 6;
     const int q0 = min(l * S, total), q1 = min(q0 + S, total);
-    double fx = 0.0, fy = 0.0, fz = 0.0, en = 0.0;
+    double gx = 0.0, gy = 0.0, gz = 0.0, en = 0.0;
     double vxx = 0.0, vyy = 0.0, vzz = 0.0, vxy = 0.0, vxz = 0.0, vyz = 0.0;
-    int icur = -1;
-    double4 a = make_double4(0.0, 0.0, 0.0, 0.0);
+    int jcur = -1;
+    double4 b = make_double4(0.0, 0.0, 0.0, 0.0);
     for (int q = q0; q < q1; ++q) {
         const int ent = qe[w][q];
         const int il = ent & 63, jl = c0 + (ent >> 6);
-        if (il != icur) {
-            if (icur >= 0) {
-                atomicAdd(&fi[w][0][icur], fx);
-                atomicAdd(&fi[w][1][icur], fy);
-                atomicAdd(&fi[w][2][icur], fz);
+        if (jl != jcur) {
+            if (jcur >= 0) {
+                atomicAdd(&fj[0][jcur], gx);
+                atomicAdd(&fj[1][jcur], gy);
+                atomicAdd(&fj[2][jcur], gz);
             }
-            fx = fy = fz = 0.0;
-            icur = il;
-            a = sI[il];
+            gx = gy = gz = 0.0;
+            jcur = jl;
+            b = sJ[jl];
         }
-        const double4 b = sJ[jl];
+        const double4 a = sI[il];
         const double dx = mimg_fast(a.x - b.x, box.L[0], box.inv[0]);
         const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
         const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
@@ -239,9 +245,9 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
         const double ep = qq * rinv * ec;
         const double pf = (ep + qq * ew.pref2k * ex) * (rinv * rinv);
         en += ep;
-        fx = fma(pf, dx, fx);
-        fy = fma(pf, dy, fy);
-        fz = fma(pf, dz, fz);
+        gx = fma(-pf, dx, gx);
+        gy = fma(-pf, dy, gy);
+        gz = fma(-pf, dz, gz);
         if (VIR) {
             vxx = fma(pf * dx, dx, vxx);
             vyy = fma(pf * dy, dy, vyy);
@@ -250,14 +256,14 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
             vxz = fma(pf * dx, dz, vxz);
             vyz = fma(pf * dy, dz, vyz);
         }
-        atomicAdd(&fj[0][jl], -pf * dx);
-        atomicAdd(&fj[1][jl], -pf * dy);
-        atomicAdd(&fj[2][jl], -pf * dz);
+        atomicAdd(&fi[w][0][il], pf * dx);
+        atomicAdd(&fi[w][1][il], pf * dy);
+        atomicAdd(&fi[w][2][il], pf * dz);
     }
-    if (icur >= 0) {
-        atomicAdd(&fi[w][0][icur], fx);
-        atomicAdd(&fi[w][1][icur], fy);
-        atomicAdd(&fi[w][2][icur], fz);
+    if (jcur >= 0) {
+        atomicAdd(&fj[0][jcur], gx);
+        atomicAdd(&fj[1][jcur], gy);
+        atomicAdd(&fj[2][jcur], gz);
     }
     en = wave_sum(en);
     if (l == 0) ewv[w] = en;
@@ -269,21 +275,30 @@ __global__ __launch_bounds__(256) void k_realspace(const double *__restrict__ po
         }
     }
     __syncthreads();
-    if (VIR && w == 0 && l < 6)
-        vpart[((long)e * 6 + l) * ep_stride + ep_off + t] = wv[0][l] + wv[1][l] + wv[2][l] + wv[3][l];
-    const long base_e = (long)e * nblk * 3 * NAP;
-    if (w < 3) {
-        // wave d writes component d of both slot entries of this tile
-        const int d = w;
-        const double si = fi[0][d][l] + fi[1][d][l] + fi[2][d][l] + fi[3][d][l];
+    if (VIR && threadIdx.x < 6) {
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) v += wv[k][threadIdx.x];
+        vpart[((long)e * 6 + threadIdx.x) * ep_stride + ep_off + t] = v;
+    }
+    // component d of both slot entries of this tile
+    for (int k = threadIdx.x; k < 192; k += 64 * W) {
+        const int d = k >> 6, ll = k & 63;
+        double si = 0.0;
+#pragma unroll
+        for (int v = 0; v < W; ++v) si += fi[v][d][ll];
         if (I == J) {
-            rpart[base_e + (long)I * 3 * NAP + d * NAP + i] = si + fj[d][l];
+            rpart[base_e + (long)I * 3 * NAP + d * NAP + I * 64 + ll] = si + fj[d][ll];
         } else {
-            rpart[base_e + (long)J * 3 * NAP + d * NAP + i] = si;                 // slot J
-            rpart[base_e + (long)I * 3 * NAP + d * NAP + J * 64 + l] = fj[d][l];  // slot I
+            rpart[base_e + (long)J * 3 * NAP + d * NAP + I * 64 + ll] = si;       // slot J
+            rpart[base_e + (long)I * 3 * NAP + d * NAP + J * 64 + ll] = fj[d][ll];  // slot I
         }
-    } else if (l == 0) {
-        epart[(long)e * ep_stride + ep_off + t] = ewv[0] + ewv[1] + ewv[2] + ewv[3];
+    }
+    if (threadIdx.x == 0) {
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < W; ++k) s += ewv[k];
+        epart[(long)e * ep_stride + ep_off + t] = s;
     }
 }
 

@@ -48,11 +48,13 @@ def test_culling_only_for_wide_boxes(oracle_mod):
         assert (launches > 0) == on, (n, launches)
 
 
-def test_culled_forces_unwrapped_molecules(oracle_mod):
+@pytest.mark.parametrize("rsw", [4, 2])
+def test_culled_forces_unwrapped_molecules(oracle_mod, rsw):
     """1000 H2O x 2 beads; 30% of the molecules moved by random whole box
     vectors (components -2..2 periods).  SC forces, one-bead forces and
     energies against the oracle on the same coordinates at the parity
-    tolerances, and the two device runs against each other."""
+    tolerances, and the two device runs against each other; real-space
+    tiles of rsw waves."""
     from qtip4pf_water_amd import Water
     n_side, L, P = 10, 31.07, 2
     n = n_side ** 3
@@ -62,7 +64,7 @@ def test_culled_forces_unwrapped_molecules(oracle_mod):
     shift = rng.integers(-2, 3, size=(n, 3)) * np.asarray(box)
     shift[rng.random(n) > 0.3] = 0.0
     qs = (q.reshape(P, n, 3, 3) + shift[None, :, None, :]).reshape(-1)
-    say("unwrapped molecules", n, "mols P", P, "moved", int(np.count_nonzero(shift.any(axis=1))))
+    say("unwrapped molecules", n, "mols P", P, "waves", rsw, "moved", int(np.count_nonzero(shift.any(axis=1))))
     o = oracle_mod.Oracle(n, P, box)
     assert box[0] >= 2.5 * o.ewald()["rcut"]
     point, lst, _ = o.all_lists(qs)
@@ -74,6 +76,7 @@ def test_culled_forces_unwrapped_molecules(oracle_mod):
         F1r = np.zeros(P * o.n_dof)
         e1r = o.get_forces(x, F1r, point, lst, 0)
         with Water(n, P, box) as w:
+            w.set_tuning(rs_waves=rsw)
             F, f = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
             e, sc = w.force_eval_sc_nospline(x, F, o.charge, o.mass, point, lst, fphys=f)
             F1 = np.zeros(P * o.n_dof)

@@ -47,14 +47,14 @@ def test_every_configuration_matches_oracle(oracle_mod):
     n = 0
     with Water(216, P, box) as w:
         w.set_state(q, p)
-        for lanes, mk, reg, small, usplit in itertools.product(
-                (16, 64), (1, 0), (0, 1), (0, 1), (1, 2, 4)):
+        for rsw, lanes, mk, reg, small, usplit in itertools.product(
+                (4, 2), (16, 64), (1, 0), (0, 1), (0, 1), (1, 2, 4)):
             if not mk and (small or usplit > 1):
                 continue
-            say("tuning", lanes, mk, reg, small, usplit)
+            say("tuning", rsw, lanes, mk, reg, small, usplit)
             w.set_tuning(lanes=lanes, matrix_kspace=mk, rs_registers=reg,
                          kforce_16_waves=small if mk else -1,
-                         ksum_blocks_per_chunk=usplit if mk else 0)
+                         ksum_blocks_per_chunk=usplit if mk else 0, rs_waves=rsw)
             _, _, F, f, _ = w.get_state()
             s = w.scalars()
             assert s["status"] == 0
@@ -62,7 +62,7 @@ def test_every_configuration_matches_oracle(oracle_mod):
             assert rel_atom_err(F, Fr) < 1e-9     # odd beads: the FD Hessian term
             assert single(s["energy"], er) and single(s["sc_pot"], scr)
             n += 1
-    assert n == 2 * 2 * (2 * 3 + 1)
+    assert n == 2 * 2 * 2 * (2 * 3 + 1)
 
 
 def test_autotune_caches_and_runs():

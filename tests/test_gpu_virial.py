@@ -97,6 +97,28 @@ def test_virial_all_components_match_bruteforce(oracle_mod):
     assert abs(e - eb) <= 1e-10 * 23.0
 
 
+def test_virial_same_for_both_realspace_block_shapes(oracle_mod):
+    """k_realspace with two and with four waves per tile: the same pairs
+    summed in a different order, so energy, forces and virial agree to
+    rounding (1e-13 relative)."""
+    from qtip4pf_water_amd import Water
+    q0, box = _system(1)
+    o = oracle_mod.Oracle(64, 1, box)
+    point, lst, _ = o.all_lists(q0)
+    out = []
+    for rsw in (4, 2):
+        say("virial, real-space waves", rsw)
+        with Water(64, 1, box) as w:
+            w.set_tuning(rs_waves=rsw)
+            F = np.zeros(o.n_dof)
+            e, W = w.get_forces_virial(q0, F, o.charge, point, lst, 0)
+        out.append((e, W, F))
+    (e4, W4, F4), (e2, W2, F2) = out
+    assert abs(e4 - e2) <= 1e-13 * abs(e4)
+    assert np.max(np.abs(W4 - W2)) <= 1e-13 * np.max(np.abs(W4))
+    assert np.max(np.abs(F4 - F2)) <= 1e-13 * np.max(np.abs(F4))
+
+
 def test_resident_virial_matches_per_bead_calls(oracle_mod):
     from qtip4pf_water_amd import Water
     say("resident virial P=2")
