@@ -43,6 +43,26 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
         for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) fph[o + t] += f2[o + t];
         __syncthreads();
     }
+    // this thread's dofs t = threadIdx.x + SC_THREADS r: forces, masses and
+    // positions loaded once, before the energy sum (one memory latency)
+    constexpr int SC_RUN = 4;
+    const int nt = 3 * a.NAP;
+    const bool odd = b & 1;
+    double fv[SC_RUN], mv[SC_RUN], qv[SC_RUN];
+    const bool fits = nt <= SC_RUN * SC_THREADS;
+    if (fits) {
+#pragma unroll
+        for (int r = 0; r < SC_RUN; ++r) {
+            const int t = threadIdx.x + r * SC_THREADS;
+            if (t < nt) {
+                fv[r] = fph[o + t];
+                if (odd) {
+                    mv[r] = m_atom[t % a.NAP];
+                    qv[r] = q[o + t];
+                }
+            }
+        }
+    }
     if (threadIdx.x < 64) {
         const double v = eval_energy(epart, b, ep_stride, ep_n, eself);
         if (threadIdx.x == 0) {
@@ -50,11 +70,19 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
             Vout[b] = v;
         }
     }
-    __syncthreads();
-    const double V = s_V;
-    if ((b & 1) == 0) {
-        for (int t = threadIdx.x; t < 3 * a.NAP; t += SC_THREADS) Fsc[o + t] = 2.0 * fph[o + t] / 3.0;
+    if (!odd) {
+        if (fits) {
+#pragma unroll
+            for (int r = 0; r < SC_RUN; ++r) {
+                const int t = threadIdx.x + r * SC_THREADS;
+                if (t < nt) Fsc[o + t] = 2.0 * fv[r] / 3.0;
+            }
+        } else {
+            for (int t = threadIdx.x; t < nt; t += SC_THREADS) Fsc[o + t] = 2.0 * fph[o + t] / 3.0;
+        }
+        __syncthreads();
         if (threadIdx.x == 0) {
+            const double V = s_V;
             scal[b * 4 + 0] = 2.0 * V;
             scal[b * 4 + 1] = V;
             scal[b * 4 + 2] = 0.0;
@@ -62,11 +90,25 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
         }
         return;
     }
+    // |f|^2 / m and (f / m)^2 over the real atoms: per thread in dof order,
+    // then the block sum
     double fsq = 0.0, del = 0.0;
-    for (int d = 0; d < 3; ++d) {
-        for (int at = threadIdx.x; at < a.n_atoms; at += SC_THREADS) {
-            const double f = fph[o + d * a.NAP + at];
-            const double m = m_atom[at];
+    if (fits) {
+#pragma unroll
+        for (int r = 0; r < SC_RUN; ++r) {
+            const int t = threadIdx.x + r * SC_THREADS;
+            if (t < nt && t % a.NAP < a.n_atoms) {
+                const double f = fv[r], m = mv[r];
+                fsq += f * f / m;
+                const double fm = f / m;
+                del += fm * fm;
+            }
+        }
+    } else {
+        for (int t = threadIdx.x; t < nt; t += SC_THREADS) {
+            if (t % a.NAP >= a.n_atoms) continue;
+            const double f = fph[o + t];
+            const double m = m_atom[t % a.NAP];
             fsq += f * f / m;
             const double fm = f / m;
             del += fm * fm;
@@ -75,6 +117,7 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
     fsq = block_sum(fsq, sh);
     del = block_sum(del, sh);
     if (threadIdx.x == 0) {
+        const double V = s_V;
         fsq /= 3.0 * a.omega_psq;
         del /= (double)(a.n_atoms * a.P);
         del = 1.0 / sqrt(del);
@@ -90,10 +133,19 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
     const int k = b >> 1;
     double *Pp = qpm + (long)(2 * k) * 3 * a.NAP;
     double *Pm = qpm + (long)(2 * k + 1) * 3 * a.NAP;
-    for (int d = 0; d < 3; ++d) {
-        for (int at = threadIdx.x; at < a.NAP; at += SC_THREADS) {
-            const long t = (long)d * a.NAP + at;
-            const double dq = sc_scale * fph[o + t] / m_atom[at];
+    if (fits) {
+#pragma unroll
+        for (int r = 0; r < SC_RUN; ++r) {
+            const int t = threadIdx.x + r * SC_THREADS;
+            if (t < nt) {
+                const double dq = sc_scale * fv[r] / mv[r];
+                Pp[t] = qv[r] + dq;
+                Pm[t] = qv[r] - dq;
+            }
+        }
+    } else {
+        for (int t = threadIdx.x; t < nt; t += SC_THREADS) {
+            const double dq = sc_scale * fph[o + t] / m_atom[t % a.NAP];
             Pp[t] = q[o + t] + dq;
             Pm[t] = q[o + t] - dq;
         }
@@ -137,20 +189,51 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc2(SCArgs a, const double *__re
         }
         return;
     }
+    // runs of four dofs per thread: the loads of a run are issued together
+    const int nt = 3 * a.NAP;
     if (b & 1) {
         const int k = b >> 1;
         const double fscale = scal[b * 4 + 3];
         const long op = (long)(2 * k) * 3 * a.NAP, om = (long)(2 * k + 1) * 3 * a.NAP;
-        for (int t = t0; t < 3 * a.NAP; t += dt) {
-            double F = 4.0 * fph[o + t] / 3.0;
-            const double fp = f2 ? fpm[op + t] + f2[op + t] : fpm[op + t];
-            const double fm = f2 ? fpm[om + t] + f2[om + t] : fpm[om + t];
-            F -= fscale * (fp - fm);
-            Fsc[o + t] = F;
-            if (kick) p[o + t] += a.halfdt * F;
+        for (int tr = t0; tr < nt; tr += 4 * dt) {
+            double F[4], pv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int t = tr + r * dt;
+                if (t < nt) {
+                    F[r] = 4.0 * fph[o + t] / 3.0;
+                    const double fp = f2 ? fpm[op + t] + f2[op + t] : fpm[op + t];
+                    const double fm = f2 ? fpm[om + t] + f2[om + t] : fpm[om + t];
+                    F[r] -= fscale * (fp - fm);
+                    if (kick) pv[r] = p[o + t];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int t = tr + r * dt;
+                if (t < nt) {
+                    Fsc[o + t] = F[r];
+                    if (kick) p[o + t] = pv[r] + a.halfdt * F[r];
+                }
+            }
         }
     } else if (kick) {
-        for (int t = t0; t < 3 * a.NAP; t += dt) p[o + t] += a.halfdt * Fsc[o + t];
+        for (int tr = t0; tr < nt; tr += 4 * dt) {
+            double F[4], pv[4];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int t = tr + r * dt;
+                if (t < nt) {
+                    F[r] = Fsc[o + t];
+                    pv[r] = p[o + t];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int t = tr + r * dt;
+                if (t < nt) p[o + t] = pv[r] + a.halfdt * F[r];
+            }
+        }
     }
 }
 
