@@ -328,7 +328,8 @@ int qtp_get_tuning(const qtp_ctx *ctx, int cfg[7]);
  * its forces are recomputed under the new configuration. */
 int qtp_set_tuning(qtp_ctx *ctx, const int cfg[7]);
 /* Times one force evaluation of every bead of the resident state (the force
- * work of one step, both streams, HIP events, median of 5) under each
+ * work of one step captured as a graph, replayed between HIP events, median
+ * of 5 runs of replays) under each
  * candidate configuration, keeps the fastest and recomputes the state's
  * forces with it.  The choice is cached per system (molecules, beads, box,
  * Ewald and LJ parameters) for the process, so contexts of one shape are

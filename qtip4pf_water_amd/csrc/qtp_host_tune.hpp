@@ -6,8 +6,9 @@
 // as literals or in scalar registers and two or four waves per tile, 8- or
 // 16-wave k_kforce_mf blocks, k_ksum_mf blocks per site chunk) are timed on the context's own resident
 // state: one force evaluation of every bead (both Suzuki-Chin phases, or the
-// classical phase), with HIP events on the context's stream around the
-// whole fork/join of the two streams, median of several repetitions.  Every
+// classical phase) captured as a graph, as the resident steps run it, and
+// replayed between HIP events on the context's stream; median of several
+// runs of replays.  Every
 // candidate computes the same forces up to summation order.  The fastest is
 // kept and cached per system for the process.
 #pragma once
@@ -80,26 +81,57 @@ int tune_forces(qtp_ctx *ctx)
     return run_phase(ctx, ctx->q, 1, 0, 0, ctx->fph, nullptr);
 }
 
-// Median time (ms) of n force evaluations after two untimed ones.
+// Time (ms) of one force evaluation as the resident steps run it: the
+// evaluation captured as a graph and replayed in runs between two events
+// (up to TUNE_RUN replays a run, fewer for slow evaluations: a run lasts
+// about TUNE_RUN_MS); the median over n runs after one untimed run.
+constexpr int TUNE_RUN = 8;
+constexpr float TUNE_RUN_MS = 2.0f;
 int tune_time(qtp_ctx *ctx, int n, float *ms)
 {
-    hipEvent_t a, b;
-    HIPCHK(hipEventCreate(&a));
-    HIPCHK(hipEventCreate(&b));
-    int st = QTP_OK;
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int st = tune_forces(ctx);
+    const hipError_t ec = hipStreamEndCapture(ctx->stream, &g);
+    if (st != QTP_OK) {
+        if (g) hipGraphDestroy(g);
+        return st;
+    }
+    if (ec != hipSuccess) {
+        ctx->err = std::string("autotune capture: ") + hipGetErrorString(ec);
+        return QTP_ERR_HIP;
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        ctx->err = std::string("autotune instantiate: ") + hipGetErrorString(ei);
+        return QTP_ERR_HIP;
+    }
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) st = QTP_ERR_HIP;
     std::vector<float> t;
-    for (int i = 0; i < n + 2 && st == QTP_OK; ++i) {
+    int run = 1;  // the untimed first run is one replay; it sizes the others
+    for (int i = 0; i < n + 1 && st == QTP_OK; ++i) {
         if (hipEventRecord(a, ctx->stream) != hipSuccess) st = QTP_ERR_HIP;
-        if (st == QTP_OK) st = tune_forces(ctx);
+        for (int r = 0; r < run && st == QTP_OK; ++r)
+            if (hipGraphLaunch(x, ctx->stream) != hipSuccess) st = QTP_ERR_HIP;
         if (st == QTP_OK && hipEventRecord(b, ctx->stream) != hipSuccess) st = QTP_ERR_HIP;
         if (st == QTP_OK && hipEventSynchronize(b) != hipSuccess) st = QTP_ERR_HIP;
         float e = 0.f;
         if (st == QTP_OK && hipEventElapsedTime(&e, a, b) != hipSuccess) st = QTP_ERR_HIP;
-        if (i >= 2) t.push_back(e);
+        if (i == 0)
+            run = std::max(1, std::min(TUNE_RUN, (int)(TUNE_RUN_MS / std::max(e, 1e-3f))));
+        else
+            t.push_back(e / run);
     }
-    hipEventDestroy(a);
-    hipEventDestroy(b);
-    if (st != QTP_OK) return st;
+    if (a) hipEventDestroy(a);
+    if (b) hipEventDestroy(b);
+    hipGraphExecDestroy(x);
+    if (st != QTP_OK) {
+        ctx->err = "autotune: HIP error while timing";
+        return st;
+    }
     std::sort(t.begin(), t.end());
     *ms = t[t.size() / 2];
     return QTP_OK;
