@@ -1610,7 +1610,7 @@ int qtp_set_nn(qtp_ctx *ctx, qtp_nn *nn)
 
 // ---- launch configuration and autotuner -----------------------------------------------
 
-int qtp_get_tuning(const qtp_ctx *ctx, int cfg[7])
+int qtp_get_tuning(const qtp_ctx *ctx, int cfg[8])
 {
     if (!ctx || !cfg) return QTP_ERR_ARG;
     const TuneCfg c = current_tuning(ctx);
@@ -1621,10 +1621,11 @@ int qtp_get_tuning(const qtp_ctx *ctx, int cfg[7])
     cfg[4] = c.usplit;
     cfg[5] = c.fused;
     cfg[6] = c.rsw;
+    cfg[7] = c.rsplit;
     return QTP_OK;
 }
 
-int qtp_set_tuning(qtp_ctx *ctx, const int cfg[7])
+int qtp_set_tuning(qtp_ctx *ctx, const int cfg[8])
 {
     if (!ctx || !cfg) return QTP_ERR_ARG;
     TuneCfg c;
@@ -1635,10 +1636,12 @@ int qtp_set_tuning(qtp_ctx *ctx, const int cfg[7])
     c.usplit = cfg[4];
     c.fused = cfg[5];
     c.rsw = cfg[6];
+    c.rsplit = cfg[7];
     if (!tuning_valid(ctx, c)) {
         ctx->err = "tuning: lanes 16 or 64, matrix k-space 0/1 (1 only for k_max <= 23), "
                    "registers and 16-wave blocks -1/0/1, blocks per chunk 0-4, fused "
-                   "propagation -1/0/1, real-space waves 2 or 4";
+                   "propagation -1/0/1, real-space waves 2 or 4, real-space eighths on the "
+                   "second stream 0-4";
         return QTP_ERR_ARG;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1647,7 +1650,7 @@ int qtp_set_tuning(qtp_ctx *ctx, const int cfg[7])
     return tune_refresh(ctx);
 }
 
-int qtp_autotune(qtp_ctx *ctx, int flags, double report[9])
+int qtp_autotune(qtp_ctx *ctx, int flags, double report[10])
 {
     if (!ctx) return QTP_ERR_ARG;
     if (!ctx->state_set) {
@@ -1688,6 +1691,19 @@ int qtp_autotune(qtp_ctx *ctx, int flags, double report[9])
                 best = c;
             }
         }
+        // then the real-space share of the second stream, on the best so far
+        for (int sp = 1; sp <= 4 && st == QTP_OK; ++sp) {
+            TuneCfg c = best;
+            c.rsplit = sp;
+            apply_tuning(ctx, c);
+            float ms = 0.f;
+            st = tune_time(ctx, 5, &ms);
+            ++n_timed;
+            if (st == QTP_OK && ms < best_ms * (1.0f - TUNE_MARGIN)) {
+                best_ms = ms;
+                best = c;
+            }
+        }
         if (st == QTP_OK) {
             std::lock_guard<std::mutex> g(g_tune_mu);
             g_tune_cache[key] = best;
@@ -1711,6 +1727,7 @@ int qtp_autotune(qtp_ctx *ctx, int flags, double report[9])
         report[6] = best.usplit;
         report[7] = n_timed;
         report[8] = best.rsw;
+        report[9] = best.rsplit;
     }
     return QTP_OK;
 }

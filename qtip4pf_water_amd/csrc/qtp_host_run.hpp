@@ -34,10 +34,55 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     double *vp = vout ? ctx->vpart : nullptr;
     // The k-space sums (ksum, sreduce) and the LJ/intramolecular kernel do not
     // depend on the real-space kernel: they run on the second stream, joined
-    // before kforce.  With per-kernel timing on, everything stays on one
-    // stream so each launch is timed alone.
+    // before kforce, with a share of the real-space tiles after them.  With
+    // per-kernel timing on, everything stays on one stream so each launch is
+    // timed alone.
     const bool fork = !ctx->timing;
     hipStream_t sk = fork ? ctx->s2 : s;
+    // real-space tiles: the first nt_a on this stream, the other nt_b (a
+    // tunable share, in eighths) on the second stream after the LJ kernel,
+    // where its kernels leave the chip room beside the first part
+    const int nt_b = fork ? (int)((long)ctx->ntile * ctx->rs_split / 8) : 0;
+    const int nt_a = ctx->ntile - nt_b;
+    // coefficients in SGPRs (more waves per SIMD) for large grids, where
+    // that measured faster (4096 x 32: 3.42 against 3.60 ms); literals
+    // for small ones (216 x 32: no gain, and the second stream's kernels
+    // lost more than the launch gained)
+    const bool carg = ctx->rs_carg >= 0 ? ctx->rs_carg == 1
+                                        : (long)E * ctx->ntile >= RS_CARG_BLOCKS;
+    const RsCoef cf = rs_coef();
+    auto launch_rs = [&](hipStream_t st, int t0, int nt) {
+        TimeScope ts(ctx, "realspace");
+#define QTP_RS_LAUNCH_W(VIR, CARG, CULL, W)                                                        \
+    k_realspace<VIR, CARG, CULL, W><<<E * nt, 64 * W, 0, st>>>(                                    \
+        pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles, ctx->ntile,     \
+        t0, nt, ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real, vp, cf, \
+        ctx->rs_sites, ctx->rs_box, ctx->rs_far)
+#define QTP_RS_LAUNCH(VIR, CARG, CULL)                                                             \
+    do {                                                                                           \
+        if (ctx->rs_waves == 2) QTP_RS_LAUNCH_W(VIR, CARG, CULL, 2);                              \
+        else QTP_RS_LAUNCH_W(VIR, CARG, CULL, 4);                                                  \
+    } while (0)
+        if (ctx->rs_cull) {
+            if (vout) QTP_RS_LAUNCH(true, true, true);
+            else QTP_RS_LAUNCH(false, true, true);
+        } else if (vout) {
+            if (carg) QTP_RS_LAUNCH(true, true, false);
+            else QTP_RS_LAUNCH(true, false, false);
+        } else {
+            if (carg) QTP_RS_LAUNCH(false, true, false);
+            else QTP_RS_LAUNCH(false, false, false);
+        }
+#undef QTP_RS_LAUNCH
+#undef QTP_RS_LAUNCH_W
+    };
+    auto launch_sites = [&](hipStream_t st) {
+        TimeScope ts(ctx, "sites");
+        k_sites<<<E * ctx->nblk, 64, 0, st>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
+                                              ctx->nblk, g, h, ctx->rs_sites, ctx->rs_box);
+    };
+    // both real-space parts read the culling form's site tables
+    if (ctx->rs_cull && nt_b > 0) launch_sites(s);
     if (fork) {
         HIPCHK(hipEventRecord(ctx->ev_fork, s));
         HIPCHK(hipStreamWaitEvent(sk, ctx->ev_fork, 0));
@@ -94,44 +139,10 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
                 pos, N3, ctx->n_atoms, ctx->NAP, ctx->nblk, ctx->box, ctx->spl, ctx->spl_coef,
                 ctx->fspl, ctx->epart, ctx->ep_stride, ctx->ep_spl, vp);
     }
+    if (nt_b > 0) launch_rs(sk, nt_a, nt_b);
     if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
-    if (ctx->rs_cull) {
-        TimeScope ts(ctx, "sites");
-        k_sites<<<E * ctx->nblk, 64, 0, s>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
-                                             ctx->nblk, g, h, ctx->rs_sites, ctx->rs_box);
-    }
-    {
-        TimeScope ts(ctx, "realspace");
-        // coefficients in SGPRs (more waves per SIMD) for large grids, where
-        // that measured faster (4096 x 32: 3.42 against 3.60 ms); literals
-        // for small ones (216 x 32: no gain, and the second stream's kernels
-        // lost more than the launch gained)
-        const bool carg = ctx->rs_carg >= 0 ? ctx->rs_carg == 1
-                                            : (long)E * ctx->ntile >= RS_CARG_BLOCKS;
-        const RsCoef cf = rs_coef();
-#define QTP_RS_LAUNCH_W(VIR, CARG, CULL, W)                                                        \
-    k_realspace<VIR, CARG, CULL, W><<<E * ctx->ntile, 64 * W, 0, s>>>(                             \
-        pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, ctx->nblk, g, h, ctx->tiles, ctx->ntile,     \
-        ctx->box, ctx->ew, ctx->rpart, ctx->epart, ctx->ep_stride, ctx->ep_real, vp, cf,         \
-        ctx->rs_sites, ctx->rs_box, ctx->rs_far)
-#define QTP_RS_LAUNCH(VIR, CARG, CULL)                                                             \
-    do {                                                                                           \
-        if (ctx->rs_waves == 2) QTP_RS_LAUNCH_W(VIR, CARG, CULL, 2);                              \
-        else QTP_RS_LAUNCH_W(VIR, CARG, CULL, 4);                                                  \
-    } while (0)
-        if (ctx->rs_cull) {
-            if (vout) QTP_RS_LAUNCH(true, true, true);
-            else QTP_RS_LAUNCH(false, true, true);
-        } else if (vout) {
-            if (carg) QTP_RS_LAUNCH(true, true, false);
-            else QTP_RS_LAUNCH(true, false, false);
-        } else {
-            if (carg) QTP_RS_LAUNCH(false, true, false);
-            else QTP_RS_LAUNCH(false, false, false);
-        }
-#undef QTP_RS_LAUNCH
-#undef QTP_RS_LAUNCH_W
-    }
+    if (ctx->rs_cull && nt_b == 0) launch_sites(s);
+    launch_rs(s, 0, nt_a);
     if (fork) HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
     {
         TimeScope ts(ctx, "kforce");

@@ -3,8 +3,9 @@
 //
 // The tunable launch choices of a context (lanes per molecule of
 // k_ljintra, matrix-core or vector k-space kernels, k_realspace coefficients
-// as literals or in scalar registers and two or four waves per tile, 8- or
-// 16-wave k_kforce_mf blocks, k_ksum_mf blocks per site chunk) are timed on the context's own resident
+// as literals or in scalar registers, two or four waves per tile and the
+// share of its tiles run on the second stream, 8- or 16-wave k_kforce_mf
+// blocks, k_ksum_mf blocks per site chunk) are timed on the context's own resident
 // state: one force evaluation of every bead (both Suzuki-Chin phases, or the
 // classical phase) captured as a graph, as the resident steps run it, and
 // replayed between HIP events on the context's stream; median of several
@@ -20,7 +21,7 @@ namespace {
 constexpr float TUNE_MARGIN = 0.02f;
 
 struct TuneCfg {
-    int lpm = 16, kmf = 0, carg = -1, small = -1, usplit = 0, fused = -1, rsw = RS_WAVES;
+    int lpm = 16, kmf = 0, carg = -1, small = -1, usplit = 0, fused = -1, rsw = RS_WAVES, rsplit = 0;
 };
 
 TuneCfg current_tuning(const qtp_ctx *ctx)
@@ -33,6 +34,7 @@ TuneCfg current_tuning(const qtp_ctx *ctx)
     c.usplit = ctx->ks_usplit;
     c.fused = ctx->prop_fused;
     c.rsw = ctx->rs_waves;
+    c.rsplit = ctx->rs_split;
     return c;
 }
 
@@ -45,6 +47,7 @@ bool tuning_valid(const qtp_ctx *ctx, const TuneCfg &c)
     if (c.small == 1 && ctx->kforce_mf_lds_small > 64 * 1024) return false;
     if (c.fused < -1 || c.fused > 1) return false;
     if (c.rsw != 2 && c.rsw != 4) return false;
+    if (c.rsplit < 0 || c.rsplit > 4) return false;
     return c.usplit >= 0 && c.usplit <= 4;
 }
 
@@ -58,6 +61,7 @@ void apply_tuning(qtp_ctx *ctx, const TuneCfg &c)
     ctx->ks_usplit = c.usplit;
     ctx->prop_fused = c.fused;
     ctx->rs_waves = c.rsw;
+    ctx->rs_split = c.rsplit;
     invalidate_graphs(ctx);
 }
 

@@ -19,7 +19,9 @@ namespace qtp {
 // the result is reproducible.
 // Partial site forces go to slot J (block I's sites) and slot I (block J's
 // sites): every (site, slot) has exactly one writer.
-// grid: E * ntile, 64 W threads.
+// grid: E * nt, 64 W threads: tiles t0 ... t0 + nt - 1 of the ntile of the
+// list (a launch may take a part of the list; the parts write disjoint
+// slots and partials).
 // VIR: also accumulate the pair virial r_ij (x) f_ij (6 components) into
 // vpart[(e*6 + c)*ep_stride + ep_off + tile].
 // CARG: the special functions' coefficients come from the argument cf (SGPRs)
@@ -98,6 +100,7 @@ __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__
                                                       const double *__restrict__ charge, int n_atoms,
                                                       int NAP, int nblk, double gamma, double half1mg,
                                                       const int2 *__restrict__ tiles, int ntile,
+                                                      int t0, int nt,
                                                       Box box, Ewald ew,
                                                       double *__restrict__ rpart,
                                                       double *__restrict__ epart, int ep_stride,
@@ -110,7 +113,7 @@ __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__
     static_assert(W == 2 || W == 4, "two or four waves per tile");
     constexpr int CPW = 64 / W;  // columns per wave
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
-    const int e = bid / ntile, t = bid - e * ntile;
+    const int e = bid / nt, t = t0 + (bid - e * nt);
     const int I = tiles[t].x, J = tiles[t].y;
     const int l = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

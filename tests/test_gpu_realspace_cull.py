@@ -48,13 +48,13 @@ def test_culling_only_for_wide_boxes(oracle_mod):
         assert (launches > 0) == on, (n, launches)
 
 
-@pytest.mark.parametrize("rsw", [4, 2])
-def test_culled_forces_unwrapped_molecules(oracle_mod, rsw):
+@pytest.mark.parametrize("rsw,split", [(4, 0), (2, 3)])
+def test_culled_forces_unwrapped_molecules(oracle_mod, rsw, split):
     """1000 H2O x 2 beads; 30% of the molecules moved by random whole box
     vectors (components -2..2 periods).  SC forces, one-bead forces and
     energies against the oracle on the same coordinates at the parity
     tolerances, and the two device runs against each other; real-space
-    tiles of rsw waves."""
+    tiles of rsw waves, split eighths of them on the second stream."""
     from qtip4pf_water_amd import Water
     n_side, L, P = 10, 31.07, 2
     n = n_side ** 3
@@ -76,7 +76,7 @@ def test_culled_forces_unwrapped_molecules(oracle_mod, rsw):
         F1r = np.zeros(P * o.n_dof)
         e1r = o.get_forces(x, F1r, point, lst, 0)
         with Water(n, P, box) as w:
-            w.set_tuning(rs_waves=rsw)
+            w.set_tuning(rs_waves=rsw, rs_split=split)
             F, f = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
             e, sc = w.force_eval_sc_nospline(x, F, o.charge, o.mass, point, lst, fphys=f)
             F1 = np.zeros(P * o.n_dof)

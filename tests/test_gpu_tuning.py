@@ -89,6 +89,35 @@ def test_autotune_caches_and_runs():
         assert np.array_equal(w.get_state()[0], q1)
 
 
+@pytest.mark.parametrize("rsw", [4, 2])
+def test_realspace_split_is_bitwise_neutral(rsw):
+    """Real-space tiles shared between the two streams (rs_split eighths on
+    the second): every tile is computed the same way into its own slots and
+    partials, so forces, energies and a resident run agree bit for bit with
+    the unsplit launch."""
+    from qtip4pf_water_amd import Water
+    P = 4
+    q, p, box = _state(P, 81)
+    out = []
+    for sp in range(5):
+        say("real-space split", rsw, sp)
+        with Water(216, P, box) as w:
+            w.set_thermostat("pile_l", 100.0)
+            w.set_seed(9)
+            w.set_tuning(rs_waves=rsw, rs_split=sp)
+            w.set_state(q, p)
+            _, _, F, f, _ = w.get_state()
+            e0 = w.scalars()["energy"]
+            w.run(10, is_nvt=True)
+            s = w.scalars()
+            assert s["status"] == 0
+            out.append((F, f, e0, w.get_state()[0], s["energy"]))
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0]) and np.array_equal(o[1], out[0][1])
+        assert o[2] == out[0][2] and o[4] == out[0][4]
+        assert np.array_equal(o[3], out[0][3])
+
+
 def test_vector_only_system_rejects_matrix_kspace():
     """k_max 24: 2 K = 50 z-factor entries, beyond the matrix-core kernels'
     twelve k-steps."""
