@@ -312,7 +312,7 @@ int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *t
                      int *n_samples);
 
 /* ---- launch configuration and autotuner ------------------------------------
- * No counterpart in the reference (one CPU code path).  cfg[8] = {lanes per
+ * No counterpart in the reference (one CPU code path).  cfg[9] = {lanes per
  * molecule of the LJ/intramolecular kernel (16 or 64), k-space on the matrix
  * cores (1; only for k_max <= 23) or the vector kernels (0), real-space
  * coefficients in scalar registers (1), as literals (0) or by grid size (-1),
@@ -322,22 +322,23 @@ int qtp_run_observed(qtp_ctx *ctx, int n_steps, int is_nvt, int every, double *t
  * it: for systems of 256 or more 64-dof tiles (-1), always (1), never (0;
  * the results are the same bit for bit), waves per real-space tile block (2
  * or 4), eighths of the real-space tiles launched on the second stream after
- * the LJ/intramolecular kernel (0-4)}.
+ * the LJ/intramolecular kernel (0-4), site chunks per structure-factor block
+ * of the matrix-core path (1, 2, 4, 8, 16; 0 by launch size)}.
  * Every configuration gives the same forces up to summation order. */
-int qtp_get_tuning(const qtp_ctx *ctx, int cfg[8]);
+int qtp_get_tuning(const qtp_ctx *ctx, int cfg[9]);
 /* Sets cfg (QTP_ERR_ARG for values the context cannot run); with a state set,
  * its forces are recomputed under the new configuration. */
-int qtp_set_tuning(qtp_ctx *ctx, const int cfg[8]);
+int qtp_set_tuning(qtp_ctx *ctx, const int cfg[9]);
 /* Times one force evaluation of every bead of the resident state (the force
  * work of one step captured as a graph, replayed between HIP events, median
  * of 5 runs of replays) under each
  * candidate configuration, keeps the fastest and recomputes the state's
  * forces with it.  The choice is cached per system (molecules, beads, box,
  * Ewald and LJ parameters) for the process, so contexts of one shape are
- * timed once; flags & 1 times again.  report[10] (may be NULL): best ms,
+ * timed once; flags & 1 times again.  report[11] (may be NULL): best ms,
  * default ms, the chosen cfg[0..4], candidates timed (0: from the cache),
- * the chosen cfg[6] and cfg[7]. */
-int qtp_autotune(qtp_ctx *ctx, int flags, double report[10]);
+ * the chosen cfg[6], cfg[7] and cfg[8]. */
+int qtp_autotune(qtp_ctx *ctx, int flags, double report[11]);
 
 /* ---- Verlet-list build ------------------------------------------------------
  * No counterpart in the reference, whose neighbour.cpp:12-62 tests every pair

@@ -439,7 +439,8 @@ class Water:
 
     # ------------------------------ launch configuration and autotuner
     TUNING_KEYS = ("lanes", "matrix_kspace", "rs_registers", "kforce_16_waves",
-                   "ksum_blocks_per_chunk", "fused_propagation", "rs_waves", "rs_split")
+                   "ksum_blocks_per_chunk", "fused_propagation", "rs_waves", "rs_split",
+                   "ksum_chunks_per_block")
 
     def tuning(self) -> dict:
         out = np.zeros(len(self.TUNING_KEYS), np.int32)
@@ -457,11 +458,12 @@ class Water:
     def autotune(self, retime: bool = False) -> dict:
         """Time every candidate launch configuration on the resident state's
         force evaluation and keep the fastest (cached per system shape)."""
-        r = np.zeros(10)
+        r = np.zeros(11)
         self._chk(self._L.qtp_autotune(self._h, 1 if retime else 0, ptr(r)))
         chosen = dict(zip(self.TUNING_KEYS[:5], (int(v) for v in r[2:7])))
         chosen["rs_waves"] = int(r[8])
         chosen["rs_split"] = int(r[9])
+        chosen["ksum_chunks_per_block"] = int(r[10])
         return {"best_ms": float(r[0]), "default_ms": float(r[1]), "chosen": chosen,
                 "timed": int(r[7])}
 

@@ -1610,7 +1610,7 @@ int qtp_set_nn(qtp_ctx *ctx, qtp_nn *nn)
 
 // ---- launch configuration and autotuner -----------------------------------------------
 
-int qtp_get_tuning(const qtp_ctx *ctx, int cfg[8])
+int qtp_get_tuning(const qtp_ctx *ctx, int cfg[9])
 {
     if (!ctx || !cfg) return QTP_ERR_ARG;
     const TuneCfg c = current_tuning(ctx);
@@ -1622,10 +1622,11 @@ int qtp_get_tuning(const qtp_ctx *ctx, int cfg[8])
     cfg[5] = c.fused;
     cfg[6] = c.rsw;
     cfg[7] = c.rsplit;
+    cfg[8] = c.cpb;
     return QTP_OK;
 }
 
-int qtp_set_tuning(qtp_ctx *ctx, const int cfg[8])
+int qtp_set_tuning(qtp_ctx *ctx, const int cfg[9])
 {
     if (!ctx || !cfg) return QTP_ERR_ARG;
     TuneCfg c;
@@ -1637,11 +1638,12 @@ int qtp_set_tuning(qtp_ctx *ctx, const int cfg[8])
     c.fused = cfg[5];
     c.rsw = cfg[6];
     c.rsplit = cfg[7];
+    c.cpb = cfg[8];
     if (!tuning_valid(ctx, c)) {
         ctx->err = "tuning: lanes 16 or 64, matrix k-space 0/1 (1 only for k_max <= 23), "
                    "registers and 16-wave blocks -1/0/1, blocks per chunk 0-4, fused "
                    "propagation -1/0/1, real-space waves 2 or 4, real-space eighths on the "
-                   "second stream 0-4";
+                   "second stream 0-4, structure-factor chunks per block 0/1/2/4/8/16";
         return QTP_ERR_ARG;
     }
     HIPCHK(hipStreamSynchronize(ctx->stream));
@@ -1650,7 +1652,7 @@ int qtp_set_tuning(qtp_ctx *ctx, const int cfg[8])
     return tune_refresh(ctx);
 }
 
-int qtp_autotune(qtp_ctx *ctx, int flags, double report[10])
+int qtp_autotune(qtp_ctx *ctx, int flags, double report[11])
 {
     if (!ctx) return QTP_ERR_ARG;
     if (!ctx->state_set) {
@@ -1691,10 +1693,13 @@ int qtp_autotune(qtp_ctx *ctx, int flags, double report[10])
                 best = c;
             }
         }
-        // then the real-space share of the second stream, on the best so far
-        for (int sp = 1; sp <= 4 && st == QTP_OK; ++sp) {
+        // then the real-space share of the second stream and the structure-
+        // factor chunks per block, one at a time on the best so far
+        for (int sp = 1; sp <= 4 + 5 && st == QTP_OK; ++sp) {
+            if (sp > 4 && !best.kmf) break;
             TuneCfg c = best;
-            c.rsplit = sp;
+            if (sp <= 4) c.rsplit = sp;
+            else c.cpb = 1 << (sp - 5);
             apply_tuning(ctx, c);
             float ms = 0.f;
             st = tune_time(ctx, 5, &ms);
@@ -1728,6 +1733,7 @@ int qtp_autotune(qtp_ctx *ctx, int flags, double report[10])
         report[7] = n_timed;
         report[8] = best.rsw;
         report[9] = best.rsplit;
+        report[10] = best.cpb;
     }
     return QTP_OK;
 }

@@ -7,6 +7,28 @@
 // ---------------------------------------------------------------------------
 // force phase: E single-bead evaluations (forces.cpp:468-630)
 // ---------------------------------------------------------------------------
+// Launch shape of k_ksum_mf for E evaluations: blocks per chunk group (a
+// multiple of the count that keeps KS_MAXU units per wave; more when the
+// launch has few blocks, or as tuned), chunks per block (as tuned, else the
+// most of 1, 2, 4, 8 that still leaves 2048 blocks), chunk groups.
+static void ksum_mf_shape(const qtp_ctx *ctx, int E, int *usplit, int *cpb, int *ncg)
+{
+    const int per = KS_MAXU * KS_WAVES;
+    const int req = std::max(1, (ctx->ks_nunit + per - 1) / per);
+    const int umax = std::min(4, std::max(1, ctx->ks_nunit / (req * KS_WAVES)));
+    const int mult = ctx->ks_usplit > 0 ? std::min(ctx->ks_usplit, umax)
+                                        : ((long)E * ctx->nac * req < 128 ? umax : 1);
+    *usplit = req * mult;
+    int c = ctx->ks_cpb;
+    if (c <= 0) {
+        c = 1;
+        for (int t = 2; t <= 8; t *= 2)
+            if ((long)E * ((ctx->nac + t - 1) / t) * *usplit >= 2048) c = t;
+    }
+    *cpb = c;
+    *ncg = (ctx->nac + c - 1) / c;
+}
+
 // energy partials summed per evaluation: the spline blocks only when used
 static int ep_live(const qtp_ctx *ctx) { return ctx->spl_on ? ctx->ep_stride : ctx->ep_spl; }
 
@@ -90,13 +112,11 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     {
         TimeScope ts(ctx, "ksum");
         if (ctx->kf_mf) {
-            // few chunks in the launch: several blocks per chunk
-            const int umax = std::min(4, std::max(1, ctx->ks_nunit / KS_WAVES));
-            const int usplit = ctx->ks_usplit > 0 ? std::min(ctx->ks_usplit, umax)
-                                                  : (E * ctx->nac < 128 ? umax : 1);
-            k_ksum_mf<<<E * ctx->nac * usplit, 64 * KS_WAVES, ctx->ksum_mf_lds, sk>>>(
+            int usplit, cpb, ncg;
+            ksum_mf_shape(ctx, E, &usplit, &cpb, &ncg);
+            k_ksum_mf<<<E * ncg * usplit, 64 * KS_WAVES, ctx->ksum_mf_lds, sk>>>(
                 pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP, g, h, ctx->ew, ctx->ks_col,
-                ctx->ks_ncol4, ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, usplit,
+                ctx->ks_ncol4, ctx->ks_unit, ctx->ks_nunit, ctx->G, ctx->AC, ctx->nac, cpb, usplit,
                 ctx->spart);
         } else
             k_ksum<<<E * ctx->ngc * ctx->nac, 256, ctx->ksum_lds, sk>>>(
@@ -105,7 +125,13 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     }
     {
         TimeScope ts(ctx, "sreduce");
-        k_sreduce<<<E * ctx->ngr, 256, 0, sk>>>(ctx->spart, ctx->nac, ctx->G, ctx->groups, ctx->gcf,
+        // partials per group: one per chunk (vector k_ksum) or chunk group
+        int nparts = ctx->nac;
+        if (ctx->kf_mf) {
+            int usplit, cpb;
+            ksum_mf_shape(ctx, E, &usplit, &cpb, &nparts);
+        }
+        k_sreduce<<<E * ctx->ngr, 256, 0, sk>>>(ctx->spart, nparts, ctx->G, ctx->groups, ctx->gcf,
                                                  ctx->sprime, ctx->epart, ctx->ep_stride, ctx->ep_k,
                                                  ctx->grk, 0.25 / ctx->ew.kappa2, vp,
                                                  ctx->kf_mf ? ctx->wm : nullptr,

@@ -5,7 +5,7 @@
 // k_ljintra, matrix-core or vector k-space kernels, k_realspace coefficients
 // as literals or in scalar registers, two or four waves per tile and the
 // share of its tiles run on the second stream, 8- or 16-wave k_kforce_mf
-// blocks, k_ksum_mf blocks per site chunk) are timed on the context's own resident
+// blocks, k_ksum_mf blocks per site chunk and site chunks per block) are timed on the context's own resident
 // state: one force evaluation of every bead (both Suzuki-Chin phases, or the
 // classical phase) captured as a graph, as the resident steps run it, and
 // replayed between HIP events on the context's stream; median of several
@@ -22,6 +22,7 @@ constexpr float TUNE_MARGIN = 0.02f;
 
 struct TuneCfg {
     int lpm = 16, kmf = 0, carg = -1, small = -1, usplit = 0, fused = -1, rsw = RS_WAVES, rsplit = 0;
+    int cpb = 0;
 };
 
 TuneCfg current_tuning(const qtp_ctx *ctx)
@@ -35,6 +36,7 @@ TuneCfg current_tuning(const qtp_ctx *ctx)
     c.fused = ctx->prop_fused;
     c.rsw = ctx->rs_waves;
     c.rsplit = ctx->rs_split;
+    c.cpb = ctx->ks_cpb;
     return c;
 }
 
@@ -48,6 +50,7 @@ bool tuning_valid(const qtp_ctx *ctx, const TuneCfg &c)
     if (c.fused < -1 || c.fused > 1) return false;
     if (c.rsw != 2 && c.rsw != 4) return false;
     if (c.rsplit < 0 || c.rsplit > 4) return false;
+    if (c.cpb != 0 && c.cpb != 1 && c.cpb != 2 && c.cpb != 4 && c.cpb != 8 && c.cpb != 16) return false;
     return c.usplit >= 0 && c.usplit <= 4;
 }
 
@@ -62,6 +65,7 @@ void apply_tuning(qtp_ctx *ctx, const TuneCfg &c)
     ctx->prop_fused = c.fused;
     ctx->rs_waves = c.rsw;
     ctx->rs_split = c.rsplit;
+    ctx->ks_cpb = c.cpb;
     invalidate_graphs(ctx);
 }
 

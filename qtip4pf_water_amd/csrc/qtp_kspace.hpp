@@ -92,15 +92,21 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 // eight sums of each group (A = X Y, B = X Y*: Ar = F0 - F1, Ai = F2 + F3,
 // Br = F0 + F1, Bi = F3 - F2).  A unit = 16 rows (four columns in
 // k_kforce_mf's sorted order) x 16 values of n (n-tile 0 or 1); each lane
-// forms its A operand F_p with one product of two table entries.  Output per
-// group and chunk (same place as k_ksum's sums): Re/Im Z times F0..F3, in
-// that order.
+// forms its A operand F_p with one product of two table entries.
+// A block takes cpb consecutive chunks (a chunk group) and a slice of the
+// units: it builds each chunk's factor tables in turn and keeps its units'
+// accumulators in registers across the chunks (at most KS_MAXU units per
+// wave), so one partial per group and chunk group is written: cpb times
+// fewer partials to write here and to read in k_sreduce.
+// Output per group and chunk group: Re/Im Z times F0..F3, in that order.
 // kcol[c] = (kx | ky << 16, first group, first kz, last kz) per sorted
 // column (padding columns: first kz > last kz).  usplit blocks share a chunk
-// (each builds its tables and takes 1/usplit of the units) when a launch has
-// few chunks.
-// grid: E * nac * usplit, 64 * KS_WAVES threads, ksum_mf_lds_bytes(K, AC, ncol4, nunit).
+// group, each taking 1/usplit of the units (the host makes the slices fit
+// KS_MAXU units per wave).
+// grid: E * ncg * usplit (ncg = ceil(nac / cpb)), 64 * KS_WAVES threads,
+// ksum_mf_lds_bytes(K, AC, ncol4, nunit).
 constexpr int KS_WAVES = 8;
+constexpr int KS_MAXU = 4;  // units per wave of one block (accumulator registers)
 // LDS rows of AC + 2 entries: the B operands of a quarter-wave (16 rows n at
 // one site) and the A factors of a tile's columns then fall in different banks.
 // The column and unit tables are staged in LDS behind the factor tables.
@@ -116,7 +122,7 @@ inline size_t ksum_mf_lds_bytes(int K, int AC, int ncol4, int nunit)
 __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     const double *__restrict__ pos, long pstride, const double *__restrict__ charge, int n_atoms,
     int NAP, double gamma, double half1mg, Ewald ew, const int4 *__restrict__ kcol, int ncol4,
-    const int2 *__restrict__ kunit, int nunit, int G, int AC, int nac, int usplit,
+    const int2 *__restrict__ kunit, int nunit, int G, int AC, int nac, int cpb, int usplit,
     double *__restrict__ spart)
 {
     extern __shared__ double smem[];
@@ -127,43 +133,15 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     double *Zt = smem + 4 * K * RS;  // [ZN][RS]
     int4 *sc = (int4 *)(Zt + ZN * RS);  // [ncol4] column table
     int2 *su = (int2 *)(sc + ncol4);    // [nunit] unit table
-    const int per_e = nac * usplit;
+    const int ncg = (nac + cpb - 1) / cpb;
+    const int per_e = ncg * usplit;
     const int e = blockIdx.x / per_e, r = blockIdx.x - e * per_e;
-    const int ac = r / usplit, us = r - ac * usplit;
+    const int cg = r / usplit, us = r - cg * usplit;
     const int u0 = us * nunit / usplit, u1 = (us + 1) * nunit / usplit;
+    const int ac0 = cg * cpb, ac1 = min(nac, ac0 + cpb);
     const double *X = pos + (long)e * pstride;
     for (int t = threadIdx.x; t < ncol4; t += blockDim.x) sc[t] = kcol[t];
     for (int t = threadIdx.x; t < nunit; t += blockDim.x) su[t] = kunit[t];
-    for (int t = threadIdx.x; t < 3 * AC; t += blockDim.x) {
-        // factor tables by the recurrence of kw_table (forces.cpp:212-245)
-        const int d = t / AC, j = t - d * AC;
-        const int atom = ac * AC + j;
-        const bool va = atom < n_atoms;
-        double sn = 0.0, cs = 0.0, qf = 0.0;
-        if (va) {
-            qf = d == 0 ? charge[atom] : 1.0;
-            sincos(ew.bk[d] * site_coord(X, NAP, atom, d, gamma, half1mg), &sn, &cs);
-        }
-        double cm = 1.0, sm = 0.0;
-        for (int m = 0; m < K; ++m) {
-            if (m > 0) {
-                const double cn = cs * cm - sn * sm;
-                const double snn = sn * cm + cs * sm;
-                cm = cn;
-                sm = snn;
-            }
-            const double c = va ? cm * qf : 0.0, s = va ? sm * qf : 0.0;
-            if (d < 2) {
-                T[(d * K + m) * RS + j] = make_double2(c, s);
-            } else {
-                Zt[(2 * m) * RS + j] = c;
-                Zt[(2 * m + 1) * RS + j] = s;
-            }
-        }
-        if (d == 2)
-            for (int n = 2 * K; n < ZN; ++n) Zt[n * RS + j] = 0.0;
-    }
-    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int j16 = lane & 15, gq = lane >> 4;
@@ -172,25 +150,71 @@ __global__ __launch_bounds__(64 * KS_WAVES) void k_ksum_mf(
     const int pp = lane & 3;
     const int xa = (pp == 1 || pp == 3) ? 1 : 0, yb = (pp == 1 || pp == 2) ? 1 : 0;
     const double *Td = (const double *)T;
-    double *out = spart + ((long)e * nac + ac) * G * 8;
-    for (int u = u0 + w; u < u1; u += KS_WAVES) {
-        const int2 un = su[u];
-        const int ck = sc[4 * un.x + (j16 >> 2)].x;
-        const int kx = ck & 0xffff, ky = ck >> 16;
-        const double *Xr = Td + 2 * (kx * RS + gq) + xa;  // site 4 s + gq
-        const double *Yr = Td + 2 * ((K + ky) * RS + gq) + yb;
-        const double *Zr = Zt + (16 * un.y + j16) * RS + gq;
-        mf_v4d acc = {0.0, 0.0, 0.0, 0.0};
+    mf_v4d acc[KS_MAXU];
+#pragma unroll
+    for (int k = 0; k < KS_MAXU; ++k) acc[k] = mf_v4d{0.0, 0.0, 0.0, 0.0};
+    for (int ac = ac0; ac < ac1; ++ac) {
+        if (ac > ac0) __syncthreads();  // the previous chunk's tables are read
+        for (int t = threadIdx.x; t < 3 * AC; t += blockDim.x) {
+            // factor tables by the recurrence of kw_table (forces.cpp:212-245)
+            const int d = t / AC, j = t - d * AC;
+            const int atom = ac * AC + j;
+            const bool va = atom < n_atoms;
+            double sn = 0.0, cs = 0.0, qf = 0.0;
+            if (va) {
+                qf = d == 0 ? charge[atom] : 1.0;
+                sincos(ew.bk[d] * site_coord(X, NAP, atom, d, gamma, half1mg), &sn, &cs);
+            }
+            double cm = 1.0, sm = 0.0;
+            for (int m = 0; m < K; ++m) {
+                if (m > 0) {
+                    const double cn = cs * cm - sn * sm;
+                    const double snn = sn * cm + cs * sm;
+                    cm = cn;
+                    sm = snn;
+                }
+                const double c = va ? cm * qf : 0.0, s = va ? sm * qf : 0.0;
+                if (d < 2) {
+                    T[(d * K + m) * RS + j] = make_double2(c, s);
+                } else {
+                    Zt[(2 * m) * RS + j] = c;
+                    Zt[(2 * m + 1) * RS + j] = s;
+                }
+            }
+            if (d == 2)
+                for (int n = 2 * K; n < ZN; ++n) Zt[n * RS + j] = 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KS_MAXU; ++k) {
+            const int u = u0 + w + k * KS_WAVES;
+            if (u >= u1) break;
+            const int2 un = su[u];
+            const int ck = sc[4 * un.x + (j16 >> 2)].x;
+            const int kx = ck & 0xffff, ky = ck >> 16;
+            const double *Xr = Td + 2 * (kx * RS + gq) + xa;  // site 4 s + gq
+            const double *Yr = Td + 2 * ((K + ky) * RS + gq) + yb;
+            const double *Zr = Zt + (16 * un.y + j16) * RS + gq;
+            mf_v4d a4 = acc[k];
 #pragma unroll 4
-        for (int s4 = 0; s4 < AC; s4 += 4)
-            acc = __builtin_amdgcn_mfma_f64_16x16x4f64(Xr[2 * s4] * Yr[2 * s4], Zr[s4], acc, 0, 0, 0);
+            for (int s4 = 0; s4 < AC; s4 += 4)
+                a4 = __builtin_amdgcn_mfma_f64_16x16x4f64(Xr[2 * s4] * Yr[2 * s4], Zr[s4], a4, 0, 0, 0);
+            acc[k] = a4;
+        }
+    }
+    double *out = spart + ((long)e * ncg + cg) * G * 8;
+#pragma unroll
+    for (int k = 0; k < KS_MAXU; ++k) {
+        const int u = u0 + w + k * KS_WAVES;
+        if (u >= u1) break;
+        const int2 un = su[u];
         // lane: n = 16 n-tile + j16, product gq, column 4 tile + reg
         const int n = 16 * un.y + j16, kz = n >> 1;
 #pragma unroll
         for (int reg = 0; reg < 4; ++reg) {
             const int4 cr = sc[4 * un.x + reg];
             if (kz >= cr.z && kz <= cr.w)
-                out[(long)(cr.y + kz - cr.z) * 8 + gq * 2 + (n & 1)] = acc[reg];
+                out[(long)(cr.y + kz - cr.z) * 8 + gq * 2 + (n & 1)] = acc[k][reg];
         }
     }
 }

@@ -150,3 +150,37 @@ def test_nve_conservation_both_paths(monkeypatch):
     say(f"excursions: matrix {exc['matrix']:.3e} vector {exc['vector']:.3e} Eh (KE {ke:.3e})")
     assert exc["matrix"] <= 1.5 * exc["vector"] + 1e-6 * ke
     assert exc["matrix"] < 1e-2 * ke
+
+
+@pytest.mark.parametrize("n_side,L,P,seed,n_max", [
+    (6, 18.64, 4, 61, None),   # 11 site chunks of 64
+    (6, 18.64, 2, 62, 18)])    # 13 chunks of 52; several unit slices per chunk group
+def test_structure_factor_chunk_groups(oracle_mod, n_side, L, P, seed, n_max):
+    """k_ksum_mf with 1, 2, 4, 8 and 16 site chunks per block (the block's
+    accumulators carried across its chunks, one partial per chunk group for
+    k_sreduce): forces and energies against the oracle at the parity
+    tolerances and against the one-chunk form to 1e-13 relative (the same
+    sums in another order)."""
+    from qtip4pf_water_amd import Water
+    n = n_side ** 3
+    q, box = system(n_side, L, P, seed)
+    ew = None if n_max is None else _stated(box, n_max)
+    o = oracle_mod.Oracle(n, P, box, ewald=ew)
+    point, lst, _ = o.all_lists(q)
+    Fr, fr = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+    with oracle_mod.exact_energy():
+        er, _ = o.force_eval_sc(q, Fr, fr, point, lst)
+    out = {}
+    with Water(n, P, box, ewald=ew) as w:
+        assert w.tuning()["matrix_kspace"] == 1
+        for cpb in (1, 2, 4, 8, 16):
+            say("chunk groups", n, "n_max", n_max, "chunks per block", cpb)
+            w.set_tuning(ksum_chunks_per_block=cpb)
+            F, f = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+            e, _ = w.force_eval_sc_nospline(q, F, o.charge, o.mass, point, lst, fphys=f)
+            assert rel_atom_err(f, fr) < 1e-10, cpb
+            assert abs(e - er) <= 1e-12 * abs(er), cpb
+            out[cpb] = (f, e)
+    for cpb in (2, 4, 8, 16):
+        assert rel_atom_err(out[cpb][0], out[1][0]) < 1e-13, cpb
+        assert abs(out[cpb][1] - out[1][1]) <= 1e-13 * abs(out[1][1]), cpb
