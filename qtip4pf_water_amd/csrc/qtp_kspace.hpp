@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_ksum(const double *__restrict__ pos, lo
 // grid: E * ncg * usplit (ncg = ceil(nac / cpb)), 64 * KS_WAVES threads,
 // ksum_mf_lds_bytes(K, AC, ncol4, nunit).
 constexpr int KS_WAVES = 8;
-constexpr int KS_MAXU = 4;  // units per wave of one block (accumulator registers)
+constexpr int KS_MAXU = 6;  // units per wave of one block (accumulator registers)
 // LDS rows of AC + 2 entries: the B operands of a quarter-wave (16 rows n at
 // one site) and the A factors of a tile's columns then fall in different banks.
 // The column and unit tables are staged in LDS behind the factor tables.
