@@ -67,6 +67,9 @@ struct qtp_ctx {
     int rs_waves = RS_WAVES;  // waves per k_realspace tile block (2 or 4)
     int rs_split = 0;         // eighths of the real-space tiles on the second stream (0-4)
     int ks_cpb = 0;           // structure-factor chunks per k_ksum_mf block (0: by launch size)
+    // a list check the SC step leaves to the next force phase's second stream
+    bool lists_defer = false, lists_heat = false;
+    const unsigned long long *lists_ctr = nullptr;
     // propagation as one LDS-staged kernel per step (k_prop_tile) where the
     // thermostat allows it: -1 for systems of PT_MIN_TILES tiles or more,
     // 1 always, 0 never (k_prop_fwd + k_prop_inv)

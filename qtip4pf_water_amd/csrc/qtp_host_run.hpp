@@ -29,6 +29,9 @@ static void ksum_mf_shape(const qtp_ctx *ctx, int E, int *usplit, int *cpb, int 
     *ncg = (ctx->nac + c - 1) / c;
 }
 
+static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false,
+                     const unsigned long long *ctr = nullptr, hipStream_t st = nullptr);
+
 // energy partials summed per evaluation: the spline blocks only when used
 static int ep_live(const qtp_ctx *ctx) { return ctx->spl_on ? ctx->ep_stride : ctx->ep_spl; }
 
@@ -137,6 +140,11 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
                                                  ctx->kf_mf ? ctx->wm : nullptr,
                                                  ctx->mf_Mp, 4 * ctx->mf_nks, ctx->gcol,
                                                  ctx->ew.bk[2], ctx->kf_mf ? 1 : 0);
+    }
+    // a list check deferred by the step (only the LJ kernel reads the lists)
+    if (ctx->lists_defer) {
+        ctx->lists_defer = false;
+        CHK(run_lists(ctx, 0, ctx->lists_heat, ctx->lists_ctr, sk));
     }
     {
         TimeScope ts(ctx, "ljintra");
@@ -292,11 +300,11 @@ static int run_cell_lists(qtp_ctx *ctx, int check, int force_all,
 
 // ctr: the device step counter of resident steps (forced rebuilds every
 // ctx->rebuild_every steps), or null
-static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat = false,
-                     const unsigned long long *ctr = nullptr)
+static int run_lists(qtp_ctx *ctx, int force_all, bool with_heat, const unsigned long long *ctr,
+                     hipStream_t st)
 {
     const int every = ctr ? ctx->rebuild_every : 0;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = st ? st : ctx->stream;
     const int P = ctx->P;
     HeatArgs heat{};
     if (with_heat) {
@@ -401,8 +409,19 @@ static int run_sc_step(qtp_ctx *ctx, int nvt, int host_dev, unsigned long long *
     }
     }
     HIPCHK(hipGetLastError());
-    CHK(run_lists(ctx, 0, nvt != 0, ctr));
-    CHK(run_sc_forces(ctx, 1, ctr));
+    // the one-kernel list check and rebuild: on the first force phase's
+    // second stream, ahead of the LJ kernel, so that the real-space kernel
+    // starts right after the propagation
+    if (ctx->n_mols <= NL_FUSED_MAXMOL && !use_cells(ctx)) {
+        ctx->lists_defer = true;
+        ctx->lists_heat = nvt != 0;
+        ctx->lists_ctr = ctr;
+    } else {
+        CHK(run_lists(ctx, 0, nvt != 0, ctr));
+    }
+    const int st = run_sc_forces(ctx, 1, ctr);
+    ctx->lists_defer = false;  // consumed by the first phase (also on an error)
+    CHK(st);
     if (ctx->cmf_on)
         k_cmf_acc<<<(3 * ctx->NAP + 255) / 256, 256, 0, s>>>(ctx->P, ctx->NAP, ctx->F, ctx->cmf_acc,
                                                             ctx->cmf_n);
