@@ -139,3 +139,32 @@ def test_resident_virial_matches_per_bead_calls(oracle_mod):
     for b in range(P):
         e_or = o.get_forces(q, np.zeros(P * o.n_dof), point, lst, b)
         assert abs(er[b] - e_or) <= 1e-12 * 23.0
+
+
+def test_virial_under_chunk_groups_and_realspace_split(oracle_mod):
+    """The virial form with several structure-factor chunks per block
+    (k_sreduce's k-space virial from chunk-group partials) and with real-space
+    tiles shared between the streams (its virial partials indexed by tile):
+    against the default launch, energy and virial to 1e-13 relative with
+    chunk groups (summation order), bit for bit with the split."""
+    from qtip4pf_water_amd import Water
+    q0, box = _system(1)
+    o = oracle_mod.Oracle(64, 1, box)
+    point, lst, _ = o.all_lists(q0)
+    out = {}
+    for name, kw in (("default", {}), ("chunks", {"ksum_chunks_per_block": 4}),
+                     ("split", {"rs_split": 3})):
+        say("virial launch", name)
+        with Water(64, 1, box) as w:
+            if kw:
+                w.set_tuning(**kw)
+            F = np.zeros(o.n_dof)
+            e, W = w.get_forces_virial(q0, F, o.charge, point, lst, 0)
+        out[name] = (e, W, F)
+    e0, W0, F0 = out["default"]
+    e1, W1, F1 = out["chunks"]
+    assert abs(e1 - e0) <= 1e-13 * abs(e0)
+    assert np.max(np.abs(W1 - W0)) <= 1e-13 * np.max(np.abs(W0))
+    assert np.max(np.abs(F1 - F0)) <= 1e-13 * np.max(np.abs(F0))
+    e2, W2, F2 = out["split"]
+    assert e2 == e0 and np.array_equal(W2, W0) and np.array_equal(F2, F0)
