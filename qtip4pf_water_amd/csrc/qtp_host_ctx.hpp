@@ -40,7 +40,7 @@ struct qtp_ctx {
     int lj_lpm = 16;     // lanes per molecule in k_ljintra
     bool rs_cull = false;  // k_realspace skips tiles beyond r_c (boxes >= RS_CULL_MIN_BOX r_c)
     double4 *rs_sites = nullptr;  // CULL: [E][nblk * 64] sites (x, y, z, q)
-    double *rs_box = nullptr;     // CULL: [E][nblk][6] block bounding boxes
+    double *rs_box = nullptr;     // CULL: [E][nblk][RS_BOX] block and 8-site cluster bounding boxes
     unsigned char *rs_far = nullptr;  // CULL: [E][ntile] 1 = tile skipped this evaluation
     int AC = 64, nac = 0, ngc = 0, ngr = 0;
     int ep_stride = 0, ep_lj = 0, ep_real = 0, ep_k = 0, ep_spl = 0;

@@ -310,7 +310,7 @@ static int alloc_all(qtp_ctx *ctx)
     CHK(dalloc(ctx, &ctx->rpart, (size_t)E * ctx->nblk * N3));
     if (ctx->rs_cull) {
         CHK(dalloc(ctx, &ctx->rs_sites, (size_t)E * ctx->nblk * 64));
-        CHK(dalloc(ctx, &ctx->rs_box, (size_t)E * ctx->nblk * 6));
+        CHK(dalloc(ctx, &ctx->rs_box, (size_t)E * ctx->nblk * RS_BOX));
         CHK(dalloc(ctx, &ctx->rs_far, (size_t)E * ctx->ntile));
     }
     CHK(dalloc(ctx, &ctx->spart, (size_t)E * ctx->nac * ctx->G * 8));

@@ -429,27 +429,31 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
     const bool va = lane < KF_SITES && atom < n_atoms;
     const double *X = pos + (long)e * pstride;
     double fx = 0.0, fy = 0.0, fz = 0.0, fl = 0.0;
-    // skipped tiles' slots (culling) are not read: one mask per wave, as in
-    // k_kforce_mf
+    // skipped tiles' slots (culling) are not read, as in k_kforce_mf
     const int nsl = (nblk - w + KF_WAVES - 1) / KF_WAVES;
     const bool masked = rfar && half == 0 && nsl <= 64;
-    unsigned long long live = 0ull;
+    const unsigned char *far = rfar ? rfar + (long)e * ntile : nullptr;
+    const int Bs = atom >> 6;  // the lane's site's 64-site block
+    unsigned long long live = 0ull, own = 0ull;
     if (masked) {
-        const unsigned char *far = rfar + (long)e * ntile;
         const int a0 = ab * KF_SITES;
         const int B0 = a0 >> 6, B1 = (min(a0 + KF_SITES, n_atoms) - 1) >> 6;
-        bool lv = false;
+        bool l0 = false, l1 = false;
         if (lane < nsl) {
             const int sl = w + KF_WAVES * lane;
-            lv = !far[tile_index(min(B0, sl), max(B0, sl), nblk)] ||
-                 !far[tile_index(min(B1, sl), max(B1, sl), nblk)];
+            l0 = !far[tile_index(min(B0, sl), max(B0, sl), nblk)];
+            l1 = !far[tile_index(min(B1, sl), max(B1, sl), nblk)];
         }
-        live = __ballot(lv);
+        const unsigned long long m0 = __ballot(l0), m1 = __ballot(l1);
+        live = m0 | m1;
+        own = Bs == B0 ? m0 : m1;
     }
     if (va && half == 0) {
         if (masked) {
             for (unsigned long long m = live; m; m &= m - 1ull) {
-                const int sl = w + KF_WAVES * (__ffsll((long long)m) - 1);
+                const int k = __ffsll((long long)m) - 1;
+                if (!((own >> k) & 1ull)) continue;
+                const int sl = w + KF_WAVES * k;
                 const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + atom;
                 fx += rp[0];
                 fy += rp[NAP];
@@ -457,6 +461,7 @@ __global__ __launch_bounds__(64 * W) void k_kforce(
             }
         } else {
             for (int sl = w; sl < nblk; sl += KF_WAVES) {
+                if (far && far[tile_index(min(Bs, sl), max(Bs, sl), nblk)]) continue;
                 const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + atom;
                 fx += rp[0];
                 fy += rp[NAP];
@@ -646,27 +651,34 @@ __global__ __launch_bounds__(64 * W) void k_kforce_mf(
     const int a0 = ab * MF_SITES;
     const double *X = pos + (long)e * pstride;
     double rx = 0.0, ry = 0.0, rz = 0.0;
-    // real-space slots w, w + W, ...: with culling (rfar), a slot whose tile
-    // was skipped for every 64-site block the wave's sites fall in holds
-    // zeros only and is not read (one mask per wave, built before the loop)
+    // real-space slots w, w + W, ...: with culling (rfar), the entries of a
+    // skipped tile are never written, so a site reads slot sl only if tile
+    // (its 64-site block, sl) was evaluated.  The wave's sites fall in at most
+    // two blocks, B0 and B1: one mask per block over the wave's slots, the
+    // loop over their union, each lane's loads under its own block's mask.
     const int nsl = (nblk - w + W - 1) / W;
-    unsigned long long live = 0ull;
+    unsigned long long live = 0ull, own = 0ull;
     const bool masked = rfar && nsl <= 64;
+    const unsigned char *far = rfar ? rfar + (long)e * ntile : nullptr;
+    const int Bs = (a0 + lane) >> 6;
     if (masked) {
-        const unsigned char *far = rfar + (long)e * ntile;
         const int B0 = a0 >> 6, B1 = (min(a0 + MF_SITES, n_atoms) - 1) >> 6;
-        bool lv = false;
+        bool l0 = false, l1 = false;
         if (lane < nsl) {
             const int sl = w + W * lane;
-            lv = !far[tile_index(min(B0, sl), max(B0, sl), nblk)] ||
-                 !far[tile_index(min(B1, sl), max(B1, sl), nblk)];
+            l0 = !far[tile_index(min(B0, sl), max(B0, sl), nblk)];
+            l1 = !far[tile_index(min(B1, sl), max(B1, sl), nblk)];
         }
-        live = __ballot(lv);
+        const unsigned long long m0 = __ballot(l0), m1 = __ballot(l1);
+        live = m0 | m1;
+        own = Bs == B0 ? m0 : m1;
     }
     if (lane < MF_SITES && a0 + lane < n_atoms) {
         if (masked) {
             for (unsigned long long m = live; m; m &= m - 1ull) {
-                const int sl = w + W * (__ffsll((long long)m) - 1);
+                const int k = __ffsll((long long)m) - 1;
+                if (!((own >> k) & 1ull)) continue;
+                const int sl = w + W * k;
                 const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + a0 + lane;
                 rx += rp[0];
                 ry += rp[NAP];
@@ -674,6 +686,7 @@ __global__ __launch_bounds__(64 * W) void k_kforce_mf(
             }
         } else {
             for (int sl = w; sl < nblk; sl += W) {
+                if (far && far[tile_index(min(Bs, sl), max(Bs, sl), nblk)]) continue;
                 const double *rp = rpart + ((long)e * nblk + sl) * 3 * NAP + a0 + lane;
                 rx += rp[0];
                 ry += rp[NAP];

@@ -26,13 +26,17 @@ namespace qtp {
 // vpart[(e*6 + c)*ep_stride + ep_off + tile].
 // CARG: the special functions' coefficients come from the argument cf (SGPRs)
 // instead of literals (VGPRs); the host picks it for large grids.
-// CULL: the sites come from k_sites (x, y, z, q per site and a bounding box
-// per 64-site block).  A tile whose two blocks' boxes are more than r_c apart
-// under every periodic image holds no pair within the cutoff: it writes zero
-// slots, marks itself in rfar[e * ntile + tile] (k_kforce_mf skips slots so
-// marked) and returns before loading any site.  The same pairs are evaluated
-// either way.  The host picks it for boxes several r_c wide, where most tiles
-// are far (4096 H2O: 54% of the tiles, sites in molecule order).
+// CULL: the sites come from k_sites (x, y, z, q per site and bounding boxes
+// per 64-site block and per 8-site cluster).  A tile whose two blocks' boxes
+// are more than r_c apart under every periodic image holds no pair within the
+// cutoff: it marks itself in rfar[e * ntile + tile], writes no slot entries
+// (k_kforce_mf and k_kforce read none of a tile so marked) and returns before
+// loading any site.  In the other tiles the cutoff test runs only over pairs
+// of clusters whose boxes lie within r_c (4096 H2O, r_c 9 A: a third of the
+// cluster pairs of the near tiles), one lane per site pair.  The same pairs
+// are evaluated either way.  The host picks it for boxes several r_c wide,
+// where most tiles are far (4096 H2O, r_c 9 A: 76% of the tiles, sites in
+// molecule order).
 // W: two waves per tile halve the block and double each lane's share of the
 // queue; a launch then holds fewer waves (216 H2O x 32 beads: 4224, which
 // the chip holds at once, against 8448 in two rounds with four).
@@ -63,9 +67,16 @@ __device__ __forceinline__ int tile_index(int I, int J, int nblk)
     return I * nblk - (I * (I - 1)) / 2 + (J - I);
 }
 
+// Doubles per block in the CULL form's box table: the block's bounding box
+// (lo xyz, hi xyz) at [0, 6), then the boxes of its eight 8-site clusters,
+// component c of cluster k at [6 + 8 c + k].
+constexpr int RS_BOX = 6 + 6 * 8;
+
 // Sites of each 64-site block as (x, y, z, q) (M from O, H, H; forces.cpp:
-// 558-564) and the block's bounding box (lo xyz, hi xyz), for the CULL form
-// of k_realspace.  grid: E * nblk, 64 threads.
+// 558-564), the block's bounding box and those of its 8-site clusters
+// (RS_BOX layout), for the CULL form of k_realspace.  A cluster holding no
+// site gets an empty box (lo > hi); k_realspace never tests one.
+// grid: E * nblk, 64 threads.
 __global__ __launch_bounds__(64) void k_sites(const double *__restrict__ pos, long pstride,
                                               const double *__restrict__ charge, int n_atoms,
                                               int NAP, int nblk, double gamma, double half1mg,
@@ -85,12 +96,27 @@ __global__ __launch_bounds__(64) void k_sites(const double *__restrict__ pos, lo
     sites[(long)e * nblk * 64 + s] = v;
     double lo[3] = {ok ? v.x : 1e300, ok ? v.y : 1e300, ok ? v.z : 1e300};
     double hi[3] = {ok ? v.x : -1e300, ok ? v.y : -1e300, ok ? v.z : -1e300};
-    for (int o = 32; o > 0; o >>= 1)
+    double *bb = bbox + ((long)e * nblk + B) * RS_BOX;
+    // lanes 8k ... 8k + 7 form cluster k: the first three butterfly steps
+    // stay inside it
+    for (int o = 1; o < 8; o <<= 1)
         for (int d = 0; d < 3; ++d) {
             lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
             hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
         }
-    double *bb = bbox + ((long)e * nblk + B) * 6;
+    {
+        const int c = l & 7, k = l >> 3;
+        if (c < 6) {
+            const double vc = c == 0 ? lo[0] : c == 1 ? lo[1] : c == 2 ? lo[2]
+                            : c == 3 ? hi[0] : c == 4 ? hi[1] : hi[2];
+            bb[6 + 8 * c + k] = vc;
+        }
+    }
+    for (int o = 8; o < 64; o <<= 1)
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], __shfl_xor(lo[d], o, 64));
+            hi[d] = fmax(hi[d], __shfl_xor(hi[d], o, 64));
+        }
     if (l < 3) bb[l] = l == 0 ? lo[0] : (l == 1 ? lo[1] : lo[2]);
     else if (l < 6) bb[l] = l == 3 ? hi[0] : (l == 4 ? hi[1] : hi[2]);
 }
@@ -122,8 +148,8 @@ __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__
     if (CULL) {
         bool far = false;
         if (I != J) {
-            const double *bI = bbox + ((long)e * nblk + I) * 6;
-            const double *bJ = bbox + ((long)e * nblk + J) * 6;
+            const double *bI = bbox + ((long)e * nblk + I) * RS_BOX;
+            const double *bJ = bbox + ((long)e * nblk + J) * RS_BOX;
             double gsq = 0.0;
             for (int d = 0; d < 3; ++d) {
                 const double g =
@@ -137,11 +163,6 @@ __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__
         if (far) {
             if (VIR && threadIdx.x < 6)
                 vpart[((long)e * 6 + threadIdx.x) * ep_stride + ep_off + t] = 0.0;
-            for (int k = threadIdx.x; k < 192; k += 64 * W) {
-                const int d = k >> 6, ll = k & 63;
-                rpart[base_e + (long)J * 3 * NAP + d * NAP + I * 64 + ll] = 0.0;
-                rpart[base_e + (long)I * 3 * NAP + d * NAP + J * 64 + ll] = 0.0;
-            }
             if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + t] = 0.0;
             return;
         }
@@ -174,35 +195,77 @@ __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__
     __syncthreads();
     const int c0 = CPW * w;
     const int i = I * 64 + l;
-    // pass 1: in-cutoff columns of row i (j > i; j < n_atoms)
-    unsigned int mask = 0u;
-    if (i < n_atoms) {
-        const double4 a = sI[l];
-#pragma unroll
-        for (int c = 0; c < CPW; ++c) {
-            const int jl = c0 + c;
-            const int j = J * 64 + jl;
-            const double4 b = sJ[jl];
+    int total = 0;
+    if constexpr (CULL) {
+        // pass 1 over the live 8 x 8 cluster pairs of the wave's columns:
+        // lane l tests row cluster l & 7 against column cluster l >> 3
+        // (boxes more than r_c apart under every image hold no pair within
+        // the cutoff), then each live pair's 64 site pairs take one lane
+        // each; the queue holds a cluster pair's entries by column, rows
+        // within a column
+        const int ci = l & 7, cj = l >> 3;
+        const double *cI = bbox + ((long)e * nblk + I) * RS_BOX + 6;
+        const double *cJ = bbox + ((long)e * nblk + J) * RS_BOX + 6;
+        double gsq = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double g = periodic_gap(cI[8 * d + ci], cI[8 * (3 + d) + ci], cJ[8 * d + cj],
+                                          cJ[8 * (3 + d) + cj], box.L[d], box.inv[d]);
+            gsq = fma(g, g, gsq);
+        }
+        const bool live = gsq <= ew.rcut_sq * (1.0 + 1e-9) && I * 64 + 8 * ci < n_atoms &&
+                          J * 64 + 8 * cj < n_atoms && (I != J || cj >= ci);
+        unsigned long long todo =
+            __ballot(live) & (((CPW == 64 ? 0ull : 1ull << CPW) - 1ull) << (CPW * w));
+        while (todo) {
+            const int bit = __builtin_ctzll(todo);
+            todo &= todo - 1ull;
+            const int il = 8 * (bit & 7) + (l & 7);
+            const int jl = 8 * (bit >> 3) + (l >> 3);
+            const double4 a = sI[il], b = sJ[jl];
             const double dx = mimg_fast(a.x - b.x, box.L[0], box.inv[0]);
             const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
             const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
             const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
-            if (j > i && j < n_atoms && rsq < ew.rcut_sq) mask |= 1u << c;
+            const int ii = I * 64 + il, j = J * 64 + jl;
+            const bool on = j > ii && j < n_atoms && rsq < ew.rcut_sq;
+            const unsigned long long bal = __ballot(on);
+            if (on) {
+                const int r = __builtin_amdgcn_mbcnt_hi(
+                    (unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                qe[w][total + r] = (unsigned short)(il | ((jl - c0) << 6));
+            }
+            total += __popcll(bal);
         }
-    }
-    // the wave's pair queue, column-major: column c's rows at off_c + (the
-    // lane's rank among the column's set lanes)
-    int total = 0;
+    } else {
+        // pass 1: in-cutoff columns of row i (j > i; j < n_atoms)
+        unsigned int mask = 0u;
+        if (i < n_atoms) {
+            const double4 a = sI[l];
 #pragma unroll
-    for (int c = 0; c < CPW; ++c) {
-        const bool on = (mask >> c) & 1u;
-        const unsigned long long bal = __ballot(on);
-        if (on) {
-            const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
-                                                    __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
-            qe[w][total + r] = (unsigned short)(l | (c << 6));
+            for (int c = 0; c < CPW; ++c) {
+                const int jl = c0 + c;
+                const int j = J * 64 + jl;
+                const double4 b = sJ[jl];
+                const double dx = mimg_fast(a.x - b.x, box.L[0], box.inv[0]);
+                const double dy = mimg_fast(a.y - b.y, box.L[1], box.inv[1]);
+                const double dz = mimg_fast(a.z - b.z, box.L[2], box.inv[2]);
+                const double rsq = fma(dx, dx, fma(dy, dy, dz * dz));
+                if (j > i && j < n_atoms && rsq < ew.rcut_sq) mask |= 1u << c;
+            }
         }
-        total += __popcll(bal);
+        // the wave's pair queue, column-major: column c's rows at off_c + (the
+        // lane's rank among the column's set lanes)
+#pragma unroll
+        for (int c = 0; c < CPW; ++c) {
+            const bool on = (mask >> c) & 1u;
+            const unsigned long long bal = __ballot(on);
+            if (on) {
+                const int r = __builtin_amdgcn_mbcnt_hi((unsigned)(bal >> 32),
+                                                        __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+                qe[w][total + r] = (unsigned short)(l | (c << 6));
+            }
+            total += __popcll(bal);
+        }
     }
     __syncthreads();
     // pass 2: lane l takes queue entries [l*S, l*S + S)

@@ -155,3 +155,44 @@ def test_culled_virial_matches_energy_differences(oracle_mod):
         fd[a] = -(es[0] - es[1]) / (2.0 * h)
     say("W diag", np.diag(W), "FD", fd)
     assert np.max(np.abs(np.diag(W) - fd)) <= 1e-6 * np.max(np.abs(W))
+
+
+@pytest.mark.parametrize("rsw", [4, 2])
+def test_skipped_tiles_leave_no_stale_slots(oracle_mod, rsw):
+    """A skipped tile writes no slot entries, and the force kernels must not
+    read them.  One Water object evaluates 1000 H2O x 2 beads with the
+    molecules in random order (every tile within r_c: every slot written),
+    then the same molecules in lattice order (most tiles skipped, their
+    slots still holding the first evaluation's values): forces and energies
+    of the second evaluation against the oracle at the parity tolerances,
+    both the single-bead and the Suzuki-Chin forms."""
+    from qtip4pf_water_amd import Water
+    n_side, L, P = 10, 31.07, 2
+    n = n_side ** 3
+    q0, box = cfg.lattice_water_box(n_side, L, seed=76, jitter_bohr=0.05)
+    q = cfg.replicate_beads(q0, P) + np.random.default_rng(77).normal(0, 0.03, P * 9 * n)
+    perm = np.random.default_rng(78).permutation(n)
+    qp = q.reshape(P, n, 9)[:, perm].reshape(-1)
+    o = oracle_mod.Oracle(n, P, box)
+    assert box[0] >= 2.5 * o.ewald()["rcut"]
+    say("stale slots", n, "mols P", P, "waves", rsw)
+    pp, lp, _ = o.all_lists(qp)
+    point, lst, _ = o.all_lists(q)
+    scale = o.ewald()["prefactor"] * 0.5 * float(np.sum(o.charge ** 2))
+    Fr, fr = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+    er, _ = o.force_eval_sc(q, Fr, fr, point, lst)
+    F1r = np.zeros(P * o.n_dof)
+    e1r = o.get_forces(q, F1r, point, lst, 0)
+    with Water(n, P, box) as w:
+        w.set_tuning(rs_waves=rsw)
+        w.force_eval_sc_nospline(qp, np.zeros(P * o.n_dof), o.charge, o.mass, pp, lp,
+                                 fphys=np.zeros(P * o.n_dof))
+        F, f = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+        e, _ = w.force_eval_sc_nospline(q, F, o.charge, o.mass, point, lst, fphys=f)
+        w.get_forces_nospline(qp, np.zeros(P * o.n_dof), o.charge, pp, lp, 0)
+        F1 = np.zeros(P * o.n_dof)
+        e1 = w.get_forces_nospline(q, F1, o.charge, point, lst, 0)
+    assert rel_atom_err(f, fr) < 1e-10
+    assert rel_atom_err(F1[: o.n_dof], F1r[: o.n_dof]) < 1e-10
+    assert abs(e - er) <= 1e-12 * scale
+    assert abs(e1 - e1r) <= 1e-12 * scale
