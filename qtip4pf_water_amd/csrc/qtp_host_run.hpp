@@ -101,13 +101,21 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
 #undef QTP_RS_LAUNCH
 #undef QTP_RS_LAUNCH_W
     };
-    auto launch_sites = [&](hipStream_t st) {
-        TimeScope ts(ctx, "sites");
-        k_sites<<<E * ctx->nblk, 64, 0, st>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
-                                              ctx->nblk, g, h, ctx->rs_sites, ctx->rs_box);
-    };
-    // both real-space parts read the culling form's site tables
-    if (ctx->rs_cull && nt_b > 0) launch_sites(s);
+    // the culling form's site tables and skip flags, ahead of the fork: on
+    // their own the two small kernels take a few microseconds, behind the
+    // structure-factor blocks of the other stream many times that
+    if (ctx->rs_cull) {
+        {
+            TimeScope ts(ctx, "sites");
+            k_sites<<<E * ctx->nblk, 64, 0, s>>>(pos, N3, ctx->charge, ctx->n_atoms, ctx->NAP,
+                                                 ctx->nblk, g, h, ctx->rs_sites, ctx->rs_box);
+        }
+        TimeScope ts(ctx, "tilefar");
+        const long nk = (long)E * ctx->ntile;
+        k_tilefar<<<(unsigned)((nk + 255) / 256), 256, 0, s>>>(
+            ctx->tiles, ctx->ntile, ctx->nblk, E, ctx->rs_box, ctx->box, ctx->ew, ctx->rs_far,
+            ctx->epart, ctx->ep_stride, ctx->ep_real, vp);
+    }
     if (fork) {
         HIPCHK(hipEventRecord(ctx->ev_fork, s));
         HIPCHK(hipStreamWaitEvent(sk, ctx->ev_fork, 0));
@@ -175,7 +183,6 @@ static int run_phase(qtp_ctx *ctx, const double *pos, int E, int lmode, int lbas
     }
     if (nt_b > 0) launch_rs(sk, nt_a, nt_b);
     if (fork) HIPCHK(hipEventRecord(ctx->ev_join, sk));
-    if (ctx->rs_cull && nt_b == 0) launch_sites(s);
     launch_rs(s, 0, nt_a);
     if (fork) HIPCHK(hipStreamWaitEvent(s, ctx->ev_join, 0));
     {

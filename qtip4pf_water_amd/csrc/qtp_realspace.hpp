@@ -29,9 +29,10 @@ namespace qtp {
 // CULL: the sites come from k_sites (x, y, z, q per site and bounding boxes
 // per 64-site block and per 8-site cluster).  A tile whose two blocks' boxes
 // are more than r_c apart under every periodic image holds no pair within the
-// cutoff: it marks itself in rfar[e * ntile + tile], writes no slot entries
-// (k_kforce_mf and k_kforce read none of a tile so marked) and returns before
-// loading any site.  In the other tiles the cutoff test runs only over pairs
+// cutoff: k_tilefar marks it in rfar[e * ntile + tile] and zeroes its energy
+// partial, and its block here returns after reading the flag, writing no slot
+// entries (k_kforce_mf and k_kforce read none of a tile so marked).  In the
+// other tiles the cutoff test runs only over pairs
 // of clusters whose boxes lie within r_c (4096 H2O, r_c 9 A: a third of the
 // cluster pairs of the near tiles), one lane per site pair.  The same pairs
 // are evaluated either way.  The host picks it for boxes several r_c wide,
@@ -121,6 +122,42 @@ __global__ __launch_bounds__(64) void k_sites(const double *__restrict__ pos, lo
     else if (l < 6) bb[l] = l == 3 ? hi[0] : (l == 4 ? hi[1] : hi[2]);
 }
 
+// Skip flags of the CULL form, one thread per (evaluation, tile): tile (I, J)
+// is skipped (rfar = 1) when its two blocks' boxes are more than r_c apart
+// under every periodic image; its energy partial (and virial partials, when
+// vpart is given) are zeroed here, so its k_realspace block only reads the
+// flag.  grid: ceil(E * ntile / 256), 256 threads.
+__global__ __launch_bounds__(256) void k_tilefar(const int2 *__restrict__ tiles, int ntile, int nblk,
+                                                int E, const double *__restrict__ bbox, Box box,
+                                                Ewald ew, unsigned char *__restrict__ rfar,
+                                                double *__restrict__ epart, int ep_stride,
+                                                int ep_off, double *__restrict__ vpart)
+{
+    const long k = (long)blockIdx.x * 256 + threadIdx.x;
+    if (k >= (long)E * ntile) return;
+    const int e = (int)(k / ntile), t = (int)(k - (long)e * ntile);
+    const int2 ij = tiles[t];
+    bool far = false;
+    if (ij.x != ij.y) {
+        const double *bI = bbox + ((long)e * nblk + ij.x) * RS_BOX;
+        const double *bJ = bbox + ((long)e * nblk + ij.y) * RS_BOX;
+        double gsq = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double g =
+                periodic_gap(bI[d], bI[3 + d], bJ[d], bJ[3 + d], box.L[d], box.inv[d]);
+            gsq = fma(g, g, gsq);
+        }
+        // the margin covers the rounding of the boxes and of rsq
+        far = gsq > ew.rcut_sq * (1.0 + 1e-9);
+    }
+    rfar[k] = far ? 1 : 0;
+    if (far) {
+        epart[(long)e * ep_stride + ep_off + t] = 0.0;
+        if (vpart)
+            for (int c = 0; c < 6; ++c) vpart[((long)e * 6 + c) * ep_stride + ep_off + t] = 0.0;
+    }
+}
+
 template <bool VIR, bool CARG, bool CULL, int W>
 __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__ pos, long pstride,
                                                       const double *__restrict__ charge, int n_atoms,
@@ -134,39 +171,19 @@ __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__
                                                       const RsCoef cf,
                                                       const double4 *__restrict__ sites,
                                                       const double *__restrict__ bbox,
-                                                      unsigned char *__restrict__ rfar)
+                                                      const unsigned char *__restrict__ rfar)
 {
     static_assert(W == 2 || W == 4, "two or four waves per tile");
     constexpr int CPW = 64 / W;  // columns per wave
     const int bid = xcd_remap(blockIdx.x, gridDim.x);
     const int e = bid / nt, t = t0 + (bid - e * nt);
+    // a tile k_tilefar skipped (its partials zeroed there)
+    if (CULL && rfar[(long)e * ntile + t]) return;
     const int I = tiles[t].x, J = tiles[t].y;
     const int l = threadIdx.x & 63;
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double *X = pos + (long)e * pstride;
     const long base_e = (long)e * nblk * 3 * NAP;
-    if (CULL) {
-        bool far = false;
-        if (I != J) {
-            const double *bI = bbox + ((long)e * nblk + I) * RS_BOX;
-            const double *bJ = bbox + ((long)e * nblk + J) * RS_BOX;
-            double gsq = 0.0;
-            for (int d = 0; d < 3; ++d) {
-                const double g =
-                    periodic_gap(bI[d], bI[3 + d], bJ[d], bJ[3 + d], box.L[d], box.inv[d]);
-                gsq = fma(g, g, gsq);
-            }
-            // the margin covers the rounding of the boxes and of rsq
-            far = gsq > ew.rcut_sq * (1.0 + 1e-9);
-        }
-        if (threadIdx.x == 0) rfar[(long)e * ntile + t] = far ? 1 : 0;
-        if (far) {
-            if (VIR && threadIdx.x < 6)
-                vpart[((long)e * 6 + threadIdx.x) * ep_stride + ep_off + t] = 0.0;
-            if (threadIdx.x == 0) epart[(long)e * ep_stride + ep_off + t] = 0.0;
-            return;
-        }
-    }
     __shared__ double4 sI[64], sJ[64];  // x, y, z, charge
     __shared__ double fj[3][64];
     __shared__ double fi[W][3][64];
