@@ -14,6 +14,10 @@
 
 namespace qtp {
 
+// Row bitmaps of the count pass kept for the fill pass up to this size
+// (4096 H2O x 32 beads: 64 MiB).
+constexpr size_t NL_BM_MAX_BYTES = (size_t)256 << 20;
+
 // Limits of the cell-binned build: the cell counts of one bead in LDS
 // (k_cell_build) and four row bitmaps of n_mols bits (k_nl_cell).
 constexpr int CELL_MAXCELL = 12288;
@@ -151,7 +155,9 @@ __global__ __launch_bounds__(1024) void k_cell_build(int n_mols, int NAP, int NM
 // its candidates the (2S+1)^2 runs of 2S+1 x-adjacent cells (contiguous in
 // member unless they wrap, then two runs).  FILL = false: row counts
 // (cnt[b][0][i] half, j > i; cnt[b][1][i] full, j != i); FILL = true: the
-// rows at point / fpoint (k_nl_scan's offsets).
+// rows at point / fpoint (k_nl_scan's offsets).  With bmg (P n_mols
+// ceil(n_mols/64) words) the count pass also stores each row's bitmap there
+// and the fill pass compacts those instead of testing the candidates again.
 // grid: P * ceil(n_mols / 4), 256 threads; dynamic LDS: 4 ceil(n_mols/64) u64.
 template <bool FILL>
 __global__ __launch_bounds__(256) void k_nl_cell(int n_mols, int NAP, const double *__restrict__ q,
@@ -164,7 +170,8 @@ __global__ __launch_bounds__(256) void k_nl_cell(int n_mols, int NAP, const doub
                                                  int *__restrict__ cnt,
                                                  const int *__restrict__ point,
                                                  const int *__restrict__ fpoint,
-                                                 int *__restrict__ list, int *__restrict__ flist)
+                                                 int *__restrict__ list, int *__restrict__ flist,
+                                                 unsigned long long *__restrict__ bmg)
 {
     extern __shared__ unsigned long long bm_all[];
     __shared__ int rlo[4][64], rpre[4][64];
@@ -176,70 +183,75 @@ __global__ __launch_bounds__(256) void k_nl_cell(int n_mols, int NAP, const doub
     if (i >= n_mols) return;   // wave-local from here on: no block barriers
     const int nw = (n_mols + 63) >> 6;
     unsigned long long *bm = bm_all + (long)w * nw;
-    for (int k = lane; k < nw; k += 64) bm[k] = 0ull;
-    const int *CS = cstart + (long)b * (cg.ncell + 1);
-    const int *ME = member + (long)b * n_mols;
-    const int c = cell_of[(long)b * n_mols + i];
-    const int cx = c % cg.nc[0], cy = (c / cg.nc[0]) % cg.nc[1], cz = c / (cg.nc[0] * cg.nc[1]);
-    const int span = 2 * cg.S + 1, nrun = 2 * span * span;
-    int start = 0, len = 0;
-    if (lane < nrun) {
-        const int r = lane >> 1, part = lane & 1;
-        int yy = cy + r % span - cg.S, zz = cz + r / span - cg.S;
-        yy += yy < 0 ? cg.nc[1] : (yy >= cg.nc[1] ? -cg.nc[1] : 0);
-        zz += zz < 0 ? cg.nc[2] : (zz >= cg.nc[2] ? -cg.nc[2] : 0);
-        const int base = (zz * cg.nc[1] + yy) * cg.nc[0];
-        const int x0 = cx - cg.S, x1 = cx + cg.S;
-        int lo = 0, hi = -1;
-        if (x0 < 0) {
-            if (part == 0) { lo = x0 + cg.nc[0]; hi = cg.nc[0] - 1; }
-            else { lo = 0; hi = x1; }
-        } else if (x1 >= cg.nc[0]) {
-            if (part == 0) { lo = x0; hi = cg.nc[0] - 1; }
-            else { lo = 0; hi = x1 - cg.nc[0]; }
-        } else if (part == 0) {
-            lo = x0;
-            hi = x1;
-        }
-        if (hi >= lo) {
-            start = CS[base + lo];
-            len = CS[base + hi + 1] - start;
-        }
-    }
-    int pre = len;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(pre, o, 64);
-        if (lane >= o) pre += t;
-    }
-    const int total = __shfl(pre, 63, 64);
-    rlo[w][lane] = start;
-    rpre[w][lane] = pre - len;
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    const double *Q = q + (long)b * 3 * NAP;
-    const double xi = Q[3 * i], yi = Q[NAP + 3 * i], zi = Q[2 * NAP + 3 * i];
-    for (int t0 = 0; t0 < total; t0 += 64) {
-        const int t = t0 + lane;
-        if (t < total) {
-            // last run whose prefix is <= t (runs after it start beyond t)
-            int a = 0, e = nrun;
-            while (e - a > 1) {
-                const int m = (a + e) >> 1;
-                if (rpre[w][m] <= t) a = m;
-                else e = m;
+    unsigned long long *bmrow = bmg ? bmg + ((long)b * n_mols + i) * nw : nullptr;
+    if (!(FILL && bmg)) {
+        for (int k = lane; k < nw; k += 64) bm[k] = 0ull;
+        const int *CS = cstart + (long)b * (cg.ncell + 1);
+        const int *ME = member + (long)b * n_mols;
+        const int c = cell_of[(long)b * n_mols + i];
+        const int cx = c % cg.nc[0], cy = (c / cg.nc[0]) % cg.nc[1], cz = c / (cg.nc[0] * cg.nc[1]);
+        const int span = 2 * cg.S + 1, nrun = 2 * span * span;
+        int start = 0, len = 0;
+        if (lane < nrun) {
+            const int r = lane >> 1, part = lane & 1;
+            int yy = cy + r % span - cg.S, zz = cz + r / span - cg.S;
+            yy += yy < 0 ? cg.nc[1] : (yy >= cg.nc[1] ? -cg.nc[1] : 0);
+            zz += zz < 0 ? cg.nc[2] : (zz >= cg.nc[2] ? -cg.nc[2] : 0);
+            const int base = (zz * cg.nc[1] + yy) * cg.nc[0];
+            const int x0 = cx - cg.S, x1 = cx + cg.S;
+            int lo = 0, hi = -1;
+            if (x0 < 0) {
+                if (part == 0) { lo = x0 + cg.nc[0]; hi = cg.nc[0] - 1; }
+                else { lo = 0; hi = x1; }
+            } else if (x1 >= cg.nc[0]) {
+                if (part == 0) { lo = x0; hi = cg.nc[0] - 1; }
+                else { lo = 0; hi = x1 - cg.nc[0]; }
+            } else if (part == 0) {
+                lo = x0;
+                hi = x1;
             }
-            const int j = ME[rlo[w][a] + (t - rpre[w][a])];
-            if (j != i) {
-                const double rx = mimg_exact(xi - Q[3 * j], box.L[0]);
-                const double ry = mimg_exact(yi - Q[NAP + 3 * j], box.L[1]);
-                const double rz = mimg_exact(zi - Q[2 * NAP + 3 * j], box.L[2]);
-                if (rx * rx + ry * ry + rz * rz < r_skin_sq)
-                    atomicOr(&bm[j >> 6], 1ull << (j & 63));
+            if (hi >= lo) {
+                start = CS[base + lo];
+                len = CS[base + hi + 1] - start;
             }
         }
+        int pre = len;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int t = __shfl_up(pre, o, 64);
+            if (lane >= o) pre += t;
+        }
+        const int total = __shfl(pre, 63, 64);
+        rlo[w][lane] = start;
+        rpre[w][lane] = pre - len;
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        const double *Q = q + (long)b * 3 * NAP;
+        const double xi = Q[3 * i], yi = Q[NAP + 3 * i], zi = Q[2 * NAP + 3 * i];
+        for (int t0 = 0; t0 < total; t0 += 64) {
+            const int t = t0 + lane;
+            if (t < total) {
+                // last run whose prefix is <= t (runs after it start beyond t)
+                int a = 0, e = nrun;
+                while (e - a > 1) {
+                    const int m = (a + e) >> 1;
+                    if (rpre[w][m] <= t) a = m;
+                    else e = m;
+                }
+                const int j = ME[rlo[w][a] + (t - rpre[w][a])];
+                if (j != i) {
+                    const double rx = mimg_exact(xi - Q[3 * j], box.L[0]);
+                    const double ry = mimg_exact(yi - Q[NAP + 3 * j], box.L[1]);
+                    const double rz = mimg_exact(zi - Q[2 * NAP + 3 * j], box.L[2]);
+                    if (rx * rx + ry * ry + rz * rz < r_skin_sq)
+                        atomicOr(&bm[j >> 6], 1ull << (j & 63));
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        if (bmrow)
+            for (int k = lane; k < nw; k += 64) bmrow[k] = bm[k];
     }
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     const int wi = i >> 6;
     const unsigned long long above = (i & 63) == 63 ? 0ull : (~0ull << ((i & 63) + 1));
     if (!FILL) {
@@ -266,7 +278,7 @@ __global__ __launch_bounds__(256) void k_nl_cell(int n_mols, int NAP, const doub
     int *FL = flist + (long)b * fcap;
     for (int k0 = 0; k0 < nw; k0 += 64) {
         const int k = k0 + lane;
-        unsigned long long v = k < nw ? bm[k] : 0ull;
+        unsigned long long v = k < nw ? (bmrow ? bmrow[k] : bm[k]) : 0ull;
         unsigned long long vh = k > wi ? v : (k == wi ? (v & above) : 0ull);
         const int pf = __popcll(v), ph = __popcll(vh);
         int sf = pf, sh = ph;

@@ -90,6 +90,9 @@ struct qtp_ctx {
     bool cells_ok = false;
     int list_method = 0, rebuild_every = 0;
     int *cell_of = nullptr, *crank = nullptr, *cstart = nullptr, *member = nullptr;
+    // the count pass's row bitmaps for the fill pass (when they take at most
+    // NL_BM_MAX_BYTES; otherwise the fill pass tests the candidates again)
+    unsigned long long *nl_bm = nullptr;
     double *nreb = nullptr;
     // force-phase scratch (E <= max(P,1))
     double *qpm = nullptr, *vpart = nullptr, *vout = nullptr, *rpart = nullptr, *spart = nullptr, *sprime = nullptr;
