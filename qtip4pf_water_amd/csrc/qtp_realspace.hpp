@@ -184,6 +184,25 @@ __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const double *X = pos + (long)e * pstride;
     const long base_e = (long)e * nblk * 3 * NAP;
+    // CULL: the live 8 x 8 cluster pairs of the wave's columns, from the
+    // cluster boxes alone, so their loads overlap the staging of the sites:
+    // lane l tests row cluster l & 7 against column cluster l >> 3 (boxes
+    // more than r_c apart under every image hold no pair within the cutoff)
+    unsigned long long todo = 0ull;
+    if constexpr (CULL) {
+        const int ci = l & 7, cj = l >> 3;
+        const double *cI = bbox + ((long)e * nblk + I) * RS_BOX + 6;
+        const double *cJ = bbox + ((long)e * nblk + J) * RS_BOX + 6;
+        double gsq = 0.0;
+        for (int d = 0; d < 3; ++d) {
+            const double g = periodic_gap(cI[8 * d + ci], cI[8 * (3 + d) + ci], cJ[8 * d + cj],
+                                          cJ[8 * (3 + d) + cj], box.L[d], box.inv[d]);
+            gsq = fma(g, g, gsq);
+        }
+        const bool live = gsq <= ew.rcut_sq * (1.0 + 1e-9) && I * 64 + 8 * ci < n_atoms &&
+                          J * 64 + 8 * cj < n_atoms && (I != J || cj >= ci);
+        todo = __ballot(live) & (((CPW == 64 ? 0ull : 1ull << CPW) - 1ull) << (CPW * w));
+    }
     __shared__ double4 sI[64], sJ[64];  // x, y, z, charge
     __shared__ double fj[3][64];
     __shared__ double fi[W][3][64];
@@ -214,25 +233,9 @@ __global__ __launch_bounds__(64 * W) void k_realspace(const double *__restrict__
     const int i = I * 64 + l;
     int total = 0;
     if constexpr (CULL) {
-        // pass 1 over the live 8 x 8 cluster pairs of the wave's columns:
-        // lane l tests row cluster l & 7 against column cluster l >> 3
-        // (boxes more than r_c apart under every image hold no pair within
-        // the cutoff), then each live pair's 64 site pairs take one lane
-        // each; the queue holds a cluster pair's entries by column, rows
-        // within a column
-        const int ci = l & 7, cj = l >> 3;
-        const double *cI = bbox + ((long)e * nblk + I) * RS_BOX + 6;
-        const double *cJ = bbox + ((long)e * nblk + J) * RS_BOX + 6;
-        double gsq = 0.0;
-        for (int d = 0; d < 3; ++d) {
-            const double g = periodic_gap(cI[8 * d + ci], cI[8 * (3 + d) + ci], cJ[8 * d + cj],
-                                          cJ[8 * (3 + d) + cj], box.L[d], box.inv[d]);
-            gsq = fma(g, g, gsq);
-        }
-        const bool live = gsq <= ew.rcut_sq * (1.0 + 1e-9) && I * 64 + 8 * ci < n_atoms &&
-                          J * 64 + 8 * cj < n_atoms && (I != J || cj >= ci);
-        unsigned long long todo =
-            __ballot(live) & (((CPW == 64 ? 0ull : 1ull << CPW) - 1ull) << (CPW * w));
+        // pass 1 over the live cluster pairs: each one's 64 site pairs take
+        // one lane each; the queue holds a cluster pair's entries by column,
+        // rows within a column
         while (todo) {
             const int bit = __builtin_ctzll(todo);
             todo &= todo - 1ull;
