@@ -32,20 +32,20 @@ namespace qtp {
 // cutoff: k_tilefar marks it in rfar[e * ntile + tile] and zeroes its energy
 // partial, and its block here returns after reading the flag, writing no slot
 // entries (k_kforce_mf and k_kforce read none of a tile so marked).  In the
-// other tiles the cutoff test runs only over pairs
-// of clusters whose boxes lie within r_c (4096 H2O, r_c 9 A: a third of the
-// cluster pairs of the near tiles), one lane per site pair.  The same pairs
-// are evaluated either way.  The host picks it for boxes several r_c wide,
-// where most tiles are far (4096 H2O, r_c 9 A: 76% of the tiles, sites in
-// molecule order).
+// other tiles the cutoff test runs only over pairs of clusters whose boxes
+// lie within r_c (4096 H2O, r_c 9 A: a third of the cluster pairs of the near
+// tiles), one lane per site pair.  The same pairs are evaluated either way.
+// The host picks it for boxes several r_c wide, where most tiles are far
+// (4096 H2O, r_c 9 A: 76% of the tiles, sites in molecule order).
 // W: two waves per tile halve the block and double each lane's share of the
 // queue; a launch then holds fewer waves (216 H2O x 32 beads: 4224, which
 // the chip holds at once, against 8448 in two rounds with four).
 constexpr int RS_WAVES = 4;  // the default W
 constexpr long RS_CARG_BLOCKS = 16384;  // grids at least this large use CARG
-// boxes at least this many r_c wide use CULL.  x 32 beads, steps/s against
-// the plain form: 216 H2O (2.06 r_c) -8%, 1000 (2.66) +7%, 1728 (2.91) +7%,
-// 4096 (3.36) +21%
+// boxes at least this many r_c wide use CULL.  x 32 beads, steps/s of the
+// first culling form (block boxes only, zeroed slots) against the plain one
+// at the reference's Ewald point: 216 H2O (2.06 r_c) -8%, 1000 (2.66) +7%,
+// 1728 (2.91) +7%, 4096 (3.36) +21%
 constexpr double RS_CULL_MIN_BOX = 2.5;
 
 // Smallest distance between [a0, a1] and the periodic images of [b0, b1]
