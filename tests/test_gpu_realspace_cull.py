@@ -196,3 +196,42 @@ def test_skipped_tiles_leave_no_stale_slots(oracle_mod, rsw):
     assert rel_atom_err(F1[: o.n_dof], F1r[: o.n_dof]) < 1e-10
     assert abs(e - er) <= 1e-12 * scale
     assert abs(e1 - e1r) <= 1e-12 * scale
+
+
+@pytest.mark.parametrize("rsw", [4, 2])
+def test_culled_partial_clusters(oracle_mod, rsw):
+    """999 H2O x 2 beads (the 1000-molecule lattice less its last molecule):
+    2,997 sites, so the last 64-site block holds 53 -- its seventh 8-site
+    cluster 5 sites, its eighth none -- and the cluster-pair test meets a
+    partly filled and an empty cluster.  SC and single-bead forces and
+    energies against the oracle at the parity tolerances."""
+    from qtip4pf_water_amd import Water
+    n_side, L, P = 10, 31.07, 2
+    q0, box = cfg.lattice_water_box(n_side, L, seed=79, jitter_bohr=0.05)
+    n = n_side ** 3 - 1
+    q0 = q0[: 9 * n]
+    q = cfg.replicate_beads(q0, P) + np.random.default_rng(80).normal(0, 0.03, P * 9 * n)
+    assert (3 * n) % 64 == 53
+    o = oracle_mod.Oracle(n, P, box)
+    assert box[0] >= 2.5 * o.ewald()["rcut"]
+    say("partial clusters", n, "mols P", P, "waves", rsw)
+    point, lst, _ = o.all_lists(q)
+    scale = o.ewald()["prefactor"] * 0.5 * float(np.sum(o.charge ** 2))
+    Fr, fr = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+    er, scr = o.force_eval_sc(q, Fr, fr, point, lst)
+    F1r = np.zeros(P * o.n_dof)
+    e1r = o.get_forces(q, F1r, point, lst, 0)
+    with Water(n, P, box) as w:
+        w.set_tuning(rs_waves=rsw)
+        F, f = np.zeros(P * o.n_dof), np.zeros(P * o.n_dof)
+        e, sc = w.force_eval_sc_nospline(q, F, o.charge, o.mass, point, lst, fphys=f)
+        F1 = np.zeros(P * o.n_dof)
+        e1 = w.get_forces_nospline(q, F1, o.charge, point, lst, 0)
+    assert rel_atom_err(f, fr) < 1e-10
+    assert rel_atom_err(F1[: o.n_dof], F1r[: o.n_dof]) < 1e-10
+    assert abs(e - er) <= 1e-12 * scale
+    assert abs(e1 - e1r) <= 1e-12 * scale
+    assert abs(sc - scr) <= 1e-12 * scale
+    for b in range(P):
+        sl = slice(b * o.n_dof, (b + 1) * o.n_dof)
+        assert rel_atom_err(F[sl], Fr[sl]) < (1e-10 if b % 2 == 0 else 1e-9), b
