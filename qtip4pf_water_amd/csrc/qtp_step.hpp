@@ -78,7 +78,21 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
                 if (t < nt) Fsc[o + t] = 2.0 * fv[r] / 3.0;
             }
         } else {
-            for (int t = threadIdx.x; t < nt; t += SC_THREADS) Fsc[o + t] = 2.0 * fph[o + t] / 3.0;
+            // larger beads: the same dofs in runs of SC_RUN, each run's loads
+            // issued together
+            for (int t0 = threadIdx.x; t0 < nt; t0 += SC_RUN * SC_THREADS) {
+                double fb[SC_RUN];
+#pragma unroll
+                for (int r = 0; r < SC_RUN; ++r) {
+                    const int t = t0 + r * SC_THREADS;
+                    if (t < nt) fb[r] = fph[o + t];
+                }
+#pragma unroll
+                for (int r = 0; r < SC_RUN; ++r) {
+                    const int t = t0 + r * SC_THREADS;
+                    if (t < nt) Fsc[o + t] = 2.0 * fb[r] / 3.0;
+                }
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -105,13 +119,27 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
             }
         }
     } else {
-        for (int t = threadIdx.x; t < nt; t += SC_THREADS) {
-            if (t % a.NAP >= a.n_atoms) continue;
-            const double f = fph[o + t];
-            const double m = m_atom[t % a.NAP];
-            fsq += f * f / m;
-            const double fm = f / m;
-            del += fm * fm;
+        // runs of SC_RUN dofs, loads first, sums in the same dof order
+        for (int t0 = threadIdx.x; t0 < nt; t0 += SC_RUN * SC_THREADS) {
+            double fb[SC_RUN], mb[SC_RUN];
+#pragma unroll
+            for (int r = 0; r < SC_RUN; ++r) {
+                const int t = t0 + r * SC_THREADS;
+                if (t < nt) {
+                    fb[r] = fph[o + t];
+                    mb[r] = m_atom[t % a.NAP];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < SC_RUN; ++r) {
+                const int t = t0 + r * SC_THREADS;
+                if (t < nt && t % a.NAP < a.n_atoms) {
+                    const double f = fb[r], m = mb[r];
+                    fsq += f * f / m;
+                    const double fm = f / m;
+                    del += fm * fm;
+                }
+            }
         }
     }
     fsq = block_sum(fsq, sh);
@@ -144,10 +172,26 @@ __global__ __launch_bounds__(SC_THREADS) void k_sc1(SCArgs a, const double *__re
             }
         }
     } else {
-        for (int t = threadIdx.x; t < nt; t += SC_THREADS) {
-            const double dq = sc_scale * fph[o + t] / m_atom[t % a.NAP];
-            Pp[t] = q[o + t] + dq;
-            Pm[t] = q[o + t] - dq;
+        for (int t0 = threadIdx.x; t0 < nt; t0 += SC_RUN * SC_THREADS) {
+            double fb[SC_RUN], mb[SC_RUN], qb[SC_RUN];
+#pragma unroll
+            for (int r = 0; r < SC_RUN; ++r) {
+                const int t = t0 + r * SC_THREADS;
+                if (t < nt) {
+                    fb[r] = fph[o + t];
+                    mb[r] = m_atom[t % a.NAP];
+                    qb[r] = q[o + t];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < SC_RUN; ++r) {
+                const int t = t0 + r * SC_THREADS;
+                if (t < nt) {
+                    const double dq = sc_scale * fb[r] / mb[r];
+                    Pp[t] = qb[r] + dq;
+                    Pm[t] = qb[r] - dq;
+                }
+            }
         }
     }
 }
