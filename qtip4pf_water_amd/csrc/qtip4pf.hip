@@ -262,7 +262,7 @@ void qtp_destroy(qtp_ctx *ctx)
     for (auto e : ctx->evpool) hipEventDestroy(e);
     void *ptrs[] = {ctx->q, ctx->p, ctx->F, ctx->fph, ctx->disp, ctx->m_atom, ctx->sqm_atom,
                     ctx->charge, ctx->point, ctx->list, ctx->fpoint, ctx->flist, ctx->cnt,
-                    ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart, ctx->rs_sites, ctx->rs_box, ctx->rs_far, ctx->nl_bm,
+                    ctx->flag, ctx->status, ctx->nreb, ctx->qpm, ctx->vpart, ctx->vout, ctx->rpart, ctx->rs_sites, ctx->rs_box, ctx->rs_far, ctx->nl_bm, ctx->cell_of, ctx->crank, ctx->cstart, ctx->member, ctx->cell_pos,
                     ctx->spart, ctx->sprime, ctx->flj, ctx->fspl, ctx->spl_coef, ctx->fpm, ctx->fk2, ctx->wm, ctx->gcol, ctx->mf_rt, ctx->mf_rt_small, ctx->ks_col, ctx->ks_unit, ctx->Ve, ctx->epart,
                     ctx->scal, ctx->out, ctx->heat, ctx->heat_step, ctx->step_ctr, ctx->groups,
                     ctx->gcf, ctx->grk, ctx->tiles, ctx->Cm, ctx->mode, ctx->nq, ctx->np,

@@ -40,8 +40,10 @@ __device__ __forceinline__ int cell_coord(double x, double L, double invL, int n
 // The rebuild test of update.cpp:356-385 (two largest O displacements,
 // flag = d1 + d2 > skin; or forced: force_all, or every > 0 and the step
 // about to complete is a multiple of every) and, on flagged beads, the
-// binning of the beads' O atoms: cell_of, cstart (nc + 1 offsets) and the
-// molecules of each cell in member (order within a cell arbitrary).
+// binning of the beads' O atoms: cell_of, cstart (nc + 1 offsets), the
+// molecules of each cell in member (order within a cell arbitrary) and
+// their O positions in the same order in mpos (x, y, z, 0), which k_nl_cell
+// reads with the member index instead of gathering them from q.
 // check = 0: the flags are given in flag[] (no test, nreb untouched).
 // grid: P, 1024 threads; dynamic LDS: ncell ints.
 __global__ __launch_bounds__(1024) void k_cell_build(int n_mols, int NAP, int NMP,
@@ -55,7 +57,8 @@ __global__ __launch_bounds__(1024) void k_cell_build(int n_mols, int NAP, int NM
                                                      int *__restrict__ cell_of,
                                                      int *__restrict__ crank,
                                                      int *__restrict__ cstart,
-                                                     int *__restrict__ member)
+                                                     int *__restrict__ member,
+                                                     double4 *__restrict__ mpos)
 {
     extern __shared__ int cc[];
     __shared__ double s1[16], s2[16];
@@ -148,7 +151,12 @@ __global__ __launch_bounds__(1024) void k_cell_build(int n_mols, int NAP, int NM
     if (tid == 0) CS[cg.ncell] = n_mols;
     __syncthreads();
     int *ME = member + (long)b * n_mols;
-    for (int i = tid; i < n_mols; i += 1024) ME[cc[CO[i]] + CR[i]] = i;
+    double4 *MP = mpos + (long)b * n_mols;
+    for (int i = tid; i < n_mols; i += 1024) {
+        const int m = cc[CO[i]] + CR[i];
+        ME[m] = i;
+        MP[m] = make_double4(Q[3 * i], Q[NAP + 3 * i], Q[2 * NAP + 3 * i], 0.0);
+    }
 }
 
 // Rows of the lists from the cell bins: one wavefront per molecule row,
@@ -167,6 +175,7 @@ __global__ __launch_bounds__(256) void k_nl_cell(int n_mols, int NAP, const doub
                                                  const int *__restrict__ cell_of,
                                                  const int *__restrict__ cstart,
                                                  const int *__restrict__ member,
+                                                 const double4 *__restrict__ mpos,
                                                  int *__restrict__ cnt,
                                                  const int *__restrict__ point,
                                                  const int *__restrict__ fpoint,
@@ -188,6 +197,7 @@ __global__ __launch_bounds__(256) void k_nl_cell(int n_mols, int NAP, const doub
         for (int k = lane; k < nw; k += 64) bm[k] = 0ull;
         const int *CS = cstart + (long)b * (cg.ncell + 1);
         const int *ME = member + (long)b * n_mols;
+        const double4 *MP = mpos + (long)b * n_mols;
         const int c = cell_of[(long)b * n_mols + i];
         const int cx = c % cg.nc[0], cy = (c / cg.nc[0]) % cg.nc[1], cz = c / (cg.nc[0] * cg.nc[1]);
         const int span = 2 * cg.S + 1, nrun = 2 * span * span;
@@ -237,11 +247,13 @@ __global__ __launch_bounds__(256) void k_nl_cell(int n_mols, int NAP, const doub
                     if (rpre[w][m] <= t) a = m;
                     else e = m;
                 }
-                const int j = ME[rlo[w][a] + (t - rpre[w][a])];
+                const int m = rlo[w][a] + (t - rpre[w][a]);
+                const int j = ME[m];
+                const double4 pj = MP[m];  // O of j, as in q
                 if (j != i) {
-                    const double rx = mimg_exact(xi - Q[3 * j], box.L[0]);
-                    const double ry = mimg_exact(yi - Q[NAP + 3 * j], box.L[1]);
-                    const double rz = mimg_exact(zi - Q[2 * NAP + 3 * j], box.L[2]);
+                    const double rx = mimg_exact(xi - pj.x, box.L[0]);
+                    const double ry = mimg_exact(yi - pj.y, box.L[1]);
+                    const double rz = mimg_exact(zi - pj.z, box.L[2]);
                     if (rx * rx + ry * ry + rz * rz < r_skin_sq)
                         atomicOr(&bm[j >> 6], 1ull << (j & 63));
                 }

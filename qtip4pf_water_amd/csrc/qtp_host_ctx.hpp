@@ -90,6 +90,7 @@ struct qtp_ctx {
     bool cells_ok = false;
     int list_method = 0, rebuild_every = 0;
     int *cell_of = nullptr, *crank = nullptr, *cstart = nullptr, *member = nullptr;
+    double4 *cell_pos = nullptr;  // O positions in member order
     // the count pass's row bitmaps for the fill pass (when they take at most
     // NL_BM_MAX_BYTES; otherwise the fill pass tests the candidates again)
     unsigned long long *nl_bm = nullptr;

@@ -285,21 +285,21 @@ static int run_cell_lists(qtp_ctx *ctx, int check, int force_all,
             ctx->n_mols, ctx->NAP, ctx->NMP, ctx->q, ctx->box, ctx->cg,
             ctx->drv.r_skin - ctx->drv.lj_rcut, check, force_all, ctr ? ctx->rebuild_every : 0, ctr,
             ctx->disp, ctx->flag, check ? ctx->nreb : nullptr, ctx->cell_of, ctx->crank,
-            ctx->cstart, ctx->member);
+            ctx->cstart, ctx->member, ctx->cell_pos);
     }
     {
         TimeScope ts(ctx, "nlist");
         k_nl_cell<false><<<P * nrb, 256, bm_lds, s>>>(
             ctx->n_mols, ctx->NAP, ctx->q, ctx->box, ctx->cg, ctx->ff.r_skin_sq, ctx->list_size,
-            ctx->fcap, ctx->flag, ctx->cell_of, ctx->cstart, ctx->member, ctx->cnt, nullptr, nullptr,
-            nullptr, nullptr, ctx->nl_bm);
+            ctx->fcap, ctx->flag, ctx->cell_of, ctx->cstart, ctx->member, ctx->cell_pos, ctx->cnt,
+            nullptr, nullptr, nullptr, nullptr, ctx->nl_bm);
         k_nl_scan<<<P, 1024, 0, s>>>(ctx->n_mols, ctx->list_size, ctx->fcap, ctx->flag, ctx->cnt,
                                      ctx->point, ctx->fpoint, ctx->list, ctx->disp, ctx->NMP,
                                      ctx->status);
         k_nl_cell<true><<<P * nrb, 256, bm_lds, s>>>(
             ctx->n_mols, ctx->NAP, ctx->q, ctx->box, ctx->cg, ctx->ff.r_skin_sq, ctx->list_size,
-            ctx->fcap, ctx->flag, ctx->cell_of, ctx->cstart, ctx->member, nullptr, ctx->point,
-            ctx->fpoint, ctx->list, ctx->flist, ctx->nl_bm);
+            ctx->fcap, ctx->flag, ctx->cell_of, ctx->cstart, ctx->member, ctx->cell_pos, nullptr,
+            ctx->point, ctx->fpoint, ctx->list, ctx->flist, ctx->nl_bm);
     }
     HIPCHK(hipGetLastError());
     return QTP_OK;

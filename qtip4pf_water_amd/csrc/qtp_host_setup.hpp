@@ -299,6 +299,7 @@ static int alloc_all(qtp_ctx *ctx)
         CHK(dalloc(ctx, &ctx->cell_of, (size_t)P * ctx->n_mols));
         CHK(dalloc(ctx, &ctx->crank, (size_t)P * ctx->n_mols));
         CHK(dalloc(ctx, &ctx->member, (size_t)P * ctx->n_mols));
+        CHK(dalloc(ctx, &ctx->cell_pos, (size_t)P * ctx->n_mols));
         CHK(dalloc(ctx, &ctx->cstart, (size_t)P * (ctx->cg.ncell + 1)));
         const size_t nbm = (size_t)P * ctx->n_mols * ((ctx->n_mols + 63) / 64);
         if (nbm * sizeof(unsigned long long) <= NL_BM_MAX_BYTES) CHK(dalloc(ctx, &ctx->nl_bm, nbm));
