@@ -158,3 +158,24 @@ def test_thermostat_normals_moments_and_independence(lib):
     assert abs(np.corrcoef(a[:-1], a[1:])[0, 1]) < 0.02          # index i vs i + 1
     b = np.array([_philox_normal(seed, 8, 1, i) for i in range(n)])
     assert abs(np.corrcoef(a, b)[0, 1]) < 0.02                   # step vs step + 1
+
+
+def test_destroy_frees_every_context_buffer():
+    """Every device buffer the context allocates (dalloc(ctx, &ctx->X, ...)
+    anywhere in the library's sources) is in qtp_destroy's free list, so a
+    destroyed context returns all of its memory.  A source check: it needs
+    no GPU and fails on the first buffer added without its free."""
+    import glob
+    import re
+    src_dir = os.path.join(os.path.dirname(__file__), "..", "qtip4pf_water_amd", "csrc")
+    text = ""
+    for path in sorted(glob.glob(os.path.join(src_dir, "*.hpp")) +
+                       glob.glob(os.path.join(src_dir, "*.hip"))):
+        with open(path) as f:
+            text += f.read()
+    allocated = set(re.findall(r"dalloc\(ctx,\s*&ctx->(\w+)", text))
+    body = text[text.index("void qtp_destroy(qtp_ctx *ctx)"):]
+    ptrs = body[body.index("void *ptrs[] = {"):]
+    freed = set(re.findall(r"ctx->(\w+)", ptrs[:ptrs.index("};")]))
+    assert allocated, "no dalloc calls found"
+    assert not (allocated - freed), sorted(allocated - freed)
