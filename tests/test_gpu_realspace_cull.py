@@ -109,14 +109,16 @@ def _pair_distances_near(sites, box, lo, hi):
     return np.sort(np.concatenate(out))
 
 
-def test_culled_virial_matches_energy_differences(oracle_mod):
+@pytest.mark.parametrize("split", [0, 3])
+def test_culled_virial_matches_energy_differences(oracle_mod, split):
     """The virial form of the culled kernel: 1000 H2O (box 2.66 r_c), the
     diagonal of W = -dV/d(strain) against central differences of the GPU's
     own energy in stretched boxes at fixed Ewald parameters.  Strain 1e-8,
     and r_c moved into a gap of the site-pair distances at least 2e-7 r_c
     wide on either side, so no pair crosses the cutoff; tolerance 1e-6 of the
     largest component (the differences carry ~1e-13 Eh of rounding over
-    2e-8)."""
+    2e-8).  split: eighths of the tiles on the second stream (the skipped
+    ones' virial partials zeroed ahead of both launches)."""
     from qtip4pf_water_amd import Water
     n_side, L, h = 10, 31.07, 1e-8
     n = n_side ** 3
@@ -134,12 +136,13 @@ def test_culled_virial_matches_energy_differences(oracle_mod):
     assert gaps[k] > 4e-7 * ew["rcut"], gaps[k]
     ew["rcut"] = 0.5 * (r[k] + r[k + 1])
     assert box[0] >= 2.5 * ew["rcut"]
-    say("culled virial FD", n, "mols, r_c", ew["rcut"], "gap", gaps[k])
+    say("culled virial FD", n, "mols, r_c", ew["rcut"], "gap", gaps[k], "split", split)
 
     def energy_virial(q, b):
         o = oracle_mod.Oracle(n, 1, b)
         point, lst, _ = o.all_lists(q)
         with Water(n, 1, b, ewald=ew) as w:
+            w.set_tuning(rs_split=split)
             return w.get_forces_virial(q, np.zeros(o.n_dof), o.charge, point, lst, 0)
 
     _, W = energy_virial(q0, box)
